@@ -1,0 +1,99 @@
+/*
+ * rockable_amd.h -- C ABI of the MI355X DEM time-step engine.
+ *
+ * The engine runs Rockable's velocity-Verlet step (OBB neighbour list,
+ * sphero-polyhedral sub-element contacts, ForceLaw with breakable bonds,
+ * per-body force/moment reduction, quaternion update) on one AMD Instinct
+ * MI355X (gfx950), in FP64, with results identical to Rockable's CPU path.
+ *
+ * Conventions: plain pointers and sizes; every call returns 0 on success and
+ * a negative code on error (rk_last_error() then describes it); nothing calls
+ * exit().  One engine per host thread; calls on one engine are serialised.
+ * Each engine owns one device stream and all its device memory.
+ *
+ * Reference interfaces replaced (paths relative to the Rockable tree):
+ *   rk_engine_create/destroy   class Rockable ctor/dtor (src/Core/Rockable.cpp:78-127)
+ *   rk_load_conf               Rockable::loadConf (Rockable.cpp:1085-1145) incl.
+ *                              the trailing accelerations() "fake step"
+ *   rk_load_driving_system     DrivingSystem::read (src/Core/DrivingSystem.cpp:44-268)
+ *   rk_set_force_law           conf keyword forceLaw -> Factory<ForceLaw>::Create,
+ *                              plug, init (Rockable.cpp:763-781; ForceLaw.hpp:44-55)
+ *   rk_set_body_force          conf keyword BodyForce (Rockable.cpp:1026-1040;
+ *                              src/BodyForces/BodyForce.hpp:45-57)
+ *   rk_update_nl               Rockable::UpdateNL -> UpdateNL_bruteForce /
+ *                              UpdateNL_linkCells (Rockable.cpp:1614-1809)
+ *   rk_accelerations           Rockable::accelerations (Rockable.cpp:3562-3719)
+ *   rk_step                    body of the integrate() loop (Rockable.cpp:2584-2745)
+ *   rk_integrate               Rockable::integrate (Rockable.cpp:2538-2746)
+ *   rk_save_conf               Rockable::saveConf (Rockable.cpp:440-610)
+ *   rk_get_bodies              Particles[i].pos/Q/vel/vrot/acc/arot/force/moment
+ *   rk_get_interactions        the Interactions sets (Rockable.hpp:107) and
+ *                              activeInteractions, lexicographic order
+ *                              (Interaction.hpp:104-122)
+ */
+#ifndef ROCKABLE_AMD_H
+#define ROCKABLE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rk_engine rk_engine;
+
+/* Interaction record: the fields of a conf "Interactions" line
+ * (Rockable.cpp:571-578) plus the interface index of a bond (-1 if none). */
+typedef struct {
+  int32_t i, j, type, isub, jsub, stick;
+  double n[3], dn, pos[3], vel[3], fn, ft[3], mom[3], damp;
+} rk_interaction;
+
+typedef struct {
+  int64_t nbodies, ndriven, nlisted, nactive, ninterfaces, nl_updates, steps;
+  double t, dt, total_broken_area;
+  int64_t last_candidates; /* body pairs passed by the last broad phase */
+} rk_counts;
+
+int rk_engine_create(rk_engine** out, int device);
+void rk_engine_destroy(rk_engine* eng);
+const char* rk_last_error(const rk_engine* eng);
+
+int rk_load_conf(rk_engine* eng, const char* path);
+int rk_load_driving_system(rk_engine* eng, const char* path);
+/* Returns 1 (not an error) when the name is unknown and Default is used. */
+int rk_set_force_law(rk_engine* eng, const char* name);
+/* name: AttractingPoint (x y z acceleration), PreferredDirection (axisBody.xyz
+ * axis.xyz momentMax), ViscousFluid (fluidDensity), or "" to remove. */
+int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int nargs);
+/* Options: "record_geometry" (1: keep n/pos/vel of every listed interaction
+ * current each step; 0: only at conf-save steps and for bonds),
+ * "profile" (1: time each kernel with device events). */
+int rk_set_option(rk_engine* eng, const char* name, double value);
+
+int rk_update_nl(rk_engine* eng);
+int rk_accelerations(rk_engine* eng);
+int rk_step(rk_engine* eng, int64_t nsteps);
+int rk_integrate(rk_engine* eng, const char* outdir, int64_t max_steps, int64_t* steps_done);
+int rk_save_conf(rk_engine* eng, const char* path);
+int rk_synchronize(rk_engine* eng);
+
+int rk_get_counts(rk_engine* eng, rk_counts* out);
+/* Any pointer may be NULL; arrays hold 3 doubles per body (Q: w x y z). */
+int rk_get_bodies(rk_engine* eng, double* pos, double* Q, double* vel, double* vrot, double* acc, double* arot,
+                  double* force, double* moment, size_t nbodies);
+int rk_set_bodies(rk_engine* eng, const double* pos, const double* Q, const double* vel, const double* vrot,
+                  size_t nbodies);
+/* active_only: 1 for the active interactions (dn < 0 or bonded), 0 for every
+ * listed one.  *n receives the total; at most cap records are written. */
+int rk_get_interactions(rk_engine* eng, int active_only, rk_interaction* out, size_t cap, size_t* n);
+/* Accumulated device time of a kernel ("predict", "interact", "gather",
+ * "breakage", "nl") while "profile" is on. */
+int rk_get_kernel_time(rk_engine* eng, const char* kernel, int64_t* launches, double* total_ms);
+int rk_reset_kernel_time(rk_engine* eng);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

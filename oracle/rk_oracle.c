@@ -2282,6 +2282,7 @@ static void fput_v3e(FILE* f, int prec, v3 v) {
 
 /* saveConf (Rockable.cpp:440-610) */
 int rko_save_conf(rko_sim* s, const char* path) {
+  build_active(s); /* the listed entries that were active at the last accelerations() */
   FILE* f = fopen(path, "w");
   if (!f) return seterr(s, "cannot write %s", path);
   fprintf(f, "Rockable 21-08-2022\n");
@@ -2526,7 +2527,10 @@ size_t rko_nlisted(rko_sim* s) {
   for (int k = 0; k < s->nb; k++) n += (size_t)s->inter[k].n;
   return n;
 }
-size_t rko_nactive(rko_sim* s) { return s->nactive; }
+size_t rko_nactive(rko_sim* s) {
+  build_active(s);
+  return s->nactive;
+}
 size_t rko_ninterfaces(rko_sim* s) {
   size_t n = 0;
   for (int k = 0; k < s->nifc; k++) n += (size_t)s->ifc[k].alive;

@@ -1,0 +1,152 @@
+// rk_device.hpp -- device-resident state of the DEM engine and the kernel
+// entry points.
+//
+// Layout in HBM (all FP64 unless stated):
+//   bodies   BodyKin[N]   128 B record read by every gather of the step
+//                         (pos, Q, vel, vrot, Minkowski radius, homothety,
+//                          shape id, group id)
+//            acc/arot/force/moment  3 doubles per body (xyz interleaved)
+//            mass[N], inertia[3N], cluster[N] (int)
+//   shapes   vertex table (3 doubles / vertex), edge pairs (2 int), face CSR
+//   entries  one per listed interaction, sorted by (i, j, type, isub, jsub):
+//            ei, ej (int), esub (uint32: type<<30 | isub<<15 | jsub)
+//            EState  {dn, fn, ft.xyz, mom.xyz}       64 B history record
+//            EGeom   {n.xyz, pos.xyz, vel.xyz}        72 B geometry record
+//            damp, stick (int, interface index or -1), act (uint8)
+//            contrib {f.xyz, mi.xyz, mj.xyz}          72 B per active entry
+//   CSR      istart[N+1]  entries owned by body b are [istart[b], istart[b+1])
+//            jstart[N+1], jlist[]  entries whose j == b, ascending index
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rk_math.hpp"
+
+namespace rk {
+
+struct alignas(16) BodyKin {
+  double px, py, pz;
+  double qw, qx, qy, qz;
+  double vx, vy, vz;
+  double wx, wy, wz;
+  double R;  // homothety * shape radius (Particle::MinkowskiRadius)
+  double h;  // homothety
+  int32_t shape;
+  int32_t group;
+};
+static_assert(sizeof(BodyKin) == 128, "BodyKin must stay one 128-byte record");
+
+struct alignas(16) EState {
+  double dn, fn;
+  double ftx, fty, ftz;
+  double momx, momy, momz;
+};
+static_assert(sizeof(EState) == 64, "EState is 64 bytes");
+
+struct EGeom {
+  double nx, ny, nz;
+  double px, py, pz;
+  double vx, vy, vz;
+};
+
+struct DevControl {
+  int32_t type, i;
+  double value;
+  double vx, vy, vz;
+};
+
+struct DevShapes {
+  const int32_t* nv;
+  const int32_t* ne;
+  const int32_t* nf;
+  const int32_t* voff;    // first vertex of the shape in vtx
+  const int32_t* eoff;    // first edge of the shape in edge
+  const int32_t* foff;    // first face of the shape in fstart
+  const double* vtx;      // 3 per vertex
+  const int32_t* edge;    // 2 per edge (local vertex indices)
+  const int32_t* fstart;  // per global face: start in fidx; one trailing entry
+  const int32_t* fidx;    // local vertex indices
+  const double* radius;
+  const double* obb;      // 15 per shape: c, e0, e1, e2, ext
+};
+
+// Group-pair parameter table: tab[(id * G + g1) * G + g2].
+struct DevParams {
+  const double* tab;
+  const double* damp_rate;  // per group pair: Rockable.cpp:1368-1370 rate
+  const int32_t* damp_kind; // 0 rate form, 1 en2 <= 0, 2 zero
+  int G;
+};
+
+struct DevInterfaces {
+  int n;
+  const uint64_t* key;  // (i << 32) | j, sorted
+  const double* par;    // 8 per interface: kn kt kr fn0 ft0 mom0 power dn0
+  const int32_t* isInner;
+  int32_t* alive;
+  uint8_t* brk;         // set by the force law when the yield criterion is met
+  const int32_t* bond_off;  // bonds of interface k: bond_key[bond_off[k] .. bond_off[k+1])
+  const int32_t* bond_key;  // 3 per bond: type isub jsub
+};
+
+struct Dev {
+  int nb, nDriven;
+  BodyKin* kin;
+  double* acc;
+  double* arot;
+  double* force;
+  double* moment;
+  const double* mass;
+  const double* inertia;
+  const int32_t* cluster;
+  DevShapes sh;
+  DevParams par;
+  // entries
+  int ne;
+  int32_t* ei;
+  int32_t* ej;
+  uint32_t* esub;
+  EState* est;
+  EGeom* egeo;
+  double* edamp;
+  int32_t* estick;
+  uint8_t* eact;
+  double* econ;
+  const int32_t* istart;
+  const int32_t* jstart;
+  const int32_t* jlist;
+  // driving
+  int nctrl;
+  DevControl* ctrl;
+  const int32_t* ctrl_off;   // per driven body, [ctrl_off[b], ctrl_off[b+1]) into ctrl_idx
+  const int32_t* ctrl_idx;
+  int servo, servo_icontr, servo_ids[6];
+  double servo_pressure;
+  // body force
+  int bf;
+  double bf_point[3], bf_acc, bf_axisBody[3], bf_axis[3], bf_Kr, bf_fluidDensity;
+  const double* bf_surface;  // per body pow(volume h^3, 2/3) for ViscousFluid
+  DevInterfaces ifc;
+  int paramsInInterfaces;
+  int32_t* flags;  // [0] needUpdate
+};
+
+struct StepParams {
+  double dt, dt_2, dt2_2;
+  double gx, gy, gz;
+  int law;
+};
+
+RK_HD uint32_t pack_sub(int type, int isub, int jsub) {
+  return ((uint32_t)type << 30) | ((uint32_t)isub << 15) | (uint32_t)jsub;
+}
+
+// rk_step.hip
+void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s);
+void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s);
+void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s);
+void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);
+
+}  // namespace rk

@@ -1,0 +1,1008 @@
+// rk_engine.cpp -- the engine: host orchestration of the device step and the
+// exported C ABI (include/rockable_amd.h).
+//
+// The host keeps the scalar time-loop arithmetic of Rockable::integrate
+// (t, interConfC, interVerletC accumulated in double exactly as
+// Rockable.cpp:2584-2715 does) and decides conf saves and neighbour-list
+// rebuilds from it; all per-body and per-interaction arithmetic runs on the
+// device.  There is no CPU fallback: without a usable gfx950 device every
+// entry point fails.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rockable_amd.h"
+#include "rk_device.hpp"
+#include "rk_model.hpp"
+#include "rk_nl.hpp"
+
+using namespace rk;
+
+struct KernelTimer {
+  int64_t launches = 0;
+  double ms = 0.0;
+};
+
+struct rk_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Model m;
+  bool loaded = false;
+  int record_geometry = 1;
+  int profile = 0;
+  std::map<std::string, KernelTimer> timers;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // device buffers
+  BodyKin* kin = nullptr;
+  double *acc = nullptr, *arot = nullptr, *force = nullptr, *moment = nullptr;
+  double *mass = nullptr, *inertia = nullptr, *bf_surface = nullptr;
+  int32_t* cluster = nullptr;
+  int32_t *sh_nv = nullptr, *sh_ne = nullptr, *sh_nf = nullptr, *sh_voff = nullptr, *sh_eoff = nullptr,
+          *sh_foff = nullptr, *sh_edge = nullptr, *sh_fstart = nullptr, *sh_fidx = nullptr;
+  double *sh_vtx = nullptr, *sh_radius = nullptr, *sh_obb = nullptr;
+  double *ptab = nullptr, *damp_rate = nullptr;
+  int32_t* damp_kind = nullptr;
+  uint64_t* ifc_key = nullptr;
+  double* ifc_par = nullptr;
+  int32_t *ifc_inner = nullptr, *ifc_alive = nullptr, *ifc_bond_off = nullptr, *ifc_bond_key = nullptr;
+  uint8_t* ifc_brk = nullptr;
+  DevControl* ctrl = nullptr;
+  int32_t *ctrl_off = nullptr, *ctrl_idx = nullptr;
+  int32_t* flags = nullptr;
+  EntryArrays E[2];
+  int cur = 0;
+  NLWork w;
+  NLParams nlp;
+  Dev d;
+  bool needUpdate = false;
+  int64_t steps = 0, nl_updates = 0;
+  int live_interfaces = 0;
+  std::vector<v3> deltaPos;
+  std::vector<qt> deltaQ;
+  double maxDeltaPos = 1.0, maxDeltaRot = 1.0;
+};
+
+namespace {
+
+int fail(rk_engine* e, const std::string& msg) {
+  e->err = msg;
+  return -1;
+}
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) return fail(eng, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int upload(rk_engine* eng, T*& dst, const std::vector<T>& src) {
+  if (dst) (void)hipFree(dst);
+  dst = nullptr;
+  size_t n = src.size() > 0 ? src.size() : 1;
+  HIPCHK(hipMalloc(&dst, n * sizeof(T)));
+  if (!src.empty()) HIPCHK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, eng->stream));
+  return 0;
+}
+
+void free_device(rk_engine* eng) {
+  void* ps[] = {eng->kin,      eng->acc,       eng->arot,      eng->force,     eng->moment,   eng->mass,
+                eng->inertia,  eng->bf_surface, eng->cluster,  eng->sh_nv,     eng->sh_ne,    eng->sh_nf,
+                eng->sh_voff,  eng->sh_eoff,   eng->sh_foff,   eng->sh_edge,   eng->sh_fstart, eng->sh_fidx,
+                eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->ptab,      eng->damp_rate, eng->damp_kind,
+                eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
+                eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  eng->kin = nullptr;
+  eng->acc = eng->arot = eng->force = eng->moment = eng->mass = eng->inertia = eng->bf_surface = nullptr;
+  eng->cluster = nullptr;
+  eng->sh_nv = eng->sh_ne = eng->sh_nf = eng->sh_voff = eng->sh_eoff = eng->sh_foff = eng->sh_edge = nullptr;
+  eng->sh_fstart = eng->sh_fidx = nullptr;
+  eng->sh_vtx = eng->sh_radius = eng->sh_obb = nullptr;
+  eng->ptab = eng->damp_rate = nullptr;
+  eng->damp_kind = nullptr;
+  eng->ifc_key = nullptr;
+  eng->ifc_par = nullptr;
+  eng->ifc_inner = eng->ifc_alive = eng->ifc_bond_off = eng->ifc_bond_key = nullptr;
+  eng->ifc_brk = nullptr;
+  eng->ctrl = nullptr;
+  eng->ctrl_off = eng->ctrl_idx = nullptr;
+  eng->flags = nullptr;
+  for (auto& E : eng->E) {
+    entries_free(E);
+    if (E.istart) (void)hipFree(E.istart);
+    if (E.jstart) (void)hipFree(E.jstart);
+    if (E.jlist) (void)hipFree(E.jlist);
+    E = EntryArrays();
+  }
+  nl_work_free(eng->w);
+}
+
+void assemble_dev(rk_engine* eng) {
+  Dev& d = eng->d;
+  const Model& m = eng->m;
+  EntryArrays& E = eng->E[eng->cur];
+  d.nb = (int)m.bodies.size();
+  d.nDriven = m.nDriven;
+  d.kin = eng->kin;
+  d.acc = eng->acc;
+  d.arot = eng->arot;
+  d.force = eng->force;
+  d.moment = eng->moment;
+  d.mass = eng->mass;
+  d.inertia = eng->inertia;
+  d.cluster = eng->cluster;
+  d.sh.nv = eng->sh_nv;
+  d.sh.ne = eng->sh_ne;
+  d.sh.nf = eng->sh_nf;
+  d.sh.voff = eng->sh_voff;
+  d.sh.eoff = eng->sh_eoff;
+  d.sh.foff = eng->sh_foff;
+  d.sh.vtx = eng->sh_vtx;
+  d.sh.edge = eng->sh_edge;
+  d.sh.fstart = eng->sh_fstart;
+  d.sh.fidx = eng->sh_fidx;
+  d.sh.radius = eng->sh_radius;
+  d.sh.obb = eng->sh_obb;
+  d.par.tab = eng->ptab;
+  d.par.damp_rate = eng->damp_rate;
+  d.par.damp_kind = eng->damp_kind;
+  d.par.G = std::max(1, m.ngroup);
+  d.ne = E.n;
+  d.ei = E.ei;
+  d.ej = E.ej;
+  d.esub = E.esub;
+  d.est = E.est;
+  d.egeo = E.egeo;
+  d.edamp = E.edamp;
+  d.estick = E.estick;
+  d.eact = E.eact;
+  d.econ = E.econ;
+  d.istart = E.istart;
+  d.jstart = E.jstart;
+  d.jlist = E.jlist;
+  d.nctrl = (int)m.ctrl.size();
+  d.ctrl = eng->ctrl;
+  d.ctrl_off = eng->ctrl_off;
+  d.ctrl_idx = eng->ctrl_idx;
+  d.servo = m.servo;
+  d.servo_icontr = m.servo_icontr;
+  for (int k = 0; k < 6; k++) d.servo_ids[k] = m.servo_ids[k];
+  d.servo_pressure = m.servo_pressure;
+  d.bf = m.bf;
+  d.bf_point[0] = m.bf_point.x;
+  d.bf_point[1] = m.bf_point.y;
+  d.bf_point[2] = m.bf_point.z;
+  d.bf_acc = m.bf_acc;
+  d.bf_axisBody[0] = m.bf_axisBody.x;
+  d.bf_axisBody[1] = m.bf_axisBody.y;
+  d.bf_axisBody[2] = m.bf_axisBody.z;
+  d.bf_axis[0] = m.bf_axis.x;
+  d.bf_axis[1] = m.bf_axis.y;
+  d.bf_axis[2] = m.bf_axis.z;
+  d.bf_Kr = m.bf_Kr;
+  d.bf_fluidDensity = m.bf_fluidDensity;
+  d.bf_surface = eng->bf_surface;
+  d.ifc.n = (int)m.ifc.size();
+  d.ifc.key = eng->ifc_key;
+  d.ifc.par = eng->ifc_par;
+  d.ifc.isInner = eng->ifc_inner;
+  d.ifc.alive = eng->ifc_alive;
+  d.ifc.brk = eng->ifc_brk;
+  d.ifc.bond_off = eng->ifc_bond_off;
+  d.ifc.bond_key = eng->ifc_bond_key;
+  d.paramsInInterfaces = m.paramsInInterfaces;
+  d.flags = eng->flags;
+}
+
+StepParams step_params(const rk_engine* eng) {
+  StepParams sp;
+  sp.dt = eng->m.dt;
+  sp.dt_2 = eng->m.dt_2;
+  sp.dt2_2 = eng->m.dt2_2;
+  sp.gx = eng->m.gravity.x;
+  sp.gy = eng->m.gravity.y;
+  sp.gz = eng->m.gravity.z;
+  sp.law = eng->m.law;
+  return sp;
+}
+
+// upload everything the model holds (after a conf load)
+int upload_model(rk_engine* eng) {
+  Model& m = eng->m;
+  const int nb = (int)m.bodies.size();
+  // shapes
+  std::vector<int32_t> nv, ne, nf, voff, eoff, foff, edge, fstart, fidx;
+  std::vector<double> vtx, radius, obb;
+  int maxv = 1;
+  int maxne = 0, maxnf = 0;
+  for (const Shape& s : m.shapes) {
+    voff.push_back((int)vtx.size() / 3);
+    eoff.push_back((int)edge.size() / 2);
+    foff.push_back((int)fstart.size());
+    nv.push_back(s.nv());
+    ne.push_back(s.ne());
+    nf.push_back(s.nf());
+    maxv = std::max(maxv, s.nv());
+    maxne = std::max(maxne, s.ne());
+    maxnf = std::max(maxnf, s.nf());
+    for (const v3& v : s.vertex) {
+      vtx.push_back(v.x);
+      vtx.push_back(v.y);
+      vtx.push_back(v.z);
+    }
+    for (int x : s.edge) edge.push_back(x);
+    const int base = (int)fidx.size();
+    for (int f = 0; f < s.nf(); f++) fstart.push_back(base + s.face_start[f]);
+    for (int x : s.face_idx) fidx.push_back(x);
+    radius.push_back(s.radius);
+    const obb_t& o = s.obb;
+    const double ob[15] = {o.c.x,    o.c.y,    o.c.z,    o.e[0].x, o.e[0].y, o.e[0].z, o.e[1].x, o.e[1].y,
+                           o.e[1].z, o.e[2].x, o.e[2].y, o.e[2].z, o.ext.x,  o.ext.y,  o.ext.z};
+    obb.insert(obb.end(), ob, ob + 15);
+  }
+  fstart.push_back((int)fidx.size());
+  if (maxv > 2048 || maxne > 8192 || maxnf > 8192) return fail(eng, "shape too large for the narrow phase");
+  const long maxT = (long)maxv * (maxv + maxne + maxnf) + (long)maxne * maxne + (long)maxv * (maxne + maxnf);
+  eng->nlp.maxv = maxv;
+  eng->nlp.maxwords = (int)((maxT + 63) / 64);
+  const size_t lds = sizeof(double) * 6 * maxv + sizeof(double) * ((2 * maxv + 7) / 8) + 8 * eng->nlp.maxwords;
+  if (lds > 65536) return fail(eng, "shapes have too many sub-elements for the narrow-phase LDS budget");
+  if (upload(eng, eng->sh_nv, nv) || upload(eng, eng->sh_ne, ne) || upload(eng, eng->sh_nf, nf) ||
+      upload(eng, eng->sh_voff, voff) || upload(eng, eng->sh_eoff, eoff) || upload(eng, eng->sh_foff, foff) ||
+      upload(eng, eng->sh_edge, edge) || upload(eng, eng->sh_fstart, fstart) || upload(eng, eng->sh_fidx, fidx) ||
+      upload(eng, eng->sh_vtx, vtx) || upload(eng, eng->sh_radius, radius) || upload(eng, eng->sh_obb, obb))
+    return -1;
+  // bodies
+  std::vector<BodyKin> kin(nb);
+  std::vector<double> acc(3 * nb), arot(3 * nb), zero3(3 * (size_t)nb, 0.0), mass(nb), inertia(3 * nb), surf(nb);
+  std::vector<int32_t> cl(nb);
+  for (int b = 0; b < nb; b++) {
+    const Body& p = m.bodies[b];
+    BodyKin& k = kin[b];
+    k.px = p.pos.x;
+    k.py = p.pos.y;
+    k.pz = p.pos.z;
+    k.qw = p.Q.w;
+    k.qx = p.Q.x;
+    k.qy = p.Q.y;
+    k.qz = p.Q.z;
+    k.vx = p.vel.x;
+    k.vy = p.vel.y;
+    k.vz = p.vel.z;
+    k.wx = p.vrot.x;
+    k.wy = p.vrot.y;
+    k.wz = p.vrot.z;
+    k.R = p.homothety * m.shapes[p.shape].radius;
+    k.h = p.homothety;
+    k.shape = p.shape;
+    k.group = p.group;
+    acc[3 * b] = p.acc.x;
+    acc[3 * b + 1] = p.acc.y;
+    acc[3 * b + 2] = p.acc.z;
+    arot[3 * b] = p.arot.x;
+    arot[3 * b + 1] = p.arot.y;
+    arot[3 * b + 2] = p.arot.z;
+    mass[b] = p.mass;
+    inertia[3 * b] = p.inertia.x;
+    inertia[3 * b + 1] = p.inertia.y;
+    inertia[3 * b + 2] = p.inertia.z;
+    cl[b] = p.cluster;
+    const double h = p.homothety;
+    surf[b] = pow(m.shapes[p.shape].volume * h * h * h, 2.0 / 3.0);  // BodyForce_ViscousFluid.cpp:58-59
+  }
+  if (upload(eng, eng->kin, kin) || upload(eng, eng->acc, acc) || upload(eng, eng->arot, arot) ||
+      upload(eng, eng->force, zero3) || upload(eng, eng->moment, zero3) || upload(eng, eng->mass, mass) ||
+      upload(eng, eng->inertia, inertia) || upload(eng, eng->cluster, cl) || upload(eng, eng->bf_surface, surf))
+    return -1;
+  // parameter tables (compact G = ngroup)
+  const int G = std::max(1, m.ngroup);
+  std::vector<double> tab((size_t)P_COUNT * G * G), rate((size_t)G * G);
+  std::vector<int32_t> kind((size_t)G * G);
+  for (int id = 0; id < P_COUNT; id++)
+    for (int a = 0; a < G; a++)
+      for (int b = 0; b < G; b++) tab[((size_t)id * G + a) * G + b] = m.param(id, a, b);
+  for (int a = 0; a < G; a++)
+    for (int b = 0; b < G; b++) {
+      const double en2 = m.param(P_EN2, a, b);
+      if (en2 > 0.0 && en2 < 1.0) {
+        const double logen = 0.5 * log(en2);
+        rate[a * G + b] = -logen / sqrt(logen * logen + M_PI * M_PI);
+        kind[a * G + b] = 0;
+      } else if (en2 <= 0.0) {
+        kind[a * G + b] = 1;
+      } else {
+        kind[a * G + b] = 2;
+      }
+    }
+  if (upload(eng, eng->ptab, tab) || upload(eng, eng->damp_rate, rate) || upload(eng, eng->damp_kind, kind)) return -1;
+  // interfaces
+  {
+    std::vector<uint64_t> key;
+    std::vector<double> par;
+    std::vector<int32_t> inner, alive, boff(1, 0), bkey;
+    for (const Interface& BI : m.ifc) {
+      key.push_back(((uint64_t)(uint32_t)BI.i << 32) | (uint32_t)BI.j);
+      const double p8[8] = {BI.kn, BI.kt, BI.kr, BI.fn0, BI.ft0, BI.mom0, BI.power, BI.dn0};
+      par.insert(par.end(), p8, p8 + 8);
+      inner.push_back(BI.isInner);
+      alive.push_back(BI.alive);
+      for (int x : BI.bkey) bkey.push_back(x);
+      boff.push_back((int)bkey.size() / 3);
+    }
+    std::vector<uint8_t> brk(m.ifc.size(), 0);
+    if (upload(eng, eng->ifc_key, key) || upload(eng, eng->ifc_par, par) || upload(eng, eng->ifc_inner, inner) ||
+        upload(eng, eng->ifc_alive, alive) || upload(eng, eng->ifc_bond_off, boff) ||
+        upload(eng, eng->ifc_bond_key, bkey) || upload(eng, eng->ifc_brk, brk))
+      return -1;
+    eng->live_interfaces = 0;
+    for (const Interface& BI : m.ifc) eng->live_interfaces += BI.alive;
+  }
+  // controls: list + per driven body CSR preserving list order
+  {
+    std::vector<DevControl> c;
+    for (const Control& x : m.ctrl) {
+      DevControl dc;
+      dc.type = x.type;
+      dc.i = x.i;
+      dc.value = x.value;
+      dc.vx = x.vec.x;
+      dc.vy = x.vec.y;
+      dc.vz = x.vec.z;
+      c.push_back(dc);
+    }
+    const int nd = std::max(0, std::min(m.nDriven, nb));
+    std::vector<int32_t> off(nd + 1, 0), idx;
+    for (int b = 0; b < nd; b++) {
+      for (size_t q = 0; q < m.ctrl.size(); q++)
+        if (m.ctrl[q].i == b) idx.push_back((int)q);
+      off[b + 1] = (int)idx.size();
+    }
+    if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
+  }
+  std::vector<int32_t> fl(4, 0);
+  if (upload(eng, eng->flags, fl)) return -1;
+  // loaded interactions: already lexicographic
+  EntryArrays& E = eng->E[eng->cur];
+  const int n = (int)m.inter.size();
+  if (entries_reserve(eng->w, E, n) != 0) return fail(eng, eng->w.err);
+  E.n = n;
+  if (n > 0) {
+    std::vector<int32_t> ei(n), ej(n), st(n);
+    std::vector<uint32_t> sub(n);
+    std::vector<EState> es(n);
+    std::vector<EGeom> eg(n);
+    std::vector<double> dp(n);
+    for (int k = 0; k < n; k++) {
+      const Inter& I = m.inter[k];
+      ei[k] = I.i;
+      ej[k] = I.j;
+      sub[k] = pack_sub(I.type, I.isub, I.jsub);
+      st[k] = I.stick;
+      EState s;
+      s.dn = I.dn;
+      s.fn = I.fn;
+      s.ftx = I.ft.x;
+      s.fty = I.ft.y;
+      s.ftz = I.ft.z;
+      s.momx = I.mom.x;
+      s.momy = I.mom.y;
+      s.momz = I.mom.z;
+      es[k] = s;
+      EGeom g;
+      g.nx = I.n.x;
+      g.ny = I.n.y;
+      g.nz = I.n.z;
+      g.px = I.pos.x;
+      g.py = I.pos.y;
+      g.pz = I.pos.z;
+      g.vx = I.vel.x;
+      g.vy = I.vel.y;
+      g.vz = I.vel.z;
+      eg[k] = g;
+      dp[k] = I.damp;
+    }
+    HIPCHK(hipMemcpyAsync(E.ei, ei.data(), 4 * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.ej, ej.data(), 4 * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.esub, sub.data(), 4 * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.estick, st.data(), 4 * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.est, es.data(), sizeof(EState) * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.egeo, eg.data(), sizeof(EGeom) * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.edamp, dp.data(), sizeof(double) * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+  }
+  if (nl_build_csr(eng->w, nb, E, eng->stream) != 0) return fail(eng, eng->w.err);
+  eng->nlp.DVerlet = m.DVerlet;
+  eng->nlp.dVerlet = m.dVerlet;
+  eng->nlp.linkcells = m.nl_linkcells;
+  eng->nlp.cellMinSizes[0] = m.cellMinSizes.x;
+  eng->nlp.cellMinSizes[1] = m.cellMinSizes.y;
+  eng->nlp.cellMinSizes[2] = m.cellMinSizes.z;
+  eng->nlp.boxForLinkCellsOpt = m.boxForLinkCellsOpt;
+  assemble_dev(eng);
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+void tic(rk_engine* eng) {
+  if (eng->profile) (void)hipEventRecord(eng->ev0, eng->stream);
+}
+void toc(rk_engine* eng, const char* name) {
+  if (!eng->profile) return;
+  (void)hipEventRecord(eng->ev1, eng->stream);
+  (void)hipEventSynchronize(eng->ev1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
+  KernelTimer& t = eng->timers[name];
+  t.launches++;
+  t.ms += ms;
+}
+
+// Rockable::accelerations (Rockable.cpp:3562-3719); kick fuses the closing
+// half kick of velocityVerletStep.
+int run_accelerations(rk_engine* eng, int kick, int write_geom) {
+  const StepParams sp = step_params(eng);
+  tic(eng);
+  launch_interact(eng->d, sp, write_geom || eng->record_geometry, eng->stream);
+  toc(eng, "interact");
+  tic(eng);
+  launch_gather(eng->d, sp, kick, eng->stream);
+  toc(eng, "gather");
+  if (eng->m.law == LAW_STICKED && eng->live_interfaces > 0) {
+    tic(eng);
+    launch_breakage(eng->d, sp, eng->stream);
+    toc(eng, "breakage");
+    int32_t fl = 0;
+    HIPCHK(hipMemcpyAsync(&fl, eng->flags, sizeof(int32_t), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    if (fl) {
+      // ordered accounting of the broken interfaces (Rockable.cpp:3391-3410)
+      std::vector<int32_t> alive(eng->m.ifc.size());
+      HIPCHK(hipMemcpy(alive.data(), eng->ifc_alive, 4 * alive.size(), hipMemcpyDeviceToHost));
+      eng->live_interfaces = 0;
+      for (size_t k = 0; k < alive.size(); k++) {
+        if (eng->m.ifc[k].alive && !alive[k]) eng->m.totalBrokenArea += eng->m.ifc[k].area;
+        eng->m.ifc[k].alive = alive[k];
+        eng->live_interfaces += alive[k];
+      }
+      std::vector<uint8_t> zero(eng->m.ifc.size(), 0);
+      HIPCHK(hipMemcpy(eng->ifc_brk, zero.data(), zero.size(), hipMemcpyHostToDevice));
+      int32_t z = 0;
+      HIPCHK(hipMemcpy(eng->flags, &z, sizeof(z), hipMemcpyHostToDevice));
+      eng->needUpdate = true;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int do_update_nl(rk_engine* eng) {
+  tic(eng);
+  const int nxt = 1 - eng->cur;
+  if (nl_rebuild(eng->w, eng->d, eng->nlp, eng->E[eng->cur], eng->E[nxt], eng->stream) != 0)
+    return fail(eng, std::string("neighbour list: ") + eng->w.err);
+  eng->cur = nxt;
+  assemble_dev(eng);
+  eng->nl_updates++;
+  toc(eng, "nl");
+  return 0;
+}
+
+int vv_step(rk_engine* eng, int write_geom) {
+  const StepParams sp = step_params(eng);
+  tic(eng);
+  launch_predict(eng->d, sp, eng->stream);
+  toc(eng, "predict");
+  return run_accelerations(eng, 1, write_geom);
+}
+
+int download_bodies(rk_engine* eng) {
+  Model& m = eng->m;
+  const int nb = (int)m.bodies.size();
+  std::vector<BodyKin> kin(nb);
+  std::vector<double> acc(3 * (size_t)nb), arot(3 * (size_t)nb), F(3 * (size_t)nb), M(3 * (size_t)nb);
+  if (nb == 0) return 0;
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  HIPCHK(hipMemcpy(kin.data(), eng->kin, sizeof(BodyKin) * nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(acc.data(), eng->acc, 8 * 3 * (size_t)nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(arot.data(), eng->arot, 8 * 3 * (size_t)nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(F.data(), eng->force, 8 * 3 * (size_t)nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(M.data(), eng->moment, 8 * 3 * (size_t)nb, hipMemcpyDeviceToHost));
+  for (int b = 0; b < nb; b++) {
+    Body& p = m.bodies[b];
+    const BodyKin& k = kin[b];
+    p.pos = V(k.px, k.py, k.pz);
+    p.Q.w = k.qw;
+    p.Q.x = k.qx;
+    p.Q.y = k.qy;
+    p.Q.z = k.qz;
+    p.vel = V(k.vx, k.vy, k.vz);
+    p.vrot = V(k.wx, k.wy, k.wz);
+    p.acc = V(acc[3 * b], acc[3 * b + 1], acc[3 * b + 2]);
+    p.arot = V(arot[3 * b], arot[3 * b + 1], arot[3 * b + 2]);
+    p.force = V(F[3 * b], F[3 * b + 1], F[3 * b + 2]);
+    p.moment = V(M[3 * b], M[3 * b + 1], M[3 * b + 2]);
+  }
+  return 0;
+}
+
+int download_interactions(rk_engine* eng, int active_only, std::vector<Inter>& out) {
+  const EntryArrays& E = eng->E[eng->cur];
+  const int n = E.n;
+  out.clear();
+  if (n == 0) return 0;
+  std::vector<int32_t> ei(n), ej(n), st(n);
+  std::vector<uint32_t> sub(n);
+  std::vector<EState> es(n);
+  std::vector<EGeom> eg(n);
+  std::vector<double> dp(n);
+  std::vector<uint8_t> act(n);
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  HIPCHK(hipMemcpy(ei.data(), E.ei, 4 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ej.data(), E.ej, 4 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sub.data(), E.esub, 4 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st.data(), E.estick, 4 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(es.data(), E.est, sizeof(EState) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(eg.data(), E.egeo, sizeof(EGeom) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dp.data(), E.edamp, 8 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(act.data(), E.eact, n, hipMemcpyDeviceToHost));
+  out.reserve(n);
+  for (int k = 0; k < n; k++) {
+    if (active_only && !act[k]) continue;
+    Inter I;
+    I.i = ei[k];
+    I.j = ej[k];
+    I.type = (int)(sub[k] >> 30);
+    I.isub = (int)((sub[k] >> 15) & 0x7fffu);
+    I.jsub = (int)(sub[k] & 0x7fffu);
+    I.stick = st[k];
+    I.n = V(eg[k].nx, eg[k].ny, eg[k].nz);
+    I.pos = V(eg[k].px, eg[k].py, eg[k].pz);
+    I.vel = V(eg[k].vx, eg[k].vy, eg[k].vz);
+    I.dn = es[k].dn;
+    I.fn = es[k].fn;
+    I.ft = V(es[k].ftx, es[k].fty, es[k].ftz);
+    I.mom = V(es[k].momx, es[k].momy, es[k].momz);
+    I.damp = dp[k];
+    out.push_back(I);
+  }
+  return 0;
+}
+
+int save_conf_now(rk_engine* eng, const std::string& path) {
+  if (download_bodies(eng) != 0) return -1;
+  std::vector<Inter> act;
+  if (download_interactions(eng, 1, act) != 0) return -1;
+  std::string err;
+  if (save_conf(eng->m, act, path, err) != 0) return fail(eng, err);
+  return 0;
+}
+
+int check_loaded(rk_engine* eng) {
+  if (!eng) return -1;
+  if (!eng->loaded) return fail(eng, "no configuration loaded");
+  return 0;
+}
+
+// dynamicCheckUpdateNL (Rockable.cpp:1571-1596) on downloaded velocities
+int dynamic_check(rk_engine* eng) {
+  Model& m = eng->m;
+  if (download_bodies(eng) != 0) return -1;
+  const int nb = (int)m.bodies.size();
+  for (int i = 0; i < nb; i++) {
+    eng->deltaPos[i] = vadd(eng->deltaPos[i], vsc(m.dt, m.bodies[i].vel));
+    double tmp1 = vnorm2(eng->deltaPos[i]);
+    if (tmp1 > eng->maxDeltaPos * eng->maxDeltaPos) eng->maxDeltaPos = sqrt(tmp1);
+    qt dq = qdot(m.bodies[i].Q, m.bodies[i].vrot);
+    eng->deltaQ[i].w += dq.w * m.dt;
+    eng->deltaQ[i].x += dq.x * m.dt;
+    eng->deltaQ[i].y += dq.y * m.dt;
+    eng->deltaQ[i].z += dq.z * m.dt;
+    double w = std::max(-1.0, std::min(1.0, eng->deltaQ[i].w));
+    double tmp2 = 2.0 * acos(w);
+    if (tmp2 > eng->maxDeltaRot) eng->maxDeltaRot = tmp2;
+  }
+  if (eng->maxDeltaPos >= m.dispUpdateNL || eng->maxDeltaRot >= m.angleUpdateNL) {
+    eng->needUpdate = true;
+    for (int i = 0; i < nb; i++) {
+      eng->deltaPos[i] = V(0, 0, 0);
+      eng->deltaQ[i] = qt{1.0, 0.0, 0.0, 0.0};
+    }
+    eng->maxDeltaPos = 0.0;
+    eng->maxDeltaRot = 0.0;
+  }
+  return 0;
+}
+
+void integrate_prepare(rk_engine* eng) {
+  Model& m = eng->m;
+  if (m.dynamicUpdateNL != 0) {
+    eng->deltaPos.assign(m.bodies.size(), V(0, 0, 0));
+    eng->deltaQ.assign(m.bodies.size(), qt{1.0, 0.0, 0.0, 0.0});
+    eng->maxDeltaPos = 0.0;
+    eng->maxDeltaRot = 0.0;
+  }
+  m.dt_2 = 0.5 * m.dt;
+  m.dt2_2 = m.dt_2 * m.dt;
+  m.dt2 = m.dt * m.dt;
+}
+
+// one iteration of the integrate() loop (Rockable.cpp:2584-2745)
+int loop_body(rk_engine* eng, const char* outdir) {
+  Model& m = eng->m;
+  m.t += m.dt;
+  m.interConfC += m.dt;
+  m.interVerletC += m.dt;
+  const bool conf_due = m.interConfC >= m.interConf - m.dt_2;
+  if (vv_step(eng, (conf_due && outdir) ? 1 : 0) != 0) return -1;
+  eng->steps++;
+  if (conf_due) {
+    m.iconf++;
+    if (outdir) {
+      char fn[4096];
+      snprintf(fn, sizeof fn, "%s/conf%d", outdir, m.iconf);
+      if (save_conf_now(eng, fn) != 0) return -1;
+    }
+    m.interConfC = 0.0;
+  }
+  if (eng->needUpdate || m.interVerletC >= m.interVerlet - m.dt_2) {
+    if (do_update_nl(eng) != 0) return -1;
+    if (!eng->needUpdate)
+      m.interVerletC = 0.0;
+    else
+      eng->needUpdate = false;
+  }
+  if (m.dynamicUpdateNL != 0)
+    if (dynamic_check(eng) != 0) return -1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rk_engine_create(rk_engine** out, int device) {
+  if (!out) return -1;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return -2;
+  if (hipSetDevice(device) != hipSuccess) return -2;
+  rk_engine* eng = new rk_engine();
+  eng->device = device;
+  if (hipStreamCreate(&eng->stream) != hipSuccess || hipEventCreate(&eng->ev0) != hipSuccess ||
+      hipEventCreate(&eng->ev1) != hipSuccess) {
+    delete eng;
+    return -2;
+  }
+  *out = eng;
+  return 0;
+}
+
+void rk_engine_destroy(rk_engine* eng) {
+  if (!eng) return;
+  (void)hipSetDevice(eng->device);
+  if (eng->stream) (void)hipStreamSynchronize(eng->stream);
+  free_device(eng);
+  if (eng->ev0) (void)hipEventDestroy(eng->ev0);
+  if (eng->ev1) (void)hipEventDestroy(eng->ev1);
+  if (eng->stream) (void)hipStreamDestroy(eng->stream);
+  delete eng;
+}
+
+const char* rk_last_error(const rk_engine* eng) { return eng ? eng->err.c_str() : "null engine"; }
+
+int rk_load_conf(rk_engine* eng, const char* path) {
+  if (!eng || !path) return -1;
+  (void)hipSetDevice(eng->device);
+  free_device(eng);
+  eng->m = Model();
+  eng->loaded = false;
+  eng->cur = 0;
+  eng->needUpdate = false;
+  eng->steps = 0;
+  eng->nl_updates = 0;
+  std::string err;
+  if (load_conf(eng->m, path, err) != 0) return fail(eng, err);
+  if (upload_model(eng) != 0) return -1;
+  eng->loaded = true;
+  // the "fake step" of loadConf (Rockable.cpp:1133-1138)
+  if (run_accelerations(eng, 0, 1) != 0) return -1;
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+int rk_load_driving_system(rk_engine* eng, const char* path) {
+  if (check_loaded(eng)) return -1;
+  std::string err;
+  if (load_driving(eng->m, path, err) != 0) return fail(eng, err);
+  std::vector<DevControl> c;
+  for (const Control& x : eng->m.ctrl) {
+    DevControl dc;
+    dc.type = x.type;
+    dc.i = x.i;
+    dc.value = x.value;
+    dc.vx = x.vec.x;
+    dc.vy = x.vec.y;
+    dc.vz = x.vec.z;
+    c.push_back(dc);
+  }
+  const int nd = std::max(0, std::min(eng->m.nDriven, (int)eng->m.bodies.size()));
+  std::vector<int32_t> off(nd + 1, 0), idx;
+  for (int b = 0; b < nd; b++) {
+    for (size_t q = 0; q < eng->m.ctrl.size(); q++)
+      if (eng->m.ctrl[q].i == b) idx.push_back((int)q);
+    off[b + 1] = (int)idx.size();
+  }
+  if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
+  assemble_dev(eng);
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+int rk_set_force_law(rk_engine* eng, const char* name) {
+  if (check_loaded(eng)) return -1;
+  std::string n = name ? name : "";
+  int rc = 0;
+  if (n == "Avalanche") {
+    eng->m.law = LAW_AVALANCHE;
+    eng->m.lawName = n;
+  } else if (n == "StickedLinks") {
+    eng->m.law = LAW_STICKED;
+    eng->m.lawName = n;
+  } else {
+    if (n != "Default") {
+      eng->err = "The ForceLaw named " + n + " is unknown! -> set to Default";
+      rc = 1;
+    }
+    eng->m.law = LAW_DEFAULT;
+    eng->m.lawName = "Default";
+  }
+  return rc;
+}
+
+int rk_set_body_force(rk_engine* eng, const char* name, const double* a, int nargs) {
+  if (check_loaded(eng)) return -1;
+  std::string n = name ? name : "";
+  Model& m = eng->m;
+  if (n.empty()) {
+    m.bf = BF_NONE;
+  } else if (n == "AttractingPoint" && nargs >= 4) {
+    m.bf = BF_ATTRACTING;
+    m.bf_point = V(a[0], a[1], a[2]);
+    m.bf_acc = a[3];
+  } else if (n == "PreferredDirection" && nargs >= 7) {
+    m.bf = BF_PREFDIR;
+    m.bf_axisBody = V(a[0], a[1], a[2]);
+    m.bf_axis = V(a[3], a[4], a[5]);
+    m.bf_momentMax = a[6];
+    vnormalize(&m.bf_axisBody);
+    vnormalize(&m.bf_axis);
+    m.bf_Kr = 2.0 * m.bf_momentMax / M_PI;
+  } else if (n == "ViscousFluid" && nargs >= 1) {
+    m.bf = BF_VISCOUS;
+    m.bf_fluidDensity = a[0];
+  } else {
+    return fail(eng, "unknown BodyForce '" + n + "' or wrong number of arguments");
+  }
+  assemble_dev(eng);
+  return 0;
+}
+
+int rk_set_option(rk_engine* eng, const char* name, double value) {
+  if (!eng || !name) return -1;
+  std::string n = name;
+  if (n == "record_geometry")
+    eng->record_geometry = value != 0.0;
+  else if (n == "profile")
+    eng->profile = value != 0.0;
+  else
+    return fail(eng, "unknown option " + n);
+  return 0;
+}
+
+int rk_update_nl(rk_engine* eng) {
+  if (check_loaded(eng)) return -1;
+  (void)hipSetDevice(eng->device);
+  return do_update_nl(eng);
+}
+
+int rk_accelerations(rk_engine* eng) {
+  if (check_loaded(eng)) return -1;
+  (void)hipSetDevice(eng->device);
+  if (run_accelerations(eng, 0, 1) != 0) return -1;
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+int rk_step(rk_engine* eng, int64_t nsteps) {
+  if (check_loaded(eng)) return -1;
+  (void)hipSetDevice(eng->device);
+  integrate_prepare(eng);
+  for (int64_t k = 0; k < nsteps; k++)
+    if (loop_body(eng, nullptr) != 0) return -1;
+  return 0;
+}
+
+int rk_integrate(rk_engine* eng, const char* outdir, int64_t max_steps, int64_t* steps_done) {
+  if (check_loaded(eng)) return -1;
+  (void)hipSetDevice(eng->device);
+  integrate_prepare(eng);
+  if (outdir) {
+    char fn[4096];
+    snprintf(fn, sizeof fn, "%s/conf%d", outdir, eng->m.iconf);
+    if (save_conf_now(eng, fn) != 0) return -1;
+  }
+  int64_t n = 0;
+  while (eng->m.t <= eng->m.tmax && (max_steps < 0 || n < max_steps)) {
+    if (loop_body(eng, outdir) != 0) return -1;
+    n++;
+  }
+  if (steps_done) *steps_done = n;
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+int rk_save_conf(rk_engine* eng, const char* path) {
+  if (check_loaded(eng)) return -1;
+  return save_conf_now(eng, path);
+}
+
+int rk_synchronize(rk_engine* eng) {
+  if (!eng) return -1;
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+int rk_get_counts(rk_engine* eng, rk_counts* out) {
+  if (check_loaded(eng) || !out) return -1;
+  const EntryArrays& E = eng->E[eng->cur];
+  out->nbodies = (int64_t)eng->m.bodies.size();
+  out->ndriven = eng->m.nDriven;
+  out->nlisted = E.n;
+  int64_t na = 0;
+  if (E.n > 0) {
+    std::vector<uint8_t> act(E.n);
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    HIPCHK(hipMemcpy(act.data(), E.eact, E.n, hipMemcpyDeviceToHost));
+    for (uint8_t a : act) na += a;
+  }
+  out->nactive = na;
+  out->ninterfaces = eng->live_interfaces;
+  out->nl_updates = eng->nl_updates;
+  out->steps = eng->steps;
+  out->t = eng->m.t;
+  out->dt = eng->m.dt;
+  out->total_broken_area = eng->m.totalBrokenArea;
+  out->last_candidates = eng->w.last_candidates;
+  return 0;
+}
+
+int rk_get_bodies(rk_engine* eng, double* pos, double* Q, double* vel, double* vrot, double* acc, double* arot,
+                  double* force, double* moment, size_t nbodies) {
+  if (check_loaded(eng)) return -1;
+  if (nbodies < eng->m.bodies.size()) return fail(eng, "output arrays too small");
+  if (download_bodies(eng) != 0) return -1;
+  for (size_t b = 0; b < eng->m.bodies.size(); b++) {
+    const Body& p = eng->m.bodies[b];
+    auto put = [&](double* a, v3 v) {
+      if (a) {
+        a[3 * b] = v.x;
+        a[3 * b + 1] = v.y;
+        a[3 * b + 2] = v.z;
+      }
+    };
+    put(pos, p.pos);
+    put(vel, p.vel);
+    put(vrot, p.vrot);
+    put(acc, p.acc);
+    put(arot, p.arot);
+    put(force, p.force);
+    put(moment, p.moment);
+    if (Q) {
+      Q[4 * b] = p.Q.w;
+      Q[4 * b + 1] = p.Q.x;
+      Q[4 * b + 2] = p.Q.y;
+      Q[4 * b + 3] = p.Q.z;
+    }
+  }
+  return 0;
+}
+
+int rk_set_bodies(rk_engine* eng, const double* pos, const double* Q, const double* vel, const double* vrot,
+                  size_t nbodies) {
+  if (check_loaded(eng)) return -1;
+  const int nb = (int)eng->m.bodies.size();
+  if (nbodies < (size_t)nb) return fail(eng, "input arrays too small");
+  std::vector<BodyKin> kin(nb);
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  HIPCHK(hipMemcpy(kin.data(), eng->kin, sizeof(BodyKin) * nb, hipMemcpyDeviceToHost));
+  for (int b = 0; b < nb; b++) {
+    BodyKin& k = kin[b];
+    if (pos) {
+      k.px = pos[3 * b];
+      k.py = pos[3 * b + 1];
+      k.pz = pos[3 * b + 2];
+    }
+    if (Q) {
+      k.qw = Q[4 * b];
+      k.qx = Q[4 * b + 1];
+      k.qy = Q[4 * b + 2];
+      k.qz = Q[4 * b + 3];
+    }
+    if (vel) {
+      k.vx = vel[3 * b];
+      k.vy = vel[3 * b + 1];
+      k.vz = vel[3 * b + 2];
+    }
+    if (vrot) {
+      k.wx = vrot[3 * b];
+      k.wy = vrot[3 * b + 1];
+      k.wz = vrot[3 * b + 2];
+    }
+  }
+  HIPCHK(hipMemcpy(eng->kin, kin.data(), sizeof(BodyKin) * nb, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int rk_get_interactions(rk_engine* eng, int active_only, rk_interaction* out, size_t cap, size_t* n) {
+  if (check_loaded(eng)) return -1;
+  std::vector<Inter> v;
+  if (download_interactions(eng, active_only, v) != 0) return -1;
+  if (n) *n = v.size();
+  if (out) {
+    const size_t k = std::min(cap, v.size());
+    for (size_t q = 0; q < k; q++) {
+      const Inter& I = v[q];
+      rk_interaction& o = out[q];
+      o.i = I.i;
+      o.j = I.j;
+      o.type = I.type;
+      o.isub = I.isub;
+      o.jsub = I.jsub;
+      o.stick = I.stick;
+      o.n[0] = I.n.x;
+      o.n[1] = I.n.y;
+      o.n[2] = I.n.z;
+      o.dn = I.dn;
+      o.pos[0] = I.pos.x;
+      o.pos[1] = I.pos.y;
+      o.pos[2] = I.pos.z;
+      o.vel[0] = I.vel.x;
+      o.vel[1] = I.vel.y;
+      o.vel[2] = I.vel.z;
+      o.fn = I.fn;
+      o.ft[0] = I.ft.x;
+      o.ft[1] = I.ft.y;
+      o.ft[2] = I.ft.z;
+      o.mom[0] = I.mom.x;
+      o.mom[1] = I.mom.y;
+      o.mom[2] = I.mom.z;
+      o.damp = I.damp;
+    }
+  }
+  return 0;
+}
+
+int rk_get_kernel_time(rk_engine* eng, const char* kernel, int64_t* launches, double* total_ms) {
+  if (!eng || !kernel) return -1;
+  auto it = eng->timers.find(kernel);
+  if (launches) *launches = it == eng->timers.end() ? 0 : it->second.launches;
+  if (total_ms) *total_ms = it == eng->timers.end() ? 0.0 : it->second.ms;
+  return 0;
+}
+
+int rk_reset_kernel_time(rk_engine* eng) {
+  if (!eng) return -1;
+  eng->timers.clear();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1069 @@
+// rk_nl.hip -- neighbour-list rebuild on the GPU.
+//
+// Restates UpdateNL_bruteForce (Rockable.cpp:1614-1662), UpdateNL_linkCells
+// (1667-1809) and AddOrRemoveInteractions_bruteForce (1309-1432) with the
+// sub-element predicates of Particle.cpp:88-209, keeping the reference's
+// set semantics exactly:
+//   * a key that is tested and near is kept (with its history) or created,
+//   * a key that is tested and not near is erased,
+//   * a key that is not tested (pair not a candidate, vertex gated out, or a
+//     type never tested in that direction) is retained unchanged.
+//
+// Pipeline (each step a kernel; counts come back to the host between steps):
+//   k_obb            Particle::updateObb (+ AABB for the cell grid)
+//   k_broad_*        candidate body pairs (i < j) whose enlarged OBBs overlap
+//   groups           directed (owner, other) groups = candidates x 2 directions
+//                    merged with the groups of the current list (radix sort)
+//   k_narrow         one wavefront per candidate pair: global vertices and
+//                    vertex gates in LDS, every predicate of both directions
+//                    evaluated 64 at a time, near bits kept as ballot masks,
+//                    then written as keys in (type, isub, jsub) order with a
+//                    wavefront prefix scan (no atomics per key)
+//   k_merge          per directed group, merge of the sorted old entries and
+//                    the sorted near keys (count pass, scan, write pass); the
+//                    output is again sorted by (i, j, type, isub, jsub)
+//   CSR              istart by owner; jlist = entries sorted (stably) by j
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "rk_device.hpp"
+#include "rk_model.hpp"
+#include "rk_nl.hpp"
+
+namespace rk {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+#define RK_CHECK(x)                                                                 \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      snprintf(w.err, sizeof(w.err), "%s failed: %s", #x, hipGetErrorString(e_)); \
+      return -1;                                                                    \
+    }                                                                               \
+  } while (0)
+
+__device__ __forceinline__ v3 kpos(const BodyKin& b) { return V(b.px, b.py, b.pz); }
+__device__ __forceinline__ qt kq(const BodyKin& b) {
+  qt q;
+  q.w = b.qw;
+  q.x = b.qx;
+  q.y = b.qy;
+  q.z = b.qz;
+  return q;
+}
+__device__ __forceinline__ v3 glob_vertex(const DevShapes& sh, const BodyKin& b, int v) {
+  const double* p = sh.vtx + 3 * (sh.voff[b.shape] + v);
+  v3 hv = vsc(b.h, V(p[0], p[1], p[2]));
+  return vadd(kpos(b), qrot(kq(b), hv));
+}
+__device__ __forceinline__ obb_t load_obb(const double* o) {
+  obb_t r;
+  r.c = V(o[0], o[1], o[2]);
+  r.e[0] = V(o[3], o[4], o[5]);
+  r.e[1] = V(o[6], o[7], o[8]);
+  r.e[2] = V(o[9], o[10], o[11]);
+  r.ext = V(o[12], o[13], o[14]);
+  return r;
+}
+__device__ __forceinline__ void store_obb(double* o, const obb_t& r) {
+  o[0] = r.c.x;
+  o[1] = r.c.y;
+  o[2] = r.c.z;
+  o[3] = r.e[0].x;
+  o[4] = r.e[0].y;
+  o[5] = r.e[0].z;
+  o[6] = r.e[1].x;
+  o[7] = r.e[1].y;
+  o[8] = r.e[1].z;
+  o[9] = r.e[2].x;
+  o[10] = r.e[2].y;
+  o[11] = r.e[2].z;
+  o[12] = r.ext.x;
+  o[13] = r.ext.y;
+  o[14] = r.ext.z;
+}
+
+// Particle::updateObb (Particle.cpp:78-86) and the body AABB of
+// Rockable::computeAABB (Rockable.cpp:3934-3941).
+__global__ void k_obb(Dev d, double* bobb, double* baabb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.nb) return;
+  const BodyKin P = d.kin[b];
+  obb_t o = load_obb(d.sh.obb + 15 * P.shape);
+  const qt Q = kq(P);
+  o.e[0] = qrot(Q, o.e[0]);
+  o.e[1] = qrot(Q, o.e[1]);
+  o.e[2] = qrot(Q, o.e[2]);
+  o.c = qrot(Q, o.c);
+  o.ext = vsc(P.h, o.ext);
+  o.c = vsc(P.h, o.c);
+  o.c = vadd(o.c, kpos(P));
+  store_obb(bobb + 15 * (size_t)b, o);
+  if (baabb) {
+    v3 mn = glob_vertex(d.sh, P, 0), mx = mn;
+    const int nv = d.sh.nv[P.shape];
+    for (int v = 1; v < nv; v++) {
+      v3 g = glob_vertex(d.sh, P, v);
+      mn.x = g.x < mn.x ? g.x : mn.x;
+      mn.y = g.y < mn.y ? g.y : mn.y;
+      mn.z = g.z < mn.z ? g.z : mn.z;
+      mx.x = g.x > mx.x ? g.x : mx.x;
+      mx.y = g.y > mx.y ? g.y : mx.y;
+      mx.z = g.z > mx.z ? g.z : mx.z;
+    }
+    const double r = P.R;
+    double* a = baabb + 6 * (size_t)b;
+    a[0] = mn.x - r;
+    a[1] = mn.y - r;
+    a[2] = mn.z - r;
+    a[3] = mx.x + r;
+    a[4] = mx.y + r;
+    a[5] = mx.z + r;
+  }
+}
+
+// Interfaces[i].find((i, j)) on the sorted live-interface keys.
+__device__ __forceinline__ bool has_live_interface(const DevInterfaces& f, int i, int j) {
+  if (f.n == 0) return false;
+  const uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+  int lo = 0, hi = f.n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (f.key[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < f.n && f.key[lo] == key && f.alive[lo];
+}
+
+__device__ __forceinline__ void emit_pair(uint64_t* cand, int* counter, int cap, int i, int j) {
+  const int slot = atomicAdd(counter, 1);
+  if (slot < cap) cand[slot] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+}
+
+__device__ __forceinline__ bool pair_overlap(const double* bobb, int i, int j, double half) {
+  obb_t oi = load_obb(bobb + 15 * (size_t)i);
+  obb_enlarge(&oi, half);
+  obb_t oj = load_obb(bobb + 15 * (size_t)j);
+  obb_enlarge(&oj, half);
+  return obb_intersect(oi, oj);
+}
+
+// UpdateNL_bruteForce pair loop (Rockable.cpp:1622-1658): i < j, j >= nDriven
+constexpr int kBruteChunk = 128;
+__global__ void k_broad_brute(Dev d, const double* bobb, double half, uint64_t* cand, int* counter, int cap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.nb) return;
+  int j0 = i + 1;
+  if (j0 < d.nDriven) j0 = d.nDriven;
+  const int jb = j0 + (int)blockIdx.y * kBruteChunk;
+  const int je = min(jb + kBruteChunk, d.nb);
+  for (int j = jb; j < je; j++) {
+    if (has_live_interface(d.ifc, i, j)) continue;
+    if (pair_overlap(bobb, i, j, half)) emit_pair(cand, counter, cap, i, j);
+  }
+}
+
+// --- linkCells broad phase ---------------------------------------------
+__global__ void k_box_partial(int first, int nb, const double* baabb, double* part) {
+  __shared__ double red[6][kBlock];
+  const int t = threadIdx.x;
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = first + blockIdx.x * blockDim.x + t; b < nb; b += gridDim.x * blockDim.x) {
+    const double* a = baabb + 6 * (size_t)b;
+    for (int k = 0; k < 3; k++) {
+      v[k] = a[k] < v[k] ? a[k] : v[k];
+      v[3 + k] = a[3 + k] > v[3 + k] ? a[3 + k] : v[3 + k];
+    }
+  }
+  for (int k = 0; k < 6; k++) red[k][t] = v[k];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < 3; k++) {
+        red[k][t] = red[k][t + s] < red[k][t] ? red[k][t + s] : red[k][t];
+        red[3 + k][t] = red[3 + k][t + s] > red[3 + k][t] ? red[3 + k][t + s] : red[3 + k][t];
+      }
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < 6; k++) part[6 * blockIdx.x + k] = red[k][0];
+}
+
+struct Grid {
+  int N[3];
+  double mn[3], sz[3];
+};
+
+__global__ void k_cell_assign(Dev d, const double* baabb, Grid g, int* cell_of, int* ccount, int* over, int* nover) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.nb) return;
+  const double* a = baabb + 6 * (size_t)b;
+  const bool oversized = (a[3] - a[0] > g.sz[0]) || (a[4] - a[1] > g.sz[1]) || (a[5] - a[2] > g.sz[2]);
+  if (oversized) {
+    cell_of[b] = -1;
+    over[atomicAdd(nover, 1)] = b;
+    return;
+  }
+  const BodyKin& P = d.kin[b];
+  const double p[3] = {P.px, P.py, P.pz};
+  int c3[3];
+  for (int k = 0; k < 3; k++) {
+    int q = (int)floor((p[k] - g.mn[k]) / g.sz[k]);
+    q = q < 0 ? 0 : q;
+    q = q >= g.N[k] ? g.N[k] - 1 : q;
+    c3[k] = q;
+  }
+  const int c = (c3[0] * g.N[1] + c3[1]) * g.N[2] + c3[2];
+  cell_of[b] = c;
+  atomicAdd(ccount + c, 1);
+}
+
+__global__ void k_cell_fill(int nb, const int* cell_of, const int* cstart, int* cfill, int* cbody) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int c = cell_of[b];
+  if (c < 0) return;
+  cbody[cstart[c] + atomicAdd(cfill + c, 1)] = b;
+}
+
+// main loop of UpdateNL_linkCells (Rockable.cpp:1697-1751), one thread per body
+__global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, const int* cell_of, const int* cstart,
+                              const int* cbody, uint64_t* cand, int* counter, int cap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.nb) return;
+  const int c = cell_of[i];
+  if (c < 0) return;
+  const int cz = c % g.N[2], cy = (c / g.N[2]) % g.N[1], cx = c / (g.N[2] * g.N[1]);
+  for (int dx = -1; dx <= 1; dx++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dz = -1; dz <= 1; dz++) {
+        const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= g.N[0] || ny >= g.N[1] || nz >= g.N[2]) continue;
+        const int nc = (nx * g.N[1] + ny) * g.N[2] + nz;
+        for (int r = cstart[nc]; r < cstart[nc + 1]; r++) {
+          const int j = cbody[r];
+          if (j < d.nDriven && i < d.nDriven) continue;
+          if (j <= i) continue;
+          if (has_live_interface(d.ifc, i, j)) continue;
+          if (pair_overlap(bobb, i, j, half)) emit_pair(cand, counter, cap, i, j);
+        }
+      }
+}
+
+// oversized bodies against every body of every cell (Rockable.cpp:1759-1805)
+__global__ void k_broad_over(Dev d, const double* bobb, double half, const int* over, int nover, const int* cbody,
+                             int ncell_bodies, uint64_t* cand, int* counter, int cap) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (r >= ncell_bodies || q >= nover) return;
+  const int i = over[q], j = cbody[r];
+  if (j < d.nDriven && i < d.nDriven) return;
+  if (j <= i) return;
+  if (has_live_interface(d.ifc, i, j)) return;
+  if (pair_overlap(bobb, i, j, half)) emit_pair(cand, counter, cap, i, j);
+}
+
+// --- directed groups -------------------------------------------------------
+// head of each (i, j) run of the current (sorted) entry list
+__global__ void k_old_heads(int n, const int32_t* ei, const int32_t* ej, int32_t* flag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  flag[e] = (e == 0 || ei[e] != ei[e - 1] || ej[e] != ej[e - 1]) ? 1 : 0;
+}
+__global__ void k_old_compact(int n, const int32_t* ei, const int32_t* ej, const int32_t* flag, const int32_t* pos,
+                              uint64_t* okey, int32_t* ostart) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n || !flag[e]) return;
+  okey[pos[e]] = ((uint64_t)(uint32_t)ei[e] << 32) | (uint32_t)ej[e];
+  ostart[pos[e]] = e;
+}
+// combined key list: candidates in both directions, then old groups
+constexpr uint32_t kOldTag = 0x80000000u;
+__global__ void k_fill_groups(int nc, const uint64_t* cand, int ng_old, const uint64_t* okey, uint64_t* gk,
+                              uint32_t* gv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nc) {
+    const uint64_t k = cand[t];
+    const uint32_t i = (uint32_t)(k >> 32), j = (uint32_t)(k & 0xffffffffu);
+    gk[2 * t] = k;
+    gv[2 * t] = (uint32_t)t << 1;
+    gk[2 * t + 1] = ((uint64_t)j << 32) | i;
+    gv[2 * t + 1] = ((uint32_t)t << 1) | 1u;
+  } else if (t < nc + ng_old) {
+    const int g = t - nc;
+    gk[2 * nc + g] = okey[g];
+    gv[2 * nc + g] = kOldTag | (uint32_t)g;
+  }
+}
+__global__ void k_group_heads(int n, const uint64_t* gk, int32_t* flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  flag[t] = (t == 0 || gk[t] != gk[t - 1]) ? 1 : 0;
+}
+__global__ void k_group_compact(int n, const uint64_t* gk, const uint32_t* gv, const int32_t* flag,
+                                const int32_t* pos, uint64_t* ukey, int32_t* ucand, int32_t* uold) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !flag[t]) return;
+  const int u = pos[t];
+  ukey[u] = gk[t];
+  int32_t c = -1, o = -1;
+  for (int q = t; q < n && q < t + 2 && gk[q] == gk[t]; q++) {
+    if (gv[q] & kOldTag)
+      o = (int32_t)(gv[q] & ~kOldTag);
+    else
+      c = (int32_t)gv[q];
+  }
+  ucand[u] = c;
+  uold[u] = o;
+}
+
+// --- narrow phase ------------------------------------------------------
+struct NarrowCfg {
+  int maxv;       // max vertices per shape
+  int maxwords;   // max 64-bit bitmask words per pair
+  double dmax;    // dVerlet
+};
+
+__device__ __forceinline__ v3 lds_v(const double* a, int k) { return V(a[3 * k], a[3 * k + 1], a[3 * k + 2]); }
+
+// Particle::VertexIsNearVertex/Edge/Face, EdgeIsNearEdge (Particle.cpp:88-209)
+// with the global vertices of both bodies read from LDS.
+__device__ bool near_vv(const double* gvi, const double* gvj, double Ri, double Rj, int isub, int jsub, double dmax) {
+  v3 pi = lds_v(gvi, isub), pj = lds_v(gvj, jsub);
+  double sum = Ri + Rj + dmax;
+  return vnorm2(vsub(pj, pi)) <= sum * sum;
+}
+__device__ bool near_ve(const DevShapes& sh, int shj, const double* gvi, const double* gvj, double Ri, double Rj,
+                        int isub, int jsub, double dmax) {
+  const int* ed = sh.edge + 2 * (sh.eoff[shj] + jsub);
+  v3 pi = lds_v(gvi, isub), p0 = lds_v(gvj, ed[0]), p1 = lds_v(gvj, ed[1]);
+  v3 E = vsub(p1, p0);
+  v3 v = vsub(pi, p0);
+  double r = vdot(v, E) / vdot(E, E);
+  if (r < -dmax) return false;
+  if (r > 1.0 + dmax) return false;
+  double sum = Ri + Rj + dmax;
+  return vnorm2(vsub(pi, vadd(p0, vsc(r, E)))) <= sum * sum;
+}
+__device__ bool near_vf(const DevShapes& sh, int shj, const double* gvi, const double* gvj, double Ri, double Rj,
+                        int isub, int jsub, double dmax) {
+  const int gface = sh.foff[shj] + jsub;
+  const int fs = sh.fstart[gface];
+  const int nbv = sh.fstart[gface + 1] - fs;
+  const int* fv = sh.fidx + fs;
+  v3 A = lds_v(gvj, fv[0]), B = lds_v(gvj, fv[1]), C = lds_v(gvj, fv[nbv - 1]);
+  v3 pi = lds_v(gvi, isub);
+  v3 v = vsub(pi, A);
+  v3 v1 = vsub(B, A);
+  vnormalize(&v1);
+  v3 v2 = vsub(C, A);
+  v3 n = vcross(v1, v2);
+  vnormalize(&n);
+  double dist = vdot(v, n);
+  if (dist < 0.0) {
+    n = vneg(n);
+    dist = -dist;
+  }
+  v3 P = vsub(pi, vsc(dist, n));
+  v2 = vcross(n, v1);
+  int odd = 0;
+  double ori1 = vdot(P, v1), ori2 = vdot(P, v2);
+  for (int iva = 0; iva < nbv; iva++) {
+    int ivb = iva + 1;
+    if (ivb == nbv) ivb = 0;
+    v3 a = lds_v(gvj, fv[iva]), b = lds_v(gvj, fv[ivb]);
+    double pa1 = vdot(a, v1), pb1 = vdot(b, v1);
+    double pa2 = vdot(a, v2), pb2 = vdot(b, v2);
+    if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
+      if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
+    }
+  }
+  if (odd == 1) {
+    dist -= (Ri + Rj);
+    if (dist <= dmax) return true;
+  }
+  return false;
+}
+__device__ bool near_ee(const DevShapes& sh, int shi, int shj, const double* gvi, const double* gvj, double Ri,
+                        double Rj, int isub, int jsub, double dmax) {
+  const int* edj = sh.edge + 2 * (sh.eoff[shj] + jsub);
+  const int* edi = sh.edge + 2 * (sh.eoff[shi] + isub);
+  v3 p0j = lds_v(gvj, edj[0]), p1j = lds_v(gvj, edj[1]);
+  v3 p0i = lds_v(gvi, edi[0]), p1i = lds_v(gvi, edi[1]);
+  v3 Ei = vsub(p1i, p0i);
+  v3 Ej = vsub(p1j, p0j);
+  v3 v = vsub(p0i, p0j);
+  double c = vdot(Ei, Ei), d = vdot(Ej, Ej), e = vdot(Ei, Ej);
+  double f = (c * d) - (e * e);
+  if (fabs(f) > 1.0e-12) {
+    f = 1.0 / f;
+    double a = vdot(Ei, v), b = vdot(Ej, v);
+    double s = (e * b - a * d) * f;
+    double t = (c * b - e * a) * f;
+    if (s >= 0.0 && s <= 1.0 && t >= 0.0 && t <= 1.0) {
+      v3 n = vsub(vadd(p0i, vsc(s, Ei)), vadd(p0j, vsc(t, Ej)));
+      double sum = Ri + Rj + dmax;
+      return vnorm2(n) <= sum * sum;
+    }
+  }
+  return false;
+}
+
+// vertex gate of AddOrRemoveInteractions_bruteForce (Rockable.cpp:1385-1393):
+// obb_other enlarged by dmax against an axis-aligned box around the vertex
+__device__ __forceinline__ bool vertex_gate(v3 gv, double Rowner, const obb_t& other, double dmax) {
+  obb_t sub;
+  sub.c = gv;
+  sub.e[0] = V(1, 0, 0);
+  sub.e[1] = V(0, 1, 0);
+  sub.e[2] = V(0, 0, 1);
+  sub.ext = V(0, 0, 0);
+  obb_enlarge(&sub, Rowner + dmax);
+  obb_t o = other;
+  obb_enlarge(&o, dmax);
+  return obb_intersect(o, sub);
+}
+
+struct Seg {
+  int type, n_own, n_oth;  // test grid n_own x n_oth
+  bool fwd;
+};
+
+__global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, NarrowCfg cfg, int nc, const uint64_t* cand,
+                                               uint32_t* nearbuf, int* near_counter, int near_cap, int32_t* cbase,
+                                               int32_t* cnf, int32_t* cnb) {
+  extern __shared__ double lds[];
+  const int c = blockIdx.x;
+  if (c >= nc) return;
+  const int lane = threadIdx.x;
+  const uint64_t key = cand[c];
+  const int i = (int)(key >> 32), j = (int)(key & 0xffffffffu);
+  const BodyKin Pi = d.kin[i], Pj = d.kin[j];
+  const int shi = Pi.shape, shj = Pj.shape;
+  const int nvi = d.sh.nv[shi], nvj = d.sh.nv[shj];
+  const int nei = d.sh.ne[shi], nej = d.sh.ne[shj];
+  const int nfi = d.sh.nf[shi], nfj = d.sh.nf[shj];
+  double* gvi = lds;
+  double* gvj = lds + 3 * cfg.maxv;
+  unsigned char* gate_i = (unsigned char*)(lds + 6 * cfg.maxv);
+  unsigned char* gate_j = gate_i + cfg.maxv;
+  unsigned long long* bits = (unsigned long long*)(lds + 6 * cfg.maxv + (2 * cfg.maxv + 7) / 8);
+  for (int v = lane; v < nvi; v += 64) {
+    v3 g = glob_vertex(d.sh, Pi, v);
+    gvi[3 * v] = g.x;
+    gvi[3 * v + 1] = g.y;
+    gvi[3 * v + 2] = g.z;
+  }
+  for (int v = lane; v < nvj; v += 64) {
+    v3 g = glob_vertex(d.sh, Pj, v);
+    gvj[3 * v] = g.x;
+    gvj[3 * v + 1] = g.y;
+    gvj[3 * v + 2] = g.z;
+  }
+  __syncthreads();
+  const obb_t oi = load_obb(bobb + 15 * (size_t)i);
+  const obb_t oj = load_obb(bobb + 15 * (size_t)j);
+  for (int v = lane; v < nvi; v += 64) gate_i[v] = vertex_gate(lds_v(gvi, v), Pi.R, oj, cfg.dmax) ? 1 : 0;
+  for (int v = lane; v < nvj; v += 64) gate_j[v] = vertex_gate(lds_v(gvj, v), Pj.R, oi, cfg.dmax) ? 1 : 0;
+  __syncthreads();
+
+  const Seg seg[6] = {{VV, nvi, nvj, true}, {VE, nvi, nej, true}, {VF, nvi, nfj, true},
+                      {EE, nei, nej, true}, {VE, nvj, nei, false}, {VF, nvj, nfi, false}};
+  int segstart[7];
+  segstart[0] = 0;
+  for (int s = 0; s < 6; s++) segstart[s + 1] = segstart[s] + seg[s].n_own * seg[s].n_oth;
+  const int T = segstart[6], Tf = segstart[4];
+  const int nwords = (T + 63) / 64;
+  if (nwords > cfg.maxwords) return;  // guarded on the host: shapes are checked at load
+
+  for (int base = 0; base < T; base += 64) {
+    const int t = base + lane;
+    bool nr = false;
+    if (t < T) {
+      int s = 0;
+      while (t >= segstart[s + 1]) s++;
+      const int local = t - segstart[s];
+      const int own = local / seg[s].n_oth, oth = local % seg[s].n_oth;
+      if (seg[s].fwd) {
+        if (seg[s].type == EE)
+          nr = near_ee(d.sh, shi, shj, gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
+        else if (gate_i[own]) {
+          if (seg[s].type == VV)
+            nr = near_vv(gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
+          else if (seg[s].type == VE)
+            nr = near_ve(d.sh, shj, gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
+          else
+            nr = near_vf(d.sh, shj, gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
+        }
+      } else if (gate_j[own]) {
+        if (seg[s].type == VE)
+          nr = near_ve(d.sh, shi, gvj, gvi, Pj.R, Pi.R, own, oth, cfg.dmax);
+        else
+          nr = near_vf(d.sh, shi, gvj, gvi, Pj.R, Pi.R, own, oth, cfg.dmax);
+      }
+    }
+    const unsigned long long m = __ballot(nr);
+    if (lane == 0) bits[base / 64] = m;
+  }
+  __syncthreads();
+  // counts per direction
+  int cf = 0, cb = 0;
+  for (int w = lane; w < nwords; w += 64) {
+    const unsigned long long m = bits[w];
+    const int lo = w * 64;
+    if (lo + 64 <= Tf)
+      cf += __popcll(m);
+    else if (lo >= Tf)
+      cb += __popcll(m);
+    else {
+      const int k = Tf - lo;
+      const unsigned long long fm = (k >= 64) ? ~0ull : ((1ull << k) - 1ull);
+      cf += __popcll(m & fm);
+      cb += __popcll(m & ~fm);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    cf += __shfl_down(cf, off, 64);
+    cb += __shfl_down(cb, off, 64);
+  }
+  cf = __shfl(cf, 0, 64);
+  cb = __shfl(cb, 0, 64);
+  int base_out = 0;
+  if (lane == 0) base_out = atomicAdd(near_counter, cf + cb);
+  base_out = __shfl(base_out, 0, 64);
+  if (lane == 0) {
+    cbase[c] = base_out;
+    cnf[c] = cf;
+    cnb[c] = cb;
+  }
+  if (base_out + cf + cb > near_cap) return;  // host re-runs with a larger buffer
+  // write keys in test order: rank of a set bit = its output slot
+  int running = 0;
+  for (int w0 = 0; w0 < nwords; w0 += 64) {
+    const int w = w0 + lane;
+    const unsigned long long m = (w < nwords) ? bits[w] : 0ull;
+    const int pc = __popcll(m);
+    int incl = pc;
+    for (int off = 1; off < 64; off <<= 1) {
+      int y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+    int pos = running + incl - pc;
+    unsigned long long mm = m;
+    while (mm) {
+      const int bit = __ffsll((long long)mm) - 1;
+      mm &= mm - 1ull;
+      const int t = w * 64 + bit;
+      int s = 0;
+      while (t >= segstart[s + 1]) s++;
+      const int local = t - segstart[s];
+      const int own = local / seg[s].n_oth, oth = local % seg[s].n_oth;
+      nearbuf[base_out + pos] = pack_sub(seg[s].type, own, oth);
+      pos++;
+    }
+    running += __shfl(incl, 63, 64);
+  }
+}
+
+// --- merge ---------------------------------------------------------------
+struct MergeIn {
+  int ng;
+  const uint64_t* ukey;
+  const int32_t* ucand;
+  const int32_t* uold;
+  const int32_t* ostart;  // old group start, size ng_old + 1 (trailing = n_old)
+  const int32_t* cbase;
+  const int32_t* cnf;
+  const int32_t* cnb;
+  const uint32_t* nearbuf;
+  const double* bobb;
+  double dmax;
+  // old entries
+  const int32_t* oi;
+  const int32_t* oj;
+  const uint32_t* osub;
+  const EState* oest;
+  const EGeom* ogeo;
+  const double* odamp;
+  const int32_t* ostick;
+};
+struct MergeOut {
+  int32_t* count;   // per group (count pass)
+  const int32_t* offset;  // per group (write pass)
+  int32_t* ei;
+  int32_t* ej;
+  uint32_t* esub;
+  EState* est;
+  EGeom* egeo;
+  double* edamp;
+  int32_t* estick;
+  uint8_t* eact;
+};
+
+// Damp of the body pair (Rockable.cpp:1356-1376), pair (i < j)
+__device__ double pair_damp(const Dev& d, int i, int j) {
+  const int gi = d.kin[i].group, gj = d.kin[j].group;
+  if (gi < 0 || gj < 0 || gi >= d.par.G || gj >= d.par.G) return 0.0;
+  const int gp = gi * d.par.G + gj;
+  const int kind = d.par.damp_kind[gp];
+  if (kind == 2) return 0.0;
+  const double kn = d.par.tab[(P_KN * d.par.G + gi) * d.par.G + gj];
+  double meff;
+  if (i < d.nDriven)
+    meff = d.mass[j];
+  else if (j < d.nDriven)
+    meff = d.mass[i];
+  else
+    meff = (d.mass[i] * d.mass[j]) / (d.mass[i] + d.mass[j]);
+  if (kind == 0) return d.par.damp_rate[gp] * 2.0 * sqrt(kn * meff);
+  return 2.0 * sqrt(kn * meff);
+}
+
+template <bool WRITE>
+__global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= in.ng) return;
+  const uint64_t key = in.ukey[g];
+  const int owner = (int)(key >> 32), other = (int)(key & 0xffffffffu);
+  const int cref = in.ucand[g], oref = in.uold[g];
+  int os = 0, on = 0;
+  if (oref >= 0) {
+    os = in.ostart[oref];
+    on = in.ostart[oref + 1] - os;
+  }
+  int ns = 0, nn = 0;
+  bool fwd = true;
+  if (cref >= 0) {
+    const int c = cref >> 1;
+    fwd = (cref & 1) == 0;
+    ns = in.cbase[c] + (fwd ? 0 : in.cnf[c]);
+    nn = fwd ? in.cnf[c] : in.cnb[c];
+  }
+  const bool candidate = cref >= 0;
+  const BodyKin Po = d.kin[owner];
+  obb_t oth;
+  if (candidate) oth = load_obb(in.bobb + 15 * (size_t)other);
+  const int pi = owner < other ? owner : other, pj = owner < other ? other : owner;
+  int cnt = WRITE ? out.offset[g] : 0;
+  int a = 0, b = 0;
+  while (a < on || b < nn) {
+    const uint32_t sa = (a < on) ? in.osub[os + a] : 0xffffffffu;
+    const uint32_t sb = (b < nn) ? in.nearbuf[ns + b] : 0xffffffffu;
+    int take;  // 0 old kept, 1 fresh near, 2 old matched by near, -1 old erased
+    if (b < nn && (a >= on || sb < sa)) {
+      take = 1;
+    } else if (a < on && (b >= nn || sa < sb)) {
+      bool tested = false;
+      if (candidate) {
+        const int type = (int)(sa >> 30), isub = (int)((sa >> 15) & 0x7fffu);
+        if (fwd && type == EE)
+          tested = true;
+        else if (type == VV || type == VE || type == VF) {
+          if (fwd || type != VV) tested = vertex_gate(glob_vertex(d.sh, Po, isub), Po.R, oth, in.dmax);
+        }
+      }
+      take = tested ? -1 : 0;
+    } else {
+      take = 2;
+    }
+    if (take != -1) {
+      if (WRITE) {
+        const int e = cnt;
+        out.ei[e] = owner;
+        out.ej[e] = other;
+        if (take == 1) {
+          out.esub[e] = sb;
+          EState s;
+          s.dn = s.fn = 0.0;
+          s.ftx = s.fty = s.ftz = 0.0;
+          s.momx = s.momy = s.momz = 0.0;
+          out.est[e] = s;
+          EGeom q;
+          q.nx = q.ny = q.nz = q.px = q.py = q.pz = q.vx = q.vy = q.vz = 0.0;
+          out.egeo[e] = q;
+          out.edamp[e] = pair_damp(d, pi, pj);
+          out.estick[e] = -1;
+          out.eact[e] = 0;
+        } else {
+          // a kept entry stays active as it was at the last accelerations()
+          const int o = os + a;
+          const EState st = in.oest[o];
+          out.esub[e] = in.osub[o];
+          out.est[e] = st;
+          out.egeo[e] = in.ogeo[o];
+          out.edamp[e] = in.odamp[o];
+          out.estick[e] = in.ostick[o];
+          out.eact[e] = (st.dn < 0.0 || in.ostick[o] >= 0) ? 1 : 0;
+        }
+      }
+      cnt++;
+    }
+    if (take == 1)
+      b++;
+    else if (take == 2) {
+      a++;
+      b++;
+    } else
+      a++;
+  }
+  if (!WRITE) out.count[g] = cnt;
+}
+
+// --- CSR -----------------------------------------------------------------
+__global__ void k_count_by(int n, const int32_t* key, int32_t* cnt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  atomicAdd(cnt + key[e], 1);
+}
+__global__ void k_iota(int n, int32_t* v) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) v[e] = e;
+}
+
+inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <class T>
+static int ensure(NLWork& w, T*& p, size_t& cap, size_t n) {
+  if (n <= cap && p) return 0;
+  if (p) (void)hipFree(p);
+  size_t c = n + n / 2 + 256;
+  if (hipMalloc(&p, c * sizeof(T)) != hipSuccess) {
+    snprintf(w.err, sizeof(w.err), "hipMalloc of %zu bytes failed", c * sizeof(T));
+    p = nullptr;
+    cap = 0;
+    return -1;
+  }
+  cap = c;
+  return 0;
+}
+
+static int scan_exclusive(NLWork& w, const int32_t* in, int32_t* out, int n, hipStream_t s) {
+  size_t bytes = 0;
+  RK_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, (size_t)n, rocprim::plus<int32_t>(), s));
+  if (ensure(w, w.tmp, w.tmp_cap, bytes) != 0) return -1;
+  RK_CHECK(rocprim::exclusive_scan((void*)w.tmp, bytes, in, out, 0, (size_t)n, rocprim::plus<int32_t>(), s));
+  return 0;
+}
+
+template <class K, class Vv>
+static int sort_pairs(NLWork& w, const K* kin, K* kout, const Vv* vin, Vv* vout, int n, int bits, hipStream_t s) {
+  size_t bytes = 0;
+  RK_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s));
+  if (ensure(w, w.tmp, w.tmp_cap, bytes) != 0) return -1;
+  RK_CHECK(rocprim::radix_sort_pairs((void*)w.tmp, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s));
+  return 0;
+}
+
+static int bits_for(uint64_t maxval) {
+  int b = 1;
+  while (b < 64 && (maxval >> b) != 0) b++;
+  return b;
+}
+
+int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
+  const int n = E.n;
+  if (ensure(w, E.istart, E.istart_cap, (size_t)nb + 1) || ensure(w, E.jstart, E.jstart_cap, (size_t)nb + 1) ||
+      ensure(w, E.jlist, E.jlist_cap, (size_t)(n > 0 ? n : 1)) || ensure(w, w.i32a, w.i32a_cap, (size_t)nb + 1) ||
+      ensure(w, w.i32b, w.i32b_cap, (size_t)(n > 0 ? n : 1)) || ensure(w, w.i32c, w.i32c_cap, (size_t)(n > 0 ? n : 1)))
+    return -1;
+  // istart: counts by owner, exclusive scan over nb + 1 (trailing zero)
+  RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));
+  if (n > 0) hipLaunchKernelGGL(k_count_by, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ei, w.i32a);
+  if (scan_exclusive(w, w.i32a, E.istart, nb + 1, s)) return -1;
+  // jstart
+  RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));
+  if (n > 0) hipLaunchKernelGGL(k_count_by, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ej, w.i32a);
+  if (scan_exclusive(w, w.i32a, E.jstart, nb + 1, s)) return -1;
+  // jlist: stable radix sort of entry indices by j
+  if (n > 0) {
+    hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(kBlock), 0, s, n, w.i32b);
+    RK_CHECK(hipMemcpyAsync(w.i32c, E.ej, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+    int32_t* kout = nullptr;
+    size_t kcap = 0;
+    if (ensure(w, kout, kcap, (size_t)n)) return -1;
+    int rc = sort_pairs(w, (const uint32_t*)w.i32c, (uint32_t*)kout, w.i32b, E.jlist, n,
+                        bits_for((uint64_t)(nb > 1 ? nb - 1 : 1)), s);
+    RK_CHECK(hipStreamSynchronize(s));
+    (void)hipFree(kout);
+    if (rc) return -1;
+  }
+  return 0;
+}
+
+int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArrays& nxt, hipStream_t s) {
+  const int nb = d.nb;
+  if (nb == 0) return 0;
+  const bool cells = p.linkcells != 0;
+  if (ensure(w, w.bobb, w.bobb_cap, (size_t)15 * nb)) return -1;
+  if (cells && ensure(w, w.baabb, w.baabb_cap, (size_t)6 * nb)) return -1;
+  hipLaunchKernelGGL(k_obb, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, cells ? w.baabb : nullptr);
+  const double half = 0.5 * p.DVerlet;
+
+  // ---- broad phase: candidate pairs
+  if (ensure(w, w.counter, w.counter_cap, 4)) return -1;
+  int nc = 0;
+  for (int attempt = 0; attempt < 3; attempt++) {
+    if (ensure(w, w.cand, w.cand_cap, (size_t)std::max(w.cand_hint, 1024))) return -1;
+    const int cap = (int)std::min<size_t>(w.cand_cap, 0x3fffffff);
+    RK_CHECK(hipMemsetAsync(w.counter, 0, sizeof(int) * 4, s));
+    if (!cells) {
+      const int jmax = nb - std::min(nb, std::max(1, d.nDriven));
+      const int chunks = std::max(1, (nb + kBruteChunk - 1) / kBruteChunk);
+      (void)jmax;
+      hipLaunchKernelGGL(k_broad_brute, dim3(nblk(nb), chunks), dim3(kBlock), 0, s, d, w.bobb, half, w.cand,
+                         w.counter, cap);
+    } else {
+      // bounding box of the bodies [first, nb) (Rockable.cpp:1683-1688, 3943-3947)
+      int first = (p.boxForLinkCellsOpt == 1) ? d.nDriven : 0;
+      if (first >= nb) first = 0;
+      const int pb = 64;
+      if (ensure(w, w.f64a, w.f64a_cap, (size_t)6 * pb)) return -1;
+      hipLaunchKernelGGL(k_box_partial, dim3(pb), dim3(kBlock), 0, s, first, nb, w.baabb, w.f64a);
+      std::vector<double> part(6 * pb);
+      RK_CHECK(hipMemcpyAsync(part.data(), w.f64a, sizeof(double) * 6 * pb, hipMemcpyDeviceToHost, s));
+      RK_CHECK(hipStreamSynchronize(s));
+      double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int q = 0; q < pb; q++)
+        for (int k = 0; k < 3; k++) {
+          mn[k] = std::min(mn[k], part[6 * q + k]);
+          mx[k] = std::max(mx[k], part[6 * q + 3 + k]);
+        }
+      Grid g;
+      const double ms[3] = {p.cellMinSizes[0], p.cellMinSizes[1], p.cellMinSizes[2]};
+      long ncell = 1;
+      for (int k = 0; k < 3; k++) {
+        const double len = mx[k] - mn[k];
+        int n = (ms[k] > 0.0) ? (int)floor(len / ms[k]) : 1;
+        g.N[k] = n < 1 ? 1 : n;
+        g.mn[k] = mn[k];
+        g.sz[k] = len / g.N[k];
+        ncell *= g.N[k];
+      }
+      if (ncell > (1L << 28)) {
+        snprintf(w.err, sizeof(w.err), "linkCells: %ld cells (cellMinSizes too small for the domain)", ncell);
+        return -1;
+      }
+      if (ensure(w, w.cell_of, w.cell_of_cap, (size_t)nb) || ensure(w, w.ccount, w.ccount_cap, (size_t)ncell + 1) ||
+          ensure(w, w.cstart, w.cstart_cap, (size_t)ncell + 1) || ensure(w, w.cfill, w.cfill_cap, (size_t)ncell) ||
+          ensure(w, w.cbody, w.cbody_cap, (size_t)nb) || ensure(w, w.over, w.over_cap, (size_t)nb))
+        return -1;
+      RK_CHECK(hipMemsetAsync(w.ccount, 0, sizeof(int32_t) * (ncell + 1), s));
+      RK_CHECK(hipMemsetAsync(w.cfill, 0, sizeof(int32_t) * ncell, s));
+      hipLaunchKernelGGL(k_cell_assign, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.baabb, g, w.cell_of, w.ccount,
+                         w.over, w.counter + 1);
+      if (scan_exclusive(w, w.ccount, w.cstart, (int)ncell + 1, s)) return -1;
+      hipLaunchKernelGGL(k_cell_fill, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, w.cell_of, w.cstart, w.cfill, w.cbody);
+      int hv[2];
+      RK_CHECK(hipMemcpyAsync(hv, w.counter + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+      RK_CHECK(hipStreamSynchronize(s));
+      const int nover = hv[0];
+      const int ncb = nb - nover;
+      hipLaunchKernelGGL(k_broad_cells, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
+                         w.cbody, w.cand, w.counter, cap);
+      if (nover > 0 && ncb > 0)
+        hipLaunchKernelGGL(k_broad_over, dim3(nblk(ncb), nover), dim3(kBlock), 0, s, d, w.bobb, half, w.over, nover,
+                           w.cbody, ncb, w.cand, w.counter, cap);
+    }
+    RK_CHECK(hipMemcpyAsync(&nc, w.counter, sizeof(int), hipMemcpyDeviceToHost, s));
+    RK_CHECK(hipStreamSynchronize(s));
+    if ((size_t)nc <= w.cand_cap) break;
+    w.cand_hint = nc + nc / 4;
+    if (attempt == 2) {
+      snprintf(w.err, sizeof(w.err), "broad phase overflow");
+      return -1;
+    }
+  }
+  w.cand_hint = std::max(w.cand_hint, nc);
+  w.last_candidates = nc;
+
+  // ---- narrow phase
+  if (ensure(w, w.cbase, w.cbase_cap, (size_t)std::max(nc, 1)) || ensure(w, w.cnf, w.cnf_cap, (size_t)std::max(nc, 1)) ||
+      ensure(w, w.cnb, w.cnb_cap, (size_t)std::max(nc, 1)))
+    return -1;
+  int nnear = 0;
+  if (nc > 0) {
+    NarrowCfg cfg;
+    cfg.maxv = p.maxv;
+    cfg.maxwords = p.maxwords;
+    cfg.dmax = p.dVerlet;
+    const size_t lds = sizeof(double) * 6 * p.maxv + sizeof(double) * ((2 * p.maxv + 7) / 8) +
+                       sizeof(unsigned long long) * p.maxwords;
+    for (int attempt = 0; attempt < 3; attempt++) {
+      if (ensure(w, w.nearbuf, w.nearbuf_cap, (size_t)std::max(w.near_hint, 4096))) return -1;
+      RK_CHECK(hipMemsetAsync(w.counter + 2, 0, sizeof(int), s));
+      hipLaunchKernelGGL(k_narrow, dim3(nc), dim3(64), lds, s, d, w.bobb, cfg, nc, w.cand, w.nearbuf, w.counter + 2,
+                         (int)std::min<size_t>(w.nearbuf_cap, 0x7fffffff), w.cbase, w.cnf, w.cnb);
+      RK_CHECK(hipGetLastError());
+      RK_CHECK(hipMemcpyAsync(&nnear, w.counter + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+      RK_CHECK(hipStreamSynchronize(s));
+      if ((size_t)nnear <= w.nearbuf_cap) break;
+      w.near_hint = nnear + nnear / 4;
+      if (attempt == 2) {
+        snprintf(w.err, sizeof(w.err), "narrow phase overflow");
+        return -1;
+      }
+    }
+    w.near_hint = std::max(w.near_hint, nnear);
+  }
+
+  // ---- old groups
+  const int nold = cur.n;
+  int ng_old = 0;
+  if (ensure(w, w.i32b, w.i32b_cap, (size_t)std::max(nold, 1) + 1) ||
+      ensure(w, w.i32c, w.i32c_cap, (size_t)std::max(nold, 1) + 1) ||
+      ensure(w, w.okey, w.okey_cap, (size_t)std::max(nold, 1)) ||
+      ensure(w, w.ostart, w.ostart_cap, (size_t)std::max(nold, 1) + 1))
+    return -1;
+  if (nold > 0) {
+    hipLaunchKernelGGL(k_old_heads, dim3(nblk(nold)), dim3(kBlock), 0, s, nold, cur.ei, cur.ej, w.i32b);
+    if (scan_exclusive(w, w.i32b, w.i32c, nold, s)) return -1;
+    hipLaunchKernelGGL(k_old_compact, dim3(nblk(nold)), dim3(kBlock), 0, s, nold, cur.ei, cur.ej, w.i32b, w.i32c,
+                       w.okey, w.ostart);
+    int last[2];
+    RK_CHECK(hipMemcpyAsync(&last[0], w.i32c + nold - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    RK_CHECK(hipMemcpyAsync(&last[1], w.i32b + nold - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    RK_CHECK(hipStreamSynchronize(s));
+    ng_old = last[0] + last[1];
+  }
+  RK_CHECK(hipMemcpyAsync(w.ostart + ng_old, &nold, sizeof(int), hipMemcpyHostToDevice, s));
+
+  // ---- unique directed groups
+  const int ngk = 2 * nc + ng_old;
+  int ng = 0;
+  if (ensure(w, w.gk, w.gk_cap, (size_t)std::max(ngk, 1)) || ensure(w, w.gk2, w.gk2_cap, (size_t)std::max(ngk, 1)) ||
+      ensure(w, w.gv, w.gv_cap, (size_t)std::max(ngk, 1)) || ensure(w, w.gv2, w.gv2_cap, (size_t)std::max(ngk, 1)) ||
+      ensure(w, w.gflag, w.gflag_cap, (size_t)std::max(ngk, 1)) ||
+      ensure(w, w.gpos, w.gpos_cap, (size_t)std::max(ngk, 1)) ||
+      ensure(w, w.ukey, w.ukey_cap, (size_t)std::max(ngk, 1)) ||
+      ensure(w, w.ucand, w.ucand_cap, (size_t)std::max(ngk, 1)) ||
+      ensure(w, w.uold, w.uold_cap, (size_t)std::max(ngk, 1)) ||
+      ensure(w, w.gcount, w.gcount_cap, (size_t)std::max(ngk, 1) + 1) ||
+      ensure(w, w.goff, w.goff_cap, (size_t)std::max(ngk, 1) + 1))
+    return -1;
+  if (ngk > 0) {
+    hipLaunchKernelGGL(k_fill_groups, dim3(nblk(nc + ng_old)), dim3(kBlock), 0, s, nc, w.cand, ng_old, w.okey, w.gk,
+                       w.gv);
+    if (sort_pairs(w, w.gk, w.gk2, w.gv, w.gv2, ngk, 64, s)) return -1;
+    hipLaunchKernelGGL(k_group_heads, dim3(nblk(ngk)), dim3(kBlock), 0, s, ngk, w.gk2, w.gflag);
+    if (scan_exclusive(w, w.gflag, w.gpos, ngk, s)) return -1;
+    hipLaunchKernelGGL(k_group_compact, dim3(nblk(ngk)), dim3(kBlock), 0, s, ngk, w.gk2, w.gv2, w.gflag, w.gpos,
+                       w.ukey, w.ucand, w.uold);
+    int last[2];
+    RK_CHECK(hipMemcpyAsync(&last[0], w.gpos + ngk - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    RK_CHECK(hipMemcpyAsync(&last[1], w.gflag + ngk - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    RK_CHECK(hipStreamSynchronize(s));
+    ng = last[0] + last[1];
+  }
+
+  // ---- merge: count, scan, allocate, write
+  MergeIn in;
+  in.ng = ng;
+  in.ukey = w.ukey;
+  in.ucand = w.ucand;
+  in.uold = w.uold;
+  in.ostart = w.ostart;
+  in.cbase = w.cbase;
+  in.cnf = w.cnf;
+  in.cnb = w.cnb;
+  in.nearbuf = w.nearbuf;
+  in.bobb = w.bobb;
+  in.dmax = p.dVerlet;
+  in.oi = cur.ei;
+  in.oj = cur.ej;
+  in.osub = cur.esub;
+  in.oest = cur.est;
+  in.ogeo = cur.egeo;
+  in.odamp = cur.edamp;
+  in.ostick = cur.estick;
+  MergeOut out;
+  out.count = w.gcount;
+  out.offset = w.goff;
+  int nnew = 0;
+  if (ng > 0) {
+    RK_CHECK(hipMemsetAsync(w.gcount + ng, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_merge<false>, dim3(nblk(ng)), dim3(kBlock), 0, s, d, in, out);
+    if (scan_exclusive(w, w.gcount, w.goff, ng + 1, s)) return -1;
+    RK_CHECK(hipMemcpyAsync(&nnew, w.goff + ng, sizeof(int), hipMemcpyDeviceToHost, s));
+    RK_CHECK(hipStreamSynchronize(s));
+  }
+  if (entries_reserve(w, nxt, nnew) != 0) return -1;
+  nxt.n = nnew;
+  if (ng > 0 && nnew > 0) {
+    out.ei = nxt.ei;
+    out.ej = nxt.ej;
+    out.esub = nxt.esub;
+    out.est = nxt.est;
+    out.egeo = nxt.egeo;
+    out.edamp = nxt.edamp;
+    out.estick = nxt.estick;
+    out.eact = nxt.eact;
+    hipLaunchKernelGGL(k_merge<true>, dim3(nblk(ng)), dim3(kBlock), 0, s, d, in, out);
+  }
+  RK_CHECK(hipGetLastError());
+  if (nl_build_csr(w, nb, nxt, s) != 0) return -1;
+  RK_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int entries_reserve(NLWork& w, EntryArrays& E, int n) {
+  size_t need = (size_t)(n > 0 ? n : 1);
+  if (need <= E.cap && E.ei) return 0;
+  entries_free(E);
+  size_t c = need + need / 2 + 1024;
+  bool ok = hipMalloc(&E.ei, c * sizeof(int32_t)) == hipSuccess && hipMalloc(&E.ej, c * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&E.esub, c * sizeof(uint32_t)) == hipSuccess && hipMalloc(&E.est, c * sizeof(EState)) == hipSuccess &&
+            hipMalloc(&E.egeo, c * sizeof(EGeom)) == hipSuccess && hipMalloc(&E.edamp, c * sizeof(double)) == hipSuccess &&
+            hipMalloc(&E.estick, c * sizeof(int32_t)) == hipSuccess && hipMalloc(&E.eact, c * sizeof(uint8_t)) == hipSuccess &&
+            hipMalloc(&E.econ, c * 9 * sizeof(double)) == hipSuccess;
+  if (!ok) {
+    snprintf(w.err, sizeof(w.err), "hipMalloc of %zu interaction entries failed", c);
+    entries_free(E);
+    return -1;
+  }
+  E.cap = c;
+  return 0;
+}
+
+void entries_free(EntryArrays& E) {
+  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.egeo, E.edamp, E.estick, E.eact, E.econ};
+  for (void* q : ps)
+    if (q) (void)hipFree(q);
+  E.ei = E.ej = nullptr;
+  E.esub = nullptr;
+  E.est = nullptr;
+  E.egeo = nullptr;
+  E.edamp = nullptr;
+  E.estick = nullptr;
+  E.eact = nullptr;
+  E.econ = nullptr;
+  E.cap = 0;
+}
+
+void nl_work_free(NLWork& w) {
+  void* ps[] = {w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
+                w.gk,    w.gk2,   w.gv,      w.gv2,   w.gflag, w.gpos,  w.ukey,  w.ucand,   w.uold,  w.gcount,
+                w.goff,  w.tmp,   w.i32a,    w.i32b,  w.i32c,  w.f64a,  w.cell_of, w.ccount, w.cstart, w.cfill,
+                w.cbody, w.over};
+  for (void* q : ps)
+    if (q) (void)hipFree(q);
+  NLWork z;
+  w = z;
+}
+
+}  // namespace rk

@@ -1,0 +1,92 @@
+// rk_nl.hpp -- neighbour-list rebuild: entry arrays and work buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "rk_device.hpp"
+
+namespace rk {
+
+// One sorted interaction list (see rk_device.hpp for the layout).
+struct EntryArrays {
+  int n = 0;
+  size_t cap = 0;
+  int32_t* ei = nullptr;
+  int32_t* ej = nullptr;
+  uint32_t* esub = nullptr;
+  EState* est = nullptr;
+  EGeom* egeo = nullptr;
+  double* edamp = nullptr;
+  int32_t* estick = nullptr;
+  uint8_t* eact = nullptr;
+  double* econ = nullptr;
+  int32_t* istart = nullptr;
+  size_t istart_cap = 0;
+  int32_t* jstart = nullptr;
+  size_t jstart_cap = 0;
+  int32_t* jlist = nullptr;
+  size_t jlist_cap = 0;
+};
+
+struct NLParams {
+  double DVerlet = 0.0, dVerlet = 0.0;
+  int linkcells = 0;
+  double cellMinSizes[3] = {1.0, 1.0, 1.0};
+  int boxForLinkCellsOpt = 0;
+  int maxv = 1;      // max vertices per shape
+  int maxwords = 1;  // max 64-bit words of predicate bits per body pair
+};
+
+struct NLWork {
+  char err[256] = {0};
+  int cand_hint = 0, near_hint = 0, last_candidates = 0;
+#define RK_BUF(T, name) \
+  T* name = nullptr;    \
+  size_t name##_cap = 0;
+  RK_BUF(double, bobb)
+  RK_BUF(double, baabb)
+  RK_BUF(int, counter)
+  RK_BUF(uint64_t, cand)
+  RK_BUF(int32_t, cbase)
+  RK_BUF(int32_t, cnf)
+  RK_BUF(int32_t, cnb)
+  RK_BUF(uint32_t, nearbuf)
+  RK_BUF(uint64_t, okey)
+  RK_BUF(int32_t, ostart)
+  RK_BUF(uint64_t, gk)
+  RK_BUF(uint64_t, gk2)
+  RK_BUF(uint32_t, gv)
+  RK_BUF(uint32_t, gv2)
+  RK_BUF(int32_t, gflag)
+  RK_BUF(int32_t, gpos)
+  RK_BUF(uint64_t, ukey)
+  RK_BUF(int32_t, ucand)
+  RK_BUF(int32_t, uold)
+  RK_BUF(int32_t, gcount)
+  RK_BUF(int32_t, goff)
+  RK_BUF(unsigned char, tmp)
+  RK_BUF(int32_t, i32a)
+  RK_BUF(int32_t, i32b)
+  RK_BUF(int32_t, i32c)
+  RK_BUF(double, f64a)
+  RK_BUF(int32_t, cell_of)
+  RK_BUF(int32_t, ccount)
+  RK_BUF(int32_t, cstart)
+  RK_BUF(int32_t, cfill)
+  RK_BUF(int32_t, cbody)
+  RK_BUF(int32_t, over)
+#undef RK_BUF
+};
+
+// Rebuild the list: reads `cur` (sorted), writes `nxt` (sorted, with CSR).
+int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArrays& nxt, hipStream_t s);
+// istart / jstart / jlist of a sorted list.
+int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s);
+int entries_reserve(NLWork& w, EntryArrays& E, int n);
+void entries_free(EntryArrays& E);
+void nl_work_free(NLWork& w);
+
+}  // namespace rk

@@ -1,0 +1,827 @@
+// rk_step.hip -- the per-step kernels of the velocity-Verlet DEM step on
+// CDNA4 (gfx950), FP64 throughout, no floating-point atomics.
+//
+//   k_predict   Rockable::velocityVerletStep predictor (Rockable.cpp:1979-2055)
+//               + ServoFunction (DrivingSystem.cpp:242-268) and
+//               velocityControlledDrive (Rockable.cpp:1856-1900)
+//   k_interact  update_interactions (Rockable.cpp:2968-2974 ->
+//               Interaction.cpp:147-326) fused with build_activeInteractions
+//               (3005-3029) and ForceLaw::computeInteraction (3035-3046,
+//               ForceLaw_*.cpp); writes each active entry's force and its two
+//               moment contributions (incrementResultants, 2752-2789)
+//   k_gather    initialise_particle_forces_and_moments (2869-2929) +
+//               compute_resultants (3267-3272) as a per-body ordered sum over
+//               the body's incident entries in global key order, which is the
+//               serial order of the reference, + compute_accelerations_from_
+//               resultants (3419-3552) + the closing kick (2120-2155)
+//   k_breakage  breakage_of_interfaces (3350-3413)
+#include <hip/hip_runtime.h>
+
+#include "rk_device.hpp"
+#include "rk_model.hpp"
+
+namespace rk {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ v3 kpos(const BodyKin& b) { return V(b.px, b.py, b.pz); }
+__device__ __forceinline__ v3 kvel(const BodyKin& b) { return V(b.vx, b.vy, b.vz); }
+__device__ __forceinline__ v3 kvrot(const BodyKin& b) { return V(b.wx, b.wy, b.wz); }
+__device__ __forceinline__ qt kq(const BodyKin& b) {
+  qt q;
+  q.w = b.qw;
+  q.x = b.qx;
+  q.y = b.qy;
+  q.z = b.qz;
+  return q;
+}
+__device__ __forceinline__ v3 ld3(const double* p) { return V(p[0], p[1], p[2]); }
+__device__ __forceinline__ void st3(double* p, v3 v) {
+  p[0] = v.x;
+  p[1] = v.y;
+  p[2] = v.z;
+}
+
+// Particle::GlobVertex (Particle.cpp:55-57): pos + Q * (homothety * vertex)
+__device__ __forceinline__ v3 glob_vertex(const DevShapes& sh, const BodyKin& b, int v) {
+  const double* p = sh.vtx + 3 * (sh.voff[b.shape] + v);
+  v3 hv = vsc(b.h, V(p[0], p[1], p[2]));
+  return vadd(kpos(b), qrot(kq(b), hv));
+}
+
+// v(Qj) - v(Qi) (Interaction.cpp:162)
+__device__ __forceinline__ v3 rel_vel(v3 pos, const BodyKin& Pi, const BodyKin& Pj) {
+  v3 a = vsub(kvel(Pj), vcross(vsub(pos, kpos(Pj)), kvrot(Pj)));
+  v3 b = vsub(kvel(Pi), vcross(vsub(pos, kpos(Pi)), kvrot(Pi)));
+  return vsub(a, b);
+}
+
+__device__ __forceinline__ double prm(const DevParams& p, int id, int g1, int g2) {
+  if (g1 < 0 || g2 < 0 || g1 >= p.G || g2 >= p.G) return 0.0;
+  return p.tab[(id * p.G + g1) * p.G + g2];
+}
+
+struct Kin {
+  v3 n, pos, vel;
+  double dn, prev_dn;
+  bool deact;
+};
+
+// even-odd crossing test of P against face f of body j (Interaction.cpp:236-253)
+__device__ bool face_crossing(const DevShapes& sh, const BodyKin& Pj, int gface, int nbv, v3 P, v3 v1, v3 v2) {
+  int odd = 0;
+  double ori1 = vdot(P, v1), ori2 = vdot(P, v2);
+  const int* fv = sh.fidx + sh.fstart[gface];
+  v3 A = glob_vertex(sh, Pj, fv[0]);
+  for (int iva = 0; iva < nbv; iva++) {
+    int ivb = iva + 1;
+    if (ivb == nbv) ivb = 0;
+    v3 B = glob_vertex(sh, Pj, fv[ivb]);
+    double pa1 = vdot(A, v1), pb1 = vdot(B, v1);
+    double pa2 = vdot(A, v2), pb2 = vdot(B, v2);
+    if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
+      if (pa1 + (ori2 - pa2) / (pb2 - pa2) * (pb1 - pa1) < ori1) odd = 1 - odd;
+    }
+    A = B;
+  }
+  return odd != 0;
+}
+
+// Interaction::UpdateDispatcher (Interaction.cpp:147-326)
+__device__ void update_kinematics(const DevShapes& sh, int type, int isub, int jsub, const BodyKin& Pi,
+                                  const BodyKin& Pj, Kin& k) {
+  const double Ri = Pi.R, Rj = Pj.R;
+  if (type == VV) {
+    v3 posi = glob_vertex(sh, Pi, isub);
+    v3 posj = glob_vertex(sh, Pj, jsub);
+    k.n = vsub(posi, posj);
+    k.dn = vnormalize(&k.n) - (Ri + Rj);
+    k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
+    k.vel = rel_vel(k.pos, Pi, Pj);
+  } else if (type == VE) {
+    const int* ed = sh.edge + 2 * (sh.eoff[Pj.shape] + jsub);
+    v3 posj1 = glob_vertex(sh, Pj, ed[0]);
+    v3 posj2 = glob_vertex(sh, Pj, ed[1]);
+    v3 posi = glob_vertex(sh, Pi, isub);
+    v3 E = vsub(posj2, posj1);
+    v3 v = vsub(posi, posj1);
+    double r = vdot(v, E) / vdot(E, E);
+    if (r <= 0.0 || r >= 1.0) {
+      k.deact = true;
+      return;
+    }
+    k.n = vsub(posi, vadd(posj1, vsc(r, E)));
+    k.dn = vnormalize(&k.n) - (Ri + Rj);
+    k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
+    k.vel = rel_vel(k.pos, Pi, Pj);
+  } else if (type == VF) {
+    const int gface = sh.foff[Pj.shape] + jsub;
+    const int fs = sh.fstart[gface];
+    const int nbv = sh.fstart[gface + 1] - fs;
+    const int* fv = sh.fidx + fs;
+    v3 A = glob_vertex(sh, Pj, fv[0]);
+    v3 B = glob_vertex(sh, Pj, fv[1]);
+    v3 C = glob_vertex(sh, Pj, fv[nbv - 1]);
+    v3 posi = glob_vertex(sh, Pi, isub);
+    v3 v = vsub(posi, A);
+    v3 v1 = vsub(B, A);
+    vnormalize(&v1);
+    v3 v2 = vsub(C, A);
+    k.n = vcross(v1, v2);
+    vnormalize(&k.n);
+    double dist = vdot(v, k.n);
+    if (dist < 0.0) {
+      k.n = vneg(k.n);
+      dist = -dist;
+    }
+    v3 P = vsub(posi, vsc(dist, k.n));
+    v2 = vcross(k.n, v1);
+    if (face_crossing(sh, Pj, gface, nbv, P, v1, v2)) {
+      k.dn = dist - (Ri + Rj);
+      k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
+      k.vel = rel_vel(k.pos, Pi, Pj);
+    } else {
+      k.deact = true;  // n keeps the new face normal, pos and vel stay stale
+    }
+  } else {
+    const int* edi = sh.edge + 2 * (sh.eoff[Pi.shape] + isub);
+    const int* edj = sh.edge + 2 * (sh.eoff[Pj.shape] + jsub);
+    v3 posi1 = glob_vertex(sh, Pi, edi[0]);
+    v3 posi2 = glob_vertex(sh, Pi, edi[1]);
+    v3 posj1 = glob_vertex(sh, Pj, edj[0]);
+    v3 posj2 = glob_vertex(sh, Pj, edj[1]);
+    v3 Ei = vsub(posi2, posi1);
+    v3 Ej = vsub(posj2, posj1);
+    v3 v = vsub(posi1, posj1);
+    double c = vdot(Ei, Ei), d = vdot(Ej, Ej), e = vdot(Ei, Ej);
+    double f = (c * d) - (e * e);
+    if (fabs(f) > 1.0e-12) {
+      f = 1.0 / f;
+      double a = vdot(Ei, v), b = vdot(Ej, v);
+      double s = (e * b - a * d) * f;
+      double t = (c * b - e * a) * f;
+      if (s <= 0.0 || s >= 1.0 || t <= 0.0 || t >= 1.0) {
+        k.deact = true;
+        return;
+      }
+      k.n = vsub(vadd(posi1, vsc(s, Ei)), vadd(posj1, vsc(t, Ej)));
+      k.dn = vnormalize(&k.n) - (Ri + Rj);
+      k.pos = vsub(vadd(posi1, vsc(s, Ei)), vsc(Ri + 0.5 * k.dn, k.n));
+      k.vel = rel_vel(k.pos, Pi, Pj);
+    } else {
+      k.deact = true;
+    }
+  }
+}
+
+struct Hist {
+  double fn;
+  v3 ft, mom;
+};
+
+// ForceLaw_Default::computeInteraction (ForceLaw_Default.cpp:57-131), FT_CORR off
+__device__ void law_default(const Dev& d, const StepParams& sp, const Kin& k, double damp, const BodyKin& Pi,
+                            const BodyKin& Pj, Hist& h) {
+  if (k.dn > 0.0) {
+    h.fn = 0.0;
+    h.ft = V(0, 0, 0);
+    h.mom = V(0, 0, 0);
+    return;
+  }
+  const int g1 = Pi.group, g2 = Pj.group;
+  const double kn = prm(d.par, P_KN, g1, g2), kt = prm(d.par, P_KT, g1, g2), mu = prm(d.par, P_MU, g1, g2);
+  const double kr = prm(d.par, P_KR, g1, g2), mur = prm(d.par, P_MUR, g1, g2);
+  double vn = vdot(k.vel, k.n);
+  double fne = -kn * k.dn;
+  double fnv = damp * vn;
+  h.fn = fne + fnv;
+  if (h.fn < 0.0) h.fn = 0.0;
+  v3 vt = vsub(k.vel, vsc(vn, k.n));
+  h.ft = vadd(h.ft, vsc(kt, vsc(sp.dt, vt)));
+  double threshold = fabs(mu * fne);
+  double ft_square = vdot(h.ft, h.ft);
+  if (ft_square > 0.0 && ft_square >= threshold * threshold) h.ft = vsc(threshold / sqrt(ft_square), h.ft);
+  if (kr > 0.0) {
+    h.mom = vadd(h.mom, vsc(sp.dt, vsc(kr, vsub(kvrot(Pj), kvrot(Pi)))));
+    double threshold_mom = fabs(mur * h.fn);
+    double mom_square = vdot(h.mom, h.mom);
+    if (mom_square > 0.0 && mom_square >= threshold_mom * threshold_mom)
+      h.mom = vsc(threshold_mom / sqrt(mom_square), h.mom);
+  }
+}
+
+// ForceLaw_Avalanche::computeInteraction (ForceLaw_Avalanche.cpp:57-136)
+__device__ void law_avalanche(const Dev& d, const StepParams& sp, const Kin& k, int i, const BodyKin& Pi,
+                              const BodyKin& Pj, Hist& h) {
+  if (k.dn > 0.0) {
+    h.fn = 0.0;
+    h.ft = V(0, 0, 0);
+    h.mom = V(0, 0, 0);
+    return;
+  }
+  const int g1 = Pi.group, g2 = Pj.group;
+  const double kn = prm(d.par, P_KN, g1, g2), en2 = prm(d.par, P_EN2, g1, g2), kt = prm(d.par, P_KT, g1, g2);
+  const double mu = prm(d.par, P_MU, g1, g2), kr = prm(d.par, P_KR, g1, g2), mur = prm(d.par, P_MUR, g1, g2);
+  double prev_dn = k.prev_dn;
+  if (prev_dn > 0.0) prev_dn = 0.0;
+  double delta_Dn = k.dn - prev_dn;
+  if (delta_Dn > 0.0)
+    h.fn = -kn * en2 * k.dn;
+  else if (delta_Dn < 0.0)
+    h.fn += -kn * delta_Dn;
+  if (h.fn < 0.0) h.fn = 0.0;
+  v3 vt = vsub(k.vel, vsc(vdot(k.vel, k.n), k.n));
+  h.ft = vadd(h.ft, vsc(kt, vsc(sp.dt, vt)));
+  double threshold_ft = fabs(mu * h.fn);
+  double ft_square = vdot(h.ft, h.ft);
+  if (ft_square > 0.0 && ft_square > threshold_ft * threshold_ft)
+    h.ft = vsc(threshold_ft / sqrt(ft_square), h.ft);
+  h.mom = vadd(h.mom, vsc(sp.dt, vsc(kr, vsub(kvrot(Pj), kvrot(Pi)))));
+  const BodyKin& P = (i < d.nDriven) ? Pj : Pi;
+  v3 branch = vsub(k.pos, kpos(P));
+  double r = vdot(branch, kvrot(P)) / vdot(kvrot(P), kvrot(P));
+  branch = vsub(branch, vsc(r, kvrot(P)));
+  double threshold_mom = fabs(mur * sqrt(vnorm2(branch)) * h.fn);
+  double mom_square = vdot(h.mom, h.mom);
+  if (mom_square > 0.0 && mom_square > threshold_mom * threshold_mom)
+    h.mom = vsc(threshold_mom / sqrt(mom_square), h.mom);
+}
+
+// ForceLaw_StickedLinks::computeInteraction (ForceLaw_StickedLinks.cpp:75-215)
+__device__ void law_sticked(const Dev& d, const StepParams& sp, const Kin& k, double damp, int stick, int i, int j,
+                            const BodyKin& Pi, const BodyKin& Pj, Hist& h, bool paramsInInterfaces) {
+  if (stick >= 0) {
+    double kn = 0.0, kt = 0.0, kr = 0.0, fn0 = 1.0, ft0 = 1.0, mom0 = 1.0, power = 1.0;
+    const double* bp = d.ifc.par + 8 * stick;
+    double dn0 = bp[7];
+    const int g1 = Pi.group, g2 = Pj.group;
+    if (paramsInInterfaces) {
+      kn = bp[0];
+      kt = bp[1];
+      kr = bp[2];
+      fn0 = bp[3];
+      ft0 = bp[4];
+      mom0 = bp[5];
+      power = bp[6];
+    } else if (d.cluster[i] == d.cluster[j]) {
+      kn = prm(d.par, P_KN_IN, g1, g2);
+      kt = prm(d.par, P_KT_IN, g1, g2);
+      kr = prm(d.par, P_KR_IN, g1, g2);
+      fn0 = prm(d.par, P_FN0_IN, g1, g2);
+      ft0 = prm(d.par, P_FT0_IN, g1, g2);
+      mom0 = prm(d.par, P_MOM0_IN, g1, g2);
+      power = prm(d.par, P_POW_IN, g1, g2);
+      dn0 = 0.0;
+    } else {
+      kn = prm(d.par, P_KN_OUT, g1, g2);
+      kt = prm(d.par, P_KT_OUT, g1, g2);
+      kr = prm(d.par, P_KR_OUT, g1, g2);
+      fn0 = prm(d.par, P_FN0_OUT, g1, g2);
+      ft0 = prm(d.par, P_FT0_OUT, g1, g2);
+      mom0 = prm(d.par, P_MOM0_OUT, g1, g2);
+      power = prm(d.par, P_POW_OUT, g1, g2);
+    }
+    double vn = vdot(k.vel, k.n);
+    double fne = -kn * (k.dn - dn0);
+    double fnv = damp * vn;
+    h.fn = fne + fnv;
+    v3 vt = vsub(k.vel, vsc(vn, k.n));
+    h.ft = vadd(h.ft, vsc(kt, vsc(sp.dt, vt)));
+    v3 ftv = vsc(damp, vt);
+    h.ft = vadd(h.ft, ftv);
+    h.mom = vadd(h.mom, vsc(sp.dt, vsc(kr, vsub(kvrot(Pj), kvrot(Pi)))));
+    double yield = pow(sqrt(vnorm2(h.ft)) / ft0, power) + pow(sqrt(vnorm2(h.mom)) / mom0, power) - h.fn / fn0 - 1.0;
+    if (yield > 0.0) d.ifc.brk[stick] = 1;  // idempotent byte store, consumed after the resultants
+  } else {
+    if (k.dn > 0.0) {
+      h.fn = 0.0;
+      h.ft = V(0, 0, 0);
+      h.mom = V(0, 0, 0);
+      return;
+    }
+    const int g1 = Pi.group, g2 = Pj.group;
+    const double kn = prm(d.par, P_KN, g1, g2), kt = prm(d.par, P_KT, g1, g2), mu = prm(d.par, P_MU, g1, g2);
+    const double kr = prm(d.par, P_KR, g1, g2), mur = prm(d.par, P_MUR, g1, g2);
+    double vn = vdot(k.vel, k.n);
+    double fne = -kn * k.dn;
+    double fnv = damp * vn;
+    h.fn = fne + fnv;
+    if (h.fn < 0.0) h.fn = 0.0;
+    v3 vt = vsub(k.vel, vsc(vn, k.n));
+    h.ft = vadd(h.ft, vsc(kt, vsc(sp.dt, vt)));
+    double threshold = fabs(mu * h.fn);
+    double ft_square = vdot(h.ft, h.ft);
+    if (ft_square > 0.0 && ft_square >= threshold * threshold) h.ft = vsc(threshold / sqrt(ft_square), h.ft);
+    h.mom = vadd(h.mom, vsc(sp.dt, vsc(kr, vsub(kvrot(Pj), kvrot(Pi)))));
+    double threshold_mom = fabs(mur * h.fn);
+    double mom_square = vdot(h.mom, h.mom);
+    if (mom_square > 0.0 && mom_square >= threshold_mom * threshold_mom)
+      h.mom = vsc(threshold_mom / sqrt(mom_square), h.mom);
+  }
+}
+
+template <int LAW>
+__global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int write_geom_all, int pii) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.ne) return;
+  const int i = d.ei[e], j = d.ej[e];
+  const uint32_t sub = d.esub[e];
+  const int type = (int)(sub >> 30), isub = (int)((sub >> 15) & 0x7fffu), jsub = (int)(sub & 0x7fffu);
+  const BodyKin Pi = d.kin[i];
+  const BodyKin Pj = d.kin[j];
+  const int stick = d.estick[e];
+  const double dn_old = d.est[e].dn;
+
+  Kin k;
+  k.deact = false;
+  k.prev_dn = dn_old;
+  if (stick >= 0) {  // a bond may run on stale geometry after a deactivation
+    const EGeom g = d.egeo[e];
+    k.n = V(g.nx, g.ny, g.nz);
+    k.pos = V(g.px, g.py, g.pz);
+    k.vel = V(g.vx, g.vy, g.vz);
+  }
+  update_kinematics(d.sh, type, isub, jsub, Pi, Pj, k);
+
+  Hist h;
+  if (k.deact) {  // Interaction::deactivate (Interaction.cpp:123-128)
+    k.dn = 0.0;
+    k.prev_dn = 0.0;
+    h.fn = 0.0;
+    h.ft = V(0, 0, 0);
+    h.mom = V(0, 0, 0);
+  }
+  const bool active = (k.dn < 0.0) || (stick >= 0);
+  if (active) {
+    if (!k.deact) {
+      const EState s = d.est[e];
+      h.fn = s.fn;
+      h.ft = V(s.ftx, s.fty, s.ftz);
+      h.mom = V(s.momx, s.momy, s.momz);
+    }
+    const double damp = d.edamp[e];
+    if (LAW == LAW_AVALANCHE)
+      law_avalanche(d, sp, k, i, Pi, Pj, h);
+    else if (LAW == LAW_STICKED)
+      law_sticked(d, sp, k, damp, stick, i, j, Pi, Pj, h, pii != 0);
+    else
+      law_default(d, sp, k, damp, Pi, Pj, h);
+    EState o;
+    o.dn = k.dn;
+    o.fn = h.fn;
+    o.ftx = h.ft.x;
+    o.fty = h.ft.y;
+    o.ftz = h.ft.z;
+    o.momx = h.mom.x;
+    o.momy = h.mom.y;
+    o.momz = h.mom.z;
+    d.est[e] = o;
+    // incrementResultants (Rockable.cpp:2756-2764), both sides
+    v3 f = vadd(vsc(h.fn, k.n), h.ft);
+    v3 Ci = vsub(k.pos, kpos(Pi));
+    v3 Cj = vsub(k.pos, vadd(kpos(Pj), V(0.0, 0.0, 0.0)));
+    v3 mi = vadd(vcross(Ci, f), h.mom);
+    v3 mj = vsub(vcross(Cj, vneg(f)), h.mom);
+    double* c = d.econ + 9 * (size_t)e;
+    st3(c, f);
+    st3(c + 3, mi);
+    st3(c + 6, mj);
+  } else if (k.deact) {
+    EState o;
+    o.dn = 0.0;
+    o.fn = 0.0;
+    o.ftx = o.fty = o.ftz = 0.0;
+    o.momx = o.momy = o.momz = 0.0;
+    d.est[e] = o;
+  } else {
+    d.est[e].dn = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
+  }
+  // Geometry is persistent for bonds (a deactivated bond keeps its stale n,
+  // pos, vel); for other entries it is written when an output is due.
+  if (stick >= 0 || (write_geom_all && !k.deact)) {
+    EGeom g;
+    g.nx = k.n.x;
+    g.ny = k.n.y;
+    g.nz = k.n.z;
+    g.px = k.pos.x;
+    g.py = k.pos.y;
+    g.pz = k.pos.z;
+    g.vx = k.vel.x;
+    g.vy = k.vel.y;
+    g.vz = k.vel.z;
+    d.egeo[e] = g;
+  }
+  d.eact[e] = active ? 1 : 0;
+}
+
+__device__ __forceinline__ v3 euler_domega(v3 I, v3 w, v3 M) {
+  return V((M.x - (I.z - I.y) * w.y * w.z) / I.x, (M.y - (I.x - I.z) * w.z * w.x) / I.y,
+           (M.z - (I.y - I.x) * w.x * w.y) / I.z);
+}
+
+__device__ void body_force(const Dev& d, int b, const BodyKin& P, v3& F, v3& M) {
+  if (d.bf == BF_ATTRACTING) {  // BodyForce_AttractingPoint.cpp:51-59
+    v3 dir = vsub(V(d.bf_point[0], d.bf_point[1], d.bf_point[2]), kpos(P));
+    double l2 = vnorm2(dir);
+    if (l2 > 0.0) {
+      double inv = 1.0 / sqrt(l2);
+      dir = V(dir.x * inv, dir.y * inv, dir.z * inv);
+      F = vsc(d.mass[b] * d.bf_acc, dir);
+    }
+  } else if (d.bf == BF_PREFDIR) {  // BodyForce_PreferredDirection.cpp:54-75
+    v3 axis = V(d.bf_axis[0], d.bf_axis[1], d.bf_axis[2]);
+    v3 axisB = qrot(kq(P), V(d.bf_axisBody[0], d.bf_axisBody[1], d.bf_axisBody[2]));
+    vnormalize(&axisB);
+    v3 u = vcross(axis, axisB);
+    vnormalize(&u);
+    double cosa = vdot(axisB, axis);
+    double angle = acos(cosa);
+    double sgn = -1.0;
+    if (cosa < 0.0) {
+      sgn = 1.0;
+      angle = M_PI - angle;
+    }
+    M = vsc(angle, vsc(d.bf_Kr * sgn, u));
+  } else if (d.bf == BF_VISCOUS) {  // BodyForce_ViscousFluid.cpp:51-62
+    v3 velSqr = V(P.vx * P.vx, P.vy * P.vy, P.vz * P.vz);
+    F = vsc(0.5 * 0.47 * d.bf_fluidDensity * d.bf_surface[b], velSqr);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.nb) return;
+  BodyKin P = d.kin[b];
+  const bool driven = b < d.nDriven;
+  // initialise_particle_forces_and_moments (Rockable.cpp:2869-2929)
+  v3 F = V(0.0, 0.0, 0.0), M = V(0.0, 0.0, 0.0);
+  if (!driven && d.bf != BF_NONE) {
+    v3 fm = V(0, 0, 0), mm = V(0, 0, 0);
+    body_force(d, b, P, fm, mm);
+    F = vadd(F, fm);
+    M = vadd(M, mm);
+  }
+  int c0 = 0, c1 = 0;
+  if (driven && d.nctrl > 0) {
+    c0 = d.ctrl_off[b];
+    c1 = d.ctrl_off[b + 1];
+    for (int q = c0; q < c1; q++) {
+      const DevControl C = d.ctrl[d.ctrl_idx[q]];
+      switch (C.type) {
+        case C_X_FOR: F.x = C.value; break;
+        case C_Y_FOR: F.y = C.value; break;
+        case C_Z_FOR: F.z = C.value; break;
+        case C_XROT_MOM: M.x = C.value; break;
+        case C_YROT_MOM: M.y = C.value; break;
+        case C_ZROT_MOM: M.z = C.value; break;
+        case C_XYZROT_MOM: M = V(C.vx, C.vy, C.vz); break;
+        default: break;
+      }
+    }
+  }
+  // compute_resultants: incident active entries in ascending global index
+  const int is = d.istart[b], ie = d.istart[b + 1];
+  int q = d.jstart[b];
+  const int qe = d.jstart[b + 1];
+  for (; q < qe; q++) {
+    const int e = d.jlist[q];
+    if (e >= is) break;
+    if (d.eact[e]) {
+      const double* c = d.econ + 9 * (size_t)e;
+      F = vsub(F, ld3(c));
+      M = vadd(M, ld3(c + 6));
+    }
+  }
+  for (int e = is; e < ie; e++) {
+    if (d.eact[e]) {
+      const double* c = d.econ + 9 * (size_t)e;
+      F = vadd(F, ld3(c));
+      M = vadd(M, ld3(c + 3));
+    }
+  }
+  for (; q < qe; q++) {
+    const int e = d.jlist[q];
+    if (d.eact[e]) {
+      const double* c = d.econ + 9 * (size_t)e;
+      F = vsub(F, ld3(c));
+      M = vadd(M, ld3(c + 6));
+    }
+  }
+  st3(d.force + 3 * (size_t)b, F);
+  st3(d.moment + 3 * (size_t)b, M);
+
+  // compute_accelerations_from_resultants (Rockable.cpp:3419-3552)
+  const double m = d.mass[b];
+  const v3 I = ld3(d.inertia + 3 * (size_t)b);
+  const qt Q = kq(P);
+  v3 acc, arot;
+  if (driven) {
+    acc = V(0.0, 0.0, 0.0);
+    arot = V(0.0, 0.0, 0.0);
+    for (int qq = c0; qq < c1; qq++) {
+      const DevControl C = d.ctrl[d.ctrl_idx[qq]];
+      switch (C.type) {
+        case C_X_FOR: acc.x = F.x / m; break;
+        case C_Y_FOR: acc.y = F.y / m; break;
+        case C_Z_FOR: acc.z = F.z / m; break;
+        case C_XROT_MOM:
+        case C_YROT_MOM:
+        case C_ZROT_MOM:
+        case C_XYZROT_MOM: {
+          qt Qinv = qconj(Q);
+          v3 w;
+          if (C.type == C_XROT_MOM)
+            w = V(P.wx, 0.0, 0.0);
+          else if (C.type == C_YROT_MOM)
+            w = V(0.0, P.wy, 0.0);
+          else if (C.type == C_ZROT_MOM)
+            w = V(0.0, 0.0, P.wz);
+          else
+            w = kvrot(P);
+          v3 omega = qrot(Qinv, w);
+          v3 Mb = qrot(Qinv, M);
+          v3 dw = qrot(Q, euler_domega(I, omega, Mb));
+          if (C.type == C_XROT_MOM)
+            arot.x = dw.x;
+          else if (C.type == C_YROT_MOM)
+            arot.y = dw.y;
+          else if (C.type == C_ZROT_MOM)
+            arot.z = dw.z;
+          else
+            arot = dw;
+        } break;
+        default: break;
+      }
+    }
+  } else {
+    acc = vadd(vdiv(F, m), V(sp.gx, sp.gy, sp.gz));
+    qt Qinv = qconj(Q);
+    v3 omega = qrot(Qinv, kvrot(P));
+    v3 Mb = qrot(Qinv, M);
+    arot = qrot(Q, euler_domega(I, omega, Mb));
+  }
+  st3(d.acc + 3 * (size_t)b, acc);
+  st3(d.arot + 3 * (size_t)b, arot);
+  if (!kick) return;
+  // closing half kick (Rockable.cpp:2120-2155)
+  if (driven) {
+    for (int qq = c0; qq < c1; qq++) {
+      const DevControl C = d.ctrl[d.ctrl_idx[qq]];
+      switch (C.type) {
+        case C_X_FOR: P.vx += sp.dt_2 * acc.x; break;
+        case C_Y_FOR: P.vy += sp.dt_2 * acc.y; break;
+        case C_Z_FOR: P.vz += sp.dt_2 * acc.z; break;
+        case C_XROT_MOM: P.wx += sp.dt_2 * arot.x; break;
+        case C_YROT_MOM: P.wy += sp.dt_2 * arot.y; break;
+        case C_ZROT_MOM: P.wz += sp.dt_2 * arot.z; break;
+        case C_XYZROT_MOM: {
+          v3 w = vadd(kvrot(P), vsc(sp.dt_2, arot));
+          P.wx = w.x;
+          P.wy = w.y;
+          P.wz = w.z;
+        } break;
+        default: break;
+      }
+    }
+  } else {
+    v3 v = vadd(kvel(P), vsc(sp.dt_2, acc));
+    v3 w = vadd(kvrot(P), vsc(sp.dt_2, arot));
+    P.vx = v.x;
+    P.vy = v.y;
+    P.vz = v.z;
+    P.wx = w.x;
+    P.wy = w.y;
+    P.wz = w.z;
+  }
+  BodyKin* out = d.kin + b;
+  out->vx = P.vx;
+  out->vy = P.vy;
+  out->vz = P.vz;
+  out->wx = P.wx;
+  out->wy = P.wy;
+  out->wz = P.wz;
+}
+
+__global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0 && d.nctrl > 0) {
+    // ServoFunction (DrivingSystem.cpp:242-268), then the controlled bodies in
+    // list order (Rockable.cpp:1856-1900, 1988-2033), all by one thread.
+    if (d.servo != 0) {
+      const int* id = d.servo_ids;
+      const double Xmin = d.kin[id[0]].px + d.kin[id[0]].R;
+      const double Xmax = d.kin[id[1]].px - d.kin[id[1]].R;
+      const double Ymin = d.kin[id[2]].py + d.kin[id[2]].R;
+      const double Ymax = d.kin[id[3]].py - d.kin[id[3]].R;
+      const double Zmin = d.kin[id[4]].pz + d.kin[id[4]].R;
+      const double Zmax = d.kin[id[5]].pz - d.kin[id[5]].R;
+      const double dX = fabs(Xmax - Xmin), dY = fabs(Ymax - Ymin), dZ = fabs(Zmax - Zmin);
+      const double surfX = dY * dZ, surfY = dX * dZ, surfZ = dY * dX;
+      d.ctrl[d.servo_icontr + 1].value = -d.servo_pressure * surfX;
+      if (d.servo == 1) d.ctrl[d.servo_icontr + 3].value = -d.servo_pressure * surfY;
+      d.ctrl[d.servo_icontr + 5].value = -d.servo_pressure * surfZ;
+    }
+    for (int c = 0; c < d.nctrl; c++) {
+      const DevControl C = d.ctrl[c];
+      BodyKin* p = d.kin + C.i;
+      switch (C.type) {
+        case C_X_VEL:
+          p->vx = C.value;
+          p->px += sp.dt * C.value;
+          break;
+        case C_Y_VEL:
+          p->vy = C.value;
+          p->py += sp.dt * C.value;
+          break;
+        case C_Z_VEL:
+          p->vz = C.value;
+          p->pz += sp.dt * C.value;
+          break;
+        case C_XROT_VEL:
+        case C_YROT_VEL:
+        case C_ZROT_VEL:
+        case C_XYZROT_VEL: {
+          v3 w;
+          if (C.type == C_XROT_VEL)
+            w = V(C.value, 0.0, 0.0);
+          else if (C.type == C_YROT_VEL)
+            w = V(0.0, C.value, 0.0);
+          else if (C.type == C_ZROT_VEL)
+            w = V(0.0, 0.0, C.value);
+          else
+            w = V(C.vx, C.vy, C.vz);
+          qt Q = kq(*p);
+          Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+          qnormalize(&Q);
+          p->qw = Q.w;
+          p->qx = Q.x;
+          p->qy = Q.y;
+          p->qz = Q.z;
+        } break;
+        default: break;
+      }
+    }
+    for (int c = 0; c < d.nctrl; c++) {
+      const DevControl C = d.ctrl[c];
+      BodyKin* p = d.kin + C.i;
+      const double* a = d.acc + 3 * (size_t)C.i;
+      const double* ar = d.arot + 3 * (size_t)C.i;
+      switch (C.type) {
+        case C_X_FOR:
+          p->px += sp.dt * p->vx + sp.dt2_2 * a[0];
+          p->vx += sp.dt_2 * a[0];
+          break;
+        case C_Y_FOR:
+          p->py += sp.dt * p->vy + sp.dt2_2 * a[1];
+          p->vy += sp.dt_2 * a[1];
+          break;
+        case C_Z_FOR:
+          p->pz += sp.dt * p->vz + sp.dt2_2 * a[2];
+          p->vz += sp.dt_2 * a[2];
+          break;
+        case C_XROT_MOM:
+        case C_YROT_MOM:
+        case C_ZROT_MOM:
+        case C_XYZROT_MOM: {
+          v3 w;
+          if (C.type == C_XROT_MOM)
+            w = V(p->wx * 1.0, p->wx * 0.0, p->wx * 0.0);
+          else if (C.type == C_YROT_MOM)
+            w = V(p->wy * 0.0, p->wy * 1.0, p->wy * 0.0);
+          else if (C.type == C_ZROT_MOM)
+            w = V(p->wz * 0.0, p->wz * 0.0, p->wz * 1.0);
+          else
+            w = V(p->wx, p->wy, p->wz);
+          qt Q = kq(*p);
+          Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+          qnormalize(&Q);
+          p->qw = Q.w;
+          p->qx = Q.x;
+          p->qy = Q.y;
+          p->qz = Q.z;
+          if (C.type == C_XROT_MOM)
+            p->wx += sp.dt_2 * ar[0];
+          else if (C.type == C_YROT_MOM)
+            p->wy += sp.dt_2 * ar[1];
+          else if (C.type == C_ZROT_MOM)
+            p->wz += sp.dt_2 * ar[2];
+          else {
+            v3 nw = vadd(V(p->wx, p->wy, p->wz), vsc(sp.dt_2, V(ar[0], ar[1], ar[2])));
+            p->wx = nw.x;
+            p->wy = nw.y;
+            p->wz = nw.z;
+          }
+        } break;
+        default: break;
+      }
+    }
+  }
+  if (b < d.nDriven || b >= d.nb) return;
+  // free bodies (Rockable.cpp:2036-2055)
+  BodyKin P = d.kin[b];
+  const v3 a = ld3(d.acc + 3 * (size_t)b);
+  const v3 ar = ld3(d.arot + 3 * (size_t)b);
+  v3 pos = vadd(kpos(P), vadd(vsc(sp.dt, kvel(P)), vsc(sp.dt2_2, a)));
+  v3 vel = vadd(kvel(P), vsc(sp.dt_2, a));
+  qt Q = kq(P);
+  const v3 w = kvrot(P);
+  Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+  Q = qadd_scaled(Q, qddot(Q, w, ar), sp.dt2_2);
+  qnormalize(&Q);
+  v3 w2 = vadd(w, vsc(sp.dt_2, ar));
+  BodyKin* out = d.kin + b;
+  out->px = pos.x;
+  out->py = pos.y;
+  out->pz = pos.z;
+  out->qw = Q.w;
+  out->qx = Q.x;
+  out->qy = Q.y;
+  out->qz = Q.z;
+  out->vx = vel.x;
+  out->vy = vel.y;
+  out->vz = vel.z;
+  out->wx = w2.x;
+  out->wy = w2.y;
+  out->wz = w2.z;
+}
+
+// locate entry (owner, other, sub) in the owner's sorted run
+__device__ int find_entry(const Dev& d, int owner, int other, uint32_t sub) {
+  int lo = d.istart[owner], hi = d.istart[owner + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    bool less = (d.ej[mid] < other) || (d.ej[mid] == other && d.esub[mid] < sub);
+    if (less)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo < d.istart[owner + 1] && d.ej[lo] == other && d.esub[lo] == sub) return lo;
+  return -1;
+}
+
+__global__ void k_breakage(Dev d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.ifc.n || !d.ifc.brk[k] || !d.ifc.alive[k]) return;
+  const uint64_t key = d.ifc.key[k];
+  const int bi = (int)(key >> 32), bj = (int)(key & 0xffffffffu);
+  const int g1 = d.kin[bi].group, g2 = d.kin[bj].group;
+  for (int q = d.ifc.bond_off[k]; q < d.ifc.bond_off[k + 1]; q++) {
+    const int* bk = d.ifc.bond_key + 3 * q;
+    const int e = find_entry(d, bi, bj, pack_sub(bk[0], bk[1], bk[2]));
+    if (e < 0) continue;
+    d.estick[e] = -1;
+    EState s = d.est[e];
+    if (s.dn >= 0.0) {
+      s.fn = 0.0;
+      s.ftx = s.fty = s.ftz = 0.0;
+    } else {
+      v3 T = V(s.ftx, s.fty, s.ftz);
+      double FT = vnormalize(&T);
+      double mu = prm(d.par, P_MU, g1, g2);
+      double thr = fabs(mu * s.fn);
+      if (FT > thr) {
+        v3 ft = vsc(thr, T);
+        s.ftx = ft.x;
+        s.fty = ft.y;
+        s.ftz = ft.z;
+      }
+    }
+    d.est[e] = s;
+  }
+  d.ifc.alive[k] = 0;
+  d.flags[0] = 1;  // needUpdate
+}
+
+}  // namespace
+
+void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s) {
+  const int n = d.nb > 0 ? d.nb : 1;
+  hipLaunchKernelGGL(k_predict, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp);
+}
+
+void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
+  if (d.ne <= 0) return;
+  dim3 g((d.ne + kBlock - 1) / kBlock), bl(kBlock);
+  if (sp.law == LAW_AVALANCHE)
+    hipLaunchKernelGGL(k_interact<LAW_AVALANCHE>, g, bl, 0, s, d, sp, write_geom_all, 0);
+  else if (sp.law == LAW_STICKED)
+    hipLaunchKernelGGL(k_interact<LAW_STICKED>, g, bl, 0, s, d, sp, write_geom_all, d.paramsInInterfaces);
+  else
+    hipLaunchKernelGGL(k_interact<LAW_DEFAULT>, g, bl, 0, s, d, sp, write_geom_all, 0);
+}
+
+void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) {
+  if (d.nb <= 0) return;
+  hipLaunchKernelGGL(k_gather, dim3((d.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp, kick);
+}
+
+void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s) {
+  (void)sp;
+  if (d.ifc.n <= 0) return;
+  hipLaunchKernelGGL(k_breakage, dim3((d.ifc.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d);
+}
+
+}  // namespace rk

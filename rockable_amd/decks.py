@@ -182,6 +182,7 @@ class Deck:
     iconf: int = 0
     extra: list = field(default_factory=list)         # extra raw lines
     body_force: Optional[str] = None
+    box_for_linkcells_opt: int = 0
     dynamic_update_nl: int = 0
     disp_update_nl: float = 1.0
     angle_update_nl: float = 1.0
@@ -217,6 +218,7 @@ def write_conf(path, deck: Deck):
     for name, g1, g2, v in deck.params:
         o.append("%s %d %d %s" % (name, g1, g2, _r(v)))
     o.append("cellMinSizes %s %s %s" % tuple(_r(x) for x in deck.cell_min_sizes))
+    o.append("boxForLinkCellsOpt %d" % deck.box_for_linkcells_opt)
     o.append("iconf %d" % deck.iconf)
     o.append("nDriven %d" % deck.nDriven)
     o.extend(deck.extra)
@@ -247,3 +249,134 @@ def contact_params(kn, kt, en2, mu, kr=0.0, mur=0.0, groups=((0, 0),)):
         p += [("knContact", g1, g2, kn), ("ktContact", g1, g2, kt), ("en2Contact", g1, g2, en2),
               ("muContact", g1, g2, mu), ("krContact", g1, g2, kr), ("murContact", g1, g2, mur)]
     return p
+
+
+# ---------------------------------------------------------------------------
+# seeded synthetic workloads
+# ---------------------------------------------------------------------------
+def box_walls(lo, hi, radius, names=("WALLZ0", "WALLX0", "WALLX1", "WALLY0", "WALLY1"), top=False):
+    """Driven square plates bounding the box [lo, hi]: floor (and lid) plus four
+    side walls, each a 4-vertex face (its quaternion turns the body xy plane
+    onto the wall plane).  Returns (shapes, bodies)."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    c = 0.5 * (lo + hi)
+    L = hi - lo
+    half = 0.5 * float(L.max()) + 2 * radius
+    shp = square_plane_shape("WALL", half, radius)
+    s2 = np.sqrt(0.5)
+    q_x = (s2, 0.0, s2, 0.0)     # 90 deg about y: body z -> global x
+    q_y = (s2, -s2, 0.0, 0.0)    # -90 deg about x: body z -> global y
+    bodies = [Body("WALL", group=1, pos=(c[0], c[1], lo[2] - radius))]
+    bodies += [Body("WALL", group=1, pos=(lo[0] - radius, c[1], c[2]), Q=q_x),
+               Body("WALL", group=1, pos=(hi[0] + radius, c[1], c[2]), Q=q_x),
+               Body("WALL", group=1, pos=(c[0], lo[1] - radius, c[2]), Q=q_y),
+               Body("WALL", group=1, pos=(c[0], hi[1] + radius, c[2]), Q=q_y)]
+    if top:
+        bodies.append(Body("WALL", group=1, pos=(c[0], c[1], hi[2] + radius)))
+    return [shp], bodies
+
+
+def sphere_lattice_deck(n_side=8, R=0.5e-3, spacing_factor=2.2, seed=1, vmax=0.01, kn=1e6, en=0.5, mu=0.4,
+                        dt=1e-6, steps=10000, nl="bruteForce"):
+    """Plain method, CPU-runnable case: n_side^3 single-vertex spheres of
+    radius R on a lattice of spacing spacing_factor * R, dropping under gravity
+    into a fixed floor and four walls (5 driven plates first), velocities
+    uniform in [-vmax, vmax] from the seed; Default law."""
+    rng = np.random.default_rng(seed)
+    a = spacing_factor * R
+    gap = a - 2 * R
+    lo = np.array([0.0, 0.0, 0.0])
+    hi = np.array([n_side * a, n_side * a, n_side * a + 4 * a])
+    wall_shapes, walls = box_walls(lo, hi, 0.1 * R)
+    shapes = wall_shapes + [sphere_shape("SPHERE", R)]
+    deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=100 * dt,
+                interConf=10.0, DVerlet=0.2 * R, dVerlet=0.1 * R, density={0: 2700.0, 1: 2700.0},
+                gravity=(0.0, 0.0, -9.81), update_nl=nl, nDriven=len(walls),
+                params=contact_params(kn, 0.8 * kn, en * en, mu, groups=((0, 0), (0, 1))))
+    deck.bodies = list(walls)
+    for ix in range(n_side):
+        for iy in range(n_side):
+            for iz in range(n_side):
+                p = lo + np.array([R + 0.5 * gap + ix * a, R + 0.5 * gap + iy * a, R + 0.5 * gap + iz * a])
+                v = rng.uniform(-vmax, vmax, size=3)
+                deck.bodies.append(Body("SPHERE", group=0, pos=tuple(p), vel=tuple(v)))
+    deck.cell_min_sizes = (2.0 * a, 2.0 * a, 2.0 * a)
+    return shapes, deck
+
+
+def hull_shape_pool(n_shapes, seed, nmin=8, nmax=14, axes=(1.0, 0.8, 0.6), radius_frac=0.05):
+    """Unit-size convex sphero-polyhedra: hull of nmin..nmax random points on an
+    ellipsoid of semi-axes 0.5 * axes, Minkowski radius radius_frac."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_shapes):
+        n = int(rng.integers(nmin, nmax + 1))
+        while True:
+            u = rng.normal(size=(n, 3))
+            u /= np.linalg.norm(u, axis=1)[:, None]
+            pts = u * (0.5 * np.asarray(axes))
+            try:
+                out.append(hull_shape("HULL%03d" % k, pts, radius_frac))
+                break
+            except Exception:
+                continue
+    return out
+
+
+def random_quaternions(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    q[q[:, 0] < 0] *= -1
+    return q
+
+
+def hull_packing_deck(n_bodies=50000, seed=2, n_shapes=64, size_range=(4e-3, 6e-3), box_in_sizes=(40, 40, 200),
+                      kn=1e7, en=0.3, mu=0.6, dt=2e-7, steps=20000, vmax=0.5, law="Avalanche", gap_factor=1.05,
+                      interverlet_steps=200, lattice=None):
+    """Plain GPU form: convex sphero-polyhedra (hulls of 8-14 points on an
+    ellipsoid 1 : 0.8 : 0.6, size uniform in size_range, Minkowski radius 5 %
+    of the size) on a jittered lattice inside a box of box_in_sizes mean
+    sizes, random orientations, velocities uniform in [-vmax, vmax], under
+    gravity; linkCells neighbour list; the given force law."""
+    rng = np.random.default_rng(seed)
+    pool = hull_shape_pool(n_shapes, seed)
+    smean = 0.5 * (size_range[0] + size_range[1])
+    box = np.array(box_in_sizes, float) * smean
+    bound = max(float(np.linalg.norm(np.asarray(s.vertex), axis=1).max()) + s.radius for s in pool)
+    a = gap_factor * 2.0 * bound * size_range[1]
+    nx, ny = max(1, int(box[0] // a)), max(1, int(box[1] // a))
+    nz = int(np.ceil(n_bodies / (nx * ny)))
+    if lattice is not None:
+        nx, ny, nz = lattice
+    lo = np.zeros(3)
+    hi = np.array([nx * a, ny * a, max(box[2], nz * a)])
+    wall_shapes, walls = box_walls(lo, hi, 0.05 * smean)
+    deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=interverlet_steps * dt,
+                interConf=1e3, DVerlet=0.1 * smean, dVerlet=0.05 * smean, density={0: 2700.0, 1: 2700.0},
+                gravity=(0.0, 0.0, -9.81), update_nl="linkCells", nDriven=len(walls), force_law=law,
+                params=contact_params(kn, 0.8 * kn, en * en, mu, groups=((0, 0), (0, 1))))
+    deck.cell_min_sizes = (a, a, a)
+    deck.box_for_linkcells_opt = 1   # cell grid over the free bodies; the walls are oversized
+    deck.bodies = list(walls)
+    q = random_quaternions(rng, n_bodies)
+    k = 0
+    for iz in range(nz):
+        for iy in range(ny):
+            for ix in range(nx):
+                if k >= n_bodies:
+                    break
+                p = lo + a * (np.array([ix, iy, iz]) + 0.5)
+                size = rng.uniform(*size_range)
+                v = rng.uniform(-vmax, vmax, size=3)
+                deck.bodies.append(Body(pool[int(rng.integers(len(pool)))].name, group=0, homothety=size,
+                                        pos=tuple(p), vel=tuple(v), Q=tuple(q[k])))
+                k += 1
+    return wall_shapes + pool, deck
+
+
+def write_deck(dirpath, shapes, deck, name="deck.conf"):
+    import os
+    os.makedirs(dirpath, exist_ok=True)
+    write_shp(os.path.join(dirpath, deck.shape_file), shapes)
+    write_conf(os.path.join(dirpath, name), deck)
+    return os.path.join(dirpath, name)

@@ -1,0 +1,181 @@
+"""Python host mirror of the engine's C ABI (include/rockable_amd.h).
+
+`Engine` follows the reference's Rockable box: load a conf deck (which runs
+the initial accelerations() "fake step", Rockable.cpp:1133-1138), rebuild the
+neighbour list, run steps of the integrate() loop, save conf files.  All
+arithmetic runs in librockable_amd.so on the GPU; if the library or the GPU is
+missing, construction fails with an error -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librockable_amd.so")
+
+INTERACTION_DTYPE = np.dtype([
+    ("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("isub", "<i4"), ("jsub", "<i4"), ("stick", "<i4"),
+    ("n", "<f8", 3), ("dn", "<f8"), ("pos", "<f8", 3), ("vel", "<f8", 3), ("fn", "<f8"),
+    ("ft", "<f8", 3), ("mom", "<f8", 3), ("damp", "<f8"),
+])
+
+
+class RkCounts(ctypes.Structure):
+    _fields_ = [("nbodies", ctypes.c_int64), ("ndriven", ctypes.c_int64), ("nlisted", ctypes.c_int64),
+                ("nactive", ctypes.c_int64), ("ninterfaces", ctypes.c_int64), ("nl_updates", ctypes.c_int64),
+                ("steps", ctypes.c_int64), ("t", ctypes.c_double), ("dt", ctypes.c_double),
+                ("total_broken_area", ctypes.c_double), ("last_candidates", ctypes.c_int64)]
+
+
+EXPORTED = ["rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_conf", "rk_load_driving_system",
+            "rk_set_force_law", "rk_set_body_force", "rk_set_option", "rk_update_nl", "rk_accelerations",
+            "rk_step", "rk_integrate", "rk_save_conf", "rk_synchronize", "rk_get_counts", "rk_get_bodies",
+            "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time"]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load librockable_amd.so and declare the C ABI; raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("librockable_amd.so is not built (run `python -m rockable_amd.build`)")
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    L.rk_engine_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
+    L.rk_engine_destroy.argtypes = [P]
+    L.rk_last_error.argtypes = [P]
+    L.rk_last_error.restype = ctypes.c_char_p
+    L.rk_load_conf.argtypes = [P, ctypes.c_char_p]
+    L.rk_load_driving_system.argtypes = [P, ctypes.c_char_p]
+    L.rk_set_force_law.argtypes = [P, ctypes.c_char_p]
+    L.rk_set_body_force.argtypes = [P, ctypes.c_char_p, P, ctypes.c_int]
+    L.rk_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_double]
+    L.rk_update_nl.argtypes = [P]
+    L.rk_accelerations.argtypes = [P]
+    L.rk_step.argtypes = [P, ctypes.c_int64]
+    L.rk_integrate.argtypes = [P, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    L.rk_save_conf.argtypes = [P, ctypes.c_char_p]
+    L.rk_synchronize.argtypes = [P]
+    L.rk_get_counts.argtypes = [P, ctypes.POINTER(RkCounts)]
+    L.rk_get_bodies.argtypes = [P] + [P] * 8 + [ctypes.c_size_t]
+    L.rk_set_bodies.argtypes = [P] + [P] * 4 + [ctypes.c_size_t]
+    L.rk_get_interactions.argtypes = [P, ctypes.c_int, P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.rk_get_kernel_time.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_double)]
+    L.rk_reset_kernel_time.argtypes = [P]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Engine:
+    """One Rockable box on one GPU."""
+
+    def __init__(self, conf_path=None, device=0, driving=None):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.rk_engine_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise RuntimeError("rk_engine_create failed (%d): no usable GPU device %d" % (rc, device))
+        self.h = h
+        if conf_path is not None:
+            self.load_conf(conf_path)
+        if driving is not None:
+            self.load_driving_system(driving)
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(self.L.rk_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rk_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_conf(self, path):
+        self._check(self.L.rk_load_conf(self.h, os.fspath(path).encode()))
+
+    def load_driving_system(self, path):
+        self._check(self.L.rk_load_driving_system(self.h, os.fspath(path).encode()))
+
+    def set_force_law(self, name):
+        return self._check(self.L.rk_set_force_law(self.h, name.encode()))
+
+    def set_body_force(self, name, args=()):
+        a = np.ascontiguousarray(args, dtype=np.float64)
+        self._check(self.L.rk_set_body_force(self.h, name.encode(), _ptr(a), len(a)))
+
+    def set_option(self, name, value):
+        self._check(self.L.rk_set_option(self.h, name.encode(), float(value)))
+
+    def update_nl(self):
+        self._check(self.L.rk_update_nl(self.h))
+
+    def accelerations(self):
+        self._check(self.L.rk_accelerations(self.h))
+
+    def step(self, n=1):
+        self._check(self.L.rk_step(self.h, int(n)))
+
+    def integrate(self, outdir=None, max_steps=-1):
+        done = ctypes.c_int64(0)
+        self._check(self.L.rk_integrate(self.h, os.fspath(outdir).encode() if outdir else None, int(max_steps),
+                                        ctypes.byref(done)))
+        return int(done.value)
+
+    def save_conf(self, path):
+        self._check(self.L.rk_save_conf(self.h, os.fspath(path).encode()))
+
+    def synchronize(self):
+        self._check(self.L.rk_synchronize(self.h))
+
+    def counts(self):
+        c = RkCounts()
+        self._check(self.L.rk_get_counts(self.h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in RkCounts._fields_}
+
+    def bodies(self):
+        n = self.counts()["nbodies"]
+        out = {k: np.zeros((n, 3)) for k in ("pos", "vel", "vrot", "acc", "arot", "force", "moment")}
+        out["Q"] = np.zeros((n, 4))
+        self._check(self.L.rk_get_bodies(self.h, _ptr(out["pos"]), _ptr(out["Q"]), _ptr(out["vel"]),
+                                         _ptr(out["vrot"]), _ptr(out["acc"]), _ptr(out["arot"]),
+                                         _ptr(out["force"]), _ptr(out["moment"]), n))
+        return out
+
+    def set_bodies(self, pos=None, Q=None, vel=None, vrot=None):
+        n = self.counts()["nbodies"]
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        pos, Q, vel, vrot = c(pos), c(Q), c(vel), c(vrot)
+        self._check(self.L.rk_set_bodies(self.h, _ptr(pos), _ptr(Q), _ptr(vel), _ptr(vrot), n))
+
+    def interactions(self, active=False):
+        n = ctypes.c_size_t(0)
+        self._check(self.L.rk_get_interactions(self.h, 1 if active else 0, None, 0, ctypes.byref(n)))
+        arr = np.zeros(n.value, dtype=INTERACTION_DTYPE)
+        if n.value:
+            self._check(self.L.rk_get_interactions(self.h, 1 if active else 0, _ptr(arr), n.value, ctypes.byref(n)))
+        return arr
+
+    def kernel_time(self, name):
+        k = ctypes.c_int64(0)
+        ms = ctypes.c_double(0.0)
+        self._check(self.L.rk_get_kernel_time(self.h, name.encode(), ctypes.byref(k), ctypes.byref(ms)))
+        return int(k.value), float(ms.value)
+
+    def reset_kernel_time(self):
+        self._check(self.L.rk_reset_kernel_time(self.h))
