@@ -39,6 +39,12 @@ struct rk_engine {
   int profile = 0;
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // profile mode: start/end event pairs recorded on the engine stream and
+  // resolved lazily, so that timing adds no host synchronisation
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<std::string, int>> ev_pending;  // (kernel, index of start event)
+  size_t ev_used = 0;
+  int tic_index = -1;
   // device buffers
   BodyKin* kin = nullptr;
   double *acc = nullptr, *arot = nullptr, *force = nullptr, *moment = nullptr;
@@ -432,18 +438,43 @@ int upload_model(rk_engine* eng) {
   return 0;
 }
 
+hipEvent_t pool_event(rk_engine* eng) {
+  if (eng->ev_used == eng->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    eng->ev_pool.push_back(e);
+  }
+  return eng->ev_pool[eng->ev_used++];
+}
+
+void resolve_events(rk_engine* eng) {
+  if (eng->ev_pending.empty()) return;
+  (void)hipStreamSynchronize(eng->stream);
+  for (const auto& pr : eng->ev_pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, eng->ev_pool[pr.second], eng->ev_pool[pr.second + 1]);
+    KernelTimer& t = eng->timers[pr.first];
+    t.launches++;
+    t.ms += ms;
+  }
+  eng->ev_pending.clear();
+  eng->ev_used = 0;
+}
+
 void tic(rk_engine* eng) {
-  if (eng->profile) (void)hipEventRecord(eng->ev0, eng->stream);
+  if (!eng->profile) return;
+  if (eng->ev_used + 2 > 4096) resolve_events(eng);
+  hipEvent_t a = pool_event(eng);
+  hipEvent_t b = pool_event(eng);
+  if (!a || !b) return;
+  eng->tic_index = (int)eng->ev_used - 2;
+  (void)hipEventRecord(a, eng->stream);
 }
 void toc(rk_engine* eng, const char* name) {
-  if (!eng->profile) return;
-  (void)hipEventRecord(eng->ev1, eng->stream);
-  (void)hipEventSynchronize(eng->ev1);
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
-  KernelTimer& t = eng->timers[name];
-  t.launches++;
-  t.ms += ms;
+  if (!eng->profile || eng->tic_index < 0) return;
+  (void)hipEventRecord(eng->ev_pool[eng->tic_index + 1], eng->stream);
+  eng->ev_pending.push_back({name, eng->tic_index});
+  eng->tic_index = -1;
 }
 
 // Rockable::accelerations (Rockable.cpp:3562-3719); kick fuses the closing
@@ -691,6 +722,7 @@ void rk_engine_destroy(rk_engine* eng) {
   (void)hipSetDevice(eng->device);
   if (eng->stream) (void)hipStreamSynchronize(eng->stream);
   free_device(eng);
+  for (hipEvent_t e : eng->ev_pool) (void)hipEventDestroy(e);
   if (eng->ev0) (void)hipEventDestroy(eng->ev0);
   if (eng->ev1) (void)hipEventDestroy(eng->ev1);
   if (eng->stream) (void)hipStreamDestroy(eng->stream);
@@ -993,6 +1025,7 @@ int rk_get_interactions(rk_engine* eng, int active_only, rk_interaction* out, si
 
 int rk_get_kernel_time(rk_engine* eng, const char* kernel, int64_t* launches, double* total_ms) {
   if (!eng || !kernel) return -1;
+  resolve_events(eng);
   auto it = eng->timers.find(kernel);
   if (launches) *launches = it == eng->timers.end() ? 0 : it->second.launches;
   if (total_ms) *total_ms = it == eng->timers.end() ? 0.0 : it->second.ms;
@@ -1001,6 +1034,7 @@ int rk_get_kernel_time(rk_engine* eng, const char* kernel, int64_t* launches, do
 
 int rk_reset_kernel_time(rk_engine* eng) {
   if (!eng) return -1;
+  resolve_events(eng);
   eng->timers.clear();
   return 0;
 }

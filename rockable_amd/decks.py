@@ -330,47 +330,142 @@ def random_quaternions(rng, n):
     return q
 
 
+def small_rotations(rng, n, max_angle):
+    """Quaternions of rotations by angles uniform in [0, max_angle] about
+    random axes."""
+    ax = rng.normal(size=(n, 3))
+    ax /= np.linalg.norm(ax, axis=1)[:, None]
+    ang = rng.uniform(0.0, max_angle, size=n)
+    return np.concatenate([np.cos(0.5 * ang)[:, None], np.sin(0.5 * ang)[:, None] * ax], axis=1)
+
+
 def hull_packing_deck(n_bodies=50000, seed=2, n_shapes=64, size_range=(4e-3, 6e-3), box_in_sizes=(40, 40, 200),
                       kn=1e7, en=0.3, mu=0.6, dt=2e-7, steps=20000, vmax=0.5, law="Avalanche", gap_factor=1.05,
-                      interverlet_steps=200, lattice=None):
+                      interverlet_steps=200, lattice=None, aligned=False, tilt_deg=10.0):
     """Plain GPU form: convex sphero-polyhedra (hulls of 8-14 points on an
     ellipsoid 1 : 0.8 : 0.6, size uniform in size_range, Minkowski radius 5 %
-    of the size) on a jittered lattice inside a box of box_in_sizes mean
-    sizes, random orientations, velocities uniform in [-vmax, vmax], under
-    gravity; linkCells neighbour list; the given force law."""
+    of the size) inside a box of box_in_sizes mean sizes under gravity, with
+    velocities uniform in [-vmax, vmax]; linkCells neighbour list; the given
+    force law.  Bodies sit on a lattice: with aligned=False each gets a random
+    orientation and the spacing is the largest circumscribed diameter times
+    gap_factor; with aligned=True (a compact pour) each body keeps its principal
+    axes along the lattice axes up to a random tilt of at most tilt_deg, and the
+    spacing along each axis is the largest extent along it times gap_factor."""
     rng = np.random.default_rng(seed)
     pool = hull_shape_pool(n_shapes, seed)
     smean = 0.5 * (size_range[0] + size_range[1])
     box = np.array(box_in_sizes, float) * smean
-    bound = max(float(np.linalg.norm(np.asarray(s.vertex), axis=1).max()) + s.radius for s in pool)
-    a = gap_factor * 2.0 * bound * size_range[1]
-    nx, ny = max(1, int(box[0] // a)), max(1, int(box[1] // a))
+    if aligned:
+        ext = np.max([np.asarray(s.extent()) for s in pool], axis=0)
+        t = np.radians(tilt_deg)
+        tilted = np.array([ext[0] * np.cos(t) + max(ext[1], ext[2]) * np.sin(t),
+                           ext[1] * np.cos(t) + max(ext[0], ext[2]) * np.sin(t),
+                           ext[2] * np.cos(t) + max(ext[0], ext[1]) * np.sin(t)])
+        a3 = gap_factor * 2.0 * tilted * size_range[1]
+    else:
+        bound = max(float(np.linalg.norm(np.asarray(s.vertex), axis=1).max()) + s.radius for s in pool)
+        a3 = np.full(3, gap_factor * 2.0 * bound * size_range[1])
+    nx, ny = max(1, int(box[0] // a3[0])), max(1, int(box[1] // a3[1]))
     nz = int(np.ceil(n_bodies / (nx * ny)))
     if lattice is not None:
         nx, ny, nz = lattice
     lo = np.zeros(3)
-    hi = np.array([nx * a, ny * a, max(box[2], nz * a)])
+    hi = np.array([nx * a3[0], ny * a3[1], max(box[2], nz * a3[2])])
     wall_shapes, walls = box_walls(lo, hi, 0.05 * smean)
     deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=interverlet_steps * dt,
                 interConf=1e3, DVerlet=0.1 * smean, dVerlet=0.05 * smean, density={0: 2700.0, 1: 2700.0},
                 gravity=(0.0, 0.0, -9.81), update_nl="linkCells", nDriven=len(walls), force_law=law,
                 params=contact_params(kn, 0.8 * kn, en * en, mu, groups=((0, 0), (0, 1))))
-    deck.cell_min_sizes = (a, a, a)
+    amax = float(a3.max())
+    deck.cell_min_sizes = (amax, amax, amax)
     deck.box_for_linkcells_opt = 1   # cell grid over the free bodies; the walls are oversized
     deck.bodies = list(walls)
-    q = random_quaternions(rng, n_bodies)
+    q = small_rotations(rng, n_bodies, np.radians(tilt_deg)) if aligned else random_quaternions(rng, n_bodies)
     k = 0
     for iz in range(nz):
         for iy in range(ny):
             for ix in range(nx):
                 if k >= n_bodies:
                     break
-                p = lo + a * (np.array([ix, iy, iz]) + 0.5)
+                p = lo + a3 * (np.array([ix, iy, iz]) + 0.5)
                 size = rng.uniform(*size_range)
                 v = rng.uniform(-vmax, vmax, size=3)
                 deck.bodies.append(Body(pool[int(rng.integers(len(pool)))].name, group=0, homothety=size,
                                         pos=tuple(p), vel=tuple(v), Q=tuple(q[k])))
                 k += 1
+    return wall_shapes + pool, deck
+
+
+def quat_matrix(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def compact_pour_deck(n_bodies=50000, seed=2, n_shapes=64, size_range=(4e-3, 6e-3), box_in_sizes=(40, 40, 200),
+                      kn=1e7, en=0.3, mu=0.6, dt=2e-7, steps=20000, vmax=0.5, law="Avalanche", gap_frac=0.02,
+                      tilt_deg=15.0, interverlet_steps=100, width_in_sizes=None):
+    """Plain GPU form as a compact pour: the convex sphero-polyhedra of
+    hull_packing_deck (hulls of 8-14 points on an ellipsoid 1 : 0.8 : 0.6,
+    size uniform in size_range, Minkowski radius 5 %), principal axes along
+    the box axes up to a random tilt of at most tilt_deg, sorted by size
+    (largest at the bottom) and packed row by row, layer by layer from the
+    floor of a box of box_in_sizes mean sizes, each body's own rotated
+    bounding box apart from its neighbours by gap_frac of the mean size;
+    velocities uniform in [-vmax, vmax]; gravity; linkCells; the given law."""
+    rng = np.random.default_rng(seed)
+    pool = hull_shape_pool(n_shapes, seed)
+    smean = 0.5 * (size_range[0] + size_range[1])
+    box = np.array(box_in_sizes, float) * smean
+    if width_in_sizes is not None:
+        box[0] = box[1] = width_in_sizes * smean
+    gap = gap_frac * smean
+    sizes = np.sort(rng.uniform(*size_range, size=n_bodies))[::-1]
+    shape_idx = rng.integers(len(pool), size=n_bodies)
+    q = small_rotations(rng, n_bodies, np.radians(tilt_deg))
+    vel = rng.uniform(-vmax, vmax, size=(n_bodies, 3))
+    pos = np.zeros((n_bodies, 3))
+    x = y = z = 0.0
+    row_h = 0.0   # max y extent in the current row
+    lay_h = 0.0   # max z extent in the current layer
+    row_start = 0
+    layer_start = 0
+    placed_rows = []
+    for k in range(n_bodies):
+        s = pool[shape_idx[k]]
+        R = quat_matrix(q[k])
+        e = np.asarray(s.extent()) * sizes[k]
+        c = R @ (np.asarray(s.obb_center) * sizes[k])
+        half = np.abs(R) @ e
+        if x + 2 * half[0] > box[0]:
+            x = 0.0
+            y += row_h + gap
+            row_h = 0.0
+        if y + 2 * half[1] > box[1]:
+            y = 0.0
+            z += lay_h + gap
+            lay_h = 0.0
+            row_h = 0.0
+        pos[k] = np.array([x + half[0], y + half[1], z + half[2]]) - c
+        x += 2 * half[0] + gap
+        row_h = max(row_h, 2 * half[1])
+        lay_h = max(lay_h, 2 * half[2])
+    top = z + lay_h
+    lo = np.zeros(3)
+    hi = np.array([box[0], box[1], max(box[2], top)])
+    wall_shapes, walls = box_walls(lo, hi, 0.05 * smean)
+    deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=interverlet_steps * dt,
+                interConf=1e3, DVerlet=0.1 * smean, dVerlet=0.05 * smean, density={0: 2700.0, 1: 2700.0},
+                gravity=(0.0, 0.0, -9.81), update_nl="linkCells", nDriven=len(walls), force_law=law,
+                params=contact_params(kn, 0.8 * kn, en * en, mu, groups=((0, 0), (0, 1))))
+    cell = 1.1 * (2 * max(float(np.max(s.extent())) for s in pool) * size_range[1]) + 0.1 * smean
+    deck.cell_min_sizes = (cell, cell, cell)
+    deck.box_for_linkcells_opt = 1
+    deck.bodies = list(walls)
+    for k in range(n_bodies):
+        deck.bodies.append(Body(pool[shape_idx[k]].name, group=0, homothety=float(sizes[k]), pos=tuple(pos[k]),
+                                vel=tuple(vel[k]), Q=tuple(q[k])))
     return wall_shapes + pool, deck
 
 
