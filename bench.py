@@ -2,26 +2,36 @@
 
 Workload (the configuration the metric is quoted on): 50,000 convex
 sphero-polyhedra -- each the convex hull of 8-14 random points on an
-ellipsoid 1 : 0.8 : 0.6, size uniform in [4, 6] mm, Minkowski radius 5 % of
-the size -- poured into a box 40 x 40 x 200 mean sizes under gravity,
-linkCells neighbour list, ForceLaw Avalanche (kn = 1e7, kt = 0.8 kn, en = 0.3,
-mu = 0.6), dt = 2e-7 s, FP64.  The pour is synthetic: bodies packed row by
-row from the floor (compact, principal axes along the box axes up to 3 deg),
-random velocities in [-0.5, 0.5] m/s, seed 2.
+ellipsoid 1 : 0.8 : 0.6 (slivers under a quarter of the ellipsoid's volume
+drawn again), size uniform in [4, 6] mm, Minkowski radius 5 % of the size --
+poured into a box 40 x 40 x 200 mean sizes under gravity, linkCells
+neighbour list, ForceLaw Avalanche (kn = 1e7, kt = 0.8 kn, en = 0.3, mu =
+0.6), dt = 2e-7 s, FP64.
+
+The pour is synthetic and settled by the engine itself before anything is
+timed: bodies packed row by row from the floor (principal axes along the box
+axes up to 3 deg, velocities in [-0.5, 0.5] m/s, seed 2), settled for
+`--settle` steps under 400 g with the same contact law, dt and stiffness
+(interVerlet 20 steps), then gravity is lowered to 100, 25, 6.25, 1.5625 and
+1 g, `--relax` steps each, so that the elastic spring-back of each stage
+damps out instead of throwing the packing up; bodies that left the box are
+dropped; the timed run uses interVerlet = 100 steps.  (20,000 steps at
+dt = 2e-7 s are only 4 ms: a pour that starts loose stays a dilute gas,
+hence the settling.)
 
 A "step" is one iteration of Rockable's integrate() loop: velocity-Verlet
 predictor, contact kinematics + force law, ordered per-body resultants,
 accelerations, kick, and the neighbour-list rebuild every interVerlet = 100
 steps.  value = bodies x steps / wall seconds of the timed steps (state
 resident in HBM).  The roofline object is the force phase (kinematics + law +
-resultants), the hot kernels of the step.  cpu_baseline times the CPU oracle
-(the restatement of Rockable's OpenMP DefaultParallelMode path) on a bounded
-sample of the same workload.
+resultants), the hot kernels of the step; kernel durations come from HIP
+events recorded on the engine stream inside the timed region.  cpu_baseline
+times the CPU oracle (the restatement of Rockable's OpenMP
+DefaultParallelMode path) on a bounded sample of the same settled packing.
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -32,11 +42,51 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def make_deck(n_bodies, seed, dirpath):
+def make_settled_deck(n_bodies, seed, dirpath, settle_steps, relax_steps, log):
+    """Loose compact pour -> settled under 400 g by the engine -> gravity
+    lowered to 1 g in steps of 4x (each stage long enough for the spring-back
+    of the contacts to damp out) -> bodies that left the box dropped -> conf
+    with the workload's own parameters (see the module docstring)."""
+    import numpy as np
     from rockable_amd import decks as D
+    from rockable_amd.engine import Engine
+    dt = 2e-7
+    g = 9.81
     shapes, deck = D.compact_pour_deck(n_bodies=n_bodies, seed=seed, tilt_deg=3.0, gap_frac=0.0, vmax=0.5,
-                                       law="Avalanche", interverlet_steps=100)
-    return D.write_deck(dirpath, shapes, deck), deck
+                                       law="Avalanche", dt=dt, interverlet_steps=20)
+    deck.gravity = (0.0, 0.0, -400.0 * g)
+    loose = D.write_deck(dirpath, shapes, deck, name="loose.conf")
+    e = Engine(loose, device=0)
+    e.set_option("record_geometry", 0)
+    e.update_nl()
+    t0 = time.perf_counter()
+    stages = [(400.0, settle_steps)] + [(m, relax_steps) for m in (100.0, 25.0, 6.25, 1.5625, 1.0)]
+    for mult, steps in stages:
+        e.set_option("gravity_z", -mult * g)
+        done = 0
+        while done < steps:
+            n = min(5000, steps - done)
+            e.step(n)
+            e.synchronize()
+            done += n
+        c = e.counts()
+        v = np.linalg.norm(e.bodies()["vel"][deck.nDriven:], axis=1)
+        log("settle at %g g: %d steps, %.1f s, listed/body %.2f, active/body %.2f, speed mean %.3g max %.3g m/s" % (
+            mult, steps, time.perf_counter() - t0, c["nlisted"] / c["nbodies"], c["nactive"] / c["nbodies"],
+            v.mean(), v.max()))
+    raw = os.path.join(dirpath, "settled_raw.conf")
+    e.save_conf(raw)
+    b = e.bodies()
+    e.close()
+    # keep the driven walls and every body whose centre is inside the box
+    lo = np.zeros(3)
+    hi = np.array([40.0, 40.0, 200.0]) * 5e-3
+    inside = np.all((b["pos"] >= lo) & (b["pos"] <= hi), axis=1)
+    inside[:deck.nDriven] = True
+    kept = D.conf_drop_bodies(raw, os.path.join(dirpath, "settled_kept.conf"), inside)
+    path = D.conf_with(kept, os.path.join(dirpath, "settled.conf"), gravity=(0.0, 0.0, -g), t=0.0, tmax=1.0,
+                       interConf=1.0, interVerlet=100 * dt)
+    return path, int((~inside).sum())
 
 
 def force_phase_bytes(nb, nl, law):
@@ -62,33 +112,35 @@ def cpu_baseline(deck_path, target_seconds, threads):
     dt = time.perf_counter() - t0
     nb = o.nbodies
     return {"value": nb * steps / dt, "unit": "body-steps/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle (OpenMP restatement of Rockable's DefaultParallelMode step), same deck, "
-                      "%d steps after 20 warm-up steps, %.1f s" % (steps, dt)}
+            "sample": "CPU oracle (OpenMP restatement of Rockable's DefaultParallelMode step, %d threads), the same "
+                      "settled packing, %d steps after 20 warm-up steps, %.1f s" % (threads, steps, dt)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--bodies", type=int, default=50000)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--settle", type=int, default=45000)
+    ap.add_argument("--relax", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--deck-dir", default=None)
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the path would shard by "
-              "spatial slabs or by packing, see DESIGN.md)" % a.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (how the step would shard over "
+              "GPUs is described in DESIGN.md)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
 
     from rockable_amd.engine import Engine
     d = a.deck_dir or tempfile.mkdtemp(prefix="rk_bench_")
-    t0 = time.perf_counter()
-    path, deck = make_deck(a.bodies, a.seed, d)
-    print("deck: %d bodies written in %.1f s" % (len(deck.bodies), time.perf_counter() - t0), file=sys.stderr,
-          flush=True)
+    path, dropped = make_settled_deck(a.bodies, a.seed, d, a.settle, a.relax, log)
 
     e = Engine(path, device=0)
     e.set_option("record_geometry", 0)
@@ -96,9 +148,10 @@ def main():
     t0 = time.perf_counter()
     e.step(a.warmup)
     e.synchronize()
-    print("warm-up: %d steps in %.2f s" % (a.warmup, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    log("warm-up: %d steps in %.2f s" % (a.warmup, time.perf_counter() - t0))
 
-    # timed region: K steps, device state resident, profile events on the stream
+    # timed region: K steps, device state resident; HIP events on the engine
+    # stream bracket the force phase and the neighbour-list rebuilds
     e.set_option("profile", 1)
     e.reset_kernel_time()
     c0 = e.counts()
@@ -109,40 +162,54 @@ def main():
     wall = time.perf_counter() - t0
     c1 = e.counts()
     nb = c1["nbodies"]
-    kt = {k: e.kernel_time(k) for k in ("predict", "interact", "gather", "breakage", "nl")}
+    kt = {k: e.kernel_time(k) for k in ("force", "nl")}
     value = nb * a.steps / wall
+    # per-kernel breakdown over the next K steps (one event pair per kernel)
+    e.set_option("profile", 2)
+    e.reset_kernel_time()
+    e.step(a.steps)
+    e.synchronize()
+    kt2 = {k: e.kernel_time(k) for k in ("predict", "interact", "gather", "breakage", "nl")}
+    e.set_option("profile", 0)
+    # and without events, to show what the brackets cost
+    e.synchronize()
+    t1 = time.perf_counter()
+    e.step(a.steps)
+    e.synchronize()
+    wall_noev = time.perf_counter() - t1
+    log("timed: %.4f ms/step with force-phase events, %.4f ms/step without" % (1e3 * wall / a.steps,
+                                                                             1e3 * wall_noev / a.steps))
 
     # roofline of the force phase (interact + gather), per launch
-    n_int, ms_int = kt["interact"]
-    n_gat, ms_gat = kt["gather"]
+    n_force, ms_force = kt["force"]
     nl_listed = 0.5 * (c0["nlisted"] + c1["nlisted"])
     bytes_per = force_phase_bytes(nb, nl_listed, "Avalanche")
-    t_force = (ms_int / max(n_int, 1) + ms_gat / max(n_gat, 1)) * 1e-3
+    t_force = ms_force / max(n_force, 1) * 1e-3
     achieved = bytes_per / t_force / 1e9
 
     out = {
         "metric": "body-steps/s", "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "plain GPU form: %d convex sphero-polyhedra (hulls of 8-14 points), compact pour, "
-                               "linkCells, ForceLaw Avalanche, dt 2e-7 s" % nb,
-                   "bodies": nb, "listed_per_body": c1["nlisted"] / nb, "active_per_body": c1["nactive"] / nb,
+        "config": {"workload": "plain GPU form: %d convex sphero-polyhedra (hulls of 8-14 points on a 1:0.8:0.6 "
+                               "ellipsoid, 4-6 mm), settled pour in a 40x40x200-size box, linkCells, ForceLaw "
+                               "Avalanche, dt 2e-7 s, interVerlet 100 steps" % nb,
+                   "bodies": nb, "bodies_dropped_after_settling": dropped,
+                   "listed_per_body": c1["nlisted"] / nb, "active_per_body": c1["nactive"] / nb,
                    "nl_rebuilds_in_timed_region": c1["nl_updates"] - c0["nl_updates"],
-                   "kernel_ms_per_step": {k: (v[1] / max(v[0], 1)) for k, v in kt.items() if v[0]},
+                   "force_phase_ms": 1e3 * t_force,
+                   "nl_rebuild_ms": kt["nl"][1] / max(kt["nl"][0], 1),
+                   "kernel_ms_per_launch": {k: (v[1] / max(v[0], 1)) for k, v in kt2.items() if v[0]},
+                   "ms_per_step_without_events": 1e3 * wall_noev / a.steps,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "force phase (k_interact + k_gather)", "algorithmic_bytes_per_launch": bytes_per},
+                     "kernel": "force phase (k_interact + k_gather + k_gather_heavy)",
+                     "algorithmic_bytes_per_launch": bytes_per},
     }
     if not a.no_cpu:
         e.close()
         out["cpu_baseline"] = cpu_baseline(path, a.cpu_seconds, a.cpu_threads)
-        try:
-            out["cpu_baseline"]["cpu_model"] = subprocess.run(
-                "lscpu | grep 'Model name' | head -1", shell=True, capture_output=True, text=True).stdout.split(
-                ":")[-1].strip()
-        except Exception:
-            pass
     print(json.dumps(out), flush=True)
 
 

@@ -69,7 +69,15 @@ int rk_set_force_law(rk_engine* eng, const char* name);
 int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int nargs);
 /* Options: "record_geometry" (1: keep n/pos/vel of every listed interaction
  * current each step; 0: only at conf-save steps and for bonds),
- * "profile" (1: time each kernel with device events). */
+ * "profile" (device events on the engine stream: 1 brackets the force phase
+ * "force" and the neighbour-list rebuild "nl"; 2 brackets every kernel),
+ * "heavy_min" (slot count above which a body's resultant is summed by a
+ * whole workgroup; default 256),
+ * "exact_sums" (1: the resultants of fixed and velocity-driven walls are
+ * summed in the reference's serial order too; 0, the default: by a fixed
+ * tree, which changes only their reported force and moment by rounding),
+ * "gravity_x", "gravity_y", "gravity_z" (the conf keyword gravity, changed
+ * between steps). */
 int rk_set_option(rk_engine* eng, const char* name, double value);
 
 int rk_update_nl(rk_engine* eng);
@@ -88,8 +96,9 @@ int rk_set_bodies(rk_engine* eng, const double* pos, const double* Q, const doub
 /* active_only: 1 for the active interactions (dn < 0 or bonded), 0 for every
  * listed one.  *n receives the total; at most cap records are written. */
 int rk_get_interactions(rk_engine* eng, int active_only, rk_interaction* out, size_t cap, size_t* n);
-/* Accumulated device time of a kernel ("predict", "interact", "gather",
- * "breakage", "nl") while "profile" is on. */
+/* Accumulated device time of a bracket while "profile" is on: "force" and
+ * "nl" at level 1; "predict", "interact", "gather", "breakage", "nl" at
+ * level 2. */
 int rk_get_kernel_time(rk_engine* eng, const char* kernel, int64_t* launches, double* total_ms);
 int rk_reset_kernel_time(rk_engine* eng);
 

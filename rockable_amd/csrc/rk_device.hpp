@@ -13,9 +13,15 @@
 //            EState  {dn, fn, ft.xyz, mom.xyz}       64 B history record
 //            EGeom   {n.xyz, pos.xyz, vel.xyz}        72 B geometry record
 //            damp, stick (int, interface index or -1), act (uint8)
-//            contrib {f.xyz, mi.xyz, mj.xyz}          72 B per active entry
+//            spos[2n]     the entry's two resultant slots (i side, j side)
 //   CSR      istart[N+1]  entries owned by body b are [istart[b], istart[b+1])
 //            jstart[N+1], jlist[]  entries whose j == b, ascending index
+//   slots    body b's incident entries in the serial order of the reference's
+//            resultant loop (entries with j == b below b's own range, b's own
+//            range, entries with j == b above it) are the slots
+//            [soff[b], soff[b+1]); per slot an activity byte sact and the
+//            six signed terms sval {F.xyz, M.xyz} (48 B) that the entry adds
+//            to that body: i side (f, mi), j side (-f, mj)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -113,10 +119,20 @@ struct Dev {
   double* edamp;
   int32_t* estick;
   uint8_t* eact;
-  double* econ;
   const int32_t* istart;
   const int32_t* jstart;
   const int32_t* jlist;
+  // resultant slots
+  const int32_t* spos;
+  const int32_t* soff;
+  uint8_t* sact;
+  double* sval;
+  const int32_t* heavy;   // bodies with more than heavy_min slots
+  int nheavy, heavy_min;
+  // per body: 1 if its force and moment feed nothing back into the motion
+  // (a driven body without force or moment controls)
+  const uint8_t* passive;
+  int exact_sums;         // 1: serial sums for passive bodies too
   // driving
   int nctrl;
   DevControl* ctrl;

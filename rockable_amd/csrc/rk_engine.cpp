@@ -62,6 +62,8 @@ struct rk_engine {
   DevControl* ctrl = nullptr;
   int32_t *ctrl_off = nullptr, *ctrl_idx = nullptr;
   int32_t* flags = nullptr;
+  uint8_t* passive = nullptr;
+  int exact_sums = 0;
   EntryArrays E[2];
   int cur = 0;
   NLWork w;
@@ -104,7 +106,7 @@ void free_device(rk_engine* eng) {
                 eng->sh_voff,  eng->sh_eoff,   eng->sh_foff,   eng->sh_edge,   eng->sh_fstart, eng->sh_fidx,
                 eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->ptab,      eng->damp_rate, eng->damp_kind,
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
-                eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags};
+                eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   eng->kin = nullptr;
@@ -122,14 +124,32 @@ void free_device(rk_engine* eng) {
   eng->ctrl = nullptr;
   eng->ctrl_off = eng->ctrl_idx = nullptr;
   eng->flags = nullptr;
+  eng->passive = nullptr;
   for (auto& E : eng->E) {
     entries_free(E);
     if (E.istart) (void)hipFree(E.istart);
     if (E.jstart) (void)hipFree(E.jstart);
     if (E.jlist) (void)hipFree(E.jlist);
+    if (E.soff) (void)hipFree(E.soff);
+    if (E.heavy) (void)hipFree(E.heavy);
     E = EntryArrays();
   }
   nl_work_free(eng->w);
+}
+
+// 1 for a driven body whose force and moment feed nothing back into the
+// motion: no force or moment control acts on it (a fixed or velocity-driven
+// wall).  Its resultant is only reported.
+std::vector<uint8_t> passive_flags(const Model& m) {
+  const int nb = (int)m.bodies.size();
+  std::vector<uint8_t> p(nb, 0);
+  for (int b = 0; b < std::min(m.nDriven, nb); b++) p[b] = 1;
+  for (const Control& c : m.ctrl) {
+    const bool fm = c.type == C_X_FOR || c.type == C_Y_FOR || c.type == C_Z_FOR || c.type == C_XROT_MOM ||
+                    c.type == C_YROT_MOM || c.type == C_ZROT_MOM || c.type == C_XYZROT_MOM;
+    if (fm && c.i >= 0 && c.i < nb) p[c.i] = 0;
+  }
+  return p;
 }
 
 void assemble_dev(rk_engine* eng) {
@@ -171,7 +191,15 @@ void assemble_dev(rk_engine* eng) {
   d.edamp = E.edamp;
   d.estick = E.estick;
   d.eact = E.eact;
-  d.econ = E.econ;
+  d.spos = E.spos;
+  d.soff = E.soff;
+  d.sact = E.sact;
+  d.sval = E.sval;
+  d.heavy = E.heavy;
+  d.nheavy = E.nheavy;
+  d.heavy_min = eng->w.heavy_min;
+  d.passive = eng->passive;
+  d.exact_sums = eng->exact_sums;
   d.istart = E.istart;
   d.jstart = E.jstart;
   d.jlist = E.jlist;
@@ -373,6 +401,7 @@ int upload_model(rk_engine* eng) {
       off[b + 1] = (int)idx.size();
     }
     if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
+    if (upload(eng, eng->passive, passive_flags(m))) return -1;
   }
   std::vector<int32_t> fl(4, 0);
   if (upload(eng, eng->flags, fl)) return -1;
@@ -461,8 +490,11 @@ void resolve_events(rk_engine* eng) {
   eng->ev_used = 0;
 }
 
-void tic(rk_engine* eng) {
-  if (!eng->profile) return;
+// Timing brackets: level 1 brackets the force phase (k_interact + gather
+// kernels, name "force") and the neighbour-list rebuild ("nl"); level 2
+// brackets every kernel on its own.
+void tic(rk_engine* eng, int level) {
+  if (eng->profile == 0 || eng->profile != level) return;
   if (eng->ev_used + 2 > 4096) resolve_events(eng);
   hipEvent_t a = pool_event(eng);
   hipEvent_t b = pool_event(eng);
@@ -470,8 +502,8 @@ void tic(rk_engine* eng) {
   eng->tic_index = (int)eng->ev_used - 2;
   (void)hipEventRecord(a, eng->stream);
 }
-void toc(rk_engine* eng, const char* name) {
-  if (!eng->profile || eng->tic_index < 0) return;
+void toc(rk_engine* eng, const char* name, int level) {
+  if (eng->profile != level || eng->tic_index < 0) return;
   (void)hipEventRecord(eng->ev_pool[eng->tic_index + 1], eng->stream);
   eng->ev_pending.push_back({name, eng->tic_index});
   eng->tic_index = -1;
@@ -481,16 +513,18 @@ void toc(rk_engine* eng, const char* name) {
 // half kick of velocityVerletStep.
 int run_accelerations(rk_engine* eng, int kick, int write_geom) {
   const StepParams sp = step_params(eng);
-  tic(eng);
+  tic(eng, 1);
+  tic(eng, 2);
   launch_interact(eng->d, sp, write_geom || eng->record_geometry, eng->stream);
-  toc(eng, "interact");
-  tic(eng);
+  toc(eng, "interact", 2);
+  tic(eng, 2);
   launch_gather(eng->d, sp, kick, eng->stream);
-  toc(eng, "gather");
+  toc(eng, "gather", 2);
+  toc(eng, "force", 1);
   if (eng->m.law == LAW_STICKED && eng->live_interfaces > 0) {
-    tic(eng);
+    tic(eng, 2);
     launch_breakage(eng->d, sp, eng->stream);
-    toc(eng, "breakage");
+    toc(eng, "breakage", 2);
     int32_t fl = 0;
     HIPCHK(hipMemcpyAsync(&fl, eng->flags, sizeof(int32_t), hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
@@ -516,22 +550,22 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom) {
 }
 
 int do_update_nl(rk_engine* eng) {
-  tic(eng);
+  tic(eng, eng->profile);
   const int nxt = 1 - eng->cur;
   if (nl_rebuild(eng->w, eng->d, eng->nlp, eng->E[eng->cur], eng->E[nxt], eng->stream) != 0)
     return fail(eng, std::string("neighbour list: ") + eng->w.err);
   eng->cur = nxt;
   assemble_dev(eng);
   eng->nl_updates++;
-  toc(eng, "nl");
+  toc(eng, "nl", eng->profile);
   return 0;
 }
 
 int vv_step(rk_engine* eng, int write_geom) {
   const StepParams sp = step_params(eng);
-  tic(eng);
+  tic(eng, 2);
   launch_predict(eng->d, sp, eng->stream);
-  toc(eng, "predict");
+  toc(eng, "predict", 2);
   return run_accelerations(eng, 1, write_geom);
 }
 
@@ -774,6 +808,7 @@ int rk_load_driving_system(rk_engine* eng, const char* path) {
     off[b + 1] = (int)idx.size();
   }
   if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
+  if (upload(eng, eng->passive, passive_flags(eng->m))) return -1;
   assemble_dev(eng);
   HIPCHK(hipStreamSynchronize(eng->stream));
   return 0;
@@ -833,9 +868,28 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
   std::string n = name;
   if (n == "record_geometry")
     eng->record_geometry = value != 0.0;
-  else if (n == "profile")
-    eng->profile = value != 0.0;
-  else
+  else if (n == "profile") {
+    if (!(value == 0.0 || value == 1.0 || value == 2.0)) return fail(eng, "profile must be 0, 1 or 2");
+    resolve_events(eng);
+    eng->profile = (int)value;
+  }
+  else if (n == "exact_sums") {
+    eng->exact_sums = value != 0.0;
+    if (eng->loaded) assemble_dev(eng);
+  } else if (n == "gravity_x" || n == "gravity_y" || n == "gravity_z") {
+    // the conf keyword "gravity" (Rockable.cpp:978-980), changed between steps
+    if (check_loaded(eng)) return -1;
+    (n == "gravity_x" ? eng->m.gravity.x : n == "gravity_y" ? eng->m.gravity.y : eng->m.gravity.z) = value;
+  } else if (n == "heavy_min") {
+    if (!(value >= 0.0 && value < 2.0e9)) return fail(eng, "heavy_min must be in [0, 2e9)");
+    eng->w.heavy_min = (int)value;
+    if (eng->loaded) {
+      (void)hipSetDevice(eng->device);
+      if (nl_build_heavy(eng->w, (int)eng->m.bodies.size(), eng->E[eng->cur], eng->stream) != 0)
+        return fail(eng, eng->w.err);
+      assemble_dev(eng);
+    }
+  } else
     return fail(eng, "unknown option " + n);
   return 0;
 }

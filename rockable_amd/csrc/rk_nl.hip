@@ -731,6 +731,56 @@ __global__ void k_iota(int n, int32_t* v) {
   if (e < n) v[e] = e;
 }
 
+// --- resultant slots -------------------------------------------------------
+// compute_resultants (Rockable.cpp:3267-3272) walks the bodies in index order
+// and each body's interactions in key order, adding to Particles[i] and
+// subtracting from Particles[j]; so body b receives, in this order, the
+// entries (a, b) with a < b, then its own entries (b, c), then the entries
+// (a, b) with a > b.  Body b's slots are [soff[b], soff[b+1]) with
+// soff[b] = istart[b] + jstart[b]; nlow[b] counts b's j-side entries whose
+// index is below b's own range.
+__global__ void k_slot_body(int nb, const int32_t* istart, const int32_t* jstart, const int32_t* jlist,
+                            int32_t* soff, int32_t* nlow) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nb) return;
+  soff[b] = istart[b] + jstart[b];
+  if (b == nb) return;
+  int lo = jstart[b], hi = jstart[b + 1];
+  const int is = istart[b];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (jlist[mid] < is)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  nlow[b] = lo - jstart[b];
+}
+__global__ void k_slot_i(int n, const int32_t* ei, const int32_t* istart, const int32_t* soff, const int32_t* nlow,
+                         int32_t* spos) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int b = ei[e];
+  spos[2 * e] = soff[b] + nlow[b] + (e - istart[b]);
+}
+__global__ void k_slot_j(int n, const int32_t* ej, const int32_t* jlist, const int32_t* jstart,
+                         const int32_t* istart, const int32_t* soff, int32_t* spos) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int e = jlist[q];
+  const int b = ej[e];
+  const int above = e >= istart[b] ? istart[b + 1] - istart[b] : 0;
+  spos[2 * e + 1] = soff[b] + (q - jstart[b]) + above;
+}
+__global__ void k_heavy_flag(int nb, const int32_t* soff, int heavy_min, int32_t* flag) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) flag[b] = (soff[b + 1] - soff[b]) > heavy_min ? 1 : 0;
+}
+__global__ void k_heavy_list(int nb, const int32_t* flag, const int32_t* pos, int32_t* heavy) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb && flag[b]) heavy[pos[b]] = b;
+}
+
 inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -803,6 +853,33 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
     (void)hipFree(kout);
     if (rc) return -1;
   }
+  // resultant slots
+  if (ensure(w, E.soff, E.soff_cap, (size_t)nb + 1)) return -1;
+  hipLaunchKernelGGL(k_slot_body, dim3(nblk(nb + 1)), dim3(kBlock), 0, s, nb, E.istart, E.jstart, E.jlist, E.soff,
+                     w.i32a);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_slot_i, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ei, E.istart, E.soff, w.i32a, E.spos);
+    hipLaunchKernelGGL(k_slot_j, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ej, E.jlist, E.jstart, E.istart, E.soff,
+                       E.spos);
+  }
+  RK_CHECK(hipGetLastError());
+  return nl_build_heavy(w, nb, E, s);
+}
+
+int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
+  E.nheavy = 0;
+  if (nb <= 0 || !E.soff) return 0;
+  if (ensure(w, E.heavy, E.heavy_cap, (size_t)nb) || ensure(w, w.i32a, w.i32a_cap, (size_t)nb + 1) ||
+      ensure(w, w.i32c, w.i32c_cap, (size_t)nb + 1))
+    return -1;
+  hipLaunchKernelGGL(k_heavy_flag, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, E.soff, w.heavy_min, w.i32a);
+  if (scan_exclusive(w, w.i32a, w.i32c, nb, s)) return -1;
+  hipLaunchKernelGGL(k_heavy_list, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, w.i32a, w.i32c, E.heavy);
+  int32_t tail[2] = {0, 0};
+  RK_CHECK(hipMemcpyAsync(&tail[0], w.i32c + nb - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  RK_CHECK(hipMemcpyAsync(&tail[1], w.i32a + nb - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  RK_CHECK(hipStreamSynchronize(s));
+  E.nheavy = tail[0] + tail[1];
   return 0;
 }
 
@@ -1030,7 +1107,9 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
             hipMalloc(&E.esub, c * sizeof(uint32_t)) == hipSuccess && hipMalloc(&E.est, c * sizeof(EState)) == hipSuccess &&
             hipMalloc(&E.egeo, c * sizeof(EGeom)) == hipSuccess && hipMalloc(&E.edamp, c * sizeof(double)) == hipSuccess &&
             hipMalloc(&E.estick, c * sizeof(int32_t)) == hipSuccess && hipMalloc(&E.eact, c * sizeof(uint8_t)) == hipSuccess &&
-            hipMalloc(&E.econ, c * 9 * sizeof(double)) == hipSuccess;
+            hipMalloc(&E.spos, c * 2 * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&E.sact, c * 2 * sizeof(uint8_t)) == hipSuccess &&
+            hipMalloc(&E.sval, c * 12 * sizeof(double)) == hipSuccess;
   if (!ok) {
     snprintf(w.err, sizeof(w.err), "hipMalloc of %zu interaction entries failed", c);
     entries_free(E);
@@ -1041,7 +1120,7 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
 }
 
 void entries_free(EntryArrays& E) {
-  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.egeo, E.edamp, E.estick, E.eact, E.econ};
+  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.egeo, E.edamp, E.estick, E.eact, E.spos, E.sact, E.sval};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   E.ei = E.ej = nullptr;
@@ -1051,7 +1130,9 @@ void entries_free(EntryArrays& E) {
   E.edamp = nullptr;
   E.estick = nullptr;
   E.eact = nullptr;
-  E.econ = nullptr;
+  E.spos = nullptr;
+  E.sact = nullptr;
+  E.sval = nullptr;
   E.cap = 0;
 }
 

@@ -22,13 +22,20 @@ struct EntryArrays {
   double* edamp = nullptr;
   int32_t* estick = nullptr;
   uint8_t* eact = nullptr;
-  double* econ = nullptr;
+  int32_t* spos = nullptr;   // 2 per entry
+  uint8_t* sact = nullptr;   // 2 per entry
+  double* sval = nullptr;    // 12 per entry
   int32_t* istart = nullptr;
   size_t istart_cap = 0;
   int32_t* jstart = nullptr;
   size_t jstart_cap = 0;
   int32_t* jlist = nullptr;
   size_t jlist_cap = 0;
+  int32_t* soff = nullptr;
+  size_t soff_cap = 0;
+  int32_t* heavy = nullptr;
+  size_t heavy_cap = 0;
+  int nheavy = 0;
 };
 
 struct NLParams {
@@ -43,6 +50,7 @@ struct NLParams {
 struct NLWork {
   char err[256] = {0};
   int cand_hint = 0, near_hint = 0, last_candidates = 0;
+  int heavy_min = 256;  // slot count above which a body's sum runs block-wide
 #define RK_BUF(T, name) \
   T* name = nullptr;    \
   size_t name##_cap = 0;
@@ -83,8 +91,11 @@ struct NLWork {
 
 // Rebuild the list: reads `cur` (sorted), writes `nxt` (sorted, with CSR).
 int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArrays& nxt, hipStream_t s);
-// istart / jstart / jlist of a sorted list.
+// istart / jstart / jlist, resultant slots and the heavy-body list of a
+// sorted list.
 int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s);
+// The heavy-body list alone (after a change of w.heavy_min).
+int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, hipStream_t s);
 int entries_reserve(NLWork& w, EntryArrays& E, int n);
 void entries_free(EntryArrays& E);
 void nl_work_free(NLWork& w);

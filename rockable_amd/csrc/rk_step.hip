@@ -7,13 +7,15 @@
 //   k_interact  update_interactions (Rockable.cpp:2968-2974 ->
 //               Interaction.cpp:147-326) fused with build_activeInteractions
 //               (3005-3029) and ForceLaw::computeInteraction (3035-3046,
-//               ForceLaw_*.cpp); writes each active entry's force and its two
-//               moment contributions (incrementResultants, 2752-2789)
+//               ForceLaw_*.cpp); writes each active entry's terms for body i
+//               and body j (incrementResultants, 2752-2789) into the two
+//               resultant slots of the entry
 //   k_gather    initialise_particle_forces_and_moments (2869-2929) +
 //               compute_resultants (3267-3272) as a per-body ordered sum over
-//               the body's incident entries in global key order, which is the
+//               the body's slots, which hold its incident entries in the
 //               serial order of the reference, + compute_accelerations_from_
 //               resultants (3419-3552) + the closing kick (2120-2155)
+//   k_gather_heavy  the same for bodies with many slots, one workgroup each
 //   k_breakage  breakage_of_interfaces (3350-3413)
 #include <hip/hip_runtime.h>
 
@@ -333,6 +335,7 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
   const BodyKin Pj = d.kin[j];
   const int stick = d.estick[e];
   const double dn_old = d.est[e].dn;
+  const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
 
   Kin k;
   k.deact = false;
@@ -384,10 +387,14 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
     v3 Cj = vsub(k.pos, vadd(kpos(Pj), V(0.0, 0.0, 0.0)));
     v3 mi = vadd(vcross(Ci, f), h.mom);
     v3 mj = vsub(vcross(Cj, vneg(f)), h.mom);
-    double* c = d.econ + 9 * (size_t)e;
-    st3(c, f);
-    st3(c + 3, mi);
-    st3(c + 6, mj);
+    // the two resultant slots: body i adds (f, mi), body j adds (-f, mj);
+    // x + (-y) is x - y exactly, so the serial sums are unchanged
+    double* ci = d.sval + 6 * (size_t)slots.x;
+    double* cj = d.sval + 6 * (size_t)slots.y;
+    st3(ci, f);
+    st3(ci + 3, mi);
+    st3(cj, vneg(f));
+    st3(cj + 3, mj);
   } else if (k.deact) {
     EState o;
     o.dn = 0.0;
@@ -414,6 +421,8 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
     d.egeo[e] = g;
   }
   d.eact[e] = active ? 1 : 0;
+  d.sact[slots.x] = active ? 1 : 0;
+  d.sact[slots.y] = active ? 1 : 0;
 }
 
 __device__ __forceinline__ v3 euler_domega(v3 I, v3 w, v3 M) {
@@ -450,20 +459,19 @@ __device__ void body_force(const Dev& d, int b, const BodyKin& P, v3& F, v3& M) 
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= d.nb) return;
-  BodyKin P = d.kin[b];
-  const bool driven = b < d.nDriven;
-  // initialise_particle_forces_and_moments (Rockable.cpp:2869-2929)
-  v3 F = V(0.0, 0.0, 0.0), M = V(0.0, 0.0, 0.0);
+// initialise_particle_forces_and_moments (Rockable.cpp:2869-2929): body force
+// for free bodies, imposed force and moment values for driven ones.
+__device__ __forceinline__ void init_resultants(const Dev& d, int b, const BodyKin& P, bool driven, v3& F, v3& M,
+                                                int& c0, int& c1) {
+  F = V(0.0, 0.0, 0.0);
+  M = V(0.0, 0.0, 0.0);
+  c0 = c1 = 0;
   if (!driven && d.bf != BF_NONE) {
     v3 fm = V(0, 0, 0), mm = V(0, 0, 0);
     body_force(d, b, P, fm, mm);
     F = vadd(F, fm);
     M = vadd(M, mm);
   }
-  int c0 = 0, c1 = 0;
   if (driven && d.nctrl > 0) {
     c0 = d.ctrl_off[b];
     c1 = d.ctrl_off[b + 1];
@@ -481,38 +489,18 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
       }
     }
   }
-  // compute_resultants: incident active entries in ascending global index
-  const int is = d.istart[b], ie = d.istart[b + 1];
-  int q = d.jstart[b];
-  const int qe = d.jstart[b + 1];
-  for (; q < qe; q++) {
-    const int e = d.jlist[q];
-    if (e >= is) break;
-    if (d.eact[e]) {
-      const double* c = d.econ + 9 * (size_t)e;
-      F = vsub(F, ld3(c));
-      M = vadd(M, ld3(c + 6));
-    }
-  }
-  for (int e = is; e < ie; e++) {
-    if (d.eact[e]) {
-      const double* c = d.econ + 9 * (size_t)e;
-      F = vadd(F, ld3(c));
-      M = vadd(M, ld3(c + 3));
-    }
-  }
-  for (; q < qe; q++) {
-    const int e = d.jlist[q];
-    if (d.eact[e]) {
-      const double* c = d.econ + 9 * (size_t)e;
-      F = vsub(F, ld3(c));
-      M = vadd(M, ld3(c + 6));
-    }
-  }
+}
+
+__device__ __forceinline__ double pick(const v3& F, const v3& M, int c) {
+  return c == 0 ? F.x : c == 1 ? F.y : c == 2 ? F.z : c == 3 ? M.x : c == 4 ? M.y : M.z;
+}
+
+// Stores the resultants, compute_accelerations_from_resultants
+// (Rockable.cpp:3419-3552) and the closing half kick (2120-2155).
+__device__ void finish_body(const Dev& d, const StepParams& sp, int kick, int b, BodyKin P, bool driven, v3 F, v3 M,
+                            int c0, int c1) {
   st3(d.force + 3 * (size_t)b, F);
   st3(d.moment + 3 * (size_t)b, M);
-
-  // compute_accelerations_from_resultants (Rockable.cpp:3419-3552)
   const double m = d.mass[b];
   const v3 I = ld3(d.inertia + 3 * (size_t)b);
   const qt Q = kq(P);
@@ -565,7 +553,6 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
   st3(d.acc + 3 * (size_t)b, acc);
   st3(d.arot + 3 * (size_t)b, arot);
   if (!kick) return;
-  // closing half kick (Rockable.cpp:2120-2155)
   if (driven) {
     for (int qq = c0; qq < c1; qq++) {
       const DevControl C = d.ctrl[d.ctrl_idx[qq]];
@@ -602,6 +589,143 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
   out->wx = P.wx;
   out->wy = P.wy;
   out->wz = P.wz;
+}
+
+// compute_resultants (Rockable.cpp:3267-3272) for bodies with at most
+// heavy_min slots.  Six lanes per body, one per component of (F, M); each
+// lane adds the body's active slots in slot order, which is the reference's
+// serial order, so every component is the reference's sum bit for bit.
+// Ten bodies per wavefront (lanes 60-63 idle).
+constexpr int kLanesPerBody = 6;
+constexpr int kBodiesPerWave = 10;
+
+__global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const int slot = lane / kLanesPerBody;
+  const int comp = lane - kLanesPerBody * slot;
+  const int b = wave * kBodiesPerWave + slot;
+  const bool valid = slot < kBodiesPerWave && b < d.nb;
+  BodyKin P;
+  v3 F = V(0.0, 0.0, 0.0), M = V(0.0, 0.0, 0.0);
+  int c0 = 0, c1 = 0;
+  bool mine = false, driven = false;
+  double s = 0.0;
+  if (valid) {
+    P = d.kin[b];
+    driven = b < d.nDriven;
+    init_resultants(d, b, P, driven, F, M, c0, c1);
+    const int k0 = d.soff[b], k1 = d.soff[b + 1];
+    mine = (k1 - k0) <= d.heavy_min;
+    s = pick(F, M, comp);
+    if (mine) {
+      const uint8_t* act = d.sact;
+      const double* val = d.sval + comp;
+      int k = k0;
+      for (; k + 4 <= k1; k += 4) {
+        const uint8_t a0 = act[k], a1 = act[k + 1], a2 = act[k + 2], a3 = act[k + 3];
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3_ = 0.0;
+        if (a0) v0 = val[6 * (size_t)k];
+        if (a1) v1 = val[6 * (size_t)(k + 1)];
+        if (a2) v2 = val[6 * (size_t)(k + 2)];
+        if (a3) v3_ = val[6 * (size_t)(k + 3)];
+        if (a0) s += v0;
+        if (a1) s += v1;
+        if (a2) s += v2;
+        if (a3) s += v3_;
+      }
+      for (; k < k1; k++)
+        if (act[k]) s += val[6 * (size_t)k];
+    }
+  }
+  const int base = kLanesPerBody * (slot < kBodiesPerWave ? slot : 0);
+  const double fx = __shfl(s, base), fy = __shfl(s, base + 1), fz = __shfl(s, base + 2);
+  const double mx = __shfl(s, base + 3), my = __shfl(s, base + 4), mz = __shfl(s, base + 5);
+  if (valid && mine && comp == 0) finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+}
+
+// compute_resultants for a body with more than heavy_min slots (a wall under
+// a packing), one workgroup per body.
+//  * passive body (its force and moment feed nothing back): a fixed-shape
+//    tree -- thread t sums slots t, t + 256, ... in order, then a pairwise
+//    tree over the threads.  Deterministic; it differs from the serial sum
+//    only by rounding (relative 1e-16 x log2 of the slot count).
+//  * otherwise (or with exact_sums): the serial sum.  Waves 1-3 stage chunks
+//    of kHeavyChunk slots into LDS (inactive slots as -0.0, the identity of
+//    addition: x + (-0.0) == x for every x), double-buffered, while lanes
+//    0-5 of wave 0 add them in order, one lane per component.
+constexpr int kHeavyChunk = 512;
+
+__global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, int kick) {
+  __shared__ double lds[2 * kHeavyChunk * 6];
+  const int b = d.heavy[blockIdx.x];
+  const int t = threadIdx.x;
+  const int k0 = d.soff[b], k1 = d.soff[b + 1];
+  const BodyKin P = d.kin[b];
+  const bool driven = b < d.nDriven;
+  v3 F, M;
+  int c0, c1;
+  init_resultants(d, b, P, driven, F, M, c0, c1);
+  const uint8_t* act = d.sact;
+  const double* val = d.sval;
+  if (d.passive[b] && !d.exact_sums) {
+    double a[6] = {-0.0, -0.0, -0.0, -0.0, -0.0, -0.0};
+    for (int k = k0 + t; k < k1; k += kBlock) {
+      if (act[k]) {
+        const double* v = val + 6 * (size_t)k;
+#pragma unroll
+        for (int c = 0; c < 6; c++) a[c] += v[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) lds[c * kBlock + t] = a[c];
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+      if (t < h)
+#pragma unroll
+        for (int c = 0; c < 6; c++) lds[c * kBlock + t] += lds[c * kBlock + t + h];
+      __syncthreads();
+    }
+    if (t == 0) {
+      const v3 Fs = V(lds[0], lds[kBlock], lds[2 * kBlock]);
+      const v3 Ms = V(lds[3 * kBlock], lds[4 * kBlock], lds[5 * kBlock]);
+      finish_body(d, sp, kick, b, P, driven, vadd(F, Fs), vadd(M, Ms), c0, c1);
+    }
+    return;
+  }
+  const int len = k1 - k0;
+  const int nchunks = (len + kHeavyChunk - 1) / kHeavyChunk;
+  const int wave = t >> 6, lane = t & 63;
+  double s = pick(F, M, lane < 6 ? lane : 0);
+  auto stage = [&](int c) {
+    double* buf = lds + (c & 1) * kHeavyChunk * 6;
+    const int kb = k0 + c * kHeavyChunk;
+    const int n = min(kHeavyChunk, k1 - kb);
+    for (int idx = t - 64; idx < n; idx += kBlock - 64) {
+      const int k = kb + idx;
+      const bool on = act[k] != 0;
+      const double* v = val + 6 * (size_t)k;
+#pragma unroll
+      for (int q = 0; q < 6; q++) buf[6 * idx + q] = on ? v[q] : -0.0;
+    }
+  };
+  if (wave > 0 && nchunks > 0) stage(0);
+  __syncthreads();
+  for (int c = 0; c < nchunks; c++) {
+    if (wave > 0) {
+      if (c + 1 < nchunks) stage(c + 1);
+    } else if (lane < 6) {
+      const double* buf = lds + (c & 1) * kHeavyChunk * 6 + lane;
+      const int n = min(kHeavyChunk, k1 - (k0 + c * kHeavyChunk));
+      for (int idx = 0; idx < n; idx++) s += buf[6 * idx];
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+    const double fx = __shfl(s, 0), fy = __shfl(s, 1), fz = __shfl(s, 2);
+    const double mx = __shfl(s, 3), my = __shfl(s, 4), mz = __shfl(s, 5);
+    if (lane == 0) finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
@@ -815,7 +939,10 @@ void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hip
 
 void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) {
   if (d.nb <= 0) return;
-  hipLaunchKernelGGL(k_gather, dim3((d.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp, kick);
+  const int waves = (d.nb + kBodiesPerWave - 1) / kBodiesPerWave;
+  const int blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+  hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(kBlock), 0, s, d, sp, kick);
+  if (d.nheavy > 0) hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
 }
 
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s) {

@@ -277,16 +277,18 @@ def box_walls(lo, hi, radius, names=("WALLZ0", "WALLX0", "WALLX1", "WALLY0", "WA
 
 
 def sphere_lattice_deck(n_side=8, R=0.5e-3, spacing_factor=2.2, seed=1, vmax=0.01, kn=1e6, en=0.5, mu=0.4,
-                        dt=1e-6, steps=10000, nl="bruteForce"):
+                        dt=1e-6, steps=10000, nl="bruteForce", dims=None):
     """Plain method, CPU-runnable case: n_side^3 single-vertex spheres of
     radius R on a lattice of spacing spacing_factor * R, dropping under gravity
     into a fixed floor and four walls (5 driven plates first), velocities
-    uniform in [-vmax, vmax] from the seed; Default law."""
+    uniform in [-vmax, vmax] from the seed; Default law.  dims = (nx, ny, nz)
+    replaces the cubic lattice by an nx x ny x nz one."""
     rng = np.random.default_rng(seed)
+    nx, ny, nz = dims if dims is not None else (n_side, n_side, n_side)
     a = spacing_factor * R
     gap = a - 2 * R
     lo = np.array([0.0, 0.0, 0.0])
-    hi = np.array([n_side * a, n_side * a, n_side * a + 4 * a])
+    hi = np.array([nx * a, ny * a, nz * a + 4 * a])
     wall_shapes, walls = box_walls(lo, hi, 0.1 * R)
     shapes = wall_shapes + [sphere_shape("SPHERE", R)]
     deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=100 * dt,
@@ -294,9 +296,9 @@ def sphere_lattice_deck(n_side=8, R=0.5e-3, spacing_factor=2.2, seed=1, vmax=0.0
                 gravity=(0.0, 0.0, -9.81), update_nl=nl, nDriven=len(walls),
                 params=contact_params(kn, 0.8 * kn, en * en, mu, groups=((0, 0), (0, 1))))
     deck.bodies = list(walls)
-    for ix in range(n_side):
-        for iy in range(n_side):
-            for iz in range(n_side):
+    for ix in range(nx):
+        for iy in range(ny):
+            for iz in range(nz):
                 p = lo + np.array([R + 0.5 * gap + ix * a, R + 0.5 * gap + iy * a, R + 0.5 * gap + iz * a])
                 v = rng.uniform(-vmax, vmax, size=3)
                 deck.bodies.append(Body("SPHERE", group=0, pos=tuple(p), vel=tuple(v)))
@@ -304,10 +306,15 @@ def sphere_lattice_deck(n_side=8, R=0.5e-3, spacing_factor=2.2, seed=1, vmax=0.0
     return shapes, deck
 
 
-def hull_shape_pool(n_shapes, seed, nmin=8, nmax=14, axes=(1.0, 0.8, 0.6), radius_frac=0.05):
+def hull_shape_pool(n_shapes, seed, nmin=8, nmax=14, axes=(1.0, 0.8, 0.6), radius_frac=0.05, min_volume_frac=0.25):
     """Unit-size convex sphero-polyhedra: hull of nmin..nmax random points on an
-    ellipsoid of semi-axes 0.5 * axes, Minkowski radius radius_frac."""
+    ellipsoid of semi-axes 0.5 * axes, Minkowski radius radius_frac.  Hulls
+    whose volume is below min_volume_frac of the ellipsoid's are drawn again:
+    such slivers have a tenth of the median rotational inertia, and with
+    several stiff contacts (kn = 1e7 N/m) their rotation outruns an explicit
+    step of dt = 2e-7 s."""
     rng = np.random.default_rng(seed)
+    vmin = min_volume_frac * (4.0 / 3.0) * np.pi * np.prod(0.5 * np.asarray(axes))
     out = []
     for k in range(n_shapes):
         n = int(rng.integers(nmin, nmax + 1))
@@ -316,10 +323,12 @@ def hull_shape_pool(n_shapes, seed, nmin=8, nmax=14, axes=(1.0, 0.8, 0.6), radiu
             u /= np.linalg.norm(u, axis=1)[:, None]
             pts = u * (0.5 * np.asarray(axes))
             try:
-                out.append(hull_shape("HULL%03d" % k, pts, radius_frac))
-                break
+                s = hull_shape("HULL%03d" % k, pts, radius_frac)
             except Exception:
                 continue
+            if s.volume >= vmin:
+                out.append(s)
+                break
     return out
 
 
@@ -475,3 +484,74 @@ def write_deck(dirpath, shapes, deck, name="deck.conf"):
     write_shp(os.path.join(dirpath, deck.shape_file), shapes)
     write_conf(os.path.join(dirpath, name), deck)
     return os.path.join(dirpath, name)
+
+
+def conf_with(src, dst, **header):
+    """Copy a conf file, replacing header keywords (the lines before
+    "Particles"): scalar keywords take a value or a tuple (gravity), pair
+    parameters are given as name=[(g1, g2, value), ...]."""
+    with open(src) as fh:
+        lines = fh.read().split("\n")
+    out = []
+    in_header = True
+    for ln in lines:
+        tok = ln.split()
+        if in_header and tok and tok[0] == "Particles":
+            in_header = False
+        if in_header and tok and tok[0] in header:
+            v = header[tok[0]]
+            if isinstance(v, list):
+                for g1, g2, x in v:
+                    if int(tok[1]) == g1 and int(tok[2]) == g2:
+                        ln = "%s %d %d %s" % (tok[0], g1, g2, _r(x))
+            elif isinstance(v, (tuple, np.ndarray)):
+                ln = tok[0] + " " + " ".join(_r(x) for x in v)
+            else:
+                ln = "%s %s" % (tok[0], v if isinstance(v, str) else _r(v))
+        out.append(ln)
+    with open(dst, "w") as fh:
+        fh.write("\n".join(out))
+    return dst
+
+
+def conf_drop_bodies(src, dst, keep):
+    """Copy a conf file without the bodies where keep is False: their
+    Particles lines go, interactions and interfaces that involve them go, and
+    the body indices of the rest are renumbered."""
+    keep = np.asarray(keep, bool)
+    new_index = np.cumsum(keep) - 1
+    with open(src) as fh:
+        lines = fh.read().split("\n")
+    out = []
+    k = 0
+    while k < len(lines):
+        ln = lines[k]
+        tok = ln.split()
+        if tok and tok[0] in ("Particles", "Interactions", "Interfaces") and len(tok) == 2:
+            n = int(tok[1])
+            k += 1
+            head, body = [], []
+            while k < len(lines) and lines[k].startswith("#"):  # column comments
+                head.append(lines[k])
+                k += 1
+            for q in range(n):
+                ln = lines[k + q]
+                if tok[0] == "Particles":
+                    if keep[q]:
+                        body.append(ln)
+                    continue
+                t = ln.split()
+                i, j = int(t[0]), int(t[1])
+                if keep[i] and keep[j]:
+                    t[0], t[1] = str(new_index[i]), str(new_index[j])
+                    body.append(" ".join(t))
+            out.append("%s %d" % (tok[0], len(body)))
+            out.extend(head)
+            out.extend(body)
+            k += n
+            continue
+        out.append(ln)
+        k += 1
+    with open(dst, "w") as fh:
+        fh.write("\n".join(out))
+    return dst

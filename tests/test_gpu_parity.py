@@ -119,3 +119,76 @@ def test_hull_packing_linkcells_bit_exact(tmp_path, law):
     I = o.interactions()
     assert o.nactive >= 3 and len(I) > 100
     assert set(I["type"].tolist()) == {0, 1, 2, 3}
+
+
+def _sphere_bed(tmp_path):
+    """3200 spheres in two layers on the floor, thrown down at 0.05 m/s under
+    100 g: the floor holds ~1600 listed entries (four 512-slot chunks of the
+    block-wide sum)."""
+    from rockable_amd import decks as D
+    shapes, deck = D.sphere_lattice_deck(dims=(40, 40, 2), vmax=0.01, spacing_factor=2.02, dt=1e-6, nl="linkCells")
+    deck.gravity = (0.0, 0.0, -981.0)
+    for b in deck.bodies[deck.nDriven:]:
+        b.vel = (b.vel[0], b.vel[1], b.vel[2] - 0.05)
+    return D.write_deck(str(tmp_path), shapes, deck)
+
+
+def _wall_scale(o, nwalls):
+    """Per wall, the sum of |terms| of its active interactions: the scale
+    against which a reordered sum is compared."""
+    I = o.interactions(active=True)
+    f = np.abs(I["fn"][:, None] * I["n"] + I["ft"])
+    sc = np.zeros((nwalls, 3))
+    for w in range(nwalls):
+        m = (I["i"] == w) | (I["j"] == w)
+        sc[w] = f[m].sum(axis=0) if m.any() else 0.0
+    return sc
+
+
+def test_heavy_bodies_serial_sum_bit_exact(tmp_path):
+    """Bodies with more slots than heavy_min summed block-wide in serial
+    order (exact_sums): every body bit-identical to the oracle, walls
+    included, over 800 steps, with the floor's sum spanning four chunks and
+    the free bodies (5-6 slots) also on the block-wide path."""
+    from oracle.rk_oracle import Oracle
+    path = _sphere_bed(tmp_path)
+    e = _engine(path)
+    e.set_option("heavy_min", 4)
+    e.set_option("exact_sums", 1)
+    o = Oracle(path)
+    e.update_nl()
+    o.update_nl()
+    I = o.interactions()
+    slots = np.bincount(I["i"], minlength=o.nbodies) + np.bincount(I["j"], minlength=o.nbodies)
+    assert slots[:5].max() > 1024
+    for _ in range(4):
+        e.step(200)
+        o.run_steps(200)
+        assert_bodies_equal(e.bodies(), o.bodies())
+    assert o.nactive > 1000
+
+
+def test_heavy_passive_walls_tree_sum(tmp_path):
+    """Default mode: fixed walls with many slots are summed by a fixed tree.
+    All motion stays bit-identical (their resultants feed nothing back); the
+    walls' force and moment agree with the serial sums to 1e-12 of the sum of
+    |terms|."""
+    from oracle.rk_oracle import Oracle
+    path = _sphere_bed(tmp_path)
+    e = _engine(path)
+    e.set_option("heavy_min", 16)
+    o = Oracle(path)
+    e.update_nl()
+    o.update_nl()
+    for _ in range(4):
+        e.step(200)
+        o.run_steps(200)
+        g, r = e.bodies(), o.bodies()
+        nw = 5
+        assert_bodies_equal(g, r, keys=("pos", "vel", "Q", "vrot", "acc", "arot"))
+        assert_bodies_equal(g, r, keys=("force", "moment"), rows=slice(nw, None))
+        sc = _wall_scale(o, nw)
+        assert np.all(np.abs(g["force"][:nw] - r["force"][:nw]) <= 1e-12 * sc.sum(axis=1)[:, None] + 1e-300)
+        lever = 0.1  # box size bound on the moment arms (m)
+        assert np.all(np.abs(g["moment"][:nw] - r["moment"][:nw]) <= 1e-12 * lever * sc.sum(axis=1)[:, None] + 1e-300)
+    assert np.abs(r["force"][0]).max() > 0.0
