@@ -5,6 +5,8 @@
 //   bodies   BodyKin[N]   128 B record read by every gather of the step
 //                         (pos, Q, vel, vrot, Minkowski radius, homothety,
 //                          shape id, group id)
+//            gv[3 NV]     global vertex positions, body b's from bvoff[b],
+//                         recomputed once per step
 //            acc/arot/force/moment  3 doubles per body (xyz interleaved)
 //            mass[N], inertia[3N], cluster[N] (int)
 //   shapes   vertex table (3 doubles / vertex), edge pairs (2 int), face CSR
@@ -57,6 +59,14 @@ struct EGeom {
   double vx, vy, vz;
 };
 
+// An active entry handed from the geometry kernel to the force-law kernel.
+struct alignas(16) ActiveKin {
+  double nx, ny, nz;
+  double px, py, pz;
+  double dn, prev_dn;
+  int32_t e, deact;
+};
+
 struct DevControl {
   int32_t type, i;
   double value;
@@ -100,6 +110,11 @@ struct DevInterfaces {
 struct Dev {
   int nb, nDriven;
   BodyKin* kin;
+  // global vertices: body b's are gv[3 * bvoff[b] ...], bvbody[k] owns slot k
+  double* gv;
+  const int32_t* bvoff;
+  const int32_t* bvbody;
+  int nvtot;
   double* acc;
   double* arot;
   double* force;
@@ -133,6 +148,10 @@ struct Dev {
   // (a driven body without force or moment controls)
   const uint8_t* passive;
   int exact_sums;         // 1: serial sums for passive bodies too
+  // active entries of the current force phase: geometry block b's in
+  // akin[b * 256, b * 256 + acount[b])
+  ActiveKin* akin;
+  int* acount;
   // driving
   int nctrl;
   DevControl* ctrl;
@@ -161,6 +180,7 @@ RK_HD uint32_t pack_sub(int type, int isub, int jsub) {
 
 // rk_step.hip
 void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s);
+void launch_glob(const Dev& d, hipStream_t s);
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s);
 void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s);
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);

@@ -63,6 +63,10 @@ struct rk_engine {
   int32_t *ctrl_off = nullptr, *ctrl_idx = nullptr;
   int32_t* flags = nullptr;
   uint8_t* passive = nullptr;
+  double* gv = nullptr;
+  int32_t *bvoff = nullptr, *bvbody = nullptr;
+  int nvtot = 0;
+  bool gv_dirty = true;  // body poses changed since the last k_glob
   int exact_sums = 0;
   EntryArrays E[2];
   int cur = 0;
@@ -106,7 +110,8 @@ void free_device(rk_engine* eng) {
                 eng->sh_voff,  eng->sh_eoff,   eng->sh_foff,   eng->sh_edge,   eng->sh_fstart, eng->sh_fidx,
                 eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->ptab,      eng->damp_rate, eng->damp_kind,
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
-                eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive};
+                eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
+                eng->gv,       eng->bvoff,     eng->bvbody};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   eng->kin = nullptr;
@@ -125,6 +130,9 @@ void free_device(rk_engine* eng) {
   eng->ctrl_off = eng->ctrl_idx = nullptr;
   eng->flags = nullptr;
   eng->passive = nullptr;
+  eng->gv = nullptr;
+  eng->bvoff = eng->bvbody = nullptr;
+  eng->nvtot = 0;
   for (auto& E : eng->E) {
     entries_free(E);
     if (E.istart) (void)hipFree(E.istart);
@@ -159,6 +167,10 @@ void assemble_dev(rk_engine* eng) {
   d.nb = (int)m.bodies.size();
   d.nDriven = m.nDriven;
   d.kin = eng->kin;
+  d.gv = eng->gv;
+  d.bvoff = eng->bvoff;
+  d.bvbody = eng->bvbody;
+  d.nvtot = eng->nvtot;
   d.acc = eng->acc;
   d.arot = eng->arot;
   d.force = eng->force;
@@ -199,6 +211,8 @@ void assemble_dev(rk_engine* eng) {
   d.nheavy = E.nheavy;
   d.heavy_min = eng->w.heavy_min;
   d.passive = eng->passive;
+  d.akin = E.akin;
+  d.acount = E.acount;
   d.exact_sums = eng->exact_sums;
   d.istart = E.istart;
   d.jstart = E.jstart;
@@ -287,8 +301,10 @@ int upload_model(rk_engine* eng) {
   if (maxv > 2048 || maxne > 8192 || maxnf > 8192) return fail(eng, "shape too large for the narrow phase");
   const long maxT = (long)maxv * (maxv + maxne + maxnf) + (long)maxne * maxne + (long)maxv * (maxne + maxnf);
   eng->nlp.maxv = maxv;
+  eng->nlp.maxe = maxne;
+  eng->nlp.maxf = maxnf;
   eng->nlp.maxwords = (int)((maxT + 63) / 64);
-  const size_t lds = sizeof(double) * 6 * maxv + sizeof(double) * ((2 * maxv + 7) / 8) + 8 * eng->nlp.maxwords;
+  const size_t lds = narrow_lds_bytes(eng->nlp);
   if (lds > 65536) return fail(eng, "shapes have too many sub-elements for the narrow-phase LDS budget");
   if (upload(eng, eng->sh_nv, nv) || upload(eng, eng->sh_ne, ne) || upload(eng, eng->sh_nf, nf) ||
       upload(eng, eng->sh_voff, voff) || upload(eng, eng->sh_eoff, eoff) || upload(eng, eng->sh_foff, foff) ||
@@ -337,6 +353,20 @@ int upload_model(rk_engine* eng) {
       upload(eng, eng->force, zero3) || upload(eng, eng->moment, zero3) || upload(eng, eng->mass, mass) ||
       upload(eng, eng->inertia, inertia) || upload(eng, eng->cluster, cl) || upload(eng, eng->bf_surface, surf))
     return -1;
+  // global-vertex table: body b's vertices from bvoff[b]
+  {
+    std::vector<int32_t> bvoff(nb + 1, 0), bvbody;
+    for (int b = 0; b < nb; b++) {
+      const int nvb = m.shapes[m.bodies[b].shape].nv();
+      bvoff[b + 1] = bvoff[b] + nvb;
+      for (int v = 0; v < nvb; v++) bvbody.push_back(b);
+    }
+    eng->nvtot = bvoff[nb];
+    if (upload(eng, eng->bvoff, bvoff) || upload(eng, eng->bvbody, bvbody) ||
+        upload(eng, eng->gv, std::vector<double>(3 * (size_t)std::max(1, eng->nvtot), 0.0)))
+      return -1;
+    eng->gv_dirty = true;
+  }
   // parameter tables (compact G = ngroup)
   const int G = std::max(1, m.ngroup);
   std::vector<double> tab((size_t)P_COUNT * G * G), rate((size_t)G * G);
@@ -490,7 +520,7 @@ void resolve_events(rk_engine* eng) {
   eng->ev_used = 0;
 }
 
-// Timing brackets: level 1 brackets the force phase (k_interact + gather
+// Timing brackets: level 1 brackets the force phase (k_geom, k_law, gather
 // kernels, name "force") and the neighbour-list rebuild ("nl"); level 2
 // brackets every kernel on its own.
 void tic(rk_engine* eng, int level) {
@@ -509,10 +539,18 @@ void toc(rk_engine* eng, const char* name, int level) {
   eng->tic_index = -1;
 }
 
+// Global vertices of the current poses (k_glob), before a kernel reads them.
+void ensure_gv(rk_engine* eng) {
+  if (!eng->gv_dirty) return;
+  launch_glob(eng->d, eng->stream);
+  eng->gv_dirty = false;
+}
+
 // Rockable::accelerations (Rockable.cpp:3562-3719); kick fuses the closing
 // half kick of velocityVerletStep.
 int run_accelerations(rk_engine* eng, int kick, int write_geom) {
   const StepParams sp = step_params(eng);
+  ensure_gv(eng);
   tic(eng, 1);
   tic(eng, 2);
   launch_interact(eng->d, sp, write_geom || eng->record_geometry, eng->stream);
@@ -550,6 +588,7 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom) {
 }
 
 int do_update_nl(rk_engine* eng) {
+  ensure_gv(eng);
   tic(eng, eng->profile);
   const int nxt = 1 - eng->cur;
   if (nl_rebuild(eng->w, eng->d, eng->nlp, eng->E[eng->cur], eng->E[nxt], eng->stream) != 0)
@@ -566,6 +605,7 @@ int vv_step(rk_engine* eng, int write_geom) {
   tic(eng, 2);
   launch_predict(eng->d, sp, eng->stream);
   toc(eng, "predict", 2);
+  eng->gv_dirty = true;
   return run_accelerations(eng, 1, write_geom);
 }
 
@@ -1035,6 +1075,7 @@ int rk_set_bodies(rk_engine* eng, const double* pos, const double* Q, const doub
     }
   }
   HIPCHK(hipMemcpy(eng->kin, kin.data(), sizeof(BodyKin) * nb, hipMemcpyHostToDevice));
+  eng->gv_dirty = true;
   return 0;
 }
 

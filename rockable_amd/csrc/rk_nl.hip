@@ -60,11 +60,6 @@ __device__ __forceinline__ qt kq(const BodyKin& b) {
   q.z = b.qz;
   return q;
 }
-__device__ __forceinline__ v3 glob_vertex(const DevShapes& sh, const BodyKin& b, int v) {
-  const double* p = sh.vtx + 3 * (sh.voff[b.shape] + v);
-  v3 hv = vsc(b.h, V(p[0], p[1], p[2]));
-  return vadd(kpos(b), qrot(kq(b), hv));
-}
 __device__ __forceinline__ obb_t load_obb(const double* o) {
   obb_t r;
   r.c = V(o[0], o[1], o[2]);
@@ -109,10 +104,11 @@ __global__ void k_obb(Dev d, double* bobb, double* baabb) {
   o.c = vadd(o.c, kpos(P));
   store_obb(bobb + 15 * (size_t)b, o);
   if (baabb) {
-    v3 mn = glob_vertex(d.sh, P, 0), mx = mn;
+    const double* gb = d.gv + 3 * (size_t)d.bvoff[b];
+    v3 mn = V(gb[0], gb[1], gb[2]), mx = mn;
     const int nv = d.sh.nv[P.shape];
     for (int v = 1; v < nv; v++) {
-      v3 g = glob_vertex(d.sh, P, v);
+      v3 g = V(gb[3 * v], gb[3 * v + 1], gb[3 * v + 2]);
       mn.x = g.x < mn.x ? g.x : mn.x;
       mn.y = g.y < mn.y ? g.y : mn.y;
       mn.z = g.z < mn.z ? g.z : mn.z;
@@ -146,9 +142,64 @@ __device__ __forceinline__ bool has_live_interface(const DevInterfaces& f, int i
   return lo < f.n && f.key[lo] == key && f.alive[lo];
 }
 
-__device__ __forceinline__ void emit_pair(uint64_t* cand, int* counter, int cap, int i, int j) {
-  const int slot = atomicAdd(counter, 1);
-  if (slot < cap) cand[slot] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+// Output counters are striped: stripe s owns counter cnt[s * kStripeStride]
+// (its own 128-byte line) and the region [s * rcap, (s + 1) * rcap) of the
+// output; a workgroup appends to stripe (its index mod kStripes), one atomic
+// per wavefront and emission point.  A single shared counter serialises at
+// one L2 channel (~10 ns per atomic, milliseconds per rebuild).
+constexpr int kStripes = 256;
+constexpr int kStripeStride = 32;
+
+__device__ __forceinline__ int stripe_of_block() {
+  return (int)((blockIdx.x + blockIdx.y * gridDim.x) & (kStripes - 1));
+}
+
+// every active lane of the wavefront calls this; lanes with emit append i, j
+__device__ __forceinline__ void emit_pair(bool emit, uint64_t* cand, int* cnt, int rcap, int i, int j) {
+  const unsigned long long m = __ballot(emit);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  const int s = stripe_of_block();
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt + s * kStripeStride, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (emit) {
+    const int k = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (k < rcap) cand[(size_t)s * rcap + k] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+  }
+}
+
+// one workgroup of kStripes threads: exclusive offsets of the stripes, their
+// total and their maximum
+__global__ void k_stripe_scan(const int* cnt, int* off, int* total_max) {
+  __shared__ int sc[kStripes], mx[kStripes];
+  const int t = threadIdx.x;
+  const int v = cnt[t * kStripeStride];
+  sc[t] = v;
+  mx[t] = v;
+  __syncthreads();
+  for (int o = 1; o < kStripes; o <<= 1) {
+    const int x = t >= o ? sc[t - o] : 0;
+    __syncthreads();
+    sc[t] += x;
+    __syncthreads();
+  }
+  for (int h = kStripes / 2; h > 0; h >>= 1) {
+    if (t < h) mx[t] = max(mx[t], mx[t + h]);
+    __syncthreads();
+  }
+  off[t] = sc[t] - v;
+  if (t == kStripes - 1) total_max[0] = sc[t];
+  if (t == 0) total_max[1] = mx[0];
+}
+
+// the stripes, concatenated
+__global__ void k_stripe_gather(const uint64_t* src, int rcap, const int* cnt, const int* off, uint64_t* dst) {
+  const int s = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = min(cnt[s * kStripeStride], rcap);
+  if (k < n) dst[off[s] + k] = src[(size_t)s * rcap + k];
 }
 
 __device__ __forceinline__ bool pair_overlap(const double* bobb, int i, int j, double half) {
@@ -161,7 +212,7 @@ __device__ __forceinline__ bool pair_overlap(const double* bobb, int i, int j, d
 
 // UpdateNL_bruteForce pair loop (Rockable.cpp:1622-1658): i < j, j >= nDriven
 constexpr int kBruteChunk = 128;
-__global__ void k_broad_brute(Dev d, const double* bobb, double half, uint64_t* cand, int* counter, int cap) {
+__global__ void k_broad_brute(Dev d, const double* bobb, double half, uint64_t* cand, int* cnt, int rcap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.nb) return;
   int j0 = i + 1;
@@ -169,8 +220,8 @@ __global__ void k_broad_brute(Dev d, const double* bobb, double half, uint64_t* 
   const int jb = j0 + (int)blockIdx.y * kBruteChunk;
   const int je = min(jb + kBruteChunk, d.nb);
   for (int j = jb; j < je; j++) {
-    if (has_live_interface(d.ifc, i, j)) continue;
-    if (pair_overlap(bobb, i, j, half)) emit_pair(cand, counter, cap, i, j);
+    const bool emit = !has_live_interface(d.ifc, i, j) && pair_overlap(bobb, i, j, half);
+    emit_pair(emit, cand, cnt, rcap, i, j);
   }
 }
 
@@ -239,7 +290,7 @@ __global__ void k_cell_fill(int nb, const int* cell_of, const int* cstart, int* 
 
 // main loop of UpdateNL_linkCells (Rockable.cpp:1697-1751), one thread per body
 __global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, const int* cell_of, const int* cstart,
-                              const int* cbody, uint64_t* cand, int* counter, int cap) {
+                              const int* cbody, uint64_t* cand, int* cnt, int rcap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.nb) return;
   const int c = cell_of[i];
@@ -253,25 +304,23 @@ __global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, co
         const int nc = (nx * g.N[1] + ny) * g.N[2] + nz;
         for (int r = cstart[nc]; r < cstart[nc + 1]; r++) {
           const int j = cbody[r];
-          if (j < d.nDriven && i < d.nDriven) continue;
-          if (j <= i) continue;
-          if (has_live_interface(d.ifc, i, j)) continue;
-          if (pair_overlap(bobb, i, j, half)) emit_pair(cand, counter, cap, i, j);
+          const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
+                            pair_overlap(bobb, i, j, half);
+          emit_pair(emit, cand, cnt, rcap, i, j);
         }
       }
 }
 
 // oversized bodies against every body of every cell (Rockable.cpp:1759-1805)
 __global__ void k_broad_over(Dev d, const double* bobb, double half, const int* over, int nover, const int* cbody,
-                             int ncell_bodies, uint64_t* cand, int* counter, int cap) {
+                             int ncell_bodies, uint64_t* cand, int* cnt, int rcap) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (r >= ncell_bodies || q >= nover) return;
   const int i = over[q], j = cbody[r];
-  if (j < d.nDriven && i < d.nDriven) return;
-  if (j <= i) return;
-  if (has_live_interface(d.ifc, i, j)) return;
-  if (pair_overlap(bobb, i, j, half)) emit_pair(cand, counter, cap, i, j);
+  const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
+                    pair_overlap(bobb, i, j, half);
+  emit_pair(emit, cand, cnt, rcap, i, j);
 }
 
 // --- directed groups -------------------------------------------------------
@@ -331,6 +380,8 @@ __global__ void k_group_compact(int n, const uint64_t* gk, const uint32_t* gv, c
 // --- narrow phase ------------------------------------------------------
 struct NarrowCfg {
   int maxv;       // max vertices per shape
+  int maxe;       // max edges per shape
+  int maxf;       // max faces per shape
   int maxwords;   // max 64-bit bitmask words per pair
   double dmax;    // dVerlet
 };
@@ -435,87 +486,224 @@ __device__ __forceinline__ bool vertex_gate(v3 gv, double Rowner, const obb_t& o
   return obb_intersect(o, sub);
 }
 
-struct Seg {
-  int type, n_own, n_oth;  // test grid n_own x n_oth
-  bool fwd;
-};
+// Sub-element culls.  They only skip tests whose predicate is false, so the
+// add / erase / retain outcome is the reference's:
+//  * EdgeIsNearEdge is true only if points of the two segments lie within
+//    sum = Ri + Rj + dmax of each other, so each edge's AABB grown by sum
+//    meets the vertex AABB of the other body;
+//  * VertexIsNearEdge accepts r in [-dmax, 1 + dmax] (r is the fraction
+//    along the edge), so the point on the line is within dmax |E|_1 of the
+//    edge's AABB;
+//  * VertexIsNearFace needs the vertex within sum of its projection P on the
+//    face plane with P inside the projected polygon, so P lies within the
+//    face's thickness along its normal of the face's AABB.
+// Every grown box also gets a margin of 1e-9 (1 + |x|max) for rounding.
+__device__ __forceinline__ bool box_meets(const double lo[3], const double hi[3], double grow, const double* blo,
+                                          const double* bhi) {
+  return lo[0] - grow <= bhi[0] && hi[0] + grow >= blo[0] && lo[1] - grow <= bhi[1] && hi[1] + grow >= blo[1] &&
+         lo[2] - grow <= bhi[2] && hi[2] + grow >= blo[2];
+}
+__device__ bool edge_gate(const DevShapes& sh, int shp, const double* gv, int e, double sum, double dmax,
+                          double margin, const double* blo, const double* bhi) {
+  const int* ed = sh.edge + 2 * (sh.eoff[shp] + e);
+  const v3 a = lds_v(gv, ed[0]), b = lds_v(gv, ed[1]);
+  const double lo[3] = {fmin(a.x, b.x), fmin(a.y, b.y), fmin(a.z, b.z)};
+  const double hi[3] = {fmax(a.x, b.x), fmax(a.y, b.y), fmax(a.z, b.z)};
+  const double l1 = (hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]);
+  return box_meets(lo, hi, sum + dmax * l1 + margin, blo, bhi);
+}
+__device__ bool face_gate(const DevShapes& sh, int shp, const double* gv, int f, double sum, double margin,
+                          const double* blo, const double* bhi) {
+  const int gface = sh.foff[shp] + f;
+  const int fs = sh.fstart[gface];
+  const int nbv = sh.fstart[gface + 1] - fs;
+  const int* fv = sh.fidx + fs;
+  const v3 A = lds_v(gv, fv[0]), B = lds_v(gv, fv[1]), C = lds_v(gv, fv[nbv - 1]);
+  v3 v1 = vsub(B, A);
+  vnormalize(&v1);
+  v3 n = vcross(v1, vsub(C, A));
+  vnormalize(&n);
+  double lo[3] = {A.x, A.y, A.z}, hi[3] = {A.x, A.y, A.z};
+  double thick = 0.0;
+  for (int k = 1; k < nbv; k++) {
+    const v3 q = lds_v(gv, fv[k]);
+    lo[0] = fmin(lo[0], q.x);
+    lo[1] = fmin(lo[1], q.y);
+    lo[2] = fmin(lo[2], q.z);
+    hi[0] = fmax(hi[0], q.x);
+    hi[1] = fmax(hi[1], q.y);
+    hi[2] = fmax(hi[2], q.z);
+    thick = fmax(thick, fabs(vdot(vsub(q, A), n)));
+  }
+  if (!(thick == thick)) return true;  // degenerate face: no cull
+  return box_meets(lo, hi, sum + 2.0 * thick + margin, blo, bhi);
+}
 
-__global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, NarrowCfg cfg, int nc, const uint64_t* cand,
-                                               uint32_t* nearbuf, int* near_counter, int near_cap, int32_t* cbase,
-                                               int32_t* cnf, int32_t* cnb) {
+
+// Wavefront compaction of the indices v < n with pred(v) into list (in
+// ascending order); returns the count.
+template <class F>
+__device__ __forceinline__ int wave_compact(int n, int lane, int* list, F pred) {
+  int base = 0;
+  for (int v0 = 0; v0 < n; v0 += 64) {
+    const int v = v0 + lane;
+    const bool f = v < n && pred(v);
+    const unsigned long long m = __ballot(f);
+    if (f) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+    base += __popcll(m);
+  }
+  return base;
+}
+
+__global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const double* baabb, NarrowCfg cfg, int nc,
+                                               const uint64_t* cand, uint32_t* nearbuf, int* near_cnt,
+                                               int near_rcap, int32_t* cbase, int32_t* cnf, int32_t* cnb) {
   extern __shared__ double lds[];
   const int c = blockIdx.x;
   if (c >= nc) return;
   const int lane = threadIdx.x;
   const uint64_t key = cand[c];
   const int i = (int)(key >> 32), j = (int)(key & 0xffffffffu);
-  const BodyKin Pi = d.kin[i], Pj = d.kin[j];
-  const int shi = Pi.shape, shj = Pj.shape;
+  const int shi = d.kin[i].shape, shj = d.kin[j].shape;
+  const double Ri = d.kin[i].R, Rj = d.kin[j].R;
   const int nvi = d.sh.nv[shi], nvj = d.sh.nv[shj];
   const int nei = d.sh.ne[shi], nej = d.sh.ne[shj];
   const int nfi = d.sh.nf[shi], nfj = d.sh.nf[shj];
   double* gvi = lds;
   double* gvj = lds + 3 * cfg.maxv;
-  unsigned char* gate_i = (unsigned char*)(lds + 6 * cfg.maxv);
-  unsigned char* gate_j = gate_i + cfg.maxv;
-  unsigned long long* bits = (unsigned long long*)(lds + 6 * cfg.maxv + (2 * cfg.maxv + 7) / 8);
-  for (int v = lane; v < nvi; v += 64) {
-    v3 g = glob_vertex(d.sh, Pi, v);
-    gvi[3 * v] = g.x;
-    gvi[3 * v + 1] = g.y;
-    gvi[3 * v + 2] = g.z;
-  }
-  for (int v = lane; v < nvj; v += 64) {
-    v3 g = glob_vertex(d.sh, Pj, v);
-    gvj[3 * v] = g.x;
-    gvj[3 * v + 1] = g.y;
-    gvj[3 * v + 2] = g.z;
-  }
-  __syncthreads();
-  const obb_t oi = load_obb(bobb + 15 * (size_t)i);
-  const obb_t oj = load_obb(bobb + 15 * (size_t)j);
-  for (int v = lane; v < nvi; v += 64) gate_i[v] = vertex_gate(lds_v(gvi, v), Pi.R, oj, cfg.dmax) ? 1 : 0;
-  for (int v = lane; v < nvj; v += 64) gate_j[v] = vertex_gate(lds_v(gvj, v), Pj.R, oi, cfg.dmax) ? 1 : 0;
-  __syncthreads();
+  unsigned long long* bits = (unsigned long long*)(lds + 6 * cfg.maxv);
+  int* lst = (int*)(bits + cfg.maxwords);
+  int* vg_l = lst;                  // gated vertices: i's as v, j's as maxv + v
+  int* vjn_l = vg_l + 2 * cfg.maxv; // vertices of j near i's box (vertex-vertex)
+  int* ef_l = vjn_l + cfg.maxv;     // gated edges and faces, tagged (see below)
+  int* cnt = ef_l + 2 * cfg.maxe + 2 * cfg.maxf;  // list sizes
 
-  const Seg seg[6] = {{VV, nvi, nvj, true}, {VE, nvi, nej, true}, {VF, nvi, nfj, true},
-                      {EE, nei, nej, true}, {VE, nvj, nei, false}, {VF, nvj, nfi, false}};
-  int segstart[7];
-  segstart[0] = 0;
-  for (int s = 0; s < 6; s++) segstart[s + 1] = segstart[s] + seg[s].n_own * seg[s].n_oth;
-  const int T = segstart[6], Tf = segstart[4];
+  // test order of the reference (AddOrRemoveInteractions_bruteForce): the
+  // bit of test (segment s, own, oth) is segstart[s] + own * n_oth[s] + oth
+  const int s1 = nvi * nvj, s2 = s1 + nvi * nej, s3 = s2 + nvi * nfj, s4 = s3 + nei * nej, s5 = s4 + nvj * nei;
+  const int T = s5 + nvj * nfi, Tf = s4;
   const int nwords = (T + 63) / 64;
   if (nwords > cfg.maxwords) return;  // guarded on the host: shapes are checked at load
-
-  for (int base = 0; base < T; base += 64) {
-    const int t = base + lane;
-    bool nr = false;
-    if (t < T) {
-      int s = 0;
-      while (t >= segstart[s + 1]) s++;
-      const int local = t - segstart[s];
-      const int own = local / seg[s].n_oth, oth = local % seg[s].n_oth;
-      if (seg[s].fwd) {
-        if (seg[s].type == EE)
-          nr = near_ee(d.sh, shi, shj, gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
-        else if (gate_i[own]) {
-          if (seg[s].type == VV)
-            nr = near_vv(gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
-          else if (seg[s].type == VE)
-            nr = near_ve(d.sh, shj, gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
-          else
-            nr = near_vf(d.sh, shj, gvi, gvj, Pi.R, Pj.R, own, oth, cfg.dmax);
-        }
-      } else if (gate_j[own]) {
-        if (seg[s].type == VE)
-          nr = near_ve(d.sh, shi, gvj, gvi, Pj.R, Pi.R, own, oth, cfg.dmax);
-        else
-          nr = near_vf(d.sh, shi, gvj, gvi, Pj.R, Pi.R, own, oth, cfg.dmax);
-      }
-    }
-    const unsigned long long m = __ballot(nr);
-    if (lane == 0) bits[base / 64] = m;
+  for (int w = lane; w < nwords; w += 64) bits[w] = 0ull;
+  {
+    const double* src_i = d.gv + 3 * (size_t)d.bvoff[i];
+    const double* src_j = d.gv + 3 * (size_t)d.bvoff[j];
+    for (int q = lane; q < 3 * nvi; q += 64) gvi[q] = src_i[q];
+    for (int q = lane; q < 3 * nvj; q += 64) gvj[q] = src_j[q];
   }
+  // body boxes (vertices +- Minkowski radius, from k_obb) for the culls
+  double bx[12];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    bx[k] = baabb[6 * (size_t)i + k];
+    bx[6 + k] = baabb[6 * (size_t)j + k];
+  }
+  double xmax = 0.0;
+#pragma unroll
+  for (int k = 0; k < 12; k++) xmax = fmax(xmax, fabs(bx[k]));
+  const double dmax = cfg.dmax;
+  const double sum = Ri + Rj + dmax;
+  const double margin = 1e-9 * (1.0 + xmax + sum);
+  const obb_t oi = load_obb(bobb + 15 * (size_t)i);
+  const obb_t oj = load_obb(bobb + 15 * (size_t)j);
+  __syncthreads();
+
+  // reference vertex gates of both bodies in one pass: i's vertices against
+  // j's OBB, j's against i's
+  const int nvg = wave_compact(nvi + nvj, lane, vg_l, [&](int v) {
+    return v < nvi ? vertex_gate(lds_v(gvi, v), Ri, oj, dmax) : vertex_gate(lds_v(gvj, v - nvi), Rj, oi, dmax);
+  });
+  // edges and faces of both bodies in one pass; tag = kind * 2^20 + index,
+  // kinds 0 edge of i, 1 edge of j, 2 face of i, 3 face of j
+  const int ne2 = nei + nej, nef = ne2 + nfi + nfj;
+  const int nefg = wave_compact(nef, lane, ef_l, [&](int q) {
+    if (q < nei) return edge_gate(d.sh, shi, gvi, q, sum, dmax, margin, bx + 6, bx + 9);
+    if (q < ne2) return edge_gate(d.sh, shj, gvj, q - nei, sum, dmax, margin, bx, bx + 3);
+    if (q < ne2 + nfi) return face_gate(d.sh, shi, gvi, q - ne2, sum, margin, bx + 6, bx + 9);
+    return face_gate(d.sh, shj, gvj, q - ne2 - nfi, sum, margin, bx, bx + 3);
+  });
+  // vertex-vertex partners: vertices of j within sum of i's box
+  const double gvv = sum + margin;
+  const int nvj_n = wave_compact(nvj, lane, vjn_l, [&](int v) {
+    const v3 p = lds_v(gvj, v);
+    return p.x >= bx[0] - gvv && p.x <= bx[3] + gvv && p.y >= bx[1] - gvv && p.y <= bx[4] + gvv &&
+           p.z >= bx[2] - gvv && p.z <= bx[5] + gvv;
+  });
+  // split the compacted lists by kind (they are in ascending q, so each kind
+  // is a contiguous run)
+  int nvi_g = 0, nei_g = 0, nej_g = 0, nfi_g = 0;
+  for (int k = lane; k < nvg; k += 64) nvi_g += vg_l[k] < nvi ? 1 : 0;
+  for (int k = lane; k < nefg; k += 64) {
+    const int q = ef_l[k];
+    nei_g += q < nei ? 1 : 0;
+    nej_g += (q >= nei && q < ne2) ? 1 : 0;
+    nfi_g += (q >= ne2 && q < ne2 + nfi) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nvi_g += __shfl_xor(nvi_g, off, 64);
+    nei_g += __shfl_xor(nei_g, off, 64);
+    nej_g += __shfl_xor(nej_g, off, 64);
+    nfi_g += __shfl_xor(nfi_g, off, 64);
+  }
+  const int nvj_g = nvg - nvi_g, nfj_g = nefg - nei_g - nej_g - nfi_g;
+  const int* vi_l = vg_l;            // values v
+  const int* vj_l = vg_l + nvi_g;    // values nvi + v
+  const int* ei_l = ef_l;            // values e
+  const int* ej_l = ef_l + nei_g;    // values nei + e
+  const int* fi_l = ej_l + nej_g;    // values ne2 + f
+  const int* fj_l = fi_l + nfi_g;    // values ne2 + nfi + f
+  __syncthreads();
+
+  // one loop per segment, so that a wavefront runs one predicate at a time
+  int n = nvi_g * nvj_n;
+  for (int k = lane; k < n; k += 64) {
+    const int a = vi_l[k / nvj_n], b = vjn_l[k % nvj_n];
+    if (near_vv(gvi, gvj, Ri, Rj, a, b, dmax)) {
+      const int bit = a * nvj + b;
+      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+    }
+  }
+  n = nvi_g * nej_g;
+  for (int k = lane; k < n; k += 64) {
+    const int a = vi_l[k / nej_g], b = ej_l[k % nej_g] - nei;
+    if (near_ve(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
+      const int bit = s1 + a * nej + b;
+      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+    }
+  }
+  n = nvi_g * nfj_g;
+  for (int k = lane; k < n; k += 64) {
+    const int a = vi_l[k / nfj_g], b = fj_l[k % nfj_g] - ne2 - nfi;
+    if (near_vf(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
+      const int bit = s2 + a * nfj + b;
+      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+    }
+  }
+  n = nei_g * nej_g;
+  for (int k = lane; k < n; k += 64) {
+    const int a = ei_l[k / nej_g], b = ej_l[k % nej_g] - nei;
+    if (near_ee(d.sh, shi, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
+      const int bit = s3 + a * nej + b;
+      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+    }
+  }
+  n = nvj_g * nei_g;
+  for (int k = lane; k < n; k += 64) {
+    const int a = vj_l[k / nei_g] - nvi, b = ei_l[k % nei_g];
+    if (near_ve(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
+      const int bit = s4 + a * nei + b;
+      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+    }
+  }
+  n = nvj_g * nfi_g;
+  for (int k = lane; k < n; k += 64) {
+    const int a = vj_l[k / nfi_g] - nvi, b = fi_l[k % nfi_g] - ne2;
+    if (near_vf(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
+      const int bit = s5 + a * nfi + b;
+      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+    }
+  }
+  (void)cnt;
   __syncthreads();
   // counts per direction
   int cf = 0, cb = 0;
@@ -539,15 +727,17 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, Narrow
   }
   cf = __shfl(cf, 0, 64);
   cb = __shfl(cb, 0, 64);
-  int base_out = 0;
-  if (lane == 0) base_out = atomicAdd(near_counter, cf + cb);
-  base_out = __shfl(base_out, 0, 64);
+  const int stripe = c & (kStripes - 1);
+  int local = 0;
+  if (lane == 0) local = atomicAdd(near_cnt + stripe * kStripeStride, cf + cb);
+  local = __shfl(local, 0, 64);
+  const int base_out = stripe * near_rcap + local;
   if (lane == 0) {
     cbase[c] = base_out;
     cnf[c] = cf;
     cnb[c] = cb;
   }
-  if (base_out + cf + cb > near_cap) return;  // host re-runs with a larger buffer
+  if (local + cf + cb > near_rcap) return;  // host re-runs with a larger buffer
   // write keys in test order: rank of a set bit = its output slot
   int running = 0;
   for (int w0 = 0; w0 < nwords; w0 += 64) {
@@ -565,11 +755,21 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, Narrow
       const int bit = __ffsll((long long)mm) - 1;
       mm &= mm - 1ull;
       const int t = w * 64 + bit;
-      int s = 0;
-      while (t >= segstart[s + 1]) s++;
-      const int local = t - segstart[s];
-      const int own = local / seg[s].n_oth, oth = local % seg[s].n_oth;
-      nearbuf[base_out + pos] = pack_sub(seg[s].type, own, oth);
+      int type, own, oth;
+      if (t < s1) {
+        type = VV, own = t / nvj, oth = t % nvj;
+      } else if (t < s2) {
+        type = VE, own = (t - s1) / nej, oth = (t - s1) % nej;
+      } else if (t < s3) {
+        type = VF, own = (t - s2) / nfj, oth = (t - s2) % nfj;
+      } else if (t < s4) {
+        type = EE, own = (t - s3) / nej, oth = (t - s3) % nej;
+      } else if (t < s5) {
+        type = VE, own = (t - s4) / nei, oth = (t - s4) % nei;
+      } else {
+        type = VF, own = (t - s5) / nfi, oth = (t - s5) % nfi;
+      }
+      nearbuf[base_out + pos] = pack_sub(type, own, oth);
       pos++;
     }
     running += __shfl(incl, 63, 64);
@@ -670,7 +870,10 @@ __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
         if (fwd && type == EE)
           tested = true;
         else if (type == VV || type == VE || type == VF) {
-          if (fwd || type != VV) tested = vertex_gate(glob_vertex(d.sh, Po, isub), Po.R, oth, in.dmax);
+          if (fwd || type != VV) {
+            const double* g = d.gv + 3 * ((size_t)d.bvoff[owner] + isub);
+            tested = vertex_gate(V(g[0], g[1], g[2]), Po.R, oth, in.dmax);
+          }
         }
       }
       take = tested ? -1 : 0;
@@ -888,23 +1091,27 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
   if (nb == 0) return 0;
   const bool cells = p.linkcells != 0;
   if (ensure(w, w.bobb, w.bobb_cap, (size_t)15 * nb)) return -1;
-  if (cells && ensure(w, w.baabb, w.baabb_cap, (size_t)6 * nb)) return -1;
-  hipLaunchKernelGGL(k_obb, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, cells ? w.baabb : nullptr);
+  if (ensure(w, w.baabb, w.baabb_cap, (size_t)6 * nb)) return -1;
+  hipLaunchKernelGGL(k_obb, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, w.baabb);
   const double half = 0.5 * p.DVerlet;
 
   // ---- broad phase: candidate pairs
-  if (ensure(w, w.counter, w.counter_cap, 4)) return -1;
+  if (ensure(w, w.counter, w.counter_cap, 4) || ensure(w, w.scnt, w.scnt_cap, (size_t)kStripes * kStripeStride) ||
+      ensure(w, w.soff, w.soff_cap, (size_t)kStripes))
+    return -1;
   int nc = 0;
   for (int attempt = 0; attempt < 3; attempt++) {
-    if (ensure(w, w.cand, w.cand_cap, (size_t)std::max(w.cand_hint, 1024))) return -1;
-    const int cap = (int)std::min<size_t>(w.cand_cap, 0x3fffffff);
+    // striped output: kStripes regions of rcap pairs
+    const int rcap = std::max(64, (w.cand_hint + kStripes - 1) / kStripes);
+    if (ensure(w, w.cstripe, w.cstripe_cap, (size_t)kStripes * rcap)) return -1;
     RK_CHECK(hipMemsetAsync(w.counter, 0, sizeof(int) * 4, s));
+    RK_CHECK(hipMemsetAsync(w.scnt, 0, sizeof(int) * kStripes * kStripeStride, s));
     if (!cells) {
       const int jmax = nb - std::min(nb, std::max(1, d.nDriven));
       const int chunks = std::max(1, (nb + kBruteChunk - 1) / kBruteChunk);
       (void)jmax;
-      hipLaunchKernelGGL(k_broad_brute, dim3(nblk(nb), chunks), dim3(kBlock), 0, s, d, w.bobb, half, w.cand,
-                         w.counter, cap);
+      hipLaunchKernelGGL(k_broad_brute, dim3(nblk(nb), chunks), dim3(kBlock), 0, s, d, w.bobb, half, w.cstripe,
+                         w.scnt, rcap);
     } else {
       // bounding box of the bodies [first, nb) (Rockable.cpp:1683-1688, 3943-3947)
       int first = (p.boxForLinkCellsOpt == 1) ? d.nDriven : 0;
@@ -952,21 +1159,28 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       const int nover = hv[0];
       const int ncb = nb - nover;
       hipLaunchKernelGGL(k_broad_cells, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
-                         w.cbody, w.cand, w.counter, cap);
+                         w.cbody, w.cstripe, w.scnt, rcap);
       if (nover > 0 && ncb > 0)
         hipLaunchKernelGGL(k_broad_over, dim3(nblk(ncb), nover), dim3(kBlock), 0, s, d, w.bobb, half, w.over, nover,
-                           w.cbody, ncb, w.cand, w.counter, cap);
+                           w.cbody, ncb, w.cstripe, w.scnt, rcap);
     }
-    RK_CHECK(hipMemcpyAsync(&nc, w.counter, sizeof(int), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
+    int tm[2] = {0, 0};
+    RK_CHECK(hipMemcpyAsync(tm, w.counter + 2, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     RK_CHECK(hipStreamSynchronize(s));
-    if ((size_t)nc <= w.cand_cap) break;
-    w.cand_hint = nc + nc / 4;
+    nc = tm[0];
+    if (tm[1] <= rcap) {
+      if (ensure(w, w.cand, w.cand_cap, (size_t)std::max(nc, 1))) return -1;
+      hipLaunchKernelGGL(k_stripe_gather, dim3((rcap + kBlock - 1) / kBlock, kStripes), dim3(kBlock), 0, s, w.cstripe,
+                         rcap, w.scnt, w.soff, w.cand);
+      break;
+    }
+    w.cand_hint = kStripes * (tm[1] + tm[1] / 4 + 64);
     if (attempt == 2) {
       snprintf(w.err, sizeof(w.err), "broad phase overflow");
       return -1;
     }
   }
-  w.cand_hint = std::max(w.cand_hint, nc);
   w.last_candidates = nc;
 
   // ---- narrow phase
@@ -977,26 +1191,31 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
   if (nc > 0) {
     NarrowCfg cfg;
     cfg.maxv = p.maxv;
+    cfg.maxe = p.maxe;
+    cfg.maxf = p.maxf;
     cfg.maxwords = p.maxwords;
     cfg.dmax = p.dVerlet;
-    const size_t lds = sizeof(double) * 6 * p.maxv + sizeof(double) * ((2 * p.maxv + 7) / 8) +
-                       sizeof(unsigned long long) * p.maxwords;
+    const size_t lds = narrow_lds_bytes(p);
     for (int attempt = 0; attempt < 3; attempt++) {
-      if (ensure(w, w.nearbuf, w.nearbuf_cap, (size_t)std::max(w.near_hint, 4096))) return -1;
-      RK_CHECK(hipMemsetAsync(w.counter + 2, 0, sizeof(int), s));
-      hipLaunchKernelGGL(k_narrow, dim3(nc), dim3(64), lds, s, d, w.bobb, cfg, nc, w.cand, w.nearbuf, w.counter + 2,
-                         (int)std::min<size_t>(w.nearbuf_cap, 0x7fffffff), w.cbase, w.cnf, w.cnb);
+      const int rcap = std::max(64, (w.near_hint + kStripes - 1) / kStripes);
+      if (ensure(w, w.nearbuf, w.nearbuf_cap, (size_t)kStripes * rcap)) return -1;
+      RK_CHECK(hipMemsetAsync(w.scnt, 0, sizeof(int) * kStripes * kStripeStride, s));
+      hipLaunchKernelGGL(k_narrow, dim3(nc), dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nc, w.cand, w.nearbuf,
+                         w.scnt, rcap, w.cbase, w.cnf, w.cnb);
       RK_CHECK(hipGetLastError());
-      RK_CHECK(hipMemcpyAsync(&nnear, w.counter + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+      hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
+      int tm[2] = {0, 0};
+      RK_CHECK(hipMemcpyAsync(tm, w.counter + 2, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
       RK_CHECK(hipStreamSynchronize(s));
-      if ((size_t)nnear <= w.nearbuf_cap) break;
-      w.near_hint = nnear + nnear / 4;
+      nnear = tm[0];
+      if (tm[1] <= rcap) break;
+      w.near_hint = kStripes * (tm[1] + tm[1] / 4 + 64);
       if (attempt == 2) {
         snprintf(w.err, sizeof(w.err), "narrow phase overflow");
         return -1;
       }
     }
-    w.near_hint = std::max(w.near_hint, nnear);
+    (void)nnear;
   }
 
   // ---- old groups
@@ -1109,7 +1328,9 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
             hipMalloc(&E.estick, c * sizeof(int32_t)) == hipSuccess && hipMalloc(&E.eact, c * sizeof(uint8_t)) == hipSuccess &&
             hipMalloc(&E.spos, c * 2 * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&E.sact, c * 2 * sizeof(uint8_t)) == hipSuccess &&
-            hipMalloc(&E.sval, c * 12 * sizeof(double)) == hipSuccess;
+            hipMalloc(&E.sval, c * 12 * sizeof(double)) == hipSuccess &&
+            hipMalloc(&E.akin, (c + 256) * sizeof(ActiveKin)) == hipSuccess &&
+            hipMalloc(&E.acount, (c / 256 + 2) * sizeof(int32_t)) == hipSuccess;
   if (!ok) {
     snprintf(w.err, sizeof(w.err), "hipMalloc of %zu interaction entries failed", c);
     entries_free(E);
@@ -1120,7 +1341,7 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
 }
 
 void entries_free(EntryArrays& E) {
-  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.egeo, E.edamp, E.estick, E.eact, E.spos, E.sact, E.sval};
+  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.egeo, E.edamp, E.estick, E.eact, E.spos, E.sact, E.sval, E.akin, E.acount};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   E.ei = E.ej = nullptr;
@@ -1133,11 +1354,13 @@ void entries_free(EntryArrays& E) {
   E.spos = nullptr;
   E.sact = nullptr;
   E.sval = nullptr;
+  E.akin = nullptr;
+  E.acount = nullptr;
   E.cap = 0;
 }
 
 void nl_work_free(NLWork& w) {
-  void* ps[] = {w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
+  void* ps[] = {w.cstripe, w.scnt, w.soff, w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
                 w.gk,    w.gk2,   w.gv,      w.gv2,   w.gflag, w.gpos,  w.ukey,  w.ucand,   w.uold,  w.gcount,
                 w.goff,  w.tmp,   w.i32a,    w.i32b,  w.i32c,  w.f64a,  w.cell_of, w.ccount, w.cstart, w.cfill,
                 w.cbody, w.over};

@@ -25,6 +25,8 @@ struct EntryArrays {
   int32_t* spos = nullptr;   // 2 per entry
   uint8_t* sact = nullptr;   // 2 per entry
   double* sval = nullptr;    // 12 per entry
+  ActiveKin* akin = nullptr; // 1 per entry
+  int32_t* acount = nullptr; // 1 per 256 entries
   int32_t* istart = nullptr;
   size_t istart_cap = 0;
   int32_t* jstart = nullptr;
@@ -44,8 +46,17 @@ struct NLParams {
   double cellMinSizes[3] = {1.0, 1.0, 1.0};
   int boxForLinkCellsOpt = 0;
   int maxv = 1;      // max vertices per shape
+  int maxe = 0;      // max edges per shape
+  int maxf = 0;      // max faces per shape
   int maxwords = 1;  // max 64-bit words of predicate bits per body pair
 };
+
+// LDS of one narrow-phase workgroup: global vertices of both bodies, the
+// predicate bits, the lists of gated vertices, edges and faces.
+inline size_t narrow_lds_bytes(const NLParams& p) {
+  return sizeof(double) * 6 * p.maxv + sizeof(unsigned long long) * p.maxwords +
+         sizeof(int) * (3 * p.maxv + 2 * p.maxe + 2 * p.maxf + 8);
+}
 
 struct NLWork {
   char err[256] = {0};
@@ -58,6 +69,9 @@ struct NLWork {
   RK_BUF(double, baabb)
   RK_BUF(int, counter)
   RK_BUF(uint64_t, cand)
+  RK_BUF(uint64_t, cstripe)
+  RK_BUF(int32_t, scnt)
+  RK_BUF(int32_t, soff)
   RK_BUF(int32_t, cbase)
   RK_BUF(int32_t, cnf)
   RK_BUF(int32_t, cnb)

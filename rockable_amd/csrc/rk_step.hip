@@ -4,10 +4,12 @@
 //   k_predict   Rockable::velocityVerletStep predictor (Rockable.cpp:1979-2055)
 //               + ServoFunction (DrivingSystem.cpp:242-268) and
 //               velocityControlledDrive (Rockable.cpp:1856-1900)
-//   k_interact  update_interactions (Rockable.cpp:2968-2974 ->
+//   k_geom      update_interactions (Rockable.cpp:2968-2974 ->
 //               Interaction.cpp:147-326) fused with build_activeInteractions
-//               (3005-3029) and ForceLaw::computeInteraction (3035-3046,
-//               ForceLaw_*.cpp); writes each active entry's terms for body i
+//               (3005-3029): contact geometry of every listed entry; the
+//               active ones go to a compact list
+//   k_law       ForceLaw::computeInteraction (3035-3046, ForceLaw_*.cpp) on
+//               the active list; writes each active entry's terms for body i
 //               and body j (incrementResultants, 2752-2789) into the two
 //               resultant slots of the entry
 //   k_gather    initialise_particle_forces_and_moments (2869-2929) +
@@ -18,6 +20,8 @@
 //   k_gather_heavy  the same for bodies with many slots, one workgroup each
 //   k_breakage  breakage_of_interfaces (3350-3413)
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "rk_device.hpp"
 #include "rk_model.hpp"
@@ -71,16 +75,19 @@ struct Kin {
   bool deact;
 };
 
+// The global vertices of a body, computed once per step by k_glob.
+__device__ __forceinline__ v3 gvtx(const double* g, int v) { return V(g[3 * v], g[3 * v + 1], g[3 * v + 2]); }
+
 // even-odd crossing test of P against face f of body j (Interaction.cpp:236-253)
-__device__ bool face_crossing(const DevShapes& sh, const BodyKin& Pj, int gface, int nbv, v3 P, v3 v1, v3 v2) {
+__device__ bool face_crossing(const DevShapes& sh, const double* gj, int gface, int nbv, v3 P, v3 v1, v3 v2) {
   int odd = 0;
   double ori1 = vdot(P, v1), ori2 = vdot(P, v2);
   const int* fv = sh.fidx + sh.fstart[gface];
-  v3 A = glob_vertex(sh, Pj, fv[0]);
+  v3 A = gvtx(gj, fv[0]);
   for (int iva = 0; iva < nbv; iva++) {
     int ivb = iva + 1;
     if (ivb == nbv) ivb = 0;
-    v3 B = glob_vertex(sh, Pj, fv[ivb]);
+    v3 B = gvtx(gj, fv[ivb]);
     double pa1 = vdot(A, v1), pb1 = vdot(B, v1);
     double pa2 = vdot(A, v2), pb2 = vdot(B, v2);
     if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
@@ -92,21 +99,23 @@ __device__ bool face_crossing(const DevShapes& sh, const BodyKin& Pj, int gface,
 }
 
 // Interaction::UpdateDispatcher (Interaction.cpp:147-326)
+// WITH_VEL: also the relative velocity at the contact point (Interaction.cpp:162).
+template <bool WITH_VEL>
 __device__ void update_kinematics(const DevShapes& sh, int type, int isub, int jsub, const BodyKin& Pi,
-                                  const BodyKin& Pj, Kin& k) {
+                                  const BodyKin& Pj, const double* gi, const double* gj, Kin& k) {
   const double Ri = Pi.R, Rj = Pj.R;
   if (type == VV) {
-    v3 posi = glob_vertex(sh, Pi, isub);
-    v3 posj = glob_vertex(sh, Pj, jsub);
+    v3 posi = gvtx(gi, isub);
+    v3 posj = gvtx(gj, jsub);
     k.n = vsub(posi, posj);
     k.dn = vnormalize(&k.n) - (Ri + Rj);
     k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
-    k.vel = rel_vel(k.pos, Pi, Pj);
+    if (WITH_VEL) k.vel = rel_vel(k.pos, Pi, Pj);
   } else if (type == VE) {
     const int* ed = sh.edge + 2 * (sh.eoff[Pj.shape] + jsub);
-    v3 posj1 = glob_vertex(sh, Pj, ed[0]);
-    v3 posj2 = glob_vertex(sh, Pj, ed[1]);
-    v3 posi = glob_vertex(sh, Pi, isub);
+    v3 posj1 = gvtx(gj, ed[0]);
+    v3 posj2 = gvtx(gj, ed[1]);
+    v3 posi = gvtx(gi, isub);
     v3 E = vsub(posj2, posj1);
     v3 v = vsub(posi, posj1);
     double r = vdot(v, E) / vdot(E, E);
@@ -117,16 +126,16 @@ __device__ void update_kinematics(const DevShapes& sh, int type, int isub, int j
     k.n = vsub(posi, vadd(posj1, vsc(r, E)));
     k.dn = vnormalize(&k.n) - (Ri + Rj);
     k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
-    k.vel = rel_vel(k.pos, Pi, Pj);
+    if (WITH_VEL) k.vel = rel_vel(k.pos, Pi, Pj);
   } else if (type == VF) {
     const int gface = sh.foff[Pj.shape] + jsub;
     const int fs = sh.fstart[gface];
     const int nbv = sh.fstart[gface + 1] - fs;
     const int* fv = sh.fidx + fs;
-    v3 A = glob_vertex(sh, Pj, fv[0]);
-    v3 B = glob_vertex(sh, Pj, fv[1]);
-    v3 C = glob_vertex(sh, Pj, fv[nbv - 1]);
-    v3 posi = glob_vertex(sh, Pi, isub);
+    v3 A = gvtx(gj, fv[0]);
+    v3 B = gvtx(gj, fv[1]);
+    v3 C = gvtx(gj, fv[nbv - 1]);
+    v3 posi = gvtx(gi, isub);
     v3 v = vsub(posi, A);
     v3 v1 = vsub(B, A);
     vnormalize(&v1);
@@ -140,20 +149,20 @@ __device__ void update_kinematics(const DevShapes& sh, int type, int isub, int j
     }
     v3 P = vsub(posi, vsc(dist, k.n));
     v2 = vcross(k.n, v1);
-    if (face_crossing(sh, Pj, gface, nbv, P, v1, v2)) {
+    if (face_crossing(sh, gj, gface, nbv, P, v1, v2)) {
       k.dn = dist - (Ri + Rj);
       k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
-      k.vel = rel_vel(k.pos, Pi, Pj);
+      if (WITH_VEL) k.vel = rel_vel(k.pos, Pi, Pj);
     } else {
       k.deact = true;  // n keeps the new face normal, pos and vel stay stale
     }
   } else {
     const int* edi = sh.edge + 2 * (sh.eoff[Pi.shape] + isub);
     const int* edj = sh.edge + 2 * (sh.eoff[Pj.shape] + jsub);
-    v3 posi1 = glob_vertex(sh, Pi, edi[0]);
-    v3 posi2 = glob_vertex(sh, Pi, edi[1]);
-    v3 posj1 = glob_vertex(sh, Pj, edj[0]);
-    v3 posj2 = glob_vertex(sh, Pj, edj[1]);
+    v3 posi1 = gvtx(gi, edi[0]);
+    v3 posi2 = gvtx(gi, edi[1]);
+    v3 posj1 = gvtx(gj, edj[0]);
+    v3 posj2 = gvtx(gj, edj[1]);
     v3 Ei = vsub(posi2, posi1);
     v3 Ej = vsub(posj2, posj1);
     v3 v = vsub(posi1, posj1);
@@ -171,7 +180,7 @@ __device__ void update_kinematics(const DevShapes& sh, int type, int isub, int j
       k.n = vsub(vadd(posi1, vsc(s, Ei)), vadd(posj1, vsc(t, Ej)));
       k.dn = vnormalize(&k.n) - (Ri + Rj);
       k.pos = vsub(vadd(posi1, vsc(s, Ei)), vsc(Ri + 0.5 * k.dn, k.n));
-      k.vel = rel_vel(k.pos, Pi, Pj);
+      if (WITH_VEL) k.vel = rel_vel(k.pos, Pi, Pj);
     } else {
       k.deact = true;
     }
@@ -324,41 +333,128 @@ __device__ void law_sticked(const Dev& d, const StepParams& sp, const Kin& k, do
   }
 }
 
-template <int LAW>
-__global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int write_geom_all, int pii) {
+// The force phase runs as two kernels.
+//  k_geom   every listed entry: contact geometry from the body poses
+//           (Interaction::UpdateDispatcher without the relative velocity).
+//           An inactive entry gets its new dn (or the zeroed history of a
+//           deactivation) and cleared activity flags; an active one (dn < 0
+//           or bonded) is appended to a compact list with its geometry.
+//  k_law    every active entry: relative velocity, the force law, the history
+//           and the two resultant slots.
+// The active entries of geometry block b go, in entry order, to
+// akin[b * kBlock, b * kBlock + acount[b]) (ballot + block prefix, no
+// atomics); k_law gives one wavefront to each block's list.
+template <bool GEOM_ALL>
+__global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
+  __shared__ int wcount[kBlock / 64];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= d.ne) return;
-  const int i = d.ei[e], j = d.ej[e];
-  const uint32_t sub = d.esub[e];
-  const int type = (int)(sub >> 30), isub = (int)((sub >> 15) & 0x7fffu), jsub = (int)(sub & 0x7fffu);
-  const BodyKin Pi = d.kin[i];
-  const BodyKin Pj = d.kin[j];
-  const int stick = d.estick[e];
-  const double dn_old = d.est[e].dn;
-  const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
-
-  Kin k;
-  k.deact = false;
-  k.prev_dn = dn_old;
-  if (stick >= 0) {  // a bond may run on stale geometry after a deactivation
-    const EGeom g = d.egeo[e];
-    k.n = V(g.nx, g.ny, g.nz);
-    k.pos = V(g.px, g.py, g.pz);
-    k.vel = V(g.vx, g.vy, g.vz);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  bool act = false;
+  ActiveKin a;
+  if (e < d.ne) {
+    const int i = d.ei[e], j = d.ej[e];
+    const uint32_t sub = d.esub[e];
+    const int type = (int)(sub >> 30), isub = (int)((sub >> 15) & 0x7fffu), jsub = (int)(sub & 0x7fffu);
+    const BodyKin& Pi = d.kin[i];
+    const BodyKin& Pj = d.kin[j];
+    const int stick = d.estick[e];
+    Kin k;
+    k.deact = false;
+    k.prev_dn = d.est[e].dn;
+    if (stick >= 0) {  // a bond may run on stale geometry after a deactivation
+      const EGeom& g = d.egeo[e];
+      k.n = V(g.nx, g.ny, g.nz);
+      k.pos = V(g.px, g.py, g.pz);
+    }
+    update_kinematics<false>(d.sh, type, isub, jsub, Pi, Pj, d.gv + 3 * (size_t)d.bvoff[i],
+                             d.gv + 3 * (size_t)d.bvoff[j], k);
+    if (k.deact) {  // Interaction::deactivate (Interaction.cpp:123-128)
+      k.dn = 0.0;
+      k.prev_dn = 0.0;
+    }
+    act = (k.dn < 0.0) || (stick >= 0);
+    if (act) {
+      a.nx = k.n.x;
+      a.ny = k.n.y;
+      a.nz = k.n.z;
+      a.px = k.pos.x;
+      a.py = k.pos.y;
+      a.pz = k.pos.z;
+      a.dn = k.dn;
+      a.prev_dn = k.prev_dn;
+      a.e = e;
+      a.deact = k.deact ? 1 : 0;
+    } else {
+      if (k.deact) {
+        EState o;
+        o.dn = 0.0;
+        o.fn = 0.0;
+        o.ftx = o.fty = o.ftz = 0.0;
+        o.momx = o.momy = o.momz = 0.0;
+        d.est[e] = o;
+      } else {
+        d.est[e].dn = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
+        if (GEOM_ALL) {
+          const v3 vel = rel_vel(k.pos, Pi, Pj);
+          EGeom g;
+          g.nx = k.n.x;
+          g.ny = k.n.y;
+          g.nz = k.n.z;
+          g.px = k.pos.x;
+          g.py = k.pos.y;
+          g.pz = k.pos.z;
+          g.vx = vel.x;
+          g.vy = vel.y;
+          g.vz = vel.z;
+          d.egeo[e] = g;
+        }
+      }
+      const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
+      d.eact[e] = 0;
+      d.sact[slots.x] = 0;
+      d.sact[slots.y] = 0;
+    }
   }
-  update_kinematics(d.sh, type, isub, jsub, Pi, Pj, k);
-
-  Hist h;
-  if (k.deact) {  // Interaction::deactivate (Interaction.cpp:123-128)
-    k.dn = 0.0;
-    k.prev_dn = 0.0;
-    h.fn = 0.0;
-    h.ft = V(0, 0, 0);
-    h.mom = V(0, 0, 0);
+  const unsigned long long m = __ballot(act);
+  if (lane == 0) wcount[wave] = __popcll(m);
+  __syncthreads();
+  int base = 0, total = 0;
+  for (int w = 0; w < kBlock / 64; w++) {
+    base += w < wave ? wcount[w] : 0;
+    total += wcount[w];
   }
-  const bool active = (k.dn < 0.0) || (stick >= 0);
-  if (active) {
-    if (!k.deact) {
+  if (act) d.akin[(size_t)blockIdx.x * kBlock + base + __popcll(m & ((1ull << lane) - 1ull))] = a;
+  if (threadIdx.x == 0) d.acount[blockIdx.x] = total;
+}
+
+template <int LAW>
+__global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_geom_all, int pii, int ngeom) {
+  const int g = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);  // geometry block
+  if (g >= ngeom) return;
+  const int lane = threadIdx.x & 63;
+  const int count = d.acount[g];
+  for (int q = lane; q < count; q += 64) {
+    const ActiveKin a = d.akin[(size_t)g * kBlock + q];
+    const int e = a.e;
+    const int i = d.ei[e], j = d.ej[e];
+    const BodyKin Pi = d.kin[i];
+    const BodyKin Pj = d.kin[j];
+    const int stick = d.estick[e];
+    Kin k;
+    k.n = V(a.nx, a.ny, a.nz);
+    k.pos = V(a.px, a.py, a.pz);
+    k.dn = a.dn;
+    k.prev_dn = a.prev_dn;
+    k.deact = a.deact != 0;
+    Hist h;
+    if (k.deact) {  // a bond keeps its stale velocity; the history restarts
+      const EGeom& g = d.egeo[e];
+      k.vel = V(g.vx, g.vy, g.vz);
+      h.fn = 0.0;
+      h.ft = V(0, 0, 0);
+      h.mom = V(0, 0, 0);
+    } else {
+      k.vel = rel_vel(k.pos, Pi, Pj);
       const EState s = d.est[e];
       h.fn = s.fn;
       h.ft = V(s.ftx, s.fty, s.ftz);
@@ -389,40 +485,32 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
     v3 mj = vsub(vcross(Cj, vneg(f)), h.mom);
     // the two resultant slots: body i adds (f, mi), body j adds (-f, mj);
     // x + (-y) is x - y exactly, so the serial sums are unchanged
+    const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
     double* ci = d.sval + 6 * (size_t)slots.x;
     double* cj = d.sval + 6 * (size_t)slots.y;
     st3(ci, f);
     st3(ci + 3, mi);
     st3(cj, vneg(f));
     st3(cj + 3, mj);
-  } else if (k.deact) {
-    EState o;
-    o.dn = 0.0;
-    o.fn = 0.0;
-    o.ftx = o.fty = o.ftz = 0.0;
-    o.momx = o.momy = o.momz = 0.0;
-    d.est[e] = o;
-  } else {
-    d.est[e].dn = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
+    // Geometry is persistent for bonds (a deactivated bond keeps its stale n,
+    // pos, vel); for other entries it is written when an output is due.
+    if (stick >= 0 || (write_geom_all && !k.deact)) {
+      EGeom g;
+      g.nx = k.n.x;
+      g.ny = k.n.y;
+      g.nz = k.n.z;
+      g.px = k.pos.x;
+      g.py = k.pos.y;
+      g.pz = k.pos.z;
+      g.vx = k.vel.x;
+      g.vy = k.vel.y;
+      g.vz = k.vel.z;
+      d.egeo[e] = g;
+    }
+    d.eact[e] = 1;
+    d.sact[slots.x] = 1;
+    d.sact[slots.y] = 1;
   }
-  // Geometry is persistent for bonds (a deactivated bond keeps its stale n,
-  // pos, vel); for other entries it is written when an output is due.
-  if (stick >= 0 || (write_geom_all && !k.deact)) {
-    EGeom g;
-    g.nx = k.n.x;
-    g.ny = k.n.y;
-    g.nz = k.n.z;
-    g.px = k.pos.x;
-    g.py = k.pos.y;
-    g.pz = k.pos.z;
-    g.vx = k.vel.x;
-    g.vy = k.vel.y;
-    g.vz = k.vel.z;
-    d.egeo[e] = g;
-  }
-  d.eact[e] = active ? 1 : 0;
-  d.sact[slots.x] = active ? 1 : 0;
-  d.sact[slots.y] = active ? 1 : 0;
 }
 
 __device__ __forceinline__ v3 euler_domega(v3 I, v3 w, v3 M) {
@@ -728,6 +816,16 @@ __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, i
   }
 }
 
+// Particle::GlobVertex (Particle.cpp:55-57) of every vertex of every body,
+// once per step: one thread per vertex.
+__global__ void __launch_bounds__(kBlock) k_glob(Dev d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nvtot) return;
+  const int b = d.bvbody[k];
+  const BodyKin& P = d.kin[b];
+  st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, k - d.bvoff[b]));
+}
+
 __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0 && d.nctrl > 0) {
@@ -926,15 +1024,27 @@ void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s) {
   hipLaunchKernelGGL(k_predict, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp);
 }
 
+void launch_glob(const Dev& d, hipStream_t s) {
+  if (d.nvtot <= 0) return;
+  hipLaunchKernelGGL(k_glob, dim3((d.nvtot + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d);
+}
+
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
   if (d.ne <= 0) return;
   dim3 g((d.ne + kBlock - 1) / kBlock), bl(kBlock);
-  if (sp.law == LAW_AVALANCHE)
-    hipLaunchKernelGGL(k_interact<LAW_AVALANCHE>, g, bl, 0, s, d, sp, write_geom_all, 0);
-  else if (sp.law == LAW_STICKED)
-    hipLaunchKernelGGL(k_interact<LAW_STICKED>, g, bl, 0, s, d, sp, write_geom_all, d.paramsInInterfaces);
+  if (write_geom_all)
+    hipLaunchKernelGGL(k_geom<true>, g, bl, 0, s, d);
   else
-    hipLaunchKernelGGL(k_interact<LAW_DEFAULT>, g, bl, 0, s, d, sp, write_geom_all, 0);
+    hipLaunchKernelGGL(k_geom<false>, g, bl, 0, s, d);
+  // one wavefront per geometry block
+  const int ngeom = (int)g.x;
+  dim3 gl((ngeom + kBlock / 64 - 1) / (kBlock / 64));
+  if (sp.law == LAW_AVALANCHE)
+    hipLaunchKernelGGL(k_law<LAW_AVALANCHE>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom);
+  else if (sp.law == LAW_STICKED)
+    hipLaunchKernelGGL(k_law<LAW_STICKED>, gl, bl, 0, s, d, sp, write_geom_all, d.paramsInInterfaces, ngeom);
+  else
+    hipLaunchKernelGGL(k_law<LAW_DEFAULT>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom);
 }
 
 void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) {
