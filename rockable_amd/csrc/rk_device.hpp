@@ -12,9 +12,11 @@
 //   shapes   vertex table (3 doubles / vertex), edge pairs (2 int), face CSR
 //   entries  one per listed interaction, sorted by (i, j, type, isub, jsub):
 //            ei, ej (int), esub (uint32: type<<30 | isub<<15 | jsub)
-//            EState  {dn, fn, ft.xyz, mom.xyz}       64 B history record
+//            edn     normal distance dn                8 B, every step
+//            EState  {fn, ft.xyz, mom.xyz}           56 B history record
 //            EGeom   {n.xyz, pos.xyz, vel.xyz}        72 B geometry record
-//            damp, stick (int, interface index or -1), act (uint8)
+//            damp, stick (int, interface index or -1), act (uint8: bit 0
+//            active at the last force phase, bit 1 history maybe non-zero)
 //            spos[2n]     the entry's two resultant slots (i side, j side)
 //   CSR      istart[N+1]  entries owned by body b are [istart[b], istart[b+1])
 //            jstart[N+1], jlist[]  entries whose j == b, ascending index
@@ -46,12 +48,15 @@ struct alignas(16) BodyKin {
 };
 static_assert(sizeof(BodyKin) == 128, "BodyKin must stay one 128-byte record");
 
-struct alignas(16) EState {
-  double dn, fn;
+// Force-law history of an entry (fn, ft, mom); its normal distance dn is
+// kept apart (edn) because every step rewrites dn of every listed entry while
+// only the active ones touch this record.
+struct EState {
+  double fn;
   double ftx, fty, ftz;
   double momx, momy, momz;
 };
-static_assert(sizeof(EState) == 64, "EState is 64 bytes");
+static_assert(sizeof(EState) == 56, "EState is 56 bytes");
 
 struct EGeom {
   double nx, ny, nz;
@@ -130,6 +135,7 @@ struct Dev {
   int32_t* ej;
   uint32_t* esub;
   EState* est;
+  double* edn;
   EGeom* egeo;
   double* edamp;
   int32_t* estick;

@@ -199,6 +199,7 @@ void assemble_dev(rk_engine* eng) {
   d.ej = E.ej;
   d.esub = E.esub;
   d.est = E.est;
+  d.edn = E.edn;
   d.egeo = E.egeo;
   d.edamp = E.edamp;
   d.estick = E.estick;
@@ -444,6 +445,8 @@ int upload_model(rk_engine* eng) {
     std::vector<int32_t> ei(n), ej(n), st(n);
     std::vector<uint32_t> sub(n);
     std::vector<EState> es(n);
+    std::vector<double> dn(n);
+    std::vector<uint8_t> act(n);
     std::vector<EGeom> eg(n);
     std::vector<double> dp(n);
     for (int k = 0; k < n; k++) {
@@ -453,7 +456,8 @@ int upload_model(rk_engine* eng) {
       sub[k] = pack_sub(I.type, I.isub, I.jsub);
       st[k] = I.stick;
       EState s;
-      s.dn = I.dn;
+      dn[k] = I.dn;
+      act[k] = ((I.dn < 0.0 || I.stick >= 0) ? 1 : 0) | 2;
       s.fn = I.fn;
       s.ftx = I.ft.x;
       s.fty = I.ft.y;
@@ -480,6 +484,8 @@ int upload_model(rk_engine* eng) {
     HIPCHK(hipMemcpyAsync(E.esub, sub.data(), 4 * n, hipMemcpyHostToDevice, eng->stream));
     HIPCHK(hipMemcpyAsync(E.estick, st.data(), 4 * n, hipMemcpyHostToDevice, eng->stream));
     HIPCHK(hipMemcpyAsync(E.est, es.data(), sizeof(EState) * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.edn, dn.data(), sizeof(double) * n, hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(E.eact, act.data(), n, hipMemcpyHostToDevice, eng->stream));
     HIPCHK(hipMemcpyAsync(E.egeo, eg.data(), sizeof(EGeom) * n, hipMemcpyHostToDevice, eng->stream));
     HIPCHK(hipMemcpyAsync(E.edamp, dp.data(), sizeof(double) * n, hipMemcpyHostToDevice, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
@@ -656,12 +662,14 @@ int download_interactions(rk_engine* eng, int active_only, std::vector<Inter>& o
   HIPCHK(hipMemcpy(sub.data(), E.esub, 4 * n, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(st.data(), E.estick, 4 * n, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(es.data(), E.est, sizeof(EState) * n, hipMemcpyDeviceToHost));
+  std::vector<double> dn(n);
+  HIPCHK(hipMemcpy(dn.data(), E.edn, sizeof(double) * n, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(eg.data(), E.egeo, sizeof(EGeom) * n, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(dp.data(), E.edamp, 8 * n, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(act.data(), E.eact, n, hipMemcpyDeviceToHost));
   out.reserve(n);
   for (int k = 0; k < n; k++) {
-    if (active_only && !act[k]) continue;
+    if (active_only && !(act[k] & 1)) continue;
     Inter I;
     I.i = ei[k];
     I.j = ej[k];
@@ -672,7 +680,7 @@ int download_interactions(rk_engine* eng, int active_only, std::vector<Inter>& o
     I.n = V(eg[k].nx, eg[k].ny, eg[k].nz);
     I.pos = V(eg[k].px, eg[k].py, eg[k].pz);
     I.vel = V(eg[k].vx, eg[k].vy, eg[k].vz);
-    I.dn = es[k].dn;
+    I.dn = dn[k];
     I.fn = es[k].fn;
     I.ft = V(es[k].ftx, es[k].fty, es[k].ftz);
     I.mom = V(es[k].momx, es[k].momy, es[k].momz);
@@ -998,7 +1006,7 @@ int rk_get_counts(rk_engine* eng, rk_counts* out) {
     std::vector<uint8_t> act(E.n);
     HIPCHK(hipStreamSynchronize(eng->stream));
     HIPCHK(hipMemcpy(act.data(), E.eact, E.n, hipMemcpyDeviceToHost));
-    for (uint8_t a : act) na += a;
+    for (uint8_t a : act) na += a & 1;
   }
   out->nactive = na;
   out->ninterfaces = eng->live_interfaces;

@@ -794,6 +794,7 @@ struct MergeIn {
   const int32_t* oj;
   const uint32_t* osub;
   const EState* oest;
+  const double* oedn;
   const EGeom* ogeo;
   const double* odamp;
   const int32_t* ostick;
@@ -805,6 +806,7 @@ struct MergeOut {
   int32_t* ej;
   uint32_t* esub;
   EState* est;
+  double* edn;
   EGeom* egeo;
   double* edamp;
   int32_t* estick;
@@ -888,10 +890,11 @@ __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
         if (take == 1) {
           out.esub[e] = sb;
           EState s;
-          s.dn = s.fn = 0.0;
+          s.fn = 0.0;
           s.ftx = s.fty = s.ftz = 0.0;
           s.momx = s.momy = s.momz = 0.0;
           out.est[e] = s;
+          out.edn[e] = 0.0;
           EGeom q;
           q.nx = q.ny = q.nz = q.px = q.py = q.pz = q.vx = q.vy = q.vz = 0.0;
           out.egeo[e] = q;
@@ -901,13 +904,14 @@ __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
         } else {
           // a kept entry stays active as it was at the last accelerations()
           const int o = os + a;
-          const EState st = in.oest[o];
+          const double dn = in.oedn[o];
           out.esub[e] = in.osub[o];
-          out.est[e] = st;
+          out.est[e] = in.oest[o];
+          out.edn[e] = dn;
           out.egeo[e] = in.ogeo[o];
           out.edamp[e] = in.odamp[o];
           out.estick[e] = in.ostick[o];
-          out.eact[e] = (st.dn < 0.0 || in.ostick[o] >= 0) ? 1 : 0;
+          out.eact[e] = ((dn < 0.0 || in.ostick[o] >= 0) ? 1 : 0) | 2;
         }
       }
       cnt++;
@@ -974,6 +978,14 @@ __global__ void k_slot_j(int n, const int32_t* ej, const int32_t* jlist, const i
   const int b = ej[e];
   const int above = e >= istart[b] ? istart[b + 1] - istart[b] : 0;
   spos[2 * e + 1] = soff[b] + (q - jstart[b]) + above;
+}
+// slot activity of a fresh list: the force phase rewrites it only on change
+__global__ void k_slot_act(int n, const int32_t* spos, const uint8_t* eact, uint8_t* sact) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint8_t a = eact[e] & 1;
+  sact[spos[2 * e]] = a;
+  sact[spos[2 * e + 1]] = a;
 }
 __global__ void k_heavy_flag(int nb, const int32_t* soff, int heavy_min, int32_t* flag) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1064,6 +1076,7 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
     hipLaunchKernelGGL(k_slot_i, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ei, E.istart, E.soff, w.i32a, E.spos);
     hipLaunchKernelGGL(k_slot_j, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ej, E.jlist, E.jstart, E.istart, E.soff,
                        E.spos);
+    hipLaunchKernelGGL(k_slot_act, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.spos, E.eact, E.sact);
   }
   RK_CHECK(hipGetLastError());
   return nl_build_heavy(w, nb, E, s);
@@ -1284,6 +1297,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
   in.oj = cur.ej;
   in.osub = cur.esub;
   in.oest = cur.est;
+  in.oedn = cur.edn;
   in.ogeo = cur.egeo;
   in.odamp = cur.edamp;
   in.ostick = cur.estick;
@@ -1305,6 +1319,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     out.ej = nxt.ej;
     out.esub = nxt.esub;
     out.est = nxt.est;
+    out.edn = nxt.edn;
     out.egeo = nxt.egeo;
     out.edamp = nxt.edamp;
     out.estick = nxt.estick;
@@ -1324,6 +1339,7 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
   size_t c = need + need / 2 + 1024;
   bool ok = hipMalloc(&E.ei, c * sizeof(int32_t)) == hipSuccess && hipMalloc(&E.ej, c * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&E.esub, c * sizeof(uint32_t)) == hipSuccess && hipMalloc(&E.est, c * sizeof(EState)) == hipSuccess &&
+            hipMalloc(&E.edn, c * sizeof(double)) == hipSuccess &&
             hipMalloc(&E.egeo, c * sizeof(EGeom)) == hipSuccess && hipMalloc(&E.edamp, c * sizeof(double)) == hipSuccess &&
             hipMalloc(&E.estick, c * sizeof(int32_t)) == hipSuccess && hipMalloc(&E.eact, c * sizeof(uint8_t)) == hipSuccess &&
             hipMalloc(&E.spos, c * 2 * sizeof(int32_t)) == hipSuccess &&
@@ -1341,12 +1357,13 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
 }
 
 void entries_free(EntryArrays& E) {
-  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.egeo, E.edamp, E.estick, E.eact, E.spos, E.sact, E.sval, E.akin, E.acount};
+  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.edn, E.egeo, E.edamp, E.estick, E.eact, E.spos, E.sact, E.sval, E.akin, E.acount};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   E.ei = E.ej = nullptr;
   E.esub = nullptr;
   E.est = nullptr;
+  E.edn = nullptr;
   E.egeo = nullptr;
   E.edamp = nullptr;
   E.estick = nullptr;

@@ -18,6 +18,7 @@ struct EntryArrays {
   int32_t* ej = nullptr;
   uint32_t* esub = nullptr;
   EState* est = nullptr;
+  double* edn = nullptr;
   EGeom* egeo = nullptr;
   double* edamp = nullptr;
   int32_t* estick = nullptr;

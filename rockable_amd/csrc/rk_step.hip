@@ -357,10 +357,10 @@ __global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
     const int type = (int)(sub >> 30), isub = (int)((sub >> 15) & 0x7fffu), jsub = (int)(sub & 0x7fffu);
     const BodyKin& Pi = d.kin[i];
     const BodyKin& Pj = d.kin[j];
-    const int stick = d.estick[e];
+    const int stick = d.ifc.n > 0 ? d.estick[e] : -1;  // bonds exist only with interfaces
     Kin k;
     k.deact = false;
-    k.prev_dn = d.est[e].dn;
+    k.prev_dn = d.edn[e];
     if (stick >= 0) {  // a bond may run on stale geometry after a deactivation
       const EGeom& g = d.egeo[e];
       k.n = V(g.nx, g.ny, g.nz);
@@ -385,15 +385,22 @@ __global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
       a.e = e;
       a.deact = k.deact ? 1 : 0;
     } else {
+      // activity byte: bit 0 active, bit 1 history maybe non-zero; the
+      // history and the slot flags are written only when they change
+      const uint8_t old = d.eact[e];
+      uint8_t now = old & 2;
       if (k.deact) {
-        EState o;
-        o.dn = 0.0;
-        o.fn = 0.0;
-        o.ftx = o.fty = o.ftz = 0.0;
-        o.momx = o.momy = o.momz = 0.0;
-        d.est[e] = o;
+        if (old & 2) {
+          EState o;
+          o.fn = 0.0;
+          o.ftx = o.fty = o.ftz = 0.0;
+          o.momx = o.momy = o.momz = 0.0;
+          d.est[e] = o;
+          now = 0;
+        }
+        d.edn[e] = 0.0;
       } else {
-        d.est[e].dn = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
+        d.edn[e] = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
         if (GEOM_ALL) {
           const v3 vel = rel_vel(k.pos, Pi, Pj);
           EGeom g;
@@ -409,10 +416,14 @@ __global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
           d.egeo[e] = g;
         }
       }
-      const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
-      d.eact[e] = 0;
-      d.sact[slots.x] = 0;
-      d.sact[slots.y] = 0;
+      if (now != old) {
+        d.eact[e] = now;
+        if (old & 1) {
+          const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
+          d.sact[slots.x] = 0;
+          d.sact[slots.y] = 0;
+        }
+      }
     }
   }
   const unsigned long long m = __ballot(act);
@@ -468,7 +479,6 @@ __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_
     else
       law_default(d, sp, k, damp, Pi, Pj, h);
     EState o;
-    o.dn = k.dn;
     o.fn = h.fn;
     o.ftx = h.ft.x;
     o.fty = h.ft.y;
@@ -477,6 +487,7 @@ __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_
     o.momy = h.mom.y;
     o.momz = h.mom.z;
     d.est[e] = o;
+    d.edn[e] = k.dn;
     // incrementResultants (Rockable.cpp:2756-2764), both sides
     v3 f = vadd(vsc(h.fn, k.n), h.ft);
     v3 Ci = vsub(k.pos, kpos(Pi));
@@ -507,9 +518,14 @@ __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_
       g.vz = k.vel.z;
       d.egeo[e] = g;
     }
-    d.eact[e] = 1;
-    d.sact[slots.x] = 1;
-    d.sact[slots.y] = 1;
+    const uint8_t old = d.eact[e];
+    if (old != 3) {
+      d.eact[e] = 3;
+      if (!(old & 1)) {
+        d.sact[slots.x] = 1;
+        d.sact[slots.y] = 1;
+      }
+    }
   }
 }
 
@@ -996,7 +1012,7 @@ __global__ void k_breakage(Dev d) {
     if (e < 0) continue;
     d.estick[e] = -1;
     EState s = d.est[e];
-    if (s.dn >= 0.0) {
+    if (d.edn[e] >= 0.0) {
       s.fn = 0.0;
       s.ftx = s.fty = s.ftz = 0.0;
     } else {
