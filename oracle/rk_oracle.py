@@ -135,6 +135,18 @@ class Oracle:
         return int(self.L.rko_nactive(self.s))
 
     @property
+    def ninterfaces(self):
+        return int(self.L.rko_ninterfaces(self.s))
+
+    @property
+    def nl_updates(self):
+        return int(self.L.rko_nl_updates(self.s))
+
+    @property
+    def total_broken_area(self):
+        return float(self.L.rko_total_broken_area(self.s))
+
+    @property
     def time(self):
         return float(self.L.rko_time(self.s))
 

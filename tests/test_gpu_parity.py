@@ -192,3 +192,85 @@ def test_heavy_passive_walls_tree_sum(tmp_path):
         lever = 0.1  # box size bound on the moment arms (m)
         assert np.all(np.abs(g["moment"][:nw] - r["moment"][:nw]) <= 1e-12 * lever * sc.sum(axis=1)[:, None] + 1e-300)
     assert np.abs(r["force"][0]).max() > 0.0
+
+
+def _kat_case(name, d):
+    """(conf path, driving file or None, steps, recompute accelerations after
+    the controls) of the known-answer decks of test_oracle_laws.py."""
+    import math
+    from tests import kat_decks as K
+    if name == "avalanche_restitution":
+        return K.two_spheres(d, 0.5, (-0.501, 0, 0), (0.501, 0, 0), v1=(0.1, 0, 0), v2=(-0.1, 0, 0),
+                             law="Avalanche", en2=0.25), None, 60000, False
+    if name == "avalanche_friction":
+        return K.two_spheres(d, 0.5, (0, 0, 0), (0.9999, 0, 0), v2=(0, 1e-2, 0), mu=0.4, law="Avalanche",
+                             density=1e15), None, 3000, False
+    if name in ("bond_breaks", "bond_holds"):
+        w = math.sqrt(1e5 / (0.5 * K.sphere_mass(0.5)))
+        v = (1.5 if name == "bond_breaks" else 0.5) * 1e-4 * w
+        return K.bonded_spheres(d, 0.5, v=v, kn=1e5, fn0=10.0), None, 150000, False
+    if name == "attracting_point":
+        return K.single_sphere(d, pos=(3.0, 0, 0), body_force="AttractingPoint 0 0 0 2.0"), None, 5000, False
+    if name == "viscous_fluid":
+        return K.single_sphere(d, gravity=(0, 0, -9.81), body_force="ViscousFluid 1000.0"), None, 5000, False
+    if name == "controls":
+        path, drv = K.driven_plate(d, ["Control _x_Vel_ 0 0.5", "Control _z_For_ 0 3000.0"])
+        return path, drv, 4000, True
+    if name == "dynamic_nl":
+        return K.single_sphere(d, vel=(0.3, 0, 0), dynamic=(0.01, 45.0)), None, 2000, False
+    raise ValueError(name)
+
+
+@pytest.mark.parametrize("name", ["avalanche_restitution", "avalanche_friction", "bond_breaks", "bond_holds",
+                                  "attracting_point", "viscous_fluid", "controls", "dynamic_nl"])
+def test_known_answer_decks_bit_exact(tmp_path, name):
+    """The decks that pin the Avalanche and StickedLinks laws, breakage, the
+    body forces, the driving controls and dynamicUpdateNL against physics:
+    the GPU engine is bit-identical to the oracle on them, including the
+    count of live interfaces and of list rebuilds."""
+    from oracle.rk_oracle import Oracle
+    path, drv, steps, reacc = _kat_case(name, str(tmp_path))
+    e = _engine(path, driving=drv)
+    o = Oracle(path, driving=drv)
+    if reacc:
+        e.accelerations()
+        o.accelerations()
+    e.update_nl()
+    o.update_nl()
+    chunk = max(1, steps // 4)
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        e.step(n)
+        o.run_steps(n)
+        done += n
+        assert_bodies_equal(e.bodies(), o.bodies())
+        assert_interactions_equal(e.interactions(), o.interactions(),
+                                  fields=("dn", "fn", "ft", "mom", "damp", "stick"))
+        c = e.counts()
+        assert c["ninterfaces"] == o.ninterfaces
+        assert c["nl_updates"] == o.nl_updates
+    if name == "bond_breaks":
+        assert o.ninterfaces == 0
+    if name == "bond_holds":
+        assert o.ninterfaces == 1
+
+
+def test_linkcells_equals_bruteforce_on_gpu(tmp_path):
+    """The same 300-hull deck with linkCells and with bruteForce gives
+    bit-identical lists and bodies on the GPU after 1000 steps (rebuilds
+    every 20), and both equal the oracle."""
+    from oracle.rk_oracle import Oracle
+    from tests import kat_decks as K
+    cells, brute = K.hull_pair_decks(str(tmp_path))
+    a, b, o = _engine(cells), _engine(brute), Oracle(cells)
+    for x in (a, b, o):
+        x.update_nl()
+    a.step(1000)
+    b.step(1000)
+    o.run_steps(1000)
+    assert_bodies_equal(a.bodies(), b.bodies())
+    assert_bodies_equal(a.bodies(), o.bodies())
+    assert_interactions_equal(a.interactions(), b.interactions())
+    assert_interactions_equal(a.interactions(), o.interactions(), fields=("dn", "fn", "ft", "mom", "damp", "stick"))
+    assert len(o.interactions()) > 100
