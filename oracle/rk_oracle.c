@@ -2406,8 +2406,10 @@ int rko_save_conf(rko_sim* s, const char* path) {
 /* ------------------------------------------------------------------------- */
 /* integrate() (Rockable.cpp:2538-2746)                                       */
 /* ------------------------------------------------------------------------- */
-static void integrate_prepare(rko_sim* s) {
-  if (s->dynamicUpdateNL != 0) {
+/* reset: the start of integrate(), which zeroes the dynamicCheckUpdateNL
+   accumulations; a continued run (rko_run_steps) keeps them. */
+static void integrate_prepare(rko_sim* s, int reset) {
+  if (s->dynamicUpdateNL != 0 && (reset || !s->deltaPos)) {
     free(s->deltaPos);
     free(s->deltaQ);
     s->deltaPos = (v3*)calloc((size_t)(s->nb > 0 ? s->nb : 1), sizeof(v3));
@@ -2448,7 +2450,7 @@ static int loop_body(rko_sim* s, const char* outdir) {
 }
 
 int64_t rko_integrate(rko_sim* s, const char* outdir, int64_t max_steps) {
-  integrate_prepare(s);
+  integrate_prepare(s, 1);
   if (outdir) {
     char fn[1024];
     snprintf(fn, sizeof fn, "%s/conf%d", outdir, s->iconf);
@@ -2463,7 +2465,7 @@ int64_t rko_integrate(rko_sim* s, const char* outdir, int64_t max_steps) {
 }
 
 int64_t rko_run_steps(rko_sim* s, int64_t nsteps) {
-  integrate_prepare(s);
+  integrate_prepare(s, 0);
   for (int64_t n = 0; n < nsteps; n++)
     if (loop_body(s, NULL) != 0) return -1;
   return nsteps;

@@ -72,6 +72,21 @@ struct alignas(16) ActiveKin {
   int32_t e, deact;
 };
 
+// dynamicCheckUpdateNL on the device (Rockable.cpp:1571-1596).  The running
+// maxima are kept as order-preserving integer keys so that atomicMax / atomicMin
+// (integer, order-independent) give the serial loop's result exactly:
+// maxDeltaPos = sqrt(max |dx|^2), and maxDeltaRot = 2 acos(min w) is compared
+// through w <= wstar, wstar being the largest w with 2 acos(w) >= angleUpdateNL.
+struct DevGuard {
+  unsigned long long maxpos2;  // bits of max |dx|^2 (a non-negative double)
+  unsigned long long minw;     // order key of min w of the accumulated dQ
+  int reset;                   // the accumulations restart at the next step
+  int nu;                      // needUpdate set by the last check
+  int frozen;                  // a rebuild is due: later step kernels do nothing
+  int pad;
+  long long frozen_step;       // the step after which the rebuild is due
+};
+
 struct DevControl {
   int32_t type, i;
   double value;
@@ -154,6 +169,12 @@ struct Dev {
   // (a driven body without force or moment controls)
   const uint8_t* passive;
   int exact_sums;         // 1: serial sums for passive bodies too
+  // device displacement guard (dynamicUpdateNL)
+  int guard_on;
+  DevGuard* guard;
+  double* gdpos;          // 3 per body
+  double* gdq;            // 4 per body
+  double disp, wstar;
   // active entries of the current force phase: geometry block b's in
   // akin[b * 256, b * 256 + acount[b])
   ActiveKin* akin;
@@ -190,5 +211,6 @@ void launch_glob(const Dev& d, hipStream_t s);
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s);
 void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s);
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);
+void launch_guard(const Dev& d, const StepParams& sp, long long step, hipStream_t s);
 
 }  // namespace rk

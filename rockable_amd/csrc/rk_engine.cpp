@@ -67,6 +67,11 @@ struct rk_engine {
   int32_t *bvoff = nullptr, *bvbody = nullptr;
   int nvtot = 0;
   bool gv_dirty = true;  // body poses changed since the last k_glob
+  // device displacement guard (dynamicUpdateNL)
+  DevGuard* guard = nullptr;
+  double *gdpos = nullptr, *gdq = nullptr;
+  double wstar = -2.0;
+  int guard_batch = 16;  // steps enqueued between two reads of the guard
   int exact_sums = 0;
   EntryArrays E[2];
   int cur = 0;
@@ -76,9 +81,6 @@ struct rk_engine {
   bool needUpdate = false;
   int64_t steps = 0, nl_updates = 0;
   int live_interfaces = 0;
-  std::vector<v3> deltaPos;
-  std::vector<qt> deltaQ;
-  double maxDeltaPos = 1.0, maxDeltaRot = 1.0;
 };
 
 namespace {
@@ -111,7 +113,7 @@ void free_device(rk_engine* eng) {
                 eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->ptab,      eng->damp_rate, eng->damp_kind,
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
                 eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
-                eng->gv,       eng->bvoff,     eng->bvbody};
+                eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   eng->kin = nullptr;
@@ -132,6 +134,8 @@ void free_device(rk_engine* eng) {
   eng->passive = nullptr;
   eng->gv = nullptr;
   eng->bvoff = eng->bvbody = nullptr;
+  eng->guard = nullptr;
+  eng->gdpos = eng->gdq = nullptr;
   eng->nvtot = 0;
   for (auto& E : eng->E) {
     entries_free(E);
@@ -213,6 +217,12 @@ void assemble_dev(rk_engine* eng) {
   d.heavy_min = eng->w.heavy_min;
   d.passive = eng->passive;
   d.akin = E.akin;
+  d.guard_on = eng->guard != nullptr;
+  d.guard = eng->guard;
+  d.gdpos = eng->gdpos;
+  d.gdq = eng->gdq;
+  d.disp = eng->m.dispUpdateNL;
+  d.wstar = eng->wstar;
   d.acount = E.acount;
   d.exact_sums = eng->exact_sums;
   d.istart = E.istart;
@@ -262,6 +272,35 @@ StepParams step_params(const rk_engine* eng) {
   sp.gz = eng->m.gravity.z;
   sp.law = eng->m.law;
   return sp;
+}
+
+// The largest w in [-1, 1] with 2 acos(w) >= angle (glibc acos, as the
+// oracle's get_angle); 2 acos is non-increasing in w, so the device compares
+// the minimum w of the accumulated rotations with it.  -2 when no w qualifies.
+double wstar_for(double angle) {
+  auto ok = [&](double w) { return 2.0 * acos(w) >= angle; };
+  if (ok(1.0)) return 1.0;
+  if (!ok(-1.0)) return -2.0;
+  auto key = [](double w) {
+    uint64_t u;
+    memcpy(&u, &w, 8);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+  };
+  auto unkey = [](uint64_t k) {
+    uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double w;
+    memcpy(&w, &u, 8);
+    return w;
+  };
+  uint64_t lo = key(-1.0), hi = key(1.0);  // ok(lo), !ok(hi)
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (ok(unkey(mid)))
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return unkey(lo);
 }
 
 // upload everything the model holds (after a conf load)
@@ -367,6 +406,24 @@ int upload_model(rk_engine* eng) {
         upload(eng, eng->gv, std::vector<double>(3 * (size_t)std::max(1, eng->nvtot), 0.0)))
       return -1;
     eng->gv_dirty = true;
+  }
+  // device displacement guard
+  if (m.dynamicUpdateNL != 0) {
+    std::vector<double> q(4 * (size_t)std::max(1, nb), 0.0);
+    for (int b = 0; b < nb; b++) q[4 * (size_t)b] = 1.0;
+    if (upload(eng, eng->gdpos, std::vector<double>(3 * (size_t)std::max(1, nb), 0.0)) || upload(eng, eng->gdq, q))
+      return -1;
+    DevGuard g0;
+    memset(&g0, 0, sizeof g0);
+    g0.minw = ~0ull;
+    if (upload(eng, eng->guard, std::vector<DevGuard>(1, g0))) return -1;
+    eng->wstar = wstar_for(m.angleUpdateNL);
+  } else {
+    void* ps[] = {eng->guard, eng->gdpos, eng->gdq};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    eng->guard = nullptr;
+    eng->gdpos = eng->gdq = nullptr;
   }
   // parameter tables (compact G = ngroup)
   const int G = std::max(1, m.ngroup);
@@ -705,47 +762,23 @@ int check_loaded(rk_engine* eng) {
   return 0;
 }
 
-// dynamicCheckUpdateNL (Rockable.cpp:1571-1596) on downloaded velocities
-int dynamic_check(rk_engine* eng) {
+// reset: the start of integrate(), which zeroes the dynamicCheckUpdateNL
+// accumulations (Rockable.cpp:2552-2560); a continued run keeps them.
+int integrate_prepare(rk_engine* eng, bool reset) {
   Model& m = eng->m;
-  if (download_bodies(eng) != 0) return -1;
-  const int nb = (int)m.bodies.size();
-  for (int i = 0; i < nb; i++) {
-    eng->deltaPos[i] = vadd(eng->deltaPos[i], vsc(m.dt, m.bodies[i].vel));
-    double tmp1 = vnorm2(eng->deltaPos[i]);
-    if (tmp1 > eng->maxDeltaPos * eng->maxDeltaPos) eng->maxDeltaPos = sqrt(tmp1);
-    qt dq = qdot(m.bodies[i].Q, m.bodies[i].vrot);
-    eng->deltaQ[i].w += dq.w * m.dt;
-    eng->deltaQ[i].x += dq.x * m.dt;
-    eng->deltaQ[i].y += dq.y * m.dt;
-    eng->deltaQ[i].z += dq.z * m.dt;
-    double w = std::max(-1.0, std::min(1.0, eng->deltaQ[i].w));
-    double tmp2 = 2.0 * acos(w);
-    if (tmp2 > eng->maxDeltaRot) eng->maxDeltaRot = tmp2;
-  }
-  if (eng->maxDeltaPos >= m.dispUpdateNL || eng->maxDeltaRot >= m.angleUpdateNL) {
-    eng->needUpdate = true;
-    for (int i = 0; i < nb; i++) {
-      eng->deltaPos[i] = V(0, 0, 0);
-      eng->deltaQ[i] = qt{1.0, 0.0, 0.0, 0.0};
-    }
-    eng->maxDeltaPos = 0.0;
-    eng->maxDeltaRot = 0.0;
-  }
-  return 0;
-}
-
-void integrate_prepare(rk_engine* eng) {
-  Model& m = eng->m;
-  if (m.dynamicUpdateNL != 0) {
-    eng->deltaPos.assign(m.bodies.size(), V(0, 0, 0));
-    eng->deltaQ.assign(m.bodies.size(), qt{1.0, 0.0, 0.0, 0.0});
-    eng->maxDeltaPos = 0.0;
-    eng->maxDeltaRot = 0.0;
-  }
   m.dt_2 = 0.5 * m.dt;
   m.dt2_2 = m.dt_2 * m.dt;
   m.dt2 = m.dt * m.dt;
+  if (reset && eng->guard) {
+    // the accumulations restart at the next step, the maxima are cleared
+    DevGuard g;
+    HIPCHK(hipMemcpy(&g, eng->guard, sizeof g, hipMemcpyDeviceToHost));
+    g.reset = 1;
+    g.maxpos2 = 0ull;
+    g.minw = ~0ull;
+    HIPCHK(hipMemcpy(eng->guard, &g, sizeof g, hipMemcpyHostToDevice));
+  }
+  return 0;
 }
 
 // one iteration of the integrate() loop (Rockable.cpp:2584-2745)
@@ -773,8 +806,92 @@ int loop_body(rk_engine* eng, const char* outdir) {
     else
       eng->needUpdate = false;
   }
-  if (m.dynamicUpdateNL != 0)
-    if (dynamic_check(eng) != 0) return -1;
+  return 0;
+}
+
+// The loop scalars of one iteration (the host keeps them; the step itself
+// runs on the device).
+struct LoopScalars {
+  double t, interConfC, interVerletC;
+  int iconf;
+  int64_t steps;
+};
+LoopScalars loop_scalars(const rk_engine* eng) {
+  return {eng->m.t, eng->m.interConfC, eng->m.interVerletC, eng->m.iconf, eng->steps};
+}
+void set_loop_scalars(rk_engine* eng, const LoopScalars& s) {
+  eng->m.t = s.t;
+  eng->m.interConfC = s.interConfC;
+  eng->m.interVerletC = s.interVerletC;
+  eng->m.iconf = s.iconf;
+  eng->steps = s.steps;
+}
+
+// The integrate() loop with dynamicUpdateNL, its displacement check on the
+// device.  Iterations are enqueued in batches without host synchronisation;
+// a batch ends where the host has work (a conf file, a scheduled rebuild,
+// bond breakage accounting) or after guard_batch iterations.  When the device
+// check finds a rebuild due after iteration s, the step kernels of the later
+// iterations do nothing: the host reads the guard once per batch, keeps the
+// loop scalars of iteration s, rebuilds and continues.  The schedule of
+// rebuilds is exactly the reference's.
+int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tmax, int64_t* ndone) {
+  Model& m = eng->m;
+  int64_t done = 0;
+  std::vector<LoopScalars> hist;
+  std::vector<char> conf_at;
+  while (done < nmax && (!until_tmax || m.t <= m.tmax)) {
+    const bool bonds = m.law == LAW_STICKED && eng->live_interfaces > 0;
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(bonds ? 1 : eng->guard_batch, nmax - done));
+    hist.assign(1, loop_scalars(eng));
+    conf_at.assign(1, 0);
+    const StepParams sp0 = step_params(eng);
+    int64_t k = 0;
+    while (k < B) {
+      m.t += m.dt;
+      m.interConfC += m.dt;
+      m.interVerletC += m.dt;
+      const bool conf_due = m.interConfC >= m.interConf - m.dt_2;
+      if (vv_step(eng, (conf_due && outdir) ? 1 : 0) != 0) return -1;
+      launch_guard(eng->d, sp0, (long long)eng->steps, eng->stream);
+      eng->steps++;
+      if (conf_due) {
+        m.iconf++;
+        m.interConfC = 0.0;
+      }
+      hist.push_back(loop_scalars(eng));
+      conf_at.push_back(conf_due ? 1 : 0);
+      k++;
+      if ((conf_due && outdir) || eng->needUpdate || m.interVerletC >= m.interVerlet - m.dt_2 ||
+          (until_tmax && m.t > m.tmax))
+        break;
+    }
+    DevGuard g;
+    HIPCHK(hipMemcpyAsync(&g, eng->guard, sizeof g, hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    int64_t executed = k;
+    bool need = eng->needUpdate;
+    if (g.frozen) {
+      executed = g.frozen_step - hist[0].steps + 1;
+      if (executed < 1 || executed > k) return fail(eng, "device guard: inconsistent frozen step");
+      set_loop_scalars(eng, hist[executed]);
+      need = true;
+      const int zero = 0;
+      HIPCHK(hipMemcpy(&eng->guard->frozen, &zero, sizeof zero, hipMemcpyHostToDevice));
+    }
+    if (conf_at[executed] && outdir) {
+      char fn[4096];
+      snprintf(fn, sizeof fn, "%s/conf%d", outdir, m.iconf);
+      if (save_conf_now(eng, fn) != 0) return -1;
+    }
+    if (need || m.interVerletC >= m.interVerlet - m.dt_2) {
+      if (do_update_nl(eng) != 0) return -1;
+      if (!need) m.interVerletC = 0.0;
+      eng->needUpdate = false;
+    }
+    done += executed;
+  }
+  if (ndone) *ndone = done;
   return 0;
 }
 
@@ -928,6 +1045,9 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     // the conf keyword "gravity" (Rockable.cpp:978-980), changed between steps
     if (check_loaded(eng)) return -1;
     (n == "gravity_x" ? eng->m.gravity.x : n == "gravity_y" ? eng->m.gravity.y : eng->m.gravity.z) = value;
+  } else if (n == "guard_batch") {
+    if (!(value >= 1.0 && value <= 1e6)) return fail(eng, "guard_batch must be in [1, 1e6]");
+    eng->guard_batch = (int)value;
   } else if (n == "heavy_min") {
     if (!(value >= 0.0 && value < 2.0e9)) return fail(eng, "heavy_min must be in [0, 2e9)");
     eng->w.heavy_min = (int)value;
@@ -959,7 +1079,8 @@ int rk_accelerations(rk_engine* eng) {
 int rk_step(rk_engine* eng, int64_t nsteps) {
   if (check_loaded(eng)) return -1;
   (void)hipSetDevice(eng->device);
-  integrate_prepare(eng);
+  if (integrate_prepare(eng, false) != 0) return -1;
+  if (eng->guard) return run_guarded(eng, nullptr, nsteps, false, nullptr);
   for (int64_t k = 0; k < nsteps; k++)
     if (loop_body(eng, nullptr) != 0) return -1;
   return 0;
@@ -968,16 +1089,20 @@ int rk_step(rk_engine* eng, int64_t nsteps) {
 int rk_integrate(rk_engine* eng, const char* outdir, int64_t max_steps, int64_t* steps_done) {
   if (check_loaded(eng)) return -1;
   (void)hipSetDevice(eng->device);
-  integrate_prepare(eng);
+  if (integrate_prepare(eng, true) != 0) return -1;
   if (outdir) {
     char fn[4096];
     snprintf(fn, sizeof fn, "%s/conf%d", outdir, eng->m.iconf);
     if (save_conf_now(eng, fn) != 0) return -1;
   }
   int64_t n = 0;
-  while (eng->m.t <= eng->m.tmax && (max_steps < 0 || n < max_steps)) {
-    if (loop_body(eng, outdir) != 0) return -1;
-    n++;
+  if (eng->guard) {
+    if (run_guarded(eng, outdir, max_steps < 0 ? INT64_MAX : max_steps, true, &n) != 0) return -1;
+  } else {
+    while (eng->m.t <= eng->m.tmax && (max_steps < 0 || n < max_steps)) {
+      if (loop_body(eng, outdir) != 0) return -1;
+      n++;
+    }
   }
   if (steps_done) *steps_done = n;
   HIPCHK(hipStreamSynchronize(eng->stream));

@@ -50,6 +50,12 @@ __device__ __forceinline__ void st3(double* p, v3 v) {
   p[2] = v.z;
 }
 
+// A rebuild found due by the device guard suspends the step kernels until
+// the host has rebuilt the list (every thread reads the same flag).
+__device__ __forceinline__ bool frozen(const Dev& d) {
+  return d.guard_on && *(volatile const int*)&d.guard->frozen;
+}
+
 // Particle::GlobVertex (Particle.cpp:55-57): pos + Q * (homothety * vertex)
 __device__ __forceinline__ v3 glob_vertex(const DevShapes& sh, const BodyKin& b, int v) {
   const double* p = sh.vtx + 3 * (sh.voff[b.shape] + v);
@@ -347,6 +353,7 @@ __device__ void law_sticked(const Dev& d, const StepParams& sp, const Kin& k, do
 template <bool GEOM_ALL>
 __global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
   __shared__ int wcount[kBlock / 64];
+  if (frozen(d)) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   bool act = false;
@@ -440,6 +447,7 @@ __global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
 
 template <int LAW>
 __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_geom_all, int pii, int ngeom) {
+  if (frozen(d)) return;
   const int g = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);  // geometry block
   if (g >= ngeom) return;
   const int lane = threadIdx.x & 63;
@@ -704,6 +712,7 @@ constexpr int kLanesPerBody = 6;
 constexpr int kBodiesPerWave = 10;
 
 __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
+  if (frozen(d)) return;
   const int lane = threadIdx.x & 63;
   const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
   const int slot = lane / kLanesPerBody;
@@ -762,6 +771,7 @@ constexpr int kHeavyChunk = 512;
 
 __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, int kick) {
   __shared__ double lds[2 * kHeavyChunk * 6];
+  if (frozen(d)) return;
   const int b = d.heavy[blockIdx.x];
   const int t = threadIdx.x;
   const int k0 = d.soff[b], k1 = d.soff[b + 1];
@@ -832,9 +842,100 @@ __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, i
   }
 }
 
+// dynamicCheckUpdateNL (Rockable.cpp:1571-1596), accumulation: deltaPos +=
+// vel dt, deltaQ += Q.dot(vrot) dt, then the running max of |deltaPos|^2 and
+// min of deltaQ.w (integer atomics on order keys; NaNs are ignored as the
+// serial comparisons ignore them).
+__device__ __forceinline__ unsigned long long wkey(double w) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(w);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__global__ void __launch_bounds__(kBlock) k_guard_acc(Dev d, StepParams sp) {
+  __shared__ unsigned long long rp[kBlock / 64], rw[kBlock / 64];
+  if (frozen(d)) return;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool r = *(volatile const int*)&d.guard->reset != 0;
+  unsigned long long kp = 0ull, kw = ~0ull;
+  if (b < d.nb) {
+    const BodyKin& P = d.kin[b];
+    v3 dp = r ? V(0.0, 0.0, 0.0) : ld3(d.gdpos + 3 * (size_t)b);
+    dp = vadd(dp, vsc(sp.dt, kvel(P)));
+    st3(d.gdpos + 3 * (size_t)b, dp);
+    const double tmp1 = vnorm2(dp);
+    if (tmp1 == tmp1) kp = (unsigned long long)__double_as_longlong(tmp1);
+    const qt dq = qdot(kq(P), kvrot(P));
+    double* g = d.gdq + 4 * (size_t)b;
+    qt q;
+    if (r) {
+      q.w = 1.0;
+      q.x = q.y = q.z = 0.0;
+    } else {
+      q.w = g[0];
+      q.x = g[1];
+      q.y = g[2];
+      q.z = g[3];
+    }
+    q.w += dq.w * sp.dt;
+    q.x += dq.x * sp.dt;
+    q.y += dq.y * sp.dt;
+    q.z += dq.z * sp.dt;
+    g[0] = q.w;
+    g[1] = q.x;
+    g[2] = q.y;
+    g[3] = q.z;
+    double w = q.w;
+    if (w > 1.0) w = 1.0;
+    if (w < -1.0) w = -1.0;
+    if (w == w) kw = wkey(w);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long op = __shfl_xor(kp, off, 64), ow = __shfl_xor(kw, off, 64);
+    kp = op > kp ? op : kp;
+    kw = ow < kw ? ow : kw;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    rp[wave] = kp;
+    rw[wave] = kw;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / 64; k++) {
+      kp = rp[k] > kp ? rp[k] : kp;
+      kw = rw[k] < kw ? rw[k] : kw;
+    }
+    if (kp) atomicMax(&d.guard->maxpos2, kp);
+    if (kw != ~0ull) atomicMin(&d.guard->minw, kw);
+  }
+}
+
+// the end of a step of the integrate() loop (Rockable.cpp:2690-2740): a
+// needUpdate left by the previous check makes the rebuild due now (the step
+// kernels then wait for the host); then the check itself
+__global__ void k_guard_check(Dev d, long long step) {
+  if (threadIdx.x != 0 || frozen(d)) return;
+  DevGuard* G = d.guard;
+  const int rebuild_now = G->nu;
+  G->nu = 0;
+  G->reset = 0;
+  const double mp = sqrt(__longlong_as_double((long long)G->maxpos2));
+  const bool rot = G->minw != ~0ull && G->minw <= wkey(d.wstar);
+  if (mp >= d.disp || rot) {
+    G->nu = 1;
+    G->reset = 1;
+    G->maxpos2 = 0ull;
+    G->minw = ~0ull;
+  }
+  if (rebuild_now) {
+    G->frozen = 1;
+    G->frozen_step = step;
+  }
+}
+
 // Particle::GlobVertex (Particle.cpp:55-57) of every vertex of every body,
 // once per step: one thread per vertex.
 __global__ void __launch_bounds__(kBlock) k_glob(Dev d) {
+  if (frozen(d)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.nvtot) return;
   const int b = d.bvbody[k];
@@ -843,6 +944,7 @@ __global__ void __launch_bounds__(kBlock) k_glob(Dev d) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
+  if (frozen(d)) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0 && d.nctrl > 0) {
     // ServoFunction (DrivingSystem.cpp:242-268), then the controlled bodies in
@@ -1069,6 +1171,11 @@ void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) 
   const int blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
   hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(kBlock), 0, s, d, sp, kick);
   if (d.nheavy > 0) hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
+}
+
+void launch_guard(const Dev& d, const StepParams& sp, long long step, hipStream_t s) {
+  if (d.nb > 0) hipLaunchKernelGGL(k_guard_acc, dim3((d.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp);
+  hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(64), 0, s, d, step);
 }
 
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s) {

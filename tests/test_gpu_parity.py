@@ -274,3 +274,61 @@ def test_linkcells_equals_bruteforce_on_gpu(tmp_path):
     assert_interactions_equal(a.interactions(), b.interactions())
     assert_interactions_equal(a.interactions(), o.interactions(), fields=("dn", "fn", "ft", "mom", "damp", "stick"))
     assert len(o.interactions()) > 100
+
+
+def _dynamic_hull_deck(tmp_path, disp=1e-3, angle_deg=15.0):
+    from rockable_amd import decks as D
+    shapes, deck = D.hull_packing_deck(n_bodies=600, lattice=(10, 10, 6), law="Avalanche", vmax=10.0,
+                                       gap_factor=1.01, interverlet_steps=100000, size_range=(5.8e-3, 6e-3))
+    deck.dynamic_update_nl = 1
+    deck.disp_update_nl = disp
+    deck.angle_update_nl = angle_deg
+    return D.write_deck(str(tmp_path), shapes, deck)
+
+
+@pytest.mark.parametrize("batch", [1, 7, 64])
+def test_device_guard_bit_exact(tmp_path, batch):
+    """dynamicUpdateNL with the displacement check on the device and the
+    host reading it once per batch of iterations: every rebuild happens at
+    the reference's iteration (same count as the oracle, whose check runs on
+    the host each iteration) and the state stays bit-identical, for batches
+    of 1, 7 and 64 iterations, across several rk_step calls."""
+    from oracle.rk_oracle import Oracle
+    path = _dynamic_hull_deck(tmp_path)
+    e = _engine(path)
+    e.set_option("guard_batch", batch)
+    o = Oracle(path)
+    e.update_nl()
+    o.update_nl()
+    for n in (150, 350, 500):
+        e.step(n)
+        o.run_steps(n)
+        assert e.counts()["nl_updates"] == o.nl_updates
+        assert_bodies_equal(e.bodies(), o.bodies())
+        assert_interactions_equal(e.interactions(), o.interactions(), fields=("dn", "fn", "ft", "mom", "damp", "stick"))
+    assert o.nl_updates > 10
+
+
+def test_device_guard_integrate_conf_files(tmp_path):
+    """integrate() with dynamicUpdateNL and conf outputs: the conf files
+    written by the engine (batches of 16) equal the oracle's line for line."""
+    import glob
+    from oracle.rk_oracle import Oracle, compare_conf
+    from rockable_amd import decks as D
+    path = _dynamic_hull_deck(tmp_path)
+    path = D.conf_with(path, str(tmp_path / "deck2.conf"), interConf=100 * 2e-7, tmax=700 * 2e-7)
+    ge, go = tmp_path / "gpu", tmp_path / "cpu"
+    ge.mkdir()
+    go.mkdir()
+    e = _engine(path)
+    e.update_nl()
+    n = e.integrate(str(ge))
+    o = Oracle(path)
+    o.update_nl()
+    o.integrate(str(go))
+    files = sorted(glob.glob(str(go / "conf*")))
+    assert n >= 699 and len(files) >= 7
+    for f in files:
+        ok, k, a, b = compare_conf(str(ge / os.path.basename(f)), f)
+        assert ok, "%s differs at line %s:\nCPU %s\nGPU %s" % (os.path.basename(f), k, b, a)
+    assert o.nl_updates == e.counts()["nl_updates"] and o.nl_updates > 5
