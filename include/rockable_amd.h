@@ -54,6 +54,8 @@ typedef struct {
   int64_t nbodies, ndriven, nlisted, nactive, ninterfaces, nl_updates, steps;
   double t, dt, total_broken_area;
   int64_t last_candidates; /* body pairs passed by the last broad phase */
+  int64_t guard_reads;     /* host reads of the device displacement guard */
+  int64_t guard_wasted;    /* iterations enqueued after a rebuild fell due (done nothing) */
 } rk_counts;
 
 int rk_engine_create(rk_engine** out, int device);
@@ -101,6 +103,37 @@ int rk_get_interactions(rk_engine* eng, int active_only, rk_interaction* out, si
  * level 2. */
 int rk_get_kernel_time(rk_engine* eng, const char* kernel, int64_t* launches, double* total_ms);
 int rk_reset_kernel_time(rk_engine* eng);
+
+/* ---------------------------------------------------------------------------
+ * Rebuild-interval proposal network.  No counterpart in the reference.  With
+ * dynamicUpdateNL the displacement check (dynamicCheckUpdateNL,
+ * Rockable.cpp:1571-1596) runs on the device and the host reads it once per
+ * batch of iterations; the network proposes the size of the next batch from
+ * 16 statistics of the packing (rk_guard_features).  The guard alone decides
+ * every rebuild, so results do not depend on the network.
+ * The network: 16 -> 64 (tanh) -> 64 (tanh) -> 1, FP32, trained on the
+ * device (Adam, mean squared error) by the library's own kernels.
+ * Parameters: W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1].
+ * ------------------------------------------------------------------------- */
+typedef struct rk_mlp rk_mlp;
+int rk_mlp_create(rk_mlp** out, int device, uint64_t seed); /* Glorot-uniform weights */
+void rk_mlp_destroy(rk_mlp* mlp);
+const char* rk_mlp_last_error(const rk_mlp* mlp);
+int rk_mlp_nparams(void);
+int rk_mlp_get_params(rk_mlp* mlp, float* out, size_t n);
+int rk_mlp_set_params(rk_mlp* mlp, const float* in, size_t n);
+/* y[s] = network(X[s][0..15]) for s < n */
+int rk_mlp_forward(rk_mlp* mlp, const float* X, size_t n, float* y);
+/* loss = mean (y - t)^2 over the n samples, grad = its gradient (nparams) */
+int rk_mlp_loss_grad(rk_mlp* mlp, const float* X, const float* t, size_t n, float* loss, float* grad);
+/* Adam on shuffled minibatches; *final_loss = loss over all n samples */
+int rk_mlp_train(rk_mlp* mlp, const float* X, const float* t, size_t n, int epochs, int batch, float lr,
+                 uint64_t seed, float* final_loss);
+/* The engine sizes its batches by the network (NULL: fixed guard_batch).
+ * The network predicts log(1 + iterations until the next rebuild falls due). */
+int rk_set_rebuild_model(rk_engine* eng, rk_mlp* mlp);
+/* The 16 statistics of the current state that the network reads. */
+int rk_guard_features(rk_engine* eng, float* out16);
 
 #ifdef __cplusplus
 }

@@ -212,5 +212,12 @@ void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hip
 void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s);
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);
 void launch_guard(const Dev& d, const StepParams& sp, long long step, hipStream_t s);
+// part: (nb / 256 + 1) * 9 doubles; feat: 16 floats
+void launch_guard_features(const Dev& d, const StepParams& sp, double disp, double angle, long long since,
+                           long long last, double* part, float* feat, hipStream_t s);
 
 }  // namespace rk
+
+// rk_mlp.hip: y[0] = network(x[0..15]) for a feature vector on the device
+struct rk_mlp;
+int rk_mlp_forward_device(rk_mlp* mlp, const float* x_dev, float* y_dev, hipStream_t s);

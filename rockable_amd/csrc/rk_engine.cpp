@@ -72,6 +72,15 @@ struct rk_engine {
   double *gdpos = nullptr, *gdq = nullptr;
   double wstar = -2.0;
   int guard_batch = 16;  // steps enqueued between two reads of the guard
+  // rebuild-interval network (not owned) and its inputs / output on the device
+  rk_mlp* model = nullptr;
+  float* feat = nullptr;   // 16
+  float* pred = nullptr;   // 1
+  double* feat_part = nullptr;
+  size_t feat_part_cap = 0;
+  int64_t pred_batch = 16;
+  int64_t guard_reads = 0, guard_wasted = 0;
+  int64_t last_rebuild_step = 0, last_interval = 0;
   int exact_sums = 0;
   EntryArrays E[2];
   int cur = 0;
@@ -113,7 +122,8 @@ void free_device(rk_engine* eng) {
                 eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->ptab,      eng->damp_rate, eng->damp_kind,
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
                 eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
-                eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq};
+                eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,
+                eng->feat,     eng->pred,      eng->feat_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   eng->kin = nullptr;
@@ -136,6 +146,9 @@ void free_device(rk_engine* eng) {
   eng->bvoff = eng->bvbody = nullptr;
   eng->guard = nullptr;
   eng->gdpos = eng->gdq = nullptr;
+  eng->feat = eng->pred = nullptr;
+  eng->feat_part = nullptr;
+  eng->feat_part_cap = 0;
   eng->nvtot = 0;
   for (auto& E : eng->E) {
     entries_free(E);
@@ -659,6 +672,8 @@ int do_update_nl(rk_engine* eng) {
   eng->cur = nxt;
   assemble_dev(eng);
   eng->nl_updates++;
+  eng->last_interval = eng->steps - eng->last_rebuild_step;
+  eng->last_rebuild_step = eng->steps;
   toc(eng, "nl", eng->profile);
   return 0;
 }
@@ -827,6 +842,39 @@ void set_loop_scalars(rk_engine* eng, const LoopScalars& s) {
   eng->steps = s.steps;
 }
 
+// The 16 network inputs of the current state into eng->feat (device).
+int compute_features(rk_engine* eng) {
+  const int np = std::max(1, ((int)eng->m.bodies.size() + 255) / 256);
+  if (!eng->feat) {
+    HIPCHK(hipMalloc(&eng->feat, 16 * sizeof(float)));
+    HIPCHK(hipMalloc(&eng->pred, sizeof(float)));
+  }
+  if (eng->feat_part_cap < (size_t)np * 9) {
+    if (eng->feat_part) (void)hipFree(eng->feat_part);
+    eng->feat_part = nullptr;
+    HIPCHK(hipMalloc(&eng->feat_part, (size_t)np * 9 * sizeof(double)));
+    eng->feat_part_cap = (size_t)np * 9;
+  }
+  launch_guard_features(eng->d, step_params(eng), eng->m.dispUpdateNL, eng->m.angleUpdateNL,
+                        (long long)(eng->steps - eng->last_rebuild_step), (long long)eng->last_interval,
+                        eng->feat_part, eng->feat, eng->stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// The next batch size proposed by the network: exp(y) - 1 iterations.
+int propose_batch(rk_engine* eng) {
+  if (compute_features(eng) != 0) return -1;
+  if (rk_mlp_forward_device(eng->model, eng->feat, eng->pred, eng->stream) != 0) return fail(eng, "network forward failed");
+  float y = 0.0f;
+  HIPCHK(hipMemcpyAsync(&y, eng->pred, sizeof y, hipMemcpyDeviceToHost, eng->stream));
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  double b = std::expm1((double)y);
+  if (!(b == b)) b = 1.0;
+  eng->pred_batch = (int64_t)std::max(1.0, std::min(b, 65536.0));
+  return 0;
+}
+
 // The integrate() loop with dynamicUpdateNL, its displacement check on the
 // device.  Iterations are enqueued in batches without host synchronisation;
 // a batch ends where the host has work (a conf file, a scheduled rebuild,
@@ -842,7 +890,8 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
   std::vector<char> conf_at;
   while (done < nmax && (!until_tmax || m.t <= m.tmax)) {
     const bool bonds = m.law == LAW_STICKED && eng->live_interfaces > 0;
-    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(bonds ? 1 : eng->guard_batch, nmax - done));
+    const int64_t want = eng->model ? eng->pred_batch : eng->guard_batch;
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(bonds ? 1 : want, nmax - done));
     hist.assign(1, loop_scalars(eng));
     conf_at.assign(1, 0);
     const StepParams sp0 = step_params(eng);
@@ -869,12 +918,14 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     DevGuard g;
     HIPCHK(hipMemcpyAsync(&g, eng->guard, sizeof g, hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
+    eng->guard_reads++;
     int64_t executed = k;
     bool need = eng->needUpdate;
     if (g.frozen) {
       executed = g.frozen_step - hist[0].steps + 1;
       if (executed < 1 || executed > k) return fail(eng, "device guard: inconsistent frozen step");
       set_loop_scalars(eng, hist[executed]);
+      eng->guard_wasted += k - executed;
       need = true;
       const int zero = 0;
       HIPCHK(hipMemcpy(&eng->guard->frozen, &zero, sizeof zero, hipMemcpyHostToDevice));
@@ -890,6 +941,7 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       eng->needUpdate = false;
     }
     done += executed;
+    if (eng->model && propose_batch(eng) != 0) return -1;
   }
   if (ndone) *ndone = done;
   return 0;
@@ -1141,6 +1193,8 @@ int rk_get_counts(rk_engine* eng, rk_counts* out) {
   out->dt = eng->m.dt;
   out->total_broken_area = eng->m.totalBrokenArea;
   out->last_candidates = eng->w.last_candidates;
+  out->guard_reads = eng->guard_reads;
+  out->guard_wasted = eng->guard_wasted;
   return 0;
 }
 
@@ -1264,6 +1318,23 @@ int rk_reset_kernel_time(rk_engine* eng) {
   if (!eng) return -1;
   resolve_events(eng);
   eng->timers.clear();
+  return 0;
+}
+
+int rk_set_rebuild_model(rk_engine* eng, rk_mlp* mlp) {
+  if (!eng) return -1;
+  eng->model = mlp;
+  eng->pred_batch = eng->guard_batch;
+  return 0;
+}
+
+int rk_guard_features(rk_engine* eng, float* out16) {
+  if (check_loaded(eng) || !out16) return -1;
+  (void)hipSetDevice(eng->device);
+  ensure_gv(eng);
+  if (compute_features(eng) != 0) return -1;
+  HIPCHK(hipMemcpyAsync(out16, eng->feat, 16 * sizeof(float), hipMemcpyDeviceToHost, eng->stream));
+  HIPCHK(hipStreamSynchronize(eng->stream));
   return 0;
 }
 

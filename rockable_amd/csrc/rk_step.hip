@@ -932,6 +932,90 @@ __global__ void k_guard_check(Dev d, long long step) {
   }
 }
 
+// The 16 statistics the rebuild-interval network reads (per-body speed, spin,
+// acceleration and accumulated displacement, and the guard's running
+// maxima), scaled by the guard's thresholds.  Pass 1: per-block partials;
+// pass 2: one workgroup sums them in block order (no atomics).
+constexpr int kFeatPart = 9;
+__global__ void __launch_bounds__(kBlock) k_guard_feat_part(Dev d, StepParams sp, double disp, double* part) {
+  __shared__ double red[kFeatPart][kBlock];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = threadIdx.x;
+  double v[kFeatPart] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (b < d.nb) {
+    const BodyKin& P = d.kin[b];
+    const double vn = sqrt(vnorm2(kvel(P))), wn = sqrt(vnorm2(kvrot(P))), an = sqrt(vnorm2(ld3(d.acc + 3 * (size_t)b)));
+    const double dx = d.gdpos ? sqrt(vnorm2(ld3(d.gdpos + 3 * (size_t)b))) : 0.0;
+    v[0] = vn;                                     // max |v|
+    v[1] = vn;                                     // sum |v|
+    v[2] = vn * vn;                                // sum |v|^2
+    v[3] = wn;                                     // max |w|
+    v[4] = wn;                                     // sum |w|
+    v[5] = an;                                     // max |a|
+    v[6] = an;                                     // sum |a|
+    v[7] = dx;                                     // sum |dx|
+    v[8] = (vn * sp.dt > 0.01 * disp) ? 1.0 : 0.0;  // moving bodies
+  }
+  for (int k = 0; k < kFeatPart; k++) red[k][t] = (v[k] == v[k]) ? v[k] : 0.0;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int k = 0; k < kFeatPart; k++) {
+        const bool mx = (k == 0 || k == 3 || k == 5);
+        red[k][t] = mx ? fmax(red[k][t], red[k][t + h]) : red[k][t] + red[k][t + h];
+      }
+    __syncthreads();
+  }
+  if (t < kFeatPart) part[(size_t)blockIdx.x * kFeatPart + t] = red[t][0];
+}
+
+__global__ void __launch_bounds__(64) k_guard_feat_final(Dev d, StepParams sp, const double* part, int nparts,
+                                                         double disp, double angle, long long since,
+                                                         long long last, float* feat) {
+  if (threadIdx.x != 0) return;
+  double a[kFeatPart] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = 0; q < nparts; q++)
+    for (int k = 0; k < kFeatPart; k++) {
+      const double x = part[(size_t)q * kFeatPart + k];
+      const bool mx = (k == 0 || k == 3 || k == 5);
+      a[k] = mx ? fmax(a[k], x) : a[k] + x;
+    }
+  const double nb = d.nb > 0 ? (double)d.nb : 1.0;
+  const double ds = disp > 0.0 ? disp : 1.0, an = angle > 0.0 ? angle : 1.0, dt = sp.dt;
+  double dxmax = 0.0, rot = 0.0;
+  if (d.guard) {
+    dxmax = sqrt(__longlong_as_double((long long)d.guard->maxpos2));
+    if (d.guard->minw != ~0ull && d.wstar < 1.0 && d.wstar >= -1.0) {
+      const unsigned long long k = d.guard->minw;
+      const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+      const double w = __longlong_as_double((long long)u);
+      rot = (1.0 - w) / (1.0 - d.wstar);
+    }
+  }
+  double f[16];
+  f[0] = dxmax / ds;
+  f[1] = rot;
+  f[2] = a[0] * dt / ds;
+  f[3] = a[1] / nb * dt / ds;
+  f[4] = sqrt(a[2] / nb) * dt / ds;
+  f[5] = a[3] * dt / an;
+  f[6] = a[4] / nb * dt / an;
+  f[7] = a[5] * dt * dt / ds;
+  f[8] = a[6] / nb * dt * dt / ds;
+  f[9] = log1p(fmax(0.0, (1.0 - f[0]) / fmax(f[2], 1e-12))) / 10.0;
+  f[10] = log1p((double)since) / 10.0;
+  f[11] = log1p((double)last) / 10.0;
+  f[12] = a[8] / nb;
+  f[13] = a[7] / nb / ds;
+  f[14] = fmin(f[2] * (double)since, 10.0);
+  f[15] = 1.0;
+  for (int k = 0; k < 16; k++) {
+    double x = f[k] == f[k] ? f[k] : 0.0;
+    x = x > 10.0 ? 10.0 : (x < -10.0 ? -10.0 : x);
+    feat[k] = (float)x;
+  }
+}
+
 // Particle::GlobVertex (Particle.cpp:55-57) of every vertex of every body,
 // once per step: one thread per vertex.
 __global__ void __launch_bounds__(kBlock) k_glob(Dev d) {
@@ -1176,6 +1260,13 @@ void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) 
 void launch_guard(const Dev& d, const StepParams& sp, long long step, hipStream_t s) {
   if (d.nb > 0) hipLaunchKernelGGL(k_guard_acc, dim3((d.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp);
   hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(64), 0, s, d, step);
+}
+
+void launch_guard_features(const Dev& d, const StepParams& sp, double disp, double angle, long long since,
+                           long long last, double* part, float* feat, hipStream_t s) {
+  const int np = std::max(1, (d.nb + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_guard_feat_part, dim3(np), dim3(kBlock), 0, s, d, sp, disp, part);
+  hipLaunchKernelGGL(k_guard_feat_final, dim3(1), dim3(64), 0, s, d, sp, part, np, disp, angle, since, last, feat);
 }
 
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s) {

@@ -25,13 +25,17 @@ class RkCounts(ctypes.Structure):
     _fields_ = [("nbodies", ctypes.c_int64), ("ndriven", ctypes.c_int64), ("nlisted", ctypes.c_int64),
                 ("nactive", ctypes.c_int64), ("ninterfaces", ctypes.c_int64), ("nl_updates", ctypes.c_int64),
                 ("steps", ctypes.c_int64), ("t", ctypes.c_double), ("dt", ctypes.c_double),
-                ("total_broken_area", ctypes.c_double), ("last_candidates", ctypes.c_int64)]
+                ("total_broken_area", ctypes.c_double), ("last_candidates", ctypes.c_int64),
+                ("guard_reads", ctypes.c_int64), ("guard_wasted", ctypes.c_int64)]
 
 
 EXPORTED = ["rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_conf", "rk_load_driving_system",
             "rk_set_force_law", "rk_set_body_force", "rk_set_option", "rk_update_nl", "rk_accelerations",
             "rk_step", "rk_integrate", "rk_save_conf", "rk_synchronize", "rk_get_counts", "rk_get_bodies",
-            "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time"]
+            "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time",
+            "rk_mlp_create", "rk_mlp_destroy", "rk_mlp_last_error", "rk_mlp_nparams", "rk_mlp_get_params",
+            "rk_mlp_set_params", "rk_mlp_forward", "rk_mlp_loss_grad", "rk_mlp_train", "rk_set_rebuild_model",
+            "rk_guard_features"]
 
 _lib = None
 
@@ -67,6 +71,20 @@ def load_library(path=LIB_PATH):
     L.rk_get_kernel_time.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_double)]
     L.rk_reset_kernel_time.argtypes = [P]
+    F = ctypes.POINTER(ctypes.c_float)
+    L.rk_mlp_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_uint64]
+    L.rk_mlp_destroy.argtypes = [P]
+    L.rk_mlp_last_error.argtypes = [P]
+    L.rk_mlp_last_error.restype = ctypes.c_char_p
+    L.rk_mlp_nparams.argtypes = []
+    L.rk_mlp_get_params.argtypes = [P, F, ctypes.c_size_t]
+    L.rk_mlp_set_params.argtypes = [P, F, ctypes.c_size_t]
+    L.rk_mlp_forward.argtypes = [P, F, ctypes.c_size_t, F]
+    L.rk_mlp_loss_grad.argtypes = [P, F, F, ctypes.c_size_t, F, F]
+    L.rk_mlp_train.argtypes = [P, F, F, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                               F]
+    L.rk_set_rebuild_model.argtypes = [P, P]
+    L.rk_guard_features.argtypes = [P, F]
     _lib = L
     return L
 
@@ -179,3 +197,76 @@ class Engine:
 
     def reset_kernel_time(self):
         self._check(self.L.rk_reset_kernel_time(self.h))
+
+    def set_rebuild_model(self, mlp):
+        """Size the batches between guard reads by the network (None: fixed)."""
+        self._model = mlp
+        self._check(self.L.rk_set_rebuild_model(self.h, mlp.h if mlp is not None else None))
+
+    def guard_features(self):
+        out = np.zeros(16, dtype=np.float32)
+        self._check(self.L.rk_guard_features(self.h, _fptr(out)))
+        return out
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The rebuild-interval network (16 -> 64 -> 64 -> 1, FP32) on a GPU."""
+
+    def __init__(self, device=0, seed=1):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.rk_mlp_create(ctypes.byref(h), device, seed)
+        if rc != 0:
+            raise RuntimeError("rk_mlp_create failed (%d): no usable GPU device %d" % (rc, device))
+        self.h = h
+        self.nparams = self.L.rk_mlp_nparams()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(self.L.rk_mlp_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rk_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.nparams, dtype=np.float32)
+        self._check(self.L.rk_mlp_get_params(self.h, _fptr(out), out.size))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.L.rk_mlp_set_params(self.h, _fptr(p), p.size))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 16)
+        y = np.zeros(len(X), dtype=np.float32)
+        self._check(self.L.rk_mlp_forward(self.h, _fptr(X), len(X), _fptr(y)))
+        return y
+
+    def loss_grad(self, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        self._check(self.L.rk_mlp_loss_grad(self.h, _fptr(X), _fptr(t), len(X), _fptr(loss), _fptr(g)))
+        return float(loss[0]), g
+
+    def train(self, X, t, epochs=50, batch=256, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(self.L.rk_mlp_train(self.h, _fptr(X), _fptr(t), len(X), epochs, batch, lr, seed, _fptr(loss)))
+        return float(loss[0])
