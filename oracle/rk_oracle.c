@@ -428,7 +428,7 @@ static const int PWRITE[P_COUNT] = {P_KN,     P_EN2,     P_KT,     P_MU,      P_
                                     P_EN2_IN, P_KN_OUT,  P_KT_OUT, P_KR_OUT,  P_FN0_OUT,  P_FT0_OUT,  P_MOM0_OUT,
                                     P_POW_OUT, P_EN2_OUT, P_GC_IN, P_GC_OUT};
 
-#define MAXG 64
+#define MAXG 256
 enum { LAW_DEFAULT = 0, LAW_AVALANCHE = 1, LAW_STICKED = 2 };
 enum { BF_NONE = 0, BF_ATTRACTING = 1, BF_PREFDIR = 2, BF_VISCOUS = 3 };
 

@@ -26,7 +26,7 @@ enum {
   P_COUNT
 };
 extern const char* const kParamName[P_COUNT];
-constexpr int kMaxGroups = 64;
+constexpr int kMaxGroups = 256;
 
 // Control codes (DrivingSystem.hpp:52-67).
 enum {

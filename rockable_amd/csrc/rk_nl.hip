@@ -256,28 +256,53 @@ struct Grid {
   double mn[3], sz[3];
 };
 
-__global__ void k_cell_assign(Dev d, const double* baabb, Grid g, int* cell_of, int* ccount, int* over, int* nover) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= d.nb) return;
-  const double* a = baabb + 6 * (size_t)b;
-  const bool oversized = (a[3] - a[0] > g.sz[0]) || (a[4] - a[1] > g.sz[1]) || (a[5] - a[2] > g.sz[2]);
-  if (oversized) {
-    cell_of[b] = -1;
-    over[atomicAdd(nover, 1)] = b;
-    return;
+// Largest distance, per dimension, from a body's position to a point of its
+// OBB enlarged by `half` (a celled body sits in the cell of its position).
+__device__ __forceinline__ void obb_reach(const double* o, const double p[3], double half, double r[3]) {
+  for (int k = 0; k < 3; k++)
+    r[k] = fabs(o[k] - p[k]) + fabs(o[3 + k]) * (o[12] + half) + fabs(o[6 + k]) * (o[13] + half) +
+           fabs(o[9 + k]) * (o[14] + half);
+}
+
+// Cell of each body (oversized bodies to their own list) and, in `reach`
+// (bit patterns of non-negative doubles, which order as integers), the
+// largest obb_reach over the celled bodies.
+__global__ void k_cell_assign(Dev d, const double* bobb, const double* baabb, double half, Grid g, int* cell_of,
+                              int* ccount, int* over, int* nover, unsigned long long* reach) {
+  __shared__ double red[3][kBlock];
+  const int t = threadIdx.x;
+  const int b = blockIdx.x * blockDim.x + t;
+  double rr[3] = {0.0, 0.0, 0.0};
+  if (b < d.nb) {
+    const double* a = baabb + 6 * (size_t)b;
+    const bool oversized = (a[3] - a[0] > g.sz[0]) || (a[4] - a[1] > g.sz[1]) || (a[5] - a[2] > g.sz[2]);
+    if (oversized) {
+      cell_of[b] = -1;
+      over[atomicAdd(nover, 1)] = b;
+    } else {
+      const BodyKin& P = d.kin[b];
+      const double p[3] = {P.px, P.py, P.pz};
+      int c3[3];
+      for (int k = 0; k < 3; k++) {
+        int q = (int)floor((p[k] - g.mn[k]) / g.sz[k]);
+        q = q < 0 ? 0 : q;
+        q = q >= g.N[k] ? g.N[k] - 1 : q;
+        c3[k] = q;
+      }
+      const int c = (c3[0] * g.N[1] + c3[1]) * g.N[2] + c3[2];
+      cell_of[b] = c;
+      atomicAdd(ccount + c, 1);
+      obb_reach(bobb + 15 * (size_t)b, p, half, rr);
+    }
   }
-  const BodyKin& P = d.kin[b];
-  const double p[3] = {P.px, P.py, P.pz};
-  int c3[3];
-  for (int k = 0; k < 3; k++) {
-    int q = (int)floor((p[k] - g.mn[k]) / g.sz[k]);
-    q = q < 0 ? 0 : q;
-    q = q >= g.N[k] ? g.N[k] - 1 : q;
-    c3[k] = q;
+  for (int k = 0; k < 3; k++) red[k][t] = rr[k];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < 3; k++) red[k][t] = red[k][t + s] > red[k][t] ? red[k][t + s] : red[k][t];
+    __syncthreads();
   }
-  const int c = (c3[0] * g.N[1] + c3[1]) * g.N[2] + c3[2];
-  cell_of[b] = c;
-  atomicAdd(ccount + c, 1);
+  if (t < 3) atomicMax(reach + t, (unsigned long long)__double_as_longlong(red[t][0]));
 }
 
 __global__ void k_cell_fill(int nb, const int* cell_of, const int* cstart, int* cfill, int* cbody) {
@@ -311,16 +336,48 @@ __global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, co
       }
 }
 
-// oversized bodies against every body of every cell (Rockable.cpp:1759-1805)
-__global__ void k_broad_over(Dev d, const double* bobb, double half, const int* over, int nover, const int* cbody,
-                             int ncell_bodies, uint64_t* cand, int* cnt, int rcap) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = blockIdx.y;
-  if (r >= ncell_bodies || q >= nover) return;
-  const int i = over[q], j = cbody[r];
-  const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
-                    pair_overlap(bobb, i, j, half);
-  emit_pair(emit, cand, cnt, rcap, i, j);
+// Oversized bodies against the bodies of the cells (Rockable.cpp:1755-1805):
+// workgroup (q, y) takes oversized body over[q] and every gridDim.y-th
+// (x, y) column of cells.  The reference visits every cell; a celled body
+// whose enlarged OBB meets the oversized body's lies within `reach` of that
+// OBB's box, so only the cells covering the box plus reach (one more on each
+// side against rounding) can hold one, and the others are skipped.
+__global__ void k_broad_over(Dev d, const double* bobb, double half, Grid g, const int* over, int nover,
+                             const int* cstart, const int* cbody, const unsigned long long* reach, uint64_t* cand,
+                             int* cnt, int rcap) {
+  const int q = blockIdx.x;
+  if (q >= nover) return;
+  const int i = over[q];
+  const double* o = bobb + 15 * (size_t)i;
+  int lo[3], hi[3];
+  for (int k = 0; k < 3; k++) {
+    const double ext =
+        fabs(o[3 + k]) * (o[12] + half) + fabs(o[6 + k]) * (o[13] + half) + fabs(o[9 + k]) * (o[14] + half);
+    const double rch = __longlong_as_double((long long)reach[k]);
+    const double top = (double)(g.N[k] - 1);
+    const double a = floor((o[k] - ext - rch - g.mn[k]) / g.sz[k]) - 1.0;
+    const double b = floor((o[k] + ext + rch - g.mn[k]) / g.sz[k]) + 1.0;
+    lo[k] = (int)fmin(fmax(a, 0.0), top);
+    hi[k] = (int)fmin(fmax(b, 0.0), top);
+  }
+  const int ny = hi[1] - lo[1] + 1;
+  const int ncol = (hi[0] - lo[0] + 1) * ny;
+  for (int col = blockIdx.y; col < ncol; col += gridDim.y) {
+    const int cx = lo[0] + col / ny, cy = lo[1] + col % ny;
+    const int cbase = (cx * g.N[1] + cy) * g.N[2];
+    const int r0 = cstart[cbase + lo[2]], r1 = cstart[cbase + hi[2] + 1];
+    for (int rb = r0; rb < r1; rb += blockDim.x) {
+      const int r = rb + (int)threadIdx.x;
+      bool emit = false;
+      int j = 0;
+      if (r < r1) {
+        j = cbody[r];
+        emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
+               pair_overlap(bobb, i, j, half);
+      }
+      emit_pair(emit, cand, cnt, rcap, i, j);
+    }
+  }
 }
 
 // --- directed groups -------------------------------------------------------
@@ -1162,8 +1219,10 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
         return -1;
       RK_CHECK(hipMemsetAsync(w.ccount, 0, sizeof(int32_t) * (ncell + 1), s));
       RK_CHECK(hipMemsetAsync(w.cfill, 0, sizeof(int32_t) * ncell, s));
-      hipLaunchKernelGGL(k_cell_assign, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.baabb, g, w.cell_of, w.ccount,
-                         w.over, w.counter + 1);
+      unsigned long long* reach = reinterpret_cast<unsigned long long*>(w.f64a);
+      RK_CHECK(hipMemsetAsync(reach, 0, sizeof(unsigned long long) * 3, s));
+      hipLaunchKernelGGL(k_cell_assign, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, w.baabb, half, g, w.cell_of,
+                         w.ccount, w.over, w.counter + 1, reach);
       if (scan_exclusive(w, w.ccount, w.cstart, (int)ncell + 1, s)) return -1;
       hipLaunchKernelGGL(k_cell_fill, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, w.cell_of, w.cstart, w.cfill, w.cbody);
       int hv[2];
@@ -1173,9 +1232,11 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       const int ncb = nb - nover;
       hipLaunchKernelGGL(k_broad_cells, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
                          w.cbody, w.cstripe, w.scnt, rcap);
-      if (nover > 0 && ncb > 0)
-        hipLaunchKernelGGL(k_broad_over, dim3(nblk(ncb), nover), dim3(kBlock), 0, s, d, w.bobb, half, w.over, nover,
-                           w.cbody, ncb, w.cstripe, w.scnt, rcap);
+      if (nover > 0 && ncb > 0) {
+        const int split = std::max(1, std::min(64, (4096 + nover - 1) / nover));
+        hipLaunchKernelGGL(k_broad_over, dim3(nover, split), dim3(kBlock), 0, s, d, w.bobb, half, g, w.over, nover,
+                           w.cstart, w.cbody, reach, w.cstripe, w.scnt, rcap);
+      }
     }
     hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
     int tm[2] = {0, 0};

@@ -192,6 +192,31 @@ def _r(x):
     return repr(float(x))
 
 
+def _body_block_text(block):
+    """Particles lines of many bodies at once: block holds arrays "names"
+    (one per body, or a single str), "group", "h", "pos", "vel", "Q" (and
+    optionally "cluster").  Columns holding only exact integers are written
+    with %d (read back exactly), the others with %.16e."""
+    import io
+    n = len(block["pos"])
+    z3 = np.zeros((n, 3))
+    cols = np.column_stack([np.asarray(block["group"], float),
+                            np.asarray(block.get("cluster", np.zeros(n)), float),
+                            np.asarray(block["h"], float), np.asarray(block["pos"], float),
+                            np.asarray(block["vel"], float), z3, np.asarray(block["Q"], float), z3, z3])
+    fmt = []
+    for k in range(cols.shape[1]):
+        c = cols[:, k]
+        fmt.append("%d" if (k < 2 or (np.all(c == np.round(c)) and np.abs(c).max(initial=0) < 2 ** 52)) else "%.16e")
+    buf = io.StringIO()
+    np.savetxt(buf, cols, fmt=fmt, delimiter=" ")
+    lines = buf.getvalue().rstrip("\n").split("\n")
+    names = block["names"]
+    if isinstance(names, str):
+        return "\n".join(names + " " + ln for ln in lines)
+    return "\n".join(nm + " " + ln for nm, ln in zip(names, lines))
+
+
 def write_conf(path, deck: Deck):
     """A conf deck readable by Rockable::loadConf (Rockable.cpp:1085-1145)."""
     o = ["Rockable 21-08-2022"]
@@ -224,10 +249,14 @@ def write_conf(path, deck: Deck):
     o.extend(deck.extra)
     o.append("shapeFile " + deck.shape_file)
     o.append("precision %d" % deck.precision)
-    o.append("Particles %d" % len(deck.bodies))
+    block = getattr(deck, "body_block", None)
+    nblock = 0 if block is None else len(block["pos"])
+    o.append("Particles %d" % (len(deck.bodies) + nblock))
     for b in deck.bodies:
         vals = [b.homothety] + list(b.pos) + list(b.vel) + list(b.acc) + list(b.Q) + list(b.vrot) + list(b.arot)
         o.append("%s %d %d %s" % (b.shape, b.group, b.cluster, " ".join(_f(x) for x in vals)))
+    if nblock:
+        o.append(_body_block_text(block))
     o.append("Interactions %d" % len(deck.interactions))
     for rec in deck.interactions:
         o.append(" ".join(str(int(x)) for x in rec[:5]) + " " + " ".join(_f(x) for x in rec[5:]))
@@ -555,3 +584,83 @@ def conf_drop_bodies(src, dst, keep):
     with open(dst, "w") as fh:
         fh.write("\n".join(out))
     return dst
+
+
+def batched_sphere_packings(n_packings=256, n_range=(500, 20000), seed=4, R=0.5e-3, spacing_factor=2.2, kn=1e6,
+                            mu_range=(0.2, 0.8), en_range=(0.1, 0.9), dt=1e-6, steps=5000, vmax=0.01,
+                            interverlet_steps=100):
+    """Batched form: n_packings independent packings of the plain CPU-runnable
+    configuration (spheres of radius R on a lattice of spacing
+    spacing_factor R dropping into a floor and four walls), of uneven sizes
+    (body counts log-uniform in n_range), each with its own Default-law
+    parameters (mu uniform in mu_range, en in en_range, en2Contact = en^2)
+    as group pair (p, p).  The packings sit on a square grid far enough apart
+    that no pair of bodies of two packings is ever near; all walls come first
+    (the driven bodies, nDriven = 5 n_packings).  Returns (shapes, deck, info)
+    where info lists each packing's body range, walls and parameters."""
+    rng = np.random.default_rng(seed)
+    a = spacing_factor * R
+    counts = np.round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), size=n_packings))).astype(int)
+    mus = rng.uniform(*mu_range, size=n_packings)
+    ens = rng.uniform(*en_range, size=n_packings)
+    dims = []
+    for n in counts:
+        nx = int(np.ceil(n ** (1.0 / 3.0)))
+        nz = int(np.ceil(n / (nx * nx)))
+        dims.append((nx, nx, nz))
+    ext = max(max(d[0], d[2] + 4) for d in dims) * a
+    pitch = ext + 20 * a
+    gridn = int(np.ceil(np.sqrt(n_packings)))
+    wall_shapes = None
+    walls, blocks, info = [], [], []
+    for p in range(n_packings):
+        nx, ny, nz = dims[p]
+        origin = np.array([(p % gridn) * pitch, (p // gridn) * pitch, 0.0])
+        lo = origin
+        hi = origin + np.array([nx * a, ny * a, nz * a + 4 * a])
+        ws, wb = box_walls(lo, hi, 0.1 * R)
+        wall_shapes = ws
+        for b in wb:
+            b.group = p
+        walls.append(wb)
+        ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+        idx = np.column_stack([ix.ravel(), iy.ravel(), iz.ravel()])[:counts[p]]
+        pos = origin + R + 0.5 * (a - 2 * R) + idx * a
+        vel = rng.uniform(-vmax, vmax, size=(counts[p], 3))
+        blocks.append((pos, vel))
+        info.append({"bodies": int(counts[p]), "mu": float(mus[p]), "en": float(ens[p])})
+    shapes = wall_shapes + [sphere_shape("SPHERE", R)]
+    params = []
+    for p in range(n_packings):
+        params += contact_params(kn, 0.8 * kn, ens[p] ** 2, mus[p], groups=((p, p),))
+    deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=interverlet_steps * dt,
+                interConf=1e3, DVerlet=0.2 * R, dVerlet=0.1 * R, density={p: 2700.0 for p in range(n_packings)},
+                gravity=(0.0, 0.0, -9.81), update_nl="linkCells", nDriven=5 * n_packings, params=params)
+    deck.bodies = [b for wb in walls for b in wb]
+    first = deck.nDriven
+    for p in range(n_packings):
+        info[p]["first"] = int(first)
+        info[p]["walls"] = list(range(5 * p, 5 * p + 5))
+        first += counts[p]
+    pos = np.concatenate([b[0] for b in blocks])
+    vel = np.concatenate([b[1] for b in blocks])
+    groups = np.concatenate([np.full(counts[p], p) for p in range(n_packings)])
+    deck.body_block = {"names": "SPHERE", "group": groups, "h": np.ones(len(pos)), "pos": pos, "vel": vel,
+                       "Q": np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1))}
+    deck.cell_min_sizes = (2.0 * a, 2.0 * a, 2.0 * a)
+    deck.box_for_linkcells_opt = 1
+    return shapes, deck, info
+
+
+def single_packing_of(shapes, deck, info, p):
+    """Packing p of a batched deck as a deck of its own (its 5 walls, then its
+    spheres), with the same parameters: the batched run must reproduce it."""
+    import copy
+    d = copy.copy(deck)
+    d.bodies = [copy.copy(b) for b in deck.bodies[5 * p:5 * p + 5]]
+    f, n = info[p]["first"] - deck.nDriven, info[p]["bodies"]
+    blk = deck.body_block
+    d.body_block = {"names": "SPHERE", "group": blk["group"][f:f + n], "h": blk["h"][f:f + n],
+                    "pos": blk["pos"][f:f + n], "vel": blk["vel"][f:f + n], "Q": blk["Q"][f:f + n]}
+    d.nDriven = 5
+    return shapes, d
