@@ -28,6 +28,14 @@ resultants), the hot kernels of the step; kernel durations come from HIP
 events recorded on the engine stream inside the timed region.  cpu_baseline
 times the CPU oracle (the restatement of Rockable's OpenMP
 DefaultParallelMode path) on a bounded sample of the same settled packing.
+
+Beside the plain form, the JSON line carries "forms": "batched" (256
+independent sphere packings in one engine, autotuned launch shapes and
+graph-captured steps, with a sample of the same packings run one engine
+after another) and "avalanche" (a 1,000,000-body polyhedral avalanche with
+the displacement-check rebuild: the plain GPU form against the batched +
+learned-rebuild form, alternating on the same trajectory).  --forms ""
+leaves them out.
 """
 import argparse
 import json
@@ -116,6 +124,143 @@ def cpu_baseline(deck_path, target_seconds, threads):
                       "settled packing, %d steps after 20 warm-up steps, %.1f s" % (threads, steps, dt)}
 
 
+def batched_form(a, log):
+    """Batched form: a.packings independent packings of uneven sizes (body
+    counts log-uniform in [500, 20000]) of the CPU-runnable configuration's
+    spheres (R = 0.5 mm on a 2.2 R lattice dropping into a floor and four
+    walls, velocities in [-0.01, 0.01] m/s), each with its own Default-law mu
+    and en, all in one engine: one launch per kernel and step covers every
+    packing.  The launch shapes and the graph length are picked by the
+    autotuner on the running batch, then a.batched_steps steps are timed.
+    Beside it, a sample of the same packings run one engine after another
+    (the unbatched way)."""
+    import numpy as np
+    from rockable_amd import decks as D
+    from rockable_amd.engine import Engine
+    d = tempfile.mkdtemp(prefix="rk_batch_")
+    t0 = time.perf_counter()
+    shapes, deck, info = D.batched_sphere_packings(n_packings=a.packings, steps=a.batched_steps)
+    path = D.write_deck(d, shapes, deck)
+    log("batched deck: %d packings, %d bodies, written in %.1f s" % (len(info), sum(p["bodies"] for p in info),
+                                                                     time.perf_counter() - t0))
+    e = Engine(path, device=0)
+    e.set_option("record_geometry", 0)
+    e.update_nl()
+    ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
+    log("autotune: %d steps, %s" % (ntune, choice))
+    c0 = e.counts()
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.step(a.batched_steps)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    c1 = e.counts()
+    nb = c1["nbodies"]
+    e.close()
+    # unbatched sample: every 32nd packing as an engine of its own, default shapes
+    rng = np.random.default_rng(0)
+    sample = sorted(rng.choice(len(info), size=min(a.unbatched_sample, len(info)), replace=False).tolist())
+    work, swall = 0, 0.0
+    for p in sample:
+        s2, d2 = D.single_packing_of(shapes, deck, info, p)
+        sp = D.write_deck(os.path.join(d, "p%d" % p), s2, d2)
+        q = Engine(sp, device=0)
+        q.set_option("record_geometry", 0)
+        q.update_nl()
+        q.step(100)
+        q.synchronize()
+        t1 = time.perf_counter()
+        q.step(a.unbatched_steps)
+        q.synchronize()
+        swall += time.perf_counter() - t1
+        work += q.counts()["nbodies"] * a.unbatched_steps
+        q.close()
+    return {"value": nb * a.batched_steps / wall, "unit": "body-steps/s", "ms_per_step": 1e3 * wall / a.batched_steps,
+            "steps": a.batched_steps, "bodies": nb, "packings": len(info),
+            "workload": "%d packings of 500-20000 spheres (R 0.5 mm, 2.2 R lattice, floor + 4 walls each), "
+                        "per-packing Default-law mu in [0.2, 0.8] and en in [0.1, 0.9], linkCells, "
+                        "dt 1e-6 s, NL every 100 steps" % len(info),
+            "autotuned": choice, "autotune_steps": ntune,
+            "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
+            "graph_launches": c1["graph_launches"] - c0["graph_launches"],
+            "unbatched_sample": {"packings": sample, "value": work / swall, "unit": "body-steps/s",
+                                 "steps_each": a.unbatched_steps,
+                                 "note": "the same packings, one engine after another, default launch shapes"}}
+
+
+def avalanche_forms(a, log):
+    """Learned-rebuild workload: a.avalanche_bodies of the plain GPU form's
+    hull polyhedra packed on the floor of a 400 x 100-size box and sent down
+    a 30 deg slope (gravity tilted; 1 m/s downslope plus velocities in
+    [-0.5, 0.5] m/s), dt 5e-7 s, the neighbour list rebuilt by the
+    displacement check alone (dynamicUpdateNL, dispUpdateNL = DVerlet / 4,
+    angleUpdateNL = 1 rad, the reference default).  One engine runs the
+    avalanche on; segments of a.avalanche_segment steps alternate between
+    the plain GPU form (the check read by the host after every step, every
+    kernel launched on its own) and the batched + learned form (batches of
+    steps sized by the rebuild-interval network, launched as captured graphs
+    of the autotuned length, the device check freezing any step after a
+    rebuild falls due).  Both forms compute the same steps, so each is timed
+    on its share of the same trajectory."""
+    import numpy as np
+    from rockable_amd import decks as D
+    from rockable_amd.engine import Engine, Mlp
+    d = tempfile.mkdtemp(prefix="rk_aval_")
+    t0 = time.perf_counter()
+    shapes, deck = D.avalanche_deck(n_bodies=a.avalanche_bodies, dt=5e-7, gap_frac=0.005)
+    path = D.write_deck(d, shapes, deck)
+    log("avalanche deck: %d bodies, written in %.1f s" % (a.avalanche_bodies, time.perf_counter() - t0))
+    e = Engine(path, device=0)
+    e.set_option("record_geometry", 0)
+    e.update_nl()
+    mlp = Mlp(device=0)
+    mlp.set_params(np.load(os.path.join(ROOT, "rockable_amd", "data", "rebuild_mlp.npy")))
+    e.step(a.avalanche_warmup)
+    ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
+    G = max(8, choice["graph_steps"])
+    log("autotune: %d steps, %s; batched form graph length %d" % (ntune, choice, G))
+    forms = {"plain": [0.0, 0, 0, 0], "batched_learned": [0.0, 0, 0, 0]}
+    done = 0
+    while done < 2 * a.avalanche_steps:
+        for name in ("plain", "batched_learned"):
+            if name == "plain":
+                e.set_rebuild_model(None)
+                e.set_option("guard_batch", 1)
+                e.set_option("graph_steps", 0)
+            else:
+                e.set_rebuild_model(mlp)
+                e.set_option("graph_steps", G)
+            c0 = e.counts()
+            e.synchronize()
+            t1 = time.perf_counter()
+            e.step(a.avalanche_segment)
+            e.synchronize()
+            f = forms[name]
+            f[0] += time.perf_counter() - t1
+            c1 = e.counts()
+            f[1] += a.avalanche_segment
+            f[2] += c1["nl_updates"] - c0["nl_updates"]
+            f[3] += c1["guard_reads"] - c0["guard_reads"]
+            done += a.avalanche_segment
+    c = e.counts()
+    nb = c["nbodies"]
+    mlp.close()
+    e.close()
+    out = {}
+    for name, (w, n, r, g) in forms.items():
+        out[name] = {"value": nb * n / w, "unit": "body-steps/s", "ms_per_step": 1e3 * w / n, "steps": n,
+                     "nl_rebuilds": r, "guard_reads": g}
+    out["batched_learned_over_plain"] = out["batched_learned"]["value"] / out["plain"]["value"]
+    out["bodies"] = nb
+    out["listed_per_body"] = c["nlisted"] / nb
+    out["active_per_body"] = c["nactive"] / nb
+    out["autotuned"] = choice
+    out["workload"] = ("%d hull polyhedra (config of the plain GPU form) down a 30 deg slope, dt 5e-7 s, "
+                       "dynamicUpdateNL (disp DVerlet/4, 1 rad), %d warm-up steps, then segments of %d steps "
+                       "alternating between the forms" % (nb, a.avalanche_warmup, a.avalanche_segment))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +274,19 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--deck-dir", default=None)
+    ap.add_argument("--packings", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=5000)
+    ap.add_argument("--unbatched-sample", type=int, default=8)
+    ap.add_argument("--unbatched-steps", type=int, default=1000)
+    ap.add_argument("--avalanche-bodies", type=int, default=1000000)
+    ap.add_argument("--avalanche-steps", type=int, default=3000, help="timed steps per form")
+    ap.add_argument("--avalanche-warmup", type=int, default=2000)
+    ap.add_argument("--avalanche-segment", type=int, default=500)
+    ap.add_argument("--tune-trial", type=int, default=50)
+    ap.add_argument("--forms", default="batched,avalanche",
+                    help="extra forms reported beside the plain one (comma list; empty for none)")
+    ap.add_argument("--prepare-only", action="store_true",
+                    help="settle the packing into --deck-dir and stop (later runs reuse it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (calibrates the HBM byte counters)")
     a = ap.parse_args()
@@ -165,6 +323,9 @@ def main():
         path, dropped = make_settled_deck(a.bodies, a.seed, d, a.settle, a.relax, log)
         meta["dropped"] = dropped
         json.dump(meta, open(meta_path, "w"))
+    if a.prepare_only:
+        log("settled packing in %s" % d)
+        return
 
     e = Engine(path, device=0)
     e.set_option("record_geometry", 0)
@@ -232,8 +393,15 @@ def main():
                      "kernel": "force phase (k_interact + k_gather + k_gather_heavy)",
                      "algorithmic_bytes_per_launch": bytes_per},
     }
+    e.close()
+    forms = [f for f in a.forms.split(",") if f]
+    if forms:
+        out["forms"] = {}
+    if "batched" in forms:
+        out["forms"]["batched"] = batched_form(a, log)
+    if "avalanche" in forms:
+        out["forms"]["avalanche"] = avalanche_forms(a, log)
     if not a.no_cpu:
-        e.close()
         out["cpu_baseline"] = cpu_baseline(path, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(out), flush=True)
 

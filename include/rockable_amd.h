@@ -56,6 +56,8 @@ typedef struct {
   int64_t last_candidates; /* body pairs passed by the last broad phase */
   int64_t guard_reads;     /* host reads of the device displacement guard */
   int64_t guard_wasted;    /* iterations enqueued after a rebuild fell due (done nothing) */
+  int64_t graph_launches;  /* runs of graph_steps steps launched as one captured graph */
+  int64_t graph_captures;  /* graphs captured (one per device view, cached) */
 } rk_counts;
 
 int rk_engine_create(rk_engine** out, int device);
@@ -79,8 +81,26 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * summed in the reference's serial order too; 0, the default: by a fixed
  * tree, which changes only their reported force and moment by rounding),
  * "gravity_x", "gravity_y", "gravity_z" (the conf keyword gravity, changed
- * between steps). */
+ * between steps),
+ * "guard_batch" (with dynamicUpdateNL and no network: iterations enqueued
+ * between two host reads of the device displacement check; default 16),
+ * "graph_steps" (n > 1: runs of n steps with no host action between them go
+ * out as one HIP graph, captured once per device view and replayed; 0 or 1,
+ * the default: every kernel launched on its own),
+ * launch shapes: "law_group" (geometry blocks per force-law wavefront: 1, 2,
+ * 4, 8; default 4), "geom_block" (contact-kinematics workgroup: 64, 128,
+ * 256, 512; default 256), "gather_waves" (wavefronts per resultant
+ * workgroup: 1, 2, 4; default 4), "body_block" (per-body kernels: 64, 128,
+ * 256; default 256).  No option changes results. */
 int rk_set_option(rk_engine* eng, const char* name, double value);
+int rk_get_option(rk_engine* eng, const char* name, double* value);
+/* Autotuner (no counterpart in the reference) of law_group, geom_block,
+ * gather_waves, body_block and graph_steps, one after the other: the
+ * simulation runs on, trial after trial of steps_per_trial steps, each
+ * candidate value twice, and the fastest (wall time per step, rebuilds
+ * excluded) is kept.  Every candidate computes the same state, so the steps
+ * taken are those rk_step would take; *steps_done receives their number. */
+int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done);
 
 int rk_update_nl(rk_engine* eng);
 int rk_accelerations(rk_engine* eng);

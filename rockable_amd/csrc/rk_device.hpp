@@ -85,6 +85,7 @@ struct DevGuard {
   int frozen;                  // a rebuild is due: later step kernels do nothing
   int pad;
   long long frozen_step;       // the step after which the rebuild is due
+  long long step;              // index of the step the next check ends
 };
 
 struct DevControl {
@@ -169,6 +170,11 @@ struct Dev {
   // (a driven body without force or moment controls)
   const uint8_t* passive;
   int exact_sums;         // 1: serial sums for passive bodies too
+  // launch shapes (any choice gives the same results)
+  int law_group;          // geometry blocks per k_law wavefront (1..8)
+  int geom_block;         // k_geom workgroup size (64, 128, 256, 512)
+  int gather_waves;       // k_gather wavefronts per workgroup (1, 2, 4)
+  int body_block;         // workgroup size of the per-body kernels (64, 128, 256)
   // device displacement guard (dynamicUpdateNL)
   int guard_on;
   DevGuard* guard;
@@ -176,7 +182,7 @@ struct Dev {
   double* gdq;            // 4 per body
   double disp, wstar;
   // active entries of the current force phase: geometry block b's in
-  // akin[b * 256, b * 256 + acount[b])
+  // akin[b * geom_block, b * geom_block + acount[b])
   ActiveKin* akin;
   int* acount;
   // driving
@@ -209,9 +215,14 @@ RK_HD uint32_t pack_sub(int type, int isub, int jsub) {
 void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s);
 void launch_glob(const Dev& d, hipStream_t s);
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s);
-void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s);
+// side (may be null): a second stream on which the workgroup-per-body sums
+// of heavy bodies run beside the main gather; fork / join: events for it
+void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s, hipStream_t side = nullptr,
+                   hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);
-void launch_guard(const Dev& d, const StepParams& sp, long long step, hipStream_t s);
+void launch_guard(const Dev& d, const StepParams& sp, hipStream_t s);
+// the step index the next guard check ends (the checks count on from it)
+void launch_guard_set_step(const Dev& d, long long step, hipStream_t s);
 // part: (nb / 256 + 1) * 9 doubles; feat: 16 floats
 void launch_guard_features(const Dev& d, const StepParams& sp, double disp, double angle, long long since,
                            long long last, double* part, float* feat, hipStream_t s);

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -39,6 +40,14 @@ struct rk_engine {
   int profile = 0;
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // second stream for the heavy-body sums, with its fork / join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int law_group = 4, geom_block = 256, gather_waves = 4, body_block = 256;
+  // host wall time of the rebuilds while the autotuner runs (it synchronises
+  // after each rebuild then)
+  bool tuning = false;
+  double nl_wall_s = 0.0;
   // profile mode: start/end event pairs recorded on the engine stream and
   // resolved lazily, so that timing adds no host synchronisation
   std::vector<hipEvent_t> ev_pool;
@@ -82,6 +91,19 @@ struct rk_engine {
   int64_t guard_reads = 0, guard_wasted = 0;
   int64_t last_rebuild_step = 0, last_interval = 0;
   int exact_sums = 0;
+  // steps enqueued but not yet launched (deferred so that runs of graph_steps
+  // steps go out as one captured graph); with_guard: they end with the
+  // displacement check
+  int graph_steps = 0;
+  int64_t pending = 0;
+  bool pending_guard = false;
+  struct StepGraph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    int64_t used = 0;
+  };
+  std::vector<StepGraph> graphs;
+  int64_t graph_captures = 0, graph_launches = 0, graph_clock = 0;
   EntryArrays E[2];
   int cur = 0;
   NLWork w;
@@ -115,7 +137,14 @@ int upload(rk_engine* eng, T*& dst, const std::vector<T>& src) {
   return 0;
 }
 
+void free_graphs(rk_engine* eng) {
+  for (auto& g : eng->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  eng->graphs.clear();
+}
+
 void free_device(rk_engine* eng) {
+  free_graphs(eng);
   void* ps[] = {eng->kin,      eng->acc,       eng->arot,      eng->force,     eng->moment,   eng->mass,
                 eng->inertia,  eng->bf_surface, eng->cluster,  eng->sh_nv,     eng->sh_ne,    eng->sh_nf,
                 eng->sh_voff,  eng->sh_eoff,   eng->sh_foff,   eng->sh_edge,   eng->sh_fstart, eng->sh_fidx,
@@ -179,6 +208,7 @@ std::vector<uint8_t> passive_flags(const Model& m) {
 
 void assemble_dev(rk_engine* eng) {
   Dev& d = eng->d;
+  memset(&d, 0, sizeof d);  // the captured-step cache compares Dev bytewise
   const Model& m = eng->m;
   EntryArrays& E = eng->E[eng->cur];
   d.nb = (int)m.bodies.size();
@@ -238,6 +268,10 @@ void assemble_dev(rk_engine* eng) {
   d.wstar = eng->wstar;
   d.acount = E.acount;
   d.exact_sums = eng->exact_sums;
+  d.law_group = eng->law_group;
+  d.geom_block = eng->geom_block;
+  d.gather_waves = eng->gather_waves;
+  d.body_block = eng->body_block;
   d.istart = E.istart;
   d.jstart = E.jstart;
   d.jlist = E.jlist;
@@ -632,7 +666,7 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom) {
   launch_interact(eng->d, sp, write_geom || eng->record_geometry, eng->stream);
   toc(eng, "interact", 2);
   tic(eng, 2);
-  launch_gather(eng->d, sp, kick, eng->stream);
+  launch_gather(eng->d, sp, kick, eng->stream, eng->side, eng->ev_fork, eng->ev_join);
   toc(eng, "gather", 2);
   toc(eng, "force", 1);
   if (eng->m.law == LAW_STICKED && eng->live_interfaces > 0) {
@@ -664,6 +698,8 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom) {
 }
 
 int do_update_nl(rk_engine* eng) {
+  if (eng->tuning) HIPCHK(hipStreamSynchronize(eng->stream));
+  const auto t0 = std::chrono::steady_clock::now();
   ensure_gv(eng);
   tic(eng, eng->profile);
   const int nxt = 1 - eng->cur;
@@ -675,6 +711,10 @@ int do_update_nl(rk_engine* eng) {
   eng->last_interval = eng->steps - eng->last_rebuild_step;
   eng->last_rebuild_step = eng->steps;
   toc(eng, "nl", eng->profile);
+  if (eng->tuning) {
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    eng->nl_wall_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
   return 0;
 }
 
@@ -685,6 +725,108 @@ int vv_step(rk_engine* eng, int write_geom) {
   toc(eng, "predict", 2);
   eng->gv_dirty = true;
   return run_accelerations(eng, 1, write_geom);
+}
+
+// One step of the loop as the device runs it (write_geom 0): the
+// velocity-Verlet step, and the displacement check with dynamicUpdateNL.
+int launch_one_step(rk_engine* eng, bool with_guard) {
+  if (vv_step(eng, 0) != 0) return -1;
+  if (with_guard) launch_guard(eng->d, step_params(eng), eng->stream);
+  return 0;
+}
+
+// What a captured run of steps depends on: the device view (pointers and
+// sizes), the step parameters and the launch options.
+std::vector<unsigned char> graph_key(const rk_engine* eng, int n, bool with_guard) {
+  const StepParams sp = step_params(eng);
+  const double spv[6] = {sp.dt, sp.dt_2, sp.dt2_2, sp.gx, sp.gy, sp.gz};
+  const int iv[5] = {n, with_guard ? 1 : 0, sp.law, eng->record_geometry, eng->live_interfaces};
+  std::vector<unsigned char> k(sizeof(Dev) + sizeof spv + sizeof iv);
+  memcpy(k.data(), &eng->d, sizeof(Dev));
+  memcpy(k.data() + sizeof(Dev), spv, sizeof spv);
+  memcpy(k.data() + sizeof(Dev) + sizeof spv, iv, sizeof iv);
+  return k;
+}
+
+// A graph of n steps for the current device view, from the cache (the last
+// few views; the two entry buffers alternate at each rebuild) or captured
+// now.  Capturing launches nothing.
+int step_graph(rk_engine* eng, int n, bool with_guard, hipGraphExec_t* out) {
+  std::vector<unsigned char> key = graph_key(eng, n, with_guard);
+  eng->graph_clock++;
+  for (auto& g : eng->graphs)
+    if (g.key == key) {
+      g.used = eng->graph_clock;
+      *out = g.exec;
+      return 0;
+    }
+  HIPCHK(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < n && rc == 0; k++) rc = launch_one_step(eng, with_guard);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(eng->stream, &graph);
+  eng->gv_dirty = false;  // each step of the graph ends with fresh global vertices
+  if (rc != 0 || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(eng, rc != 0 ? eng->err : std::string("step capture: ") + hipGetErrorString(ec));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(eng, std::string("step graph: ") + hipGetErrorString(ei));
+  eng->graph_captures++;
+  if (eng->graphs.size() >= 4) {
+    size_t old = 0;
+    for (size_t q = 1; q < eng->graphs.size(); q++)
+      if (eng->graphs[q].used < eng->graphs[old].used) old = q;
+    (void)hipGraphExecDestroy(eng->graphs[old].exec);
+    eng->graphs.erase(eng->graphs.begin() + old);
+  }
+  rk_engine::StepGraph g;
+  g.key = std::move(key);
+  g.exec = exec;
+  g.used = eng->graph_clock;
+  eng->graphs.push_back(std::move(g));
+  *out = exec;
+  return 0;
+}
+
+// Launch the deferred steps: whole runs of graph_steps as captured graphs,
+// the rest one by one.  Every host action that reads or changes the device
+// state (rebuild, conf file, guard read, counts) comes after a flush.
+int flush_steps(rk_engine* eng) {
+  int64_t n = eng->pending;
+  const bool with_guard = eng->pending_guard;
+  eng->pending = 0;
+  const int G = eng->graph_steps;
+  if (G > 1 && n >= G && eng->profile == 0) {
+    hipGraphExec_t exec = nullptr;
+    if (step_graph(eng, G, with_guard, &exec) != 0) return -1;
+    for (; n >= G; n -= G) {
+      HIPCHK(hipGraphLaunch(exec, eng->stream));
+      eng->graph_launches++;
+    }
+  }
+  for (; n > 0; n--)
+    if (launch_one_step(eng, with_guard) != 0) return -1;
+  return 0;
+}
+
+// One step of the loop from the host's side: deferred when nothing on the
+// host depends on it (no conf file written at it, no bonds whose breakage
+// the host accounts), launched now otherwise.
+int enqueue_step(rk_engine* eng, bool write_geom, bool with_guard) {
+  const bool bonds = eng->m.law == LAW_STICKED && eng->live_interfaces > 0;
+  if (eng->pending > 0 && eng->pending_guard != with_guard && flush_steps(eng) != 0) return -1;
+  if (write_geom || bonds || eng->graph_steps <= 1) {
+    if (flush_steps(eng) != 0) return -1;
+    if (vv_step(eng, write_geom ? 1 : 0) != 0) return -1;
+    if (with_guard) launch_guard(eng->d, step_params(eng), eng->stream);
+    return 0;
+  }
+  eng->pending_guard = with_guard;
+  eng->pending++;
+  return 0;
 }
 
 int download_bodies(rk_engine* eng) {
@@ -803,7 +945,7 @@ int loop_body(rk_engine* eng, const char* outdir) {
   m.interConfC += m.dt;
   m.interVerletC += m.dt;
   const bool conf_due = m.interConfC >= m.interConf - m.dt_2;
-  if (vv_step(eng, (conf_due && outdir) ? 1 : 0) != 0) return -1;
+  if (enqueue_step(eng, conf_due && outdir, false) != 0) return -1;
   eng->steps++;
   if (conf_due) {
     m.iconf++;
@@ -815,6 +957,7 @@ int loop_body(rk_engine* eng, const char* outdir) {
     m.interConfC = 0.0;
   }
   if (eng->needUpdate || m.interVerletC >= m.interVerlet - m.dt_2) {
+    if (flush_steps(eng) != 0) return -1;
     if (do_update_nl(eng) != 0) return -1;
     if (!eng->needUpdate)
       m.interVerletC = 0.0;
@@ -894,15 +1037,14 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     const int64_t B = std::max<int64_t>(1, std::min<int64_t>(bonds ? 1 : want, nmax - done));
     hist.assign(1, loop_scalars(eng));
     conf_at.assign(1, 0);
-    const StepParams sp0 = step_params(eng);
+    launch_guard_set_step(eng->d, (long long)eng->steps, eng->stream);
     int64_t k = 0;
     while (k < B) {
       m.t += m.dt;
       m.interConfC += m.dt;
       m.interVerletC += m.dt;
       const bool conf_due = m.interConfC >= m.interConf - m.dt_2;
-      if (vv_step(eng, (conf_due && outdir) ? 1 : 0) != 0) return -1;
-      launch_guard(eng->d, sp0, (long long)eng->steps, eng->stream);
+      if (enqueue_step(eng, conf_due && outdir, true) != 0) return -1;
       eng->steps++;
       if (conf_due) {
         m.iconf++;
@@ -915,6 +1057,7 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
           (until_tmax && m.t > m.tmax))
         break;
     }
+    if (flush_steps(eng) != 0) return -1;
     DevGuard g;
     HIPCHK(hipMemcpyAsync(&g, eng->guard, sizeof g, hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
@@ -960,7 +1103,9 @@ int rk_engine_create(rk_engine** out, int device) {
   rk_engine* eng = new rk_engine();
   eng->device = device;
   if (hipStreamCreate(&eng->stream) != hipSuccess || hipEventCreate(&eng->ev0) != hipSuccess ||
-      hipEventCreate(&eng->ev1) != hipSuccess) {
+      hipEventCreate(&eng->ev1) != hipSuccess || hipStreamCreate(&eng->side) != hipSuccess ||
+      hipEventCreateWithFlags(&eng->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&eng->ev_join, hipEventDisableTiming) != hipSuccess) {
     delete eng;
     return -2;
   }
@@ -976,6 +1121,10 @@ void rk_engine_destroy(rk_engine* eng) {
   for (hipEvent_t e : eng->ev_pool) (void)hipEventDestroy(e);
   if (eng->ev0) (void)hipEventDestroy(eng->ev0);
   if (eng->ev1) (void)hipEventDestroy(eng->ev1);
+  if (eng->ev_fork) (void)hipEventDestroy(eng->ev_fork);
+  if (eng->ev_join) (void)hipEventDestroy(eng->ev_join);
+  if (eng->side) (void)hipStreamSynchronize(eng->side);
+  if (eng->side) (void)hipStreamDestroy(eng->side);
   if (eng->stream) (void)hipStreamDestroy(eng->stream);
   delete eng;
 }
@@ -1097,6 +1246,31 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     // the conf keyword "gravity" (Rockable.cpp:978-980), changed between steps
     if (check_loaded(eng)) return -1;
     (n == "gravity_x" ? eng->m.gravity.x : n == "gravity_y" ? eng->m.gravity.y : eng->m.gravity.z) = value;
+  } else if (n == "law_group" || n == "geom_block" || n == "gather_waves" || n == "body_block") {
+    const int v = (int)value;
+    if ((double)v != value) return fail(eng, n + " must be an integer");
+    if (n == "law_group") {
+      if (v != 1 && v != 2 && v != 4 && v != 8) return fail(eng, "law_group must be 1, 2, 4 or 8");
+      eng->law_group = v;
+    } else if (n == "geom_block") {
+      if (v != 64 && v != 128 && v != 256 && v != 512) return fail(eng, "geom_block must be 64, 128, 256 or 512");
+      eng->geom_block = v;
+    } else if (n == "gather_waves") {
+      if (v != 1 && v != 2 && v != 4) return fail(eng, "gather_waves must be 1, 2 or 4");
+      eng->gather_waves = v;
+    } else {
+      if (v != 64 && v != 128 && v != 256) return fail(eng, "body_block must be 64, 128 or 256");
+      eng->body_block = v;
+    }
+    if (eng->loaded) assemble_dev(eng);
+  } else if (n == "graph_steps") {
+    if (!(value >= 0.0 && value <= 4096.0)) return fail(eng, "graph_steps must be in [0, 4096]");
+    if (eng->pending > 0 && eng->loaded) {
+      (void)hipSetDevice(eng->device);
+      if (flush_steps(eng) != 0) return -1;
+    }
+    eng->graph_steps = (int)value;
+    free_graphs(eng);
   } else if (n == "guard_batch") {
     if (!(value >= 1.0 && value <= 1e6)) return fail(eng, "guard_batch must be in [1, 1e6]");
     eng->guard_batch = (int)value;
@@ -1135,6 +1309,104 @@ int rk_step(rk_engine* eng, int64_t nsteps) {
   if (eng->guard) return run_guarded(eng, nullptr, nsteps, false, nullptr);
   for (int64_t k = 0; k < nsteps; k++)
     if (loop_body(eng, nullptr) != 0) return -1;
+  return flush_steps(eng);
+}
+
+namespace {
+int* tuned_option(rk_engine* eng, const std::string& n) {
+  if (n == "law_group") return &eng->law_group;
+  if (n == "geom_block") return &eng->geom_block;
+  if (n == "gather_waves") return &eng->gather_waves;
+  if (n == "body_block") return &eng->body_block;
+  if (n == "graph_steps") return &eng->graph_steps;
+  return nullptr;
+}
+}  // namespace
+
+int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
+  if (check_loaded(eng)) return -1;
+  if (steps_per_trial < 1) return fail(eng, "steps_per_trial must be positive");
+  (void)hipSetDevice(eng->device);
+  if (eng->profile != 0) return fail(eng, "autotune needs profile 0");
+  struct Knob {
+    const char* name;
+    std::vector<int> values;
+  };
+  const Knob knobs[] = {{"law_group", {1, 2, 4, 8}},
+                        {"geom_block", {64, 128, 256, 512}},
+                        {"gather_waves", {1, 2, 4}},
+                        {"body_block", {64, 128, 256}},
+                        {"graph_steps", {0, 8, 32}}};
+  int64_t done = 0;
+  eng->tuning = true;
+  // wall seconds per step of one trial, the rebuilds excluded
+  auto trial = [&](double* per_step) -> int {
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    const double nl0 = eng->nl_wall_s;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (rk_step(eng, steps_per_trial) != 0) return -1;
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    const double w = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    done += steps_per_trial;
+    *per_step = (w - (eng->nl_wall_s - nl0)) / steps_per_trial;
+    return 0;
+  };
+  int rc = 0;
+  for (const Knob& k : knobs) {
+    std::vector<double> best(k.values.size(), 1e30);
+    for (int rep = 0; rep < 2 && rc == 0; rep++)
+      for (size_t c = 0; c < k.values.size() && rc == 0; c++) {
+        if (rk_set_option(eng, k.name, k.values[c]) != 0) {
+          rc = -1;
+          break;
+        }
+        double t = 0.0;
+        // a first short run captures the graphs of the new shape, then the trial
+        const int warm = std::max(1, eng->graph_steps);
+        if (rk_step(eng, warm) != 0 || trial(&t) != 0) {
+          rc = -1;
+          break;
+        }
+        done += warm;
+        best[c] = std::min(best[c], t);
+      }
+    if (rc != 0) break;
+    size_t b = 0;
+    for (size_t c = 1; c < best.size(); c++)
+      if (best[c] < best[b]) b = c;
+    if (rk_set_option(eng, k.name, k.values[b]) != 0) {
+      rc = -1;
+      break;
+    }
+  }
+  eng->tuning = false;
+  if (steps_done) *steps_done = done;
+  return rc;
+}
+
+int rk_get_option(rk_engine* eng, const char* name, double* value) {
+  if (!eng || !name || !value) return -1;
+  const std::string n = name;
+  if (int* p = tuned_option(eng, n))
+    *value = *p;
+  else if (n == "record_geometry")
+    *value = eng->record_geometry;
+  else if (n == "profile")
+    *value = eng->profile;
+  else if (n == "exact_sums")
+    *value = eng->exact_sums;
+  else if (n == "heavy_min")
+    *value = eng->w.heavy_min;
+  else if (n == "guard_batch")
+    *value = eng->guard_batch;
+  else if (n == "gravity_x")
+    *value = eng->m.gravity.x;
+  else if (n == "gravity_y")
+    *value = eng->m.gravity.y;
+  else if (n == "gravity_z")
+    *value = eng->m.gravity.z;
+  else
+    return fail(eng, "unknown option " + n);
   return 0;
 }
 
@@ -1155,6 +1427,7 @@ int rk_integrate(rk_engine* eng, const char* outdir, int64_t max_steps, int64_t*
       if (loop_body(eng, outdir) != 0) return -1;
       n++;
     }
+    if (flush_steps(eng) != 0) return -1;
   }
   if (steps_done) *steps_done = n;
   HIPCHK(hipStreamSynchronize(eng->stream));
@@ -1195,6 +1468,8 @@ int rk_get_counts(rk_engine* eng, rk_counts* out) {
   out->last_candidates = eng->w.last_candidates;
   out->guard_reads = eng->guard_reads;
   out->guard_wasted = eng->guard_wasted;
+  out->graph_launches = eng->graph_launches;
+  out->graph_captures = eng->graph_captures;
   return 0;
 }
 

@@ -1406,8 +1406,8 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
             hipMalloc(&E.spos, c * 2 * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&E.sact, c * 2 * sizeof(uint8_t)) == hipSuccess &&
             hipMalloc(&E.sval, c * 12 * sizeof(double)) == hipSuccess &&
-            hipMalloc(&E.akin, (c + 256) * sizeof(ActiveKin)) == hipSuccess &&
-            hipMalloc(&E.acount, (c / 256 + 2) * sizeof(int32_t)) == hipSuccess;
+            hipMalloc(&E.akin, (c + 512) * sizeof(ActiveKin)) == hipSuccess &&
+            hipMalloc(&E.acount, (c / 64 + 2) * sizeof(int32_t)) == hipSuccess;
   if (!ok) {
     snprintf(w.err, sizeof(w.err), "hipMalloc of %zu interaction entries failed", c);
     entries_free(E);

@@ -26,8 +26,8 @@ struct EntryArrays {
   int32_t* spos = nullptr;   // 2 per entry
   uint8_t* sact = nullptr;   // 2 per entry
   double* sval = nullptr;    // 12 per entry
-  ActiveKin* akin = nullptr; // 1 per entry
-  int32_t* acount = nullptr; // 1 per 256 entries
+  ActiveKin* akin = nullptr; // 1 per entry (+512)
+  int32_t* acount = nullptr; // 1 per geometry block (64 entries or more)
   int32_t* istart = nullptr;
   size_t istart_cap = 0;
   int32_t* jstart = nullptr;

@@ -52,6 +52,17 @@ __device__ __forceinline__ void st3(double* p, v3 v) {
 
 // A rebuild found due by the device guard suspends the step kernels until
 // the host has rebuilt the list (every thread reads the same flag).
+// Workgroups are handed to the 8 XCDs round-robin in dispatch order (XCD =
+// block mod 8), and each XCD has its own L2.  xcd_block maps the blocks one
+// XCD runs onto one contiguous range of [0, n) (a bijection), so that the
+// bodies a block's entries read are mostly those its XCD's L2 already holds
+// (entries are sorted by body, and neighbouring indices are near in space).
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int x = b & 7, k = b >> 3;
+  const int q = n >> 3, r = n & 7;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 __device__ __forceinline__ bool frozen(const Dev& d) {
   return d.guard_on && *(volatile const int*)&d.guard->frozen;
 }
@@ -348,14 +359,38 @@ __device__ void law_sticked(const Dev& d, const StepParams& sp, const Kin& k, do
 //  k_law    every active entry: relative velocity, the force law, the history
 //           and the two resultant slots.
 // The active entries of geometry block b go, in entry order, to
-// akin[b * kBlock, b * kBlock + acount[b]) (ballot + block prefix, no
+// akin[b * BS, b * BS + acount[b]) (ballot + block prefix, no
 // atomics); k_law gives one wavefront to each block's list.
-template <bool GEOM_ALL>
-__global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
-  __shared__ int wcount[kBlock / 64];
+template <bool GEOM_ALL, int BS>
+__global__ void __launch_bounds__(BS) k_geom(Dev d) {
+  __shared__ int wcount[BS / 64];
   if (frozen(d)) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int wtype[BS / 64][5];
+  __shared__ short perm[BS];
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // The block's entries are taken in the order of their contact type (a
+  // counting sort in LDS; entries past the end last), so that a wavefront
+  // runs one branch of update_kinematics instead of all four.
+  {
+    const int e0 = blk * BS + (int)threadIdx.x;
+    const int ty = e0 < d.ne ? (int)(d.esub[e0] >> 30) : 4;
+    unsigned long long mine = 0ull;
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      const unsigned long long m = __ballot(ty == t);
+      if (ty == t) mine = m;
+      if (lane == 0) wtype[wave][t] = __popcll(m);
+    }
+    __syncthreads();
+    int rank = 0;
+    for (int t = 0; t < 5; t++)
+      for (int w = 0; w < BS / 64; w++) rank += (t < ty || (t == ty && w < wave)) ? wtype[w][t] : 0;
+    rank += __popcll(mine & ((1ull << lane) - 1ull));
+    perm[rank] = (short)threadIdx.x;
+    __syncthreads();
+  }
+  const int e = blk * BS + perm[threadIdx.x];
   bool act = false;
   ActiveKin a;
   if (e < d.ne) {
@@ -437,23 +472,45 @@ __global__ void __launch_bounds__(kBlock) k_geom(Dev d) {
   if (lane == 0) wcount[wave] = __popcll(m);
   __syncthreads();
   int base = 0, total = 0;
-  for (int w = 0; w < kBlock / 64; w++) {
+  for (int w = 0; w < BS / 64; w++) {
     base += w < wave ? wcount[w] : 0;
     total += wcount[w];
   }
-  if (act) d.akin[(size_t)blockIdx.x * kBlock + base + __popcll(m & ((1ull << lane) - 1ull))] = a;
-  if (threadIdx.x == 0) d.acount[blockIdx.x] = total;
+  if (act) d.akin[(size_t)blk * BS + base + __popcll(m & ((1ull << lane) - 1ull))] = a;
+  if (threadIdx.x == 0) d.acount[blk] = total;
 }
 
+// Each wavefront takes W (at most kLawGroup) consecutive geometry blocks
+// and spreads their active entries over its lanes, so that lanes are not left
+// idle by blocks with few active entries.
+constexpr int kLawGroup = 8;
+
 template <int LAW>
-__global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_geom_all, int pii, int ngeom) {
+__global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_geom_all, int pii, int ngeom, int W) {
   if (frozen(d)) return;
-  const int g = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);  // geometry block
-  if (g >= ngeom) return;
+  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int g0 = (int)((lb * (unsigned)blockDim.x + threadIdx.x) >> 6) * W;  // first geometry block
+  if (g0 >= ngeom) return;
   const int lane = threadIdx.x & 63;
-  const int count = d.acount[g];
-  for (int q = lane; q < count; q += 64) {
-    const ActiveKin a = d.akin[(size_t)g * kBlock + q];
+  // inclusive prefix sums of the active counts of the W blocks (lanes < W)
+  int incl = (lane < W && g0 + lane < ngeom) ? d.acount[g0 + lane] : 0;
+  for (int o = 1; o < kLawGroup; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  int pre[kLawGroup];
+#pragma unroll
+  for (int m = 0; m < kLawGroup; m++) pre[m] = __shfl(incl, m, 64);
+  const int total = pre[kLawGroup - 1];
+  for (int t = lane; t < total; t += 64) {
+    int gb = 0, before = 0;
+#pragma unroll
+    for (int m = 0; m < kLawGroup; m++)
+      if (pre[m] <= t) {
+        gb = m + 1;
+        before = pre[m];
+      }
+    const ActiveKin a = d.akin[(size_t)(g0 + gb) * d.geom_block + (t - before)];
     const int e = a.e;
     const int i = d.ei[e], j = d.ej[e];
     const BodyKin Pi = d.kin[i];
@@ -609,8 +666,8 @@ __device__ __forceinline__ double pick(const v3& F, const v3& M, int c) {
 
 // Stores the resultants, compute_accelerations_from_resultants
 // (Rockable.cpp:3419-3552) and the closing half kick (2120-2155).
-__device__ void finish_body(const Dev& d, const StepParams& sp, int kick, int b, BodyKin P, bool driven, v3 F, v3 M,
-                            int c0, int c1) {
+__device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, int kick, int b, BodyKin P,
+                                            bool driven, v3 F, v3 M, int c0, int c1) {
   st3(d.force + 3 * (size_t)b, F);
   st3(d.moment + 3 * (size_t)b, M);
   const double m = d.mass[b];
@@ -714,7 +771,8 @@ constexpr int kBodiesPerWave = 10;
 __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
   if (frozen(d)) return;
   const int lane = threadIdx.x & 63;
-  const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int wave = (int)((lb * (unsigned)blockDim.x + threadIdx.x) >> 6);
   const int slot = lane / kLanesPerBody;
   const int comp = lane - kLanesPerBody * slot;
   const int b = wave * kBodiesPerWave + slot;
@@ -900,7 +958,7 @@ __global__ void __launch_bounds__(kBlock) k_guard_acc(Dev d, StepParams sp) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < kBlock / 64; k++) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
       kp = rp[k] > kp ? rp[k] : kp;
       kw = rw[k] < kw ? rw[k] : kw;
     }
@@ -911,10 +969,13 @@ __global__ void __launch_bounds__(kBlock) k_guard_acc(Dev d, StepParams sp) {
 
 // the end of a step of the integrate() loop (Rockable.cpp:2690-2740): a
 // needUpdate left by the previous check makes the rebuild due now (the step
-// kernels then wait for the host); then the check itself
-__global__ void k_guard_check(Dev d, long long step) {
+// kernels then wait for the host); then the check itself.  The step index
+// lives on the device, so that a captured sequence of steps replays as is.
+__global__ void k_guard_check(Dev d) {
   if (threadIdx.x != 0 || frozen(d)) return;
   DevGuard* G = d.guard;
+  const long long step = G->step;
+  G->step = step + 1;
   const int rebuild_now = G->nu;
   G->nu = 0;
   G->reset = 0;
@@ -1221,45 +1282,84 @@ __global__ void k_breakage(Dev d) {
 
 }  // namespace
 
+int body_block(const Dev& d) { return d.body_block == 64 || d.body_block == 128 ? d.body_block : 256; }
+
 void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s) {
-  const int n = d.nb > 0 ? d.nb : 1;
-  hipLaunchKernelGGL(k_predict, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp);
+  const int n = d.nb > 0 ? d.nb : 1, bs = body_block(d);
+  hipLaunchKernelGGL(k_predict, dim3((n + bs - 1) / bs), dim3(bs), 0, s, d, sp);
 }
 
 void launch_glob(const Dev& d, hipStream_t s) {
   if (d.nvtot <= 0) return;
-  hipLaunchKernelGGL(k_glob, dim3((d.nvtot + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d);
+  const int bs = body_block(d);
+  hipLaunchKernelGGL(k_glob, dim3((d.nvtot + bs - 1) / bs), dim3(bs), 0, s, d);
+}
+
+template <int BS>
+void launch_geom(const Dev& d, int write_geom_all, hipStream_t s) {
+  const dim3 g((d.ne + BS - 1) / BS), bl(BS);
+  if (write_geom_all)
+    hipLaunchKernelGGL((k_geom<true, BS>), g, bl, 0, s, d);
+  else
+    hipLaunchKernelGGL((k_geom<false, BS>), g, bl, 0, s, d);
 }
 
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
   if (d.ne <= 0) return;
-  dim3 g((d.ne + kBlock - 1) / kBlock), bl(kBlock);
-  if (write_geom_all)
-    hipLaunchKernelGGL(k_geom<true>, g, bl, 0, s, d);
-  else
-    hipLaunchKernelGGL(k_geom<false>, g, bl, 0, s, d);
-  // one wavefront per geometry block
-  const int ngeom = (int)g.x;
-  dim3 gl((ngeom + kBlock / 64 - 1) / (kBlock / 64));
+  switch (d.geom_block) {
+    case 64: launch_geom<64>(d, write_geom_all, s); break;
+    case 128: launch_geom<128>(d, write_geom_all, s); break;
+    case 512: launch_geom<512>(d, write_geom_all, s); break;
+    default: launch_geom<256>(d, write_geom_all, s); break;
+  }
+  const dim3 bl(kBlock);
+  // W geometry blocks per wavefront
+  const int gbs = (d.geom_block == 64 || d.geom_block == 128 || d.geom_block == 512) ? d.geom_block : 256;
+  const int ngeom = (d.ne + gbs - 1) / gbs;
+  const int W = std::min(std::max(d.law_group, 1), kLawGroup);
+  const int waves = (ngeom + W - 1) / W;
+  dim3 gl((waves + kBlock / 64 - 1) / (kBlock / 64));
   if (sp.law == LAW_AVALANCHE)
-    hipLaunchKernelGGL(k_law<LAW_AVALANCHE>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom);
+    hipLaunchKernelGGL(k_law<LAW_AVALANCHE>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom, W);
   else if (sp.law == LAW_STICKED)
-    hipLaunchKernelGGL(k_law<LAW_STICKED>, gl, bl, 0, s, d, sp, write_geom_all, d.paramsInInterfaces, ngeom);
+    hipLaunchKernelGGL(k_law<LAW_STICKED>, gl, bl, 0, s, d, sp, write_geom_all, d.paramsInInterfaces, ngeom, W);
   else
-    hipLaunchKernelGGL(k_law<LAW_DEFAULT>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom);
+    hipLaunchKernelGGL(k_law<LAW_DEFAULT>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom, W);
 }
 
-void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) {
+void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s, hipStream_t side, hipEvent_t fork,
+                   hipEvent_t join) {
   if (d.nb <= 0) return;
   const int waves = (d.nb + kBodiesPerWave - 1) / kBodiesPerWave;
-  const int blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
-  hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(kBlock), 0, s, d, sp, kick);
-  if (d.nheavy > 0) hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
+  const int wpb = d.gather_waves == 1 || d.gather_waves == 2 ? d.gather_waves : 4;
+  const int blocks = (waves + wpb - 1) / wpb;
+  // the two kernels sum disjoint sets of bodies from the same slots
+  const bool split = d.nheavy > 0 && side && fork && join;
+  if (split) {
+    (void)hipEventRecord(fork, s);
+    (void)hipStreamWaitEvent(side, fork, 0);
+    hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, side, d, sp, kick);
+    (void)hipEventRecord(join, side);
+  }
+  hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(64 * wpb), 0, s, d, sp, kick);
+  if (split)
+    (void)hipStreamWaitEvent(s, join, 0);
+  else if (d.nheavy > 0)
+    hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
 }
 
-void launch_guard(const Dev& d, const StepParams& sp, long long step, hipStream_t s) {
-  if (d.nb > 0) hipLaunchKernelGGL(k_guard_acc, dim3((d.nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, sp);
-  hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(64), 0, s, d, step);
+__global__ void k_guard_set_step(Dev d, long long step) {
+  if (threadIdx.x == 0) d.guard->step = step;
+}
+
+void launch_guard(const Dev& d, const StepParams& sp, hipStream_t s) {
+  const int bs = body_block(d);
+  if (d.nb > 0) hipLaunchKernelGGL(k_guard_acc, dim3((d.nb + bs - 1) / bs), dim3(bs), 0, s, d, sp);
+  hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(64), 0, s, d);
+}
+
+void launch_guard_set_step(const Dev& d, long long step, hipStream_t s) {
+  hipLaunchKernelGGL(k_guard_set_step, dim3(1), dim3(64), 0, s, d, step);
 }
 
 void launch_guard_features(const Dev& d, const StepParams& sp, double disp, double angle, long long since,

@@ -507,6 +507,93 @@ def compact_pour_deck(n_bodies=50000, seed=2, n_shapes=64, size_range=(4e-3, 6e-
     return wall_shapes + pool, deck
 
 
+def quat_matrices(q):
+    """Rotation matrices of an (n, 4) array of unit quaternions (w, x, y, z)."""
+    w, x, y, z = (q[:, k] for k in range(4))
+    R = np.empty((len(q), 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - w * z)
+    R[:, 0, 2] = 2 * (x * z + w * y)
+    R[:, 1, 0] = 2 * (x * y + w * z)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - w * x)
+    R[:, 2, 0] = 2 * (x * z - w * y)
+    R[:, 2, 1] = 2 * (y * z + w * x)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def compact_layout(pool, shape_idx, sizes, q, floor, gap):
+    """Positions packing bodies row by row (along x), layer by layer from
+    z = 0, each body's rotated OBB box `gap` apart from its neighbours, on a
+    floor of floor = (Lx, Ly).  Returns (pos, top of the last layer)."""
+    ext = np.array([s.extent() for s in pool])[shape_idx] * sizes[:, None]
+    cen = np.array([s.obb_center for s in pool])[shape_idx] * sizes[:, None]
+    R = quat_matrices(q)
+    c = np.einsum("nij,nj->ni", R, cen)
+    half = np.einsum("nij,nj->ni", np.abs(R), ext)
+    pos = np.zeros((len(sizes), 3))
+    x = y = z = row_h = lay_h = 0.0
+    Lx, Ly = floor
+    hx, hy, hz = (half[:, k].tolist() for k in range(3))
+    for k in range(len(sizes)):
+        if x + 2 * hx[k] > Lx:
+            x = 0.0
+            y += row_h + gap
+            row_h = 0.0
+        if y + 2 * hy[k] > Ly:
+            y = 0.0
+            z += lay_h + gap
+            lay_h = row_h = 0.0
+        pos[k] = (x + hx[k], y + hy[k], z + hz[k])
+        x += 2 * hx[k] + gap
+        row_h = max(row_h, 2 * hy[k])
+        lay_h = max(lay_h, 2 * hz[k])
+    return pos - c, z + lay_h
+
+
+def avalanche_deck(n_bodies=1000000, seed=3, n_shapes=64, size_range=(4e-3, 6e-3), floor_in_sizes=(400, 100),
+                   slope_deg=30.0, kn=1e7, en=0.3, mu=0.6, dt=2e-7, steps=20000, vmax=0.5, flow_speed=1.0,
+                   law="Avalanche", gap_frac=0.02, tilt_deg=15.0, disp_frac=0.25, angle_deg=57.29577951308232):
+    """Learned-rebuild workload: a polyhedral avalanche of the plain GPU form's
+    bodies (hulls of 8-14 points on an ellipsoid 1 : 0.8 : 0.6, size uniform
+    in size_range, Minkowski radius 5 %) packed compactly on the floor of a
+    box floor_in_sizes mean sizes long (x) and wide (y), principal axes along
+    the box axes up to tilt_deg, sent down a slope of slope_deg (gravity
+    tilted about y, downslope along +x) at flow_speed plus velocities uniform
+    in [-vmax, vmax]; linkCells; the neighbour list rebuilt by the
+    displacement check only (dynamicUpdateNL with dispUpdateNL = disp_frac
+    DVerlet and angleUpdateNL = angle_deg, by default 1 rad as in Rockable;
+    interVerlet beyond the run)."""
+    rng = np.random.default_rng(seed)
+    pool = hull_shape_pool(n_shapes, seed)
+    smean = 0.5 * (size_range[0] + size_range[1])
+    floor = np.array(floor_in_sizes, float) * smean
+    sizes = np.sort(rng.uniform(*size_range, size=n_bodies))[::-1]
+    shape_idx = rng.integers(len(pool), size=n_bodies)
+    q = small_rotations(rng, n_bodies, np.radians(tilt_deg))
+    vel = rng.uniform(-vmax, vmax, size=(n_bodies, 3))
+    vel[:, 0] += flow_speed
+    pos, top = compact_layout(pool, shape_idx, sizes, q, floor, gap_frac * smean)
+    wall_shapes, walls = box_walls(np.zeros(3), np.array([floor[0], floor[1], top + 4 * size_range[1]]),
+                                   0.05 * smean)
+    th = np.radians(slope_deg)
+    deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=10.0 * steps * dt,
+                interConf=1e3, DVerlet=0.1 * smean, dVerlet=0.05 * smean, density={0: 2700.0, 1: 2700.0},
+                gravity=(9.81 * np.sin(th), 0.0, -9.81 * np.cos(th)), update_nl="linkCells", nDriven=len(walls),
+                force_law=law, params=contact_params(kn, 0.8 * kn, en * en, mu, groups=((0, 0), (0, 1))))
+    cell = 1.1 * (2 * max(float(np.max(s.extent())) for s in pool) * size_range[1]) + 0.1 * smean
+    deck.cell_min_sizes = (cell, cell, cell)
+    deck.box_for_linkcells_opt = 1
+    deck.dynamic_update_nl = 1
+    deck.disp_update_nl = disp_frac * deck.DVerlet
+    deck.angle_update_nl = angle_deg
+    deck.bodies = list(walls)
+    names = np.array([s.name for s in pool])[shape_idx]
+    deck.body_block = {"names": names, "group": np.zeros(n_bodies), "h": sizes, "pos": pos, "vel": vel, "Q": q}
+    return wall_shapes + pool, deck
+
+
 def write_deck(dirpath, shapes, deck, name="deck.conf"):
     import os
     os.makedirs(dirpath, exist_ok=True)

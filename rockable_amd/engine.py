@@ -26,7 +26,8 @@ class RkCounts(ctypes.Structure):
                 ("nactive", ctypes.c_int64), ("ninterfaces", ctypes.c_int64), ("nl_updates", ctypes.c_int64),
                 ("steps", ctypes.c_int64), ("t", ctypes.c_double), ("dt", ctypes.c_double),
                 ("total_broken_area", ctypes.c_double), ("last_candidates", ctypes.c_int64),
-                ("guard_reads", ctypes.c_int64), ("guard_wasted", ctypes.c_int64)]
+                ("guard_reads", ctypes.c_int64), ("guard_wasted", ctypes.c_int64),
+                ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64)]
 
 
 EXPORTED = ["rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_conf", "rk_load_driving_system",
@@ -35,7 +36,7 @@ EXPORTED = ["rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_c
             "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time",
             "rk_mlp_create", "rk_mlp_destroy", "rk_mlp_last_error", "rk_mlp_nparams", "rk_mlp_get_params",
             "rk_mlp_set_params", "rk_mlp_forward", "rk_mlp_loss_grad", "rk_mlp_train", "rk_set_rebuild_model",
-            "rk_guard_features"]
+            "rk_guard_features", "rk_get_option", "rk_autotune"]
 
 _lib = None
 
@@ -85,6 +86,8 @@ def load_library(path=LIB_PATH):
                                F]
     L.rk_set_rebuild_model.argtypes = [P, P]
     L.rk_guard_features.argtypes = [P, F]
+    L.rk_get_option.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
+    L.rk_autotune.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     _lib = L
     return L
 
@@ -139,6 +142,20 @@ class Engine:
 
     def set_option(self, name, value):
         self._check(self.L.rk_set_option(self.h, name.encode(), float(value)))
+
+    def get_option(self, name):
+        v = ctypes.c_double(0.0)
+        self._check(self.L.rk_get_option(self.h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    TUNED = ("law_group", "geom_block", "gather_waves", "body_block", "graph_steps")
+
+    def autotune(self, steps_per_trial=100):
+        """Pick the launch shapes by timing them on the running simulation
+        (the steps taken are real steps); returns (steps taken, choices)."""
+        n = ctypes.c_int64(0)
+        self._check(self.L.rk_autotune(self.h, int(steps_per_trial), ctypes.byref(n)))
+        return int(n.value), {k: int(self.get_option(k)) for k in self.TUNED}
 
     def update_nl(self):
         self._check(self.L.rk_update_nl(self.h))
