@@ -34,7 +34,7 @@ independent sphere packings in one engine, autotuned launch shapes and
 graph-captured steps, with a sample of the same packings run one engine
 after another) and "avalanche" (a 1,000,000-body polyhedral avalanche with
 the displacement-check rebuild: the plain GPU form against the batched +
-learned-rebuild form, alternating on the same trajectory).  --forms ""
+learned-rebuild form, timed on the same steps of the same trajectory).  --forms ""
 leaves them out.
 """
 import argparse
@@ -48,6 +48,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per force-phase launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# (profiles/): None until measured for the current kernels
+TRAFFIC_BYTES_PER_LAUNCH = None
 
 
 def make_settled_deck(n_bodies, seed, dirpath, settle_steps, relax_steps, log):
@@ -107,7 +110,7 @@ def force_phase_bytes(nb, nl, law):
     return 156 * nb + per_inter * nl
 
 
-def cpu_baseline(deck_path, target_seconds, threads):
+def _oracle_rate(deck_path, target_seconds, threads):
     from oracle.rk_oracle import Oracle
     o = Oracle(deck_path, threads=threads)
     o.update_nl()
@@ -118,10 +121,58 @@ def cpu_baseline(deck_path, target_seconds, threads):
         o.run_steps(20)
         steps += 20
     dt = time.perf_counter() - t0
-    nb = o.nbodies
-    return {"value": nb * steps / dt, "unit": "body-steps/s", "cores": threads, "kind": "port",
+    return o.nbodies * steps / dt, steps, dt
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(deck_path, target_seconds, threads):
+    v, steps, dt = _oracle_rate(deck_path, target_seconds, threads)
+    v1, steps1, dt1 = _oracle_rate(deck_path, max(2.0, target_seconds / 3.0), 1)
+    return {"value": v, "unit": "body-steps/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "value_1_thread": v1,
             "sample": "CPU oracle (OpenMP restatement of Rockable's DefaultParallelMode step, %d threads), the same "
-                      "settled packing, %d steps after 20 warm-up steps, %.1f s" % (threads, steps, dt)}
+                      "settled packing, %d steps after 20 warm-up steps, %.1f s; 1 thread: %d steps, %.1f s"
+                      % (threads, steps, dt, steps1, dt1)}
+
+
+def copy_bandwidth_child(mib=1024):
+    """copy_bandwidth in a child process: torch brings its own HIP runtime,
+    which must not share a process with the engine's."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-rate", str(mib)], capture_output=True,
+                       text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr.strip()[-300:])
+    return float(json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"])
+
+
+def copy_bandwidth(mib=1024, reps=10):
+    """Device-to-device copy rate (read + write bytes per second), the
+    STREAM-copy figure the roofline is also quoted against."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    rate = 2.0 * n * reps / (time.perf_counter() - t0) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return rate
 
 
 def batched_form(a, log):
@@ -192,72 +243,82 @@ def avalanche_forms(a, log):
     """Learned-rebuild workload: a.avalanche_bodies of the plain GPU form's
     hull polyhedra packed on the floor of a 400 x 100-size box and sent down
     a 30 deg slope (gravity tilted; 1 m/s downslope plus velocities in
-    [-0.5, 0.5] m/s), dt 5e-7 s, the neighbour list rebuilt by the
+    [-0.2, 0.2] m/s), dt 2e-7 s, the neighbour list rebuilt by the
     displacement check alone (dynamicUpdateNL, dispUpdateNL = DVerlet / 4,
-    angleUpdateNL = 1 rad, the reference default).  One engine runs the
-    avalanche on; segments of a.avalanche_segment steps alternate between
-    the plain GPU form (the check read by the host after every step, every
-    kernel launched on its own) and the batched + learned form (batches of
-    steps sized by the rebuild-interval network, launched as captured graphs
-    of the autotuned length, the device check freezing any step after a
-    rebuild falls due).  Both forms compute the same steps, so each is timed
-    on its share of the same trajectory."""
+    angleUpdateNL = 1 rad, the reference default).  The launch shapes are
+    autotuned on the running avalanche, then after a warm-up the same
+    a.avalanche_steps steps are timed twice, from the same state: in the
+    plain GPU form (the check read by the host after every step, every kernel
+    launched on its own) and in the batched + learned form on a second engine
+    brought to that state (batches of steps sized by the rebuild-interval
+    network and enqueued without host synchronisation, as captured graphs
+    when the autotuner finds them faster, the device check freezing any step
+    after a rebuild falls due).  The two final states must be identical."""
     import numpy as np
     from rockable_amd import decks as D
     from rockable_amd.engine import Engine, Mlp
     d = tempfile.mkdtemp(prefix="rk_aval_")
     t0 = time.perf_counter()
-    shapes, deck = D.avalanche_deck(n_bodies=a.avalanche_bodies, dt=5e-7, gap_frac=0.005)
+    shapes, deck = D.avalanche_deck(n_bodies=a.avalanche_bodies, dt=2e-7, gap_frac=0.005, vmax=0.2)
     path = D.write_deck(d, shapes, deck)
     log("avalanche deck: %d bodies, written in %.1f s" % (a.avalanche_bodies, time.perf_counter() - t0))
+    mlp = Mlp(device=0)
+    mlp.set_params(np.load(os.path.join(ROOT, "rockable_amd", "data", "rebuild_mlp.npy")))
     e = Engine(path, device=0)
     e.set_option("record_geometry", 0)
     e.update_nl()
-    mlp = Mlp(device=0)
-    mlp.set_params(np.load(os.path.join(ROOT, "rockable_amd", "data", "rebuild_mlp.npy")))
-    e.step(a.avalanche_warmup)
     ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
-    G = max(8, choice["graph_steps"])
-    log("autotune: %d steps, %s; batched form graph length %d" % (ntune, choice, G))
-    forms = {"plain": [0.0, 0, 0, 0], "batched_learned": [0.0, 0, 0, 0]}
-    done = 0
-    while done < 2 * a.avalanche_steps:
-        for name in ("plain", "batched_learned"):
-            if name == "plain":
-                e.set_rebuild_model(None)
-                e.set_option("guard_batch", 1)
-                e.set_option("graph_steps", 0)
-            else:
-                e.set_rebuild_model(mlp)
-                e.set_option("graph_steps", G)
-            c0 = e.counts()
-            e.synchronize()
-            t1 = time.perf_counter()
-            e.step(a.avalanche_segment)
-            e.synchronize()
-            f = forms[name]
-            f[0] += time.perf_counter() - t1
-            c1 = e.counts()
-            f[1] += a.avalanche_segment
-            f[2] += c1["nl_updates"] - c0["nl_updates"]
-            f[3] += c1["guard_reads"] - c0["guard_reads"]
-            done += a.avalanche_segment
-    c = e.counts()
-    nb = c["nbodies"]
+    e.step(a.avalanche_warmup)
+    start = ntune + a.avalanche_warmup
+    log("autotune: %d steps, %s" % (ntune, choice))
+    forms, finals = {}, {}
+    for name in ("plain", "batched_learned"):
+        if name == "batched_learned":
+            e = Engine(path, device=0)
+            e.set_option("record_geometry", 0)
+            for k, v in choice.items():
+                e.set_option(k, v)
+            e.update_nl()
+            e.step(start)
+            e.set_rebuild_model(mlp)
+        else:
+            e.set_rebuild_model(None)
+            e.set_option("guard_batch", 1)
+            e.set_option("graph_steps", 0)
+        c0 = e.counts()
+        e.synchronize()
+        t1 = time.perf_counter()
+        e.step(a.avalanche_steps)
+        e.synchronize()
+        wall = time.perf_counter() - t1
+        c1 = e.counts()
+        forms[name] = {"value": c1["nbodies"] * a.avalanche_steps / wall, "unit": "body-steps/s",
+                       "ms_per_step": 1e3 * wall / a.avalanche_steps, "steps": a.avalanche_steps,
+                       "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
+                       "guard_reads": c1["guard_reads"] - c0["guard_reads"],
+                       "frozen_steps": c1["guard_wasted"] - c0["guard_wasted"],
+                       "graph_launches": c1["graph_launches"] - c0["graph_launches"]}
+        b = e.bodies()
+        finals[name] = {k: b[k] for k in ("pos", "vel", "Q", "vrot")}
+        if name == "plain":
+            stats, c = e.stats(), c1
+        e.close()
+        log("avalanche %s: %.3f ms/step" % (name, forms[name]["ms_per_step"]))
     mlp.close()
-    e.close()
-    out = {}
-    for name, (w, n, r, g) in forms.items():
-        out[name] = {"value": nb * n / w, "unit": "body-steps/s", "ms_per_step": 1e3 * w / n, "steps": n,
-                     "nl_rebuilds": r, "guard_reads": g}
-    out["batched_learned_over_plain"] = out["batched_learned"]["value"] / out["plain"]["value"]
+    nb = c["nbodies"]
+    out = dict(forms)
+    out["batched_learned_over_plain"] = forms["batched_learned"]["value"] / forms["plain"]["value"]
+    out["identical_final_state"] = all(np.array_equal(finals["plain"][k], finals["batched_learned"][k])
+                                       for k in finals["plain"])
     out["bodies"] = nb
     out["listed_per_body"] = c["nlisted"] / nb
     out["active_per_body"] = c["nactive"] / nb
+    out["kinetic_energy"] = stats["kinetic_translational"] + stats["kinetic_rotational"]
+    out["coordination"] = stats["coordination"]
     out["autotuned"] = choice
-    out["workload"] = ("%d hull polyhedra (config of the plain GPU form) down a 30 deg slope, dt 5e-7 s, "
-                       "dynamicUpdateNL (disp DVerlet/4, 1 rad), %d warm-up steps, then segments of %d steps "
-                       "alternating between the forms" % (nb, a.avalanche_warmup, a.avalanche_segment))
+    out["workload"] = ("%d hull polyhedra (config of the plain GPU form) down a 30 deg slope, dt 2e-7 s, "
+                       "dynamicUpdateNL (disp DVerlet/4, 1 rad); %d steps (autotuning and warm-up) untimed, then "
+                       "the same %d steps timed in each form" % (nb, start, a.avalanche_steps))
     return out
 
 
@@ -281,15 +342,21 @@ def main():
     ap.add_argument("--avalanche-bodies", type=int, default=1000000)
     ap.add_argument("--avalanche-steps", type=int, default=3000, help="timed steps per form")
     ap.add_argument("--avalanche-warmup", type=int, default=2000)
-    ap.add_argument("--avalanche-segment", type=int, default=500)
     ap.add_argument("--tune-trial", type=int, default=50)
     ap.add_argument("--forms", default="batched,avalanche",
                     help="extra forms reported beside the plain one (comma list; empty for none)")
     ap.add_argument("--prepare-only", action="store_true",
                     help="settle the packing into --deck-dir and stop (later runs reuse it)")
+    ap.add_argument("--copy-rate", type=int, default=0, metavar="MIB",
+                    help="only print the device copy rate of MIB MiB buffers (run as a child process)")
+    ap.add_argument("--no-copy-rate", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (calibrates the HBM byte counters)")
     a = ap.parse_args()
+    if a.copy_rate > 0:
+        print(json.dumps({"copy_GBs": copy_bandwidth(a.copy_rate)}), flush=True)
+        return
     if a.copy_calibration > 0:
         import torch
         n = a.copy_calibration << 20
@@ -331,9 +398,11 @@ def main():
     e.set_option("record_geometry", 0)
     e.update_nl()
     t0 = time.perf_counter()
-    e.step(a.warmup)
+    tuned, choice = (0, None) if a.no_autotune else e.autotune(steps_per_trial=a.tune_trial)
+    e.step(max(0, a.warmup - tuned))
     e.synchronize()
-    log("warm-up: %d steps in %.2f s" % (a.warmup, time.perf_counter() - t0))
+    log("warm-up: %d steps (%d of them autotuning: %s) in %.2f s" % (max(a.warmup, tuned), tuned, choice,
+                                                                   time.perf_counter() - t0))
 
     # timed region: K steps, device state resident; HIP events on the engine
     # stream bracket the force phase and the neighbour-list rebuilds
@@ -374,7 +443,7 @@ def main():
 
     out = {
         "metric": "body-steps/s", "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "warmup": max(a.warmup, tuned), "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "plain GPU form: %d convex sphero-polyhedra (hulls of 8-14 points on a 1:0.8:0.6 "
                                "ellipsoid, 4-6 mm), settled pour in a 40x40x200-size box, linkCells, ForceLaw "
@@ -387,13 +456,21 @@ def main():
                    "nl_rebuild_ms": kt["nl"][1] / max(kt["nl"][0], 1),
                    "kernel_ms_per_launch": {k: (v[1] / max(v[0], 1)) for k, v in kt2.items() if v[0]},
                    "ms_per_step_without_events": 1e3 * wall_noev / a.steps,
+                   "autotuned": choice,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "force phase (k_interact + k_gather + k_gather_heavy)",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+                     "kernel": "force phase (k_geom + k_law + k_gather + k_gather_heavy)",
                      "algorithmic_bytes_per_launch": bytes_per},
     }
     e.close()
+    if not a.no_copy_rate:
+        try:
+            bw = copy_bandwidth_child()
+            out["roofline"]["measured_copy_GBs"] = bw
+            out["roofline"]["frac_of_measured_copy"] = achieved / bw
+        except Exception as exc:
+            log("copy bandwidth not measured: %s" % exc)
     forms = [f for f in a.forms.split(",") if f]
     if forms:
         out["forms"] = {}

@@ -60,11 +60,46 @@ typedef struct {
   int64_t graph_captures;  /* graphs captured (one per device view, cached) */
 } rk_counts;
 
+/* Statistics of the packing, reduced on the device in a fixed order
+ * (DataExtractor_dnStat.cpp and DataExtractor_TrackDamage.cpp report the
+ * overlap and damage figures; kinetic energies as in the reference's
+ * kineticEnergy output, Rockable.cpp:3958-3970).  Free bodies only for the
+ * body sums; "contacts" are the active interactions with dn < 0. */
+typedef struct {
+  double kinetic_translational;  /* sum m |v|^2 / 2 */
+  double kinetic_rotational;     /* sum w_b . (I w_b) / 2, w_b in the body frame */
+  double vmax;                   /* largest |v| */
+  int64_t nfree;                 /* free (not driven) bodies */
+  int64_t nactive;               /* active interactions (dn < 0 or bonded) */
+  int64_t ncontacts;             /* active with dn < 0 */
+  int64_t active_pairs;          /* body pairs with an active interaction */
+  double coordination;           /* 2 active_pairs / nfree */
+  double dn_mean, dn_min;        /* over the contacts (0 if none) */
+  int64_t ninterfaces;           /* live breakable interfaces */
+  int64_t broken_interfaces;     /* interfaces broken since the load */
+  double total_broken_area;      /* TrackDamage's broken area */
+} rk_stats;
+
 int rk_engine_create(rk_engine** out, int device);
 void rk_engine_destroy(rk_engine* eng);
 const char* rk_last_error(const rk_engine* eng);
 
 int rk_load_conf(rk_engine* eng, const char* path);
+/* Batched form (no counterpart in the reference): n decks, independent
+ * packings that must lie apart in space, loaded as one model -- every
+ * deck's driven bodies first, then every deck's free bodies; shapes shared by
+ * name (a name must mean one shape), groups renumbered per deck so that each
+ * keeps its own parameters; each deck's interactions, interfaces and
+ * controls (driving_paths[p], may be NULL) renumbered.  The decks must agree
+ * on forceLaw, dt, interVerlet, interConf, DVerlet, dVerlet,
+ * dynamicUpdateNL, ParamsInInterfaces, UpdateNL, cellMinSizes, gravity,
+ * BodyForce, t and tmax.  Each packing then computes what it computes alone
+ * (with dynamicUpdateNL the check is the batch's: one packing's motion
+ * rebuilds every list).  Then as rk_load_conf (the fake step included). */
+int rk_load_batch(rk_engine* eng, const char* const* conf_paths, const char* const* driving_paths, int n);
+/* Body ranges of packing p: out[0] first driven body, out[1] driven count,
+ * out[2] first free body, out[3] free count (one packing after rk_load_conf). */
+int rk_batch_packing(rk_engine* eng, int p, int64_t* out);
 int rk_load_driving_system(rk_engine* eng, const char* path);
 /* Returns 1 (not an error) when the name is unknown and Default is used. */
 int rk_set_force_law(rk_engine* eng, const char* name);
@@ -88,7 +123,7 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * out as one HIP graph, captured once per device view and replayed; 0 or 1,
  * the default: every kernel launched on its own),
  * launch shapes: "law_group" (geometry blocks per force-law wavefront: 1, 2,
- * 4, 8; default 4), "geom_block" (contact-kinematics workgroup: 64, 128,
+ * 4, 8; default 1), "geom_block" (contact-kinematics workgroup: 64, 128,
  * 256, 512; default 256), "gather_waves" (wavefronts per resultant
  * workgroup: 1, 2, 4; default 4), "body_block" (per-body kernels: 64, 128,
  * 256; default 256).  No option changes results. */
@@ -110,6 +145,7 @@ int rk_save_conf(rk_engine* eng, const char* path);
 int rk_synchronize(rk_engine* eng);
 
 int rk_get_counts(rk_engine* eng, rk_counts* out);
+int rk_get_stats(rk_engine* eng, rk_stats* out);
 /* Any pointer may be NULL; arrays hold 3 doubles per body (Q: w x y z). */
 int rk_get_bodies(rk_engine* eng, double* pos, double* Q, double* vel, double* vrot, double* acc, double* arot,
                   double* force, double* moment, size_t nbodies);

@@ -2569,6 +2569,17 @@ void rko_get_bodies(rko_sim* s, double* pos, double* vel, double* acc, double* Q
   }
 }
 
+void rko_get_mass(rko_sim* s, double* mass, double* inertia) {
+  for (int i = 0; i < s->nb; i++) {
+    if (mass) mass[i] = s->b[i].mass;
+    if (inertia) {
+      inertia[3 * i] = s->b[i].inertia.x;
+      inertia[3 * i + 1] = s->b[i].inertia.y;
+      inertia[3 * i + 2] = s->b[i].inertia.z;
+    }
+  }
+}
+
 void rko_set_bodies(rko_sim* s, const double* pos, const double* vel, const double* Q, const double* vrot) {
   for (int i = 0; i < s->nb; i++) {
     body_t* p = &s->b[i];

@@ -72,6 +72,8 @@ double rko_total_broken_area(rko_sim* s);
 void rko_get_bodies(rko_sim* s, double* pos, double* vel, double* acc, double* Q, double* vrot, double* arot,
                     double* force, double* moment);
 void rko_set_bodies(rko_sim* s, const double* pos, const double* vel, const double* Q, const double* vrot);
+/* masses (nb) and principal inertia moments (3 nb) */
+void rko_get_mass(rko_sim* s, double* mass, double* inertia);
 /* All listed interactions (active == 0) or the active ones (active == 1),
  * lexicographic order (Interaction.hpp:104-122). Returns the count; writes at
  * most cap records. */

@@ -78,6 +78,7 @@ def lib():
         L.rko_total_broken_area.argtypes = [P]
         L.rko_total_broken_area.restype = ctypes.c_double
         L.rko_get_bodies.argtypes = [P] + [P] * 8
+        L.rko_get_mass.argtypes = [P, P, P]
         L.rko_set_bodies.argtypes = [P] + [P] * 4
         L.rko_get_interactions.argtypes = [P, ctypes.c_int, P, ctypes.c_size_t]
         L.rko_get_interactions.restype = ctypes.c_size_t
@@ -182,6 +183,12 @@ class Oracle:
         self.L.rko_get_bodies(self.s, _ptr(out["pos"]), _ptr(out["vel"]), _ptr(out["acc"]), _ptr(out["Q"]),
                               _ptr(out["vrot"]), _ptr(out["arot"]), _ptr(out["force"]), _ptr(out["moment"]))
         return out
+
+    def mass(self):
+        """Masses and principal inertia moments (body frame) of the bodies."""
+        m, I = np.zeros(self.nbodies), np.zeros((self.nbodies, 3))
+        self.L.rko_get_mass(self.s, _ptr(m), _ptr(I))
+        return m, I
 
     def set_bodies(self, pos=None, vel=None, Q=None, vrot=None):
         c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)

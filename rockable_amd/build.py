@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librockable_amd.so")
-SOURCES = ["rk_io.cpp", "rk_step.hip", "rk_nl.hip", "rk_mlp.hip", "rk_engine.cpp"]
+SOURCES = ["rk_io.cpp", "rk_batch.cpp", "rk_step.hip", "rk_nl.hip", "rk_mlp.hip", "rk_engine.cpp"]
 ARCH = os.environ.get("RK_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

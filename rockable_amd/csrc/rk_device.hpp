@@ -70,7 +70,9 @@ struct alignas(16) ActiveKin {
   double px, py, pz;
   double dn, prev_dn;
   int32_t e, deact;
+  int32_t i, j;  // the entry's bodies (saves the force law a dependent load)
 };
+static_assert(sizeof(ActiveKin) == 80, "ActiveKin is 80 bytes");
 
 // dynamicCheckUpdateNL on the device (Rockable.cpp:1571-1596).  The running
 // maxima are kept as order-preserving integer keys so that atomicMax / atomicMin
@@ -134,6 +136,9 @@ struct Dev {
   // global vertices: body b's are gv[3 * bvoff[b] ...], bvbody[k] owns slot k
   double* gv;
   const int32_t* bvoff;
+  const int32_t* beoff;   // body b's edges (as sub-elements) from beoff[b], its faces from bfoff[b]
+  const int32_t* bfoff;
+  int netot, nftot;       // edges and faces of all bodies
   const int32_t* bvbody;
   int nvtot;
   double* acc;
@@ -166,6 +171,14 @@ struct Dev {
   double* sval;
   const int32_t* heavy;   // bodies with more than heavy_min slots
   int nheavy, heavy_min;
+  // passive heavy bodies: their slots in chunks of 64 (chunk c: slots
+  // [hch[c].x, hch[c].y)), heavy body h's chunks [hch_off[h], hch_off[h+1]),
+  // six partial sums per chunk in hpart
+  const int2* hch;
+  const int32_t* hch_off;
+  double* hpart;
+  int nhch;
+  int nheavy_exact;       // heavy bodies summed in serial order (k_gather_heavy)
   // per body: 1 if its force and moment feed nothing back into the motion
   // (a driven body without force or moment controls)
   const uint8_t* passive;
@@ -175,6 +188,7 @@ struct Dev {
   int geom_block;         // k_geom workgroup size (64, 128, 256, 512)
   int gather_waves;       // k_gather wavefronts per workgroup (1, 2, 4)
   int body_block;         // workgroup size of the per-body kernels (64, 128, 256)
+  int fuse_law;           // 1: geometry and law in one kernel (k_interact)
   // device displacement guard (dynamicUpdateNL)
   int guard_on;
   DevGuard* guard;
@@ -215,11 +229,13 @@ RK_HD uint32_t pack_sub(int type, int isub, int jsub) {
 void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s);
 void launch_glob(const Dev& d, hipStream_t s);
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s);
-// side (may be null): a second stream on which the workgroup-per-body sums
-// of heavy bodies run beside the main gather; fork / join: events for it
-void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s, hipStream_t side = nullptr,
-                   hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
+void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s);
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);
+// statistics: part holds stat_parts(d) doubles, out 9 doubles (KE translation,
+// KE rotation, max |v|, free bodies, active, contacts, active pairs, sum dn,
+// min dn over contacts)
+int stat_parts(const Dev& d);
+void launch_stats(const Dev& d, double* part, double* out, hipStream_t s);
 void launch_guard(const Dev& d, const StepParams& sp, hipStream_t s);
 // the step index the next guard check ends (the checks count on from it)
 void launch_guard_set_step(const Dev& d, long long step, hipStream_t s);

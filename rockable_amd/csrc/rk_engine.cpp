@@ -40,12 +40,14 @@ struct rk_engine {
   int profile = 0;
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // second stream for the heavy-body sums, with its fork / join events
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int law_group = 4, geom_block = 256, gather_waves = 4, body_block = 256;
+  int law_group = 1, geom_block = 256, gather_waves = 4, body_block = 256, fuse_law = 1;
   // host wall time of the rebuilds while the autotuner runs (it synchronises
   // after each rebuild then)
+  double* stat_part = nullptr;
+  double* stat_out = nullptr;
+  size_t stat_cap = 0;
+  int64_t loaded_interfaces = 0;
+  std::vector<PackingRange> packings;  // one per deck of a batch (one for a single deck)
   bool tuning = false;
   double nl_wall_s = 0.0;
   // profile mode: start/end event pairs recorded on the engine stream and
@@ -73,7 +75,8 @@ struct rk_engine {
   int32_t* flags = nullptr;
   uint8_t* passive = nullptr;
   double* gv = nullptr;
-  int32_t *bvoff = nullptr, *bvbody = nullptr;
+  int32_t *bvoff = nullptr, *bvbody = nullptr, *beoff = nullptr, *bfoff = nullptr;
+  int netot = 0, nftot = 0;
   int nvtot = 0;
   bool gv_dirty = true;  // body poses changed since the last k_glob
   // device displacement guard (dynamicUpdateNL)
@@ -152,7 +155,8 @@ void free_device(rk_engine* eng) {
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
                 eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
                 eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,
-                eng->feat,     eng->pred,      eng->feat_part};
+                eng->beoff,    eng->bfoff,
+                eng->feat,     eng->pred,      eng->feat_part, eng->stat_part, eng->stat_out};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   eng->kin = nullptr;
@@ -172,12 +176,15 @@ void free_device(rk_engine* eng) {
   eng->flags = nullptr;
   eng->passive = nullptr;
   eng->gv = nullptr;
-  eng->bvoff = eng->bvbody = nullptr;
+  eng->bvoff = eng->bvbody = eng->beoff = eng->bfoff = nullptr;
+  eng->netot = eng->nftot = 0;
   eng->guard = nullptr;
   eng->gdpos = eng->gdq = nullptr;
   eng->feat = eng->pred = nullptr;
   eng->feat_part = nullptr;
   eng->feat_part_cap = 0;
+  eng->stat_part = eng->stat_out = nullptr;
+  eng->stat_cap = 0;
   eng->nvtot = 0;
   for (auto& E : eng->E) {
     entries_free(E);
@@ -186,6 +193,9 @@ void free_device(rk_engine* eng) {
     if (E.jlist) (void)hipFree(E.jlist);
     if (E.soff) (void)hipFree(E.soff);
     if (E.heavy) (void)hipFree(E.heavy);
+    if (E.hch) (void)hipFree(E.hch);
+    if (E.hch_off) (void)hipFree(E.hch_off);
+    if (E.hpart) (void)hipFree(E.hpart);
     E = EntryArrays();
   }
   nl_work_free(eng->w);
@@ -216,6 +226,10 @@ void assemble_dev(rk_engine* eng) {
   d.kin = eng->kin;
   d.gv = eng->gv;
   d.bvoff = eng->bvoff;
+  d.beoff = eng->beoff;
+  d.bfoff = eng->bfoff;
+  d.netot = eng->netot;
+  d.nftot = eng->nftot;
   d.bvbody = eng->bvbody;
   d.nvtot = eng->nvtot;
   d.acc = eng->acc;
@@ -256,6 +270,11 @@ void assemble_dev(rk_engine* eng) {
   d.sact = E.sact;
   d.sval = E.sval;
   d.heavy = E.heavy;
+  d.hch = E.hch;
+  d.hch_off = E.hch_off;
+  d.hpart = E.hpart;
+  d.nhch = E.nhch;
+  d.nheavy_exact = eng->exact_sums ? E.nheavy : E.nheavy_np;
   d.nheavy = E.nheavy;
   d.heavy_min = eng->w.heavy_min;
   d.passive = eng->passive;
@@ -272,6 +291,7 @@ void assemble_dev(rk_engine* eng) {
   d.geom_block = eng->geom_block;
   d.gather_waves = eng->gather_waves;
   d.body_block = eng->body_block;
+  d.fuse_law = eng->fuse_law;
   d.istart = E.istart;
   d.jstart = E.jstart;
   d.jlist = E.jlist;
@@ -453,6 +473,16 @@ int upload_model(rk_engine* eng) {
         upload(eng, eng->gv, std::vector<double>(3 * (size_t)std::max(1, eng->nvtot), 0.0)))
       return -1;
     eng->gv_dirty = true;
+    // each body's edges and faces (for the narrow phase's cull boxes)
+    std::vector<int32_t> beoff(nb + 1, 0), bfoff(nb + 1, 0);
+    for (int b = 0; b < nb; b++) {
+      const Shape& sh = m.shapes[m.bodies[b].shape];
+      beoff[b + 1] = beoff[b] + sh.ne();
+      bfoff[b + 1] = bfoff[b] + sh.nf();
+    }
+    eng->netot = beoff[nb];
+    eng->nftot = bfoff[nb];
+    if (upload(eng, eng->beoff, beoff) || upload(eng, eng->bfoff, bfoff)) return -1;
   }
   // device displacement guard
   if (m.dynamicUpdateNL != 0) {
@@ -514,6 +544,7 @@ int upload_model(rk_engine* eng) {
       return -1;
     eng->live_interfaces = 0;
     for (const Interface& BI : m.ifc) eng->live_interfaces += BI.alive;
+    eng->loaded_interfaces = eng->live_interfaces;
   }
   // controls: list + per driven body CSR preserving list order
   {
@@ -594,7 +625,7 @@ int upload_model(rk_engine* eng) {
     HIPCHK(hipMemcpyAsync(E.edamp, dp.data(), sizeof(double) * n, hipMemcpyHostToDevice, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
   }
-  if (nl_build_csr(eng->w, nb, E, eng->stream) != 0) return fail(eng, eng->w.err);
+  if (nl_build_csr(eng->w, nb, E, eng->passive, eng->stream) != 0) return fail(eng, eng->w.err);
   eng->nlp.DVerlet = m.DVerlet;
   eng->nlp.dVerlet = m.dVerlet;
   eng->nlp.linkcells = m.nl_linkcells;
@@ -666,7 +697,7 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom) {
   launch_interact(eng->d, sp, write_geom || eng->record_geometry, eng->stream);
   toc(eng, "interact", 2);
   tic(eng, 2);
-  launch_gather(eng->d, sp, kick, eng->stream, eng->side, eng->ev_fork, eng->ev_join);
+  launch_gather(eng->d, sp, kick, eng->stream);
   toc(eng, "gather", 2);
   toc(eng, "force", 1);
   if (eng->m.law == LAW_STICKED && eng->live_interfaces > 0) {
@@ -1103,9 +1134,7 @@ int rk_engine_create(rk_engine** out, int device) {
   rk_engine* eng = new rk_engine();
   eng->device = device;
   if (hipStreamCreate(&eng->stream) != hipSuccess || hipEventCreate(&eng->ev0) != hipSuccess ||
-      hipEventCreate(&eng->ev1) != hipSuccess || hipStreamCreate(&eng->side) != hipSuccess ||
-      hipEventCreateWithFlags(&eng->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&eng->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreate(&eng->ev1) != hipSuccess) {
     delete eng;
     return -2;
   }
@@ -1121,18 +1150,14 @@ void rk_engine_destroy(rk_engine* eng) {
   for (hipEvent_t e : eng->ev_pool) (void)hipEventDestroy(e);
   if (eng->ev0) (void)hipEventDestroy(eng->ev0);
   if (eng->ev1) (void)hipEventDestroy(eng->ev1);
-  if (eng->ev_fork) (void)hipEventDestroy(eng->ev_fork);
-  if (eng->ev_join) (void)hipEventDestroy(eng->ev_join);
-  if (eng->side) (void)hipStreamSynchronize(eng->side);
-  if (eng->side) (void)hipStreamDestroy(eng->side);
   if (eng->stream) (void)hipStreamDestroy(eng->stream);
   delete eng;
 }
 
 const char* rk_last_error(const rk_engine* eng) { return eng ? eng->err.c_str() : "null engine"; }
 
-int rk_load_conf(rk_engine* eng, const char* path) {
-  if (!eng || !path) return -1;
+namespace {
+void reset_engine(rk_engine* eng) {
   (void)hipSetDevice(eng->device);
   free_device(eng);
   eng->m = Model();
@@ -1141,8 +1166,53 @@ int rk_load_conf(rk_engine* eng, const char* path) {
   eng->needUpdate = false;
   eng->steps = 0;
   eng->nl_updates = 0;
+  eng->packings.clear();
+}
+}  // namespace
+
+int rk_load_batch(rk_engine* eng, const char* const* conf_paths, const char* const* driving_paths, int n) {
+  if (!eng || !conf_paths || n < 1) return -1;
+  reset_engine(eng);
+  std::vector<Model> parts(n);
+  std::string err;
+  for (int p = 0; p < n; p++) {
+    if (!conf_paths[p]) return fail(eng, "batch: null deck path");
+    if (load_conf(parts[p], conf_paths[p], err) != 0) return fail(eng, std::string(conf_paths[p]) + ": " + err);
+    if (driving_paths && driving_paths[p] && load_driving(parts[p], driving_paths[p], err) != 0)
+      return fail(eng, std::string(driving_paths[p]) + ": " + err);
+  }
+  if (merge_models(parts, eng->m, eng->packings, err) != 0) return fail(eng, err);
+  parts.clear();
+  if (upload_model(eng) != 0) return -1;
+  // (upload_model uploads the decks' controls with the rest)
+  eng->loaded = true;
+  // the "fake step" of loadConf (Rockable.cpp:1133-1138), every deck at once
+  if (run_accelerations(eng, 0, 1) != 0) return -1;
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+int rk_batch_packing(rk_engine* eng, int p, int64_t* out) {
+  if (check_loaded(eng) || !out) return -1;
+  if (p < 0 || p >= (int)eng->packings.size()) return fail(eng, "no packing " + std::to_string(p));
+  const PackingRange& r = eng->packings[p];
+  out[0] = r.first_driven;
+  out[1] = r.n_driven;
+  out[2] = r.first_free;
+  out[3] = r.n_free;
+  return 0;
+}
+
+int rk_load_conf(rk_engine* eng, const char* path) {
+  if (!eng || !path) return -1;
+  reset_engine(eng);
   std::string err;
   if (load_conf(eng->m, path, err) != 0) return fail(eng, err);
+  PackingRange one;
+  one.n_driven = eng->m.nDriven;
+  one.first_free = eng->m.nDriven;
+  one.n_free = (int)eng->m.bodies.size() - eng->m.nDriven;
+  eng->packings.assign(1, one);
   if (upload_model(eng) != 0) return -1;
   eng->loaded = true;
   // the "fake step" of loadConf (Rockable.cpp:1133-1138)
@@ -1175,6 +1245,9 @@ int rk_load_driving_system(rk_engine* eng, const char* path) {
   }
   if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
   if (upload(eng, eng->passive, passive_flags(eng->m))) return -1;
+  // the heavy bodies' summation paths depend on the passive flags
+  if (nl_build_heavy(eng->w, (int)eng->m.bodies.size(), eng->E[eng->cur], eng->passive, eng->stream) != 0)
+    return fail(eng, eng->w.err);
   assemble_dev(eng);
   HIPCHK(hipStreamSynchronize(eng->stream));
   return 0;
@@ -1246,6 +1319,10 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     // the conf keyword "gravity" (Rockable.cpp:978-980), changed between steps
     if (check_loaded(eng)) return -1;
     (n == "gravity_x" ? eng->m.gravity.x : n == "gravity_y" ? eng->m.gravity.y : eng->m.gravity.z) = value;
+  } else if (n == "fuse_law") {
+    if (!(value == 0.0 || value == 1.0)) return fail(eng, "fuse_law must be 0 or 1");
+    eng->fuse_law = (int)value;
+    if (eng->loaded) assemble_dev(eng);
   } else if (n == "law_group" || n == "geom_block" || n == "gather_waves" || n == "body_block") {
     const int v = (int)value;
     if ((double)v != value) return fail(eng, n + " must be an integer");
@@ -1279,7 +1356,7 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     eng->w.heavy_min = (int)value;
     if (eng->loaded) {
       (void)hipSetDevice(eng->device);
-      if (nl_build_heavy(eng->w, (int)eng->m.bodies.size(), eng->E[eng->cur], eng->stream) != 0)
+      if (nl_build_heavy(eng->w, (int)eng->m.bodies.size(), eng->E[eng->cur], eng->passive, eng->stream) != 0)
         return fail(eng, eng->w.err);
       assemble_dev(eng);
     }
@@ -1319,6 +1396,7 @@ int* tuned_option(rk_engine* eng, const std::string& n) {
   if (n == "gather_waves") return &eng->gather_waves;
   if (n == "body_block") return &eng->body_block;
   if (n == "graph_steps") return &eng->graph_steps;
+  if (n == "fuse_law") return &eng->fuse_law;
   return nullptr;
 }
 }  // namespace
@@ -1332,7 +1410,8 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
     const char* name;
     std::vector<int> values;
   };
-  const Knob knobs[] = {{"law_group", {1, 2, 4, 8}},
+  const Knob knobs[] = {{"fuse_law", {0, 1}},
+                        {"law_group", {1, 2, 4, 8}},
                         {"geom_block", {64, 128, 256, 512}},
                         {"gather_waves", {1, 2, 4}},
                         {"body_block", {64, 128, 256}},
@@ -1442,6 +1521,38 @@ int rk_save_conf(rk_engine* eng, const char* path) {
 int rk_synchronize(rk_engine* eng) {
   if (!eng) return -1;
   HIPCHK(hipStreamSynchronize(eng->stream));
+  return 0;
+}
+
+int rk_get_stats(rk_engine* eng, rk_stats* out) {
+  if (check_loaded(eng) || !out) return -1;
+  (void)hipSetDevice(eng->device);
+  const size_t need = (size_t)stat_parts(eng->d);
+  if (eng->stat_cap < need) {
+    if (eng->stat_part) (void)hipFree(eng->stat_part);
+    eng->stat_part = nullptr;
+    HIPCHK(hipMalloc(&eng->stat_part, need * sizeof(double)));
+    eng->stat_cap = need;
+  }
+  if (!eng->stat_out) HIPCHK(hipMalloc(&eng->stat_out, 16 * sizeof(double)));
+  launch_stats(eng->d, eng->stat_part, eng->stat_out, eng->stream);
+  HIPCHK(hipGetLastError());
+  double v[9];
+  HIPCHK(hipMemcpyAsync(v, eng->stat_out, sizeof v, hipMemcpyDeviceToHost, eng->stream));
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  out->kinetic_translational = v[0];
+  out->kinetic_rotational = v[1];
+  out->vmax = v[2];
+  out->nfree = (int64_t)v[3];
+  out->nactive = (int64_t)v[4];
+  out->ncontacts = (int64_t)v[5];
+  out->active_pairs = (int64_t)v[6];
+  out->coordination = v[3] > 0.0 ? 2.0 * v[6] / v[3] : 0.0;
+  out->dn_mean = v[5] > 0.0 ? v[7] / v[5] : 0.0;
+  out->dn_min = v[5] > 0.0 ? v[8] : 0.0;
+  out->ninterfaces = eng->live_interfaces;
+  out->broken_interfaces = eng->loaded_interfaces - eng->live_interfaces;
+  out->total_broken_area = eng->m.totalBrokenArea;
   return 0;
 }
 

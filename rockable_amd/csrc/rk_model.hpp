@@ -162,4 +162,13 @@ int save_conf(const Model& m, const std::vector<Inter>& active, const std::strin
 // Damping coefficient of a body pair (Rockable.cpp:1356-1376).
 double pair_damp(const Model& m, int i, int j);
 
+// Body ranges of one packing in a merged batch.
+struct PackingRange {
+  int first_driven = 0, n_driven = 0, first_free = 0, n_free = 0;
+};
+// rk_batch.cpp: several loaded decks as one model (driven bodies of every
+// deck first, then free bodies; groups, shapes, interactions, interfaces and
+// controls renumbered).  Returns 0 or -1 with err set.
+int merge_models(std::vector<Model>& parts, Model& out, std::vector<PackingRange>& ranges, std::string& err);
+
 }  // namespace rk

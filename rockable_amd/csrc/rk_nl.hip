@@ -313,27 +313,28 @@ __global__ void k_cell_fill(int nb, const int* cell_of, const int* cstart, int* 
   cbody[cstart[c] + atomicAdd(cfill + c, 1)] = b;
 }
 
-// main loop of UpdateNL_linkCells (Rockable.cpp:1697-1751), one thread per body
+// main loop of UpdateNL_linkCells (Rockable.cpp:1697-1751): 32 threads per
+// body, thread q < 27 taking neighbour cell q (a thread per body alone
+// leaves the GPU nearly idle: a few hundred wavefronts, each looping over
+// 27 cells).  The candidates are sorted afterwards, so the order in which
+// they are found does not matter.
 __global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, const int* cell_of, const int* cstart,
                               const int* cbody, uint64_t* cand, int* cnt, int rcap) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.nb) return;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(t >> 5), q = (int)(t & 31);
+  if (i >= d.nb || q >= 27) return;
   const int c = cell_of[i];
   if (c < 0) return;
   const int cz = c % g.N[2], cy = (c / g.N[2]) % g.N[1], cx = c / (g.N[2] * g.N[1]);
-  for (int dx = -1; dx <= 1; dx++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dz = -1; dz <= 1; dz++) {
-        const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
-        if (nx < 0 || ny < 0 || nz < 0 || nx >= g.N[0] || ny >= g.N[1] || nz >= g.N[2]) continue;
-        const int nc = (nx * g.N[1] + ny) * g.N[2] + nz;
-        for (int r = cstart[nc]; r < cstart[nc + 1]; r++) {
-          const int j = cbody[r];
-          const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
-                            pair_overlap(bobb, i, j, half);
-          emit_pair(emit, cand, cnt, rcap, i, j);
-        }
-      }
+  const int nx = cx + q / 9 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q % 3 - 1;
+  if (nx < 0 || ny < 0 || nz < 0 || nx >= g.N[0] || ny >= g.N[1] || nz >= g.N[2]) return;
+  const int nc = (nx * g.N[1] + ny) * g.N[2] + nz;
+  for (int r = cstart[nc]; r < cstart[nc + 1]; r++) {
+    const int j = cbody[r];
+    const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
+                      pair_overlap(bobb, i, j, half);
+    emit_pair(emit, cand, cnt, rcap, i, j);
+  }
 }
 
 // Oversized bodies against the bodies of the cells (Rockable.cpp:1755-1805):
@@ -441,6 +442,9 @@ struct NarrowCfg {
   int maxf;       // max faces per shape
   int maxwords;   // max 64-bit bitmask words per pair
   double dmax;    // dVerlet
+  // the culls' boxes of every body's edges and faces (k_gate_boxes)
+  const double* gedge;
+  const double* gface;
 };
 
 __device__ __forceinline__ v3 lds_v(const double* a, int k) { return V(a[3 * k], a[3 * k + 1], a[3 * k + 2]); }
@@ -560,41 +564,85 @@ __device__ __forceinline__ bool box_meets(const double lo[3], const double hi[3]
   return lo[0] - grow <= bhi[0] && hi[0] + grow >= blo[0] && lo[1] - grow <= bhi[1] && hi[1] + grow >= blo[1] &&
          lo[2] - grow <= bhi[2] && hi[2] + grow >= blo[2];
 }
-__device__ bool edge_gate(const DevShapes& sh, int shp, const double* gv, int e, double sum, double dmax,
-                          double margin, const double* blo, const double* bhi) {
-  const int* ed = sh.edge + 2 * (sh.eoff[shp] + e);
-  const v3 a = lds_v(gv, ed[0]), b = lds_v(gv, ed[1]);
-  const double lo[3] = {fmin(a.x, b.x), fmin(a.y, b.y), fmin(a.z, b.z)};
-  const double hi[3] = {fmax(a.x, b.x), fmax(a.y, b.y), fmax(a.z, b.z)};
-  const double l1 = (hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]);
-  return box_meets(lo, hi, sum + dmax * l1 + margin, blo, bhi);
-}
-__device__ bool face_gate(const DevShapes& sh, int shp, const double* gv, int f, double sum, double margin,
-                          const double* blo, const double* bhi) {
-  const int gface = sh.foff[shp] + f;
-  const int fs = sh.fstart[gface];
-  const int nbv = sh.fstart[gface + 1] - fs;
-  const int* fv = sh.fidx + fs;
-  const v3 A = lds_v(gv, fv[0]), B = lds_v(gv, fv[1]), C = lds_v(gv, fv[nbv - 1]);
-  v3 v1 = vsub(B, A);
-  vnormalize(&v1);
-  v3 n = vcross(v1, vsub(C, A));
-  vnormalize(&n);
-  double lo[3] = {A.x, A.y, A.z}, hi[3] = {A.x, A.y, A.z};
-  double thick = 0.0;
-  for (int k = 1; k < nbv; k++) {
-    const v3 q = lds_v(gv, fv[k]);
-    lo[0] = fmin(lo[0], q.x);
-    lo[1] = fmin(lo[1], q.y);
-    lo[2] = fmin(lo[2], q.z);
-    hi[0] = fmax(hi[0], q.x);
-    hi[1] = fmax(hi[1], q.y);
-    hi[2] = fmax(hi[2], q.z);
-    thick = fmax(thick, fabs(vdot(vsub(q, A), n)));
+// The parts of the edge and face culls that depend on the body alone, once
+// per body and rebuild instead of once per candidate pair: an edge's box and
+// the L1 length of its box; a face's box and its thickness along its normal
+// (the face normal from its first two edges, as the predicates take it).
+// 7 doubles each, body b's edges from beoff[b], its faces from bfoff[b].
+__global__ void k_gate_boxes(Dev d, double* gedge, double* gface) {
+  const int b = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= d.nb) return;
+  const int shp = d.kin[b].shape;
+  const int ne = d.sh.ne[shp], nf = d.sh.nf[shp];
+  const double* gv = d.gv + 3 * (size_t)d.bvoff[b];
+  for (int k = lane; k < ne + nf; k += 64) {
+    double lo[3], hi[3], x;
+    if (k < ne) {
+      const int* ed = d.sh.edge + 2 * (d.sh.eoff[shp] + k);
+      const v3 a = lds_v(gv, ed[0]), c = lds_v(gv, ed[1]);
+      lo[0] = fmin(a.x, c.x);
+      lo[1] = fmin(a.y, c.y);
+      lo[2] = fmin(a.z, c.z);
+      hi[0] = fmax(a.x, c.x);
+      hi[1] = fmax(a.y, c.y);
+      hi[2] = fmax(a.z, c.z);
+      x = (hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]);
+    } else {
+      const int gfc = d.sh.foff[shp] + (k - ne);
+      const int fs = d.sh.fstart[gfc];
+      const int nbv = d.sh.fstart[gfc + 1] - fs;
+      const int* fv = d.sh.fidx + fs;
+      const v3 A = lds_v(gv, fv[0]), B = lds_v(gv, fv[1]), C = lds_v(gv, fv[nbv - 1]);
+      v3 v1 = vsub(B, A);
+      vnormalize(&v1);
+      v3 n = vcross(v1, vsub(C, A));
+      vnormalize(&n);
+      lo[0] = hi[0] = A.x;
+      lo[1] = hi[1] = A.y;
+      lo[2] = hi[2] = A.z;
+      x = 0.0;
+      for (int q = 1; q < nbv; q++) {
+        const v3 p = lds_v(gv, fv[q]);
+        lo[0] = fmin(lo[0], p.x);
+        lo[1] = fmin(lo[1], p.y);
+        lo[2] = fmin(lo[2], p.z);
+        hi[0] = fmax(hi[0], p.x);
+        hi[1] = fmax(hi[1], p.y);
+        hi[2] = fmax(hi[2], p.z);
+        x = fmax(x, fabs(vdot(vsub(p, A), n)));
+      }
+    }
+    double* o = k < ne ? gedge + 7 * ((size_t)d.beoff[b] + k) : gface + 7 * ((size_t)d.bfoff[b] + (k - ne));
+    o[0] = lo[0];
+    o[1] = lo[1];
+    o[2] = lo[2];
+    o[3] = hi[0];
+    o[4] = hi[1];
+    o[5] = hi[2];
+    o[6] = x;
   }
+}
+
+// The culls (they skip only tests whose predicate is provably false):
+// EdgeIsNearEdge needs points of both segments within Ri + Rj + dmax, so an
+// edge whose box grown by that (and dmax times its L1 length, for
+// VertexIsNearEdge's r in [-dmax, 1 + dmax]) misses the other body's box has
+// no test to pass; VertexIsNearFace needs the projection inside the face's
+// prism (box grown by twice the face's thickness).
+__device__ __forceinline__ bool edge_gate_pre(const double* g, double sum, double dmax, double margin,
+                                              const double* blo, const double* bhi) {
+  const double lo[3] = {g[0], g[1], g[2]}, hi[3] = {g[3], g[4], g[5]};
+  return box_meets(lo, hi, sum + dmax * g[6] + margin, blo, bhi);
+}
+__device__ __forceinline__ bool face_gate_pre(const double* g, double sum, double margin, const double* blo,
+                                              const double* bhi) {
+  const double thick = g[6];
   if (!(thick == thick)) return true;  // degenerate face: no cull
+  const double lo[3] = {g[0], g[1], g[2]}, hi[3] = {g[3], g[4], g[5]};
   return box_meets(lo, hi, sum + 2.0 * thick + margin, blo, bhi);
 }
+
 
 
 // Wavefront compaction of the indices v < n with pred(v) into list (in
@@ -673,11 +721,15 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
   // edges and faces of both bodies in one pass; tag = kind * 2^20 + index,
   // kinds 0 edge of i, 1 edge of j, 2 face of i, 3 face of j
   const int ne2 = nei + nej, nef = ne2 + nfi + nfj;
+  const double* gei = cfg.gedge + 7 * (size_t)d.beoff[i];
+  const double* gej = cfg.gedge + 7 * (size_t)d.beoff[j];
+  const double* gfi = cfg.gface + 7 * (size_t)d.bfoff[i];
+  const double* gfj = cfg.gface + 7 * (size_t)d.bfoff[j];
   const int nefg = wave_compact(nef, lane, ef_l, [&](int q) {
-    if (q < nei) return edge_gate(d.sh, shi, gvi, q, sum, dmax, margin, bx + 6, bx + 9);
-    if (q < ne2) return edge_gate(d.sh, shj, gvj, q - nei, sum, dmax, margin, bx, bx + 3);
-    if (q < ne2 + nfi) return face_gate(d.sh, shi, gvi, q - ne2, sum, margin, bx + 6, bx + 9);
-    return face_gate(d.sh, shj, gvj, q - ne2 - nfi, sum, margin, bx, bx + 3);
+    if (q < nei) return edge_gate_pre(gei + 7 * q, sum, dmax, margin, bx + 6, bx + 9);
+    if (q < ne2) return edge_gate_pre(gej + 7 * (q - nei), sum, dmax, margin, bx, bx + 3);
+    if (q < ne2 + nfi) return face_gate_pre(gfi + 7 * (q - ne2), sum, margin, bx + 6, bx + 9);
+    return face_gate_pre(gfj + 7 * (q - ne2 - nfi), sum, margin, bx, bx + 3);
   });
   // vertex-vertex partners: vertices of j within sum of i's box
   const double gvv = sum + margin;
@@ -1053,6 +1105,31 @@ __global__ void k_heavy_list(int nb, const int32_t* flag, const int32_t* pos, in
   if (b < nb && flag[b]) heavy[pos[b]] = b;
 }
 
+// Chunks of 64 slots of the heavy bodies, and the count of heavy bodies that
+// are not passive.  One workgroup: thread 0 lays out the chunk offsets
+// (a few thousand heavy bodies at most), then the threads fill the chunks.
+__global__ void k_heavy_chunks(const int32_t* heavy, int nheavy, const int32_t* soff, const uint8_t* passive,
+                               int32_t* hch_off, int2* hch, int cap, int32_t* out) {
+  if (threadIdx.x == 0) {
+    int off = 0, np = 0;
+    for (int h = 0; h < nheavy; h++) {
+      const int b = heavy[h];
+      hch_off[h] = off;
+      off += (soff[b + 1] - soff[b] + 63) / 64;
+      np += passive[b] ? 0 : 1;
+    }
+    hch_off[nheavy] = off;
+    out[0] = off;
+    out[1] = np;
+  }
+  __syncthreads();
+  for (int h = threadIdx.x; h < nheavy; h += blockDim.x) {
+    const int b = heavy[h];
+    const int k0 = soff[b], k1 = soff[b + 1];
+    for (int c = hch_off[h], k = k0; c < hch_off[h + 1] && c < cap; c++, k += 64) hch[c] = make_int2(k, min(k + 64, k1));
+  }
+}
+
 inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -1098,7 +1175,7 @@ static int bits_for(uint64_t maxval) {
   return b;
 }
 
-int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
+int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipStream_t s) {
   const int n = E.n;
   if (ensure(w, E.istart, E.istart_cap, (size_t)nb + 1) || ensure(w, E.jstart, E.jstart_cap, (size_t)nb + 1) ||
       ensure(w, E.jlist, E.jlist_cap, (size_t)(n > 0 ? n : 1)) || ensure(w, w.i32a, w.i32a_cap, (size_t)nb + 1) ||
@@ -1136,11 +1213,13 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
     hipLaunchKernelGGL(k_slot_act, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.spos, E.eact, E.sact);
   }
   RK_CHECK(hipGetLastError());
-  return nl_build_heavy(w, nb, E, s);
+  return nl_build_heavy(w, nb, E, passive, s);
 }
 
-int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
+int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipStream_t s) {
   E.nheavy = 0;
+  E.nhch = 0;
+  E.nheavy_np = 0;
   if (nb <= 0 || !E.soff) return 0;
   if (ensure(w, E.heavy, E.heavy_cap, (size_t)nb) || ensure(w, w.i32a, w.i32a_cap, (size_t)nb + 1) ||
       ensure(w, w.i32c, w.i32c_cap, (size_t)nb + 1))
@@ -1153,6 +1232,31 @@ int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, hipStream_t s) {
   RK_CHECK(hipMemcpyAsync(&tail[1], w.i32a + nb - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   RK_CHECK(hipStreamSynchronize(s));
   E.nheavy = tail[0] + tail[1];
+  if (E.nheavy == 0) return 0;
+  // chunk table: at most (slots of all bodies) / 64 + one per heavy body
+  const size_t cap = (size_t)E.cap * 2 / 64 + (size_t)E.nheavy + 1;
+  if (ensure(w, E.hch, E.hch_cap, cap) || ensure(w, E.hch_off, E.hch_off_cap, (size_t)E.nheavy + 1)) return -1;
+  if (E.hpart_cap < cap) {
+    if (E.hpart) (void)hipFree(E.hpart);
+    E.hpart = nullptr;
+    E.hpart_cap = 0;
+    if (hipMalloc(&E.hpart, cap * 6 * sizeof(double)) != hipSuccess) {
+      snprintf(w.err, sizeof(w.err), "out of device memory (heavy-body partial sums)");
+      return -1;
+    }
+    E.hpart_cap = cap;
+  }
+  hipLaunchKernelGGL(k_heavy_chunks, dim3(1), dim3(kBlock), 0, s, E.heavy, E.nheavy, E.soff, passive, E.hch_off,
+                     E.hch, (int)cap, w.i32c);
+  int32_t cnt[2] = {0, 0};
+  RK_CHECK(hipMemcpyAsync(cnt, w.i32c, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  RK_CHECK(hipStreamSynchronize(s));
+  if ((size_t)cnt[0] > cap) {
+    snprintf(w.err, sizeof(w.err), "heavy-body chunk table overflow");
+    return -1;
+  }
+  E.nhch = cnt[0];
+  E.nheavy_np = cnt[1];
   return 0;
 }
 
@@ -1230,7 +1334,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       RK_CHECK(hipStreamSynchronize(s));
       const int nover = hv[0];
       const int ncb = nb - nover;
-      hipLaunchKernelGGL(k_broad_cells, dim3(nblk(nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
+      hipLaunchKernelGGL(k_broad_cells, dim3(nblk(32L * nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
                          w.cbody, w.cstripe, w.scnt, rcap);
       if (nover > 0 && ncb > 0) {
         const int split = std::max(1, std::min(64, (4096 + nover - 1) / nover));
@@ -1269,6 +1373,13 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     cfg.maxf = p.maxf;
     cfg.maxwords = p.maxwords;
     cfg.dmax = p.dVerlet;
+    // the bodies' edge and face boxes for the culls
+    if (ensure(w, w.gedge, w.gedge_cap, (size_t)7 * std::max(d.netot, 1)) ||
+        ensure(w, w.gface, w.gface_cap, (size_t)7 * std::max(d.nftot, 1)))
+      return -1;
+    hipLaunchKernelGGL(k_gate_boxes, dim3((d.nb + 3) / 4), dim3(kBlock), 0, s, d, w.gedge, w.gface);
+    cfg.gedge = w.gedge;
+    cfg.gface = w.gface;
     const size_t lds = narrow_lds_bytes(p);
     for (int attempt = 0; attempt < 3; attempt++) {
       const int rcap = std::max(64, (w.near_hint + kStripes - 1) / kStripes);
@@ -1388,7 +1499,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     hipLaunchKernelGGL(k_merge<true>, dim3(nblk(ng)), dim3(kBlock), 0, s, d, in, out);
   }
   RK_CHECK(hipGetLastError());
-  if (nl_build_csr(w, nb, nxt, s) != 0) return -1;
+  if (nl_build_csr(w, nb, nxt, d.passive, s) != 0) return -1;
   RK_CHECK(hipStreamSynchronize(s));
   return 0;
 }
@@ -1441,7 +1552,7 @@ void nl_work_free(NLWork& w) {
   void* ps[] = {w.cstripe, w.scnt, w.soff, w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
                 w.gk,    w.gk2,   w.gv,      w.gv2,   w.gflag, w.gpos,  w.ukey,  w.ucand,   w.uold,  w.gcount,
                 w.goff,  w.tmp,   w.i32a,    w.i32b,  w.i32c,  w.f64a,  w.cell_of, w.ccount, w.cstart, w.cfill,
-                w.cbody, w.over};
+                w.cbody, w.over, w.gedge, w.gface};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   NLWork z;

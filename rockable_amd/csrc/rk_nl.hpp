@@ -39,6 +39,15 @@ struct EntryArrays {
   int32_t* heavy = nullptr;
   size_t heavy_cap = 0;
   int nheavy = 0;
+  // heavy bodies' slots in chunks of 64: chunk c covers slots [hch[c].x,
+  // hch[c].y), heavy body h its chunks [hch_off[h], hch_off[h + 1]); hpart
+  // holds each chunk's six partial sums
+  int2* hch = nullptr;
+  int32_t* hch_off = nullptr;
+  double* hpart = nullptr;
+  size_t hch_cap = 0, hch_off_cap = 0, hpart_cap = 0;
+  int nhch = 0;
+  int nheavy_np = 0;  // heavy bodies that are not passive
 };
 
 struct NLParams {
@@ -68,6 +77,8 @@ struct NLWork {
   size_t name##_cap = 0;
   RK_BUF(double, bobb)
   RK_BUF(double, baabb)
+  RK_BUF(double, gedge)
+  RK_BUF(double, gface)
   RK_BUF(int, counter)
   RK_BUF(uint64_t, cand)
   RK_BUF(uint64_t, cstripe)
@@ -108,9 +119,9 @@ struct NLWork {
 int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArrays& nxt, hipStream_t s);
 // istart / jstart / jlist, resultant slots and the heavy-body list of a
 // sorted list.
-int nl_build_csr(NLWork& w, int nb, EntryArrays& E, hipStream_t s);
+int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipStream_t s);
 // The heavy-body list alone (after a change of w.heavy_min).
-int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, hipStream_t s);
+int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipStream_t s);
 int entries_reserve(NLWork& w, EntryArrays& E, int n);
 void entries_free(EntryArrays& E);
 void nl_work_free(NLWork& w);

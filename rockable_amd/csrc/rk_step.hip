@@ -350,24 +350,24 @@ __device__ void law_sticked(const Dev& d, const StepParams& sp, const Kin& k, do
   }
 }
 
-// The force phase runs as two kernels.
-//  k_geom   every listed entry: contact geometry from the body poses
-//           (Interaction::UpdateDispatcher without the relative velocity).
-//           An inactive entry gets its new dn (or the zeroed history of a
-//           deactivation) and cleared activity flags; an active one (dn < 0
-//           or bonded) is appended to a compact list with its geometry.
-//  k_law    every active entry: relative velocity, the force law, the history
-//           and the two resultant slots.
-// The active entries of geometry block b go, in entry order, to
-// akin[b * BS, b * BS + acount[b]) (ballot + block prefix, no
-// atomics); k_law gives one wavefront to each block's list.
+// The force phase: geometry, then law, as two kernels or fused in one.
+//  geometry  every listed entry: contact geometry from the body poses
+//            (Interaction::UpdateDispatcher without the relative velocity).
+//            An inactive entry gets its new dn (or the zeroed history of a
+//            deactivation) and cleared activity flags; an active one (dn < 0
+//            or bonded) keeps its geometry for the law.
+//  law       every active entry: relative velocity, the force law, the
+//            history and the two resultant slots.
+// Two kernels: k_geom writes the active entries of geometry block b to
+// akin[b * BS, b * BS + acount[b]) (ballot + block prefix, no atomics) and
+// k_law gives a wavefront to each block's list.  Fused: k_interact keeps
+// them in LDS.
+//
+// The geometry pass of one block of BS entries (block index blk): returns
+// whether this thread's entry is active, with its kinematics in a.  Every
+// thread of the block calls it (it synchronises the block).
 template <bool GEOM_ALL, int BS>
-__global__ void __launch_bounds__(BS) k_geom(Dev d) {
-  __shared__ int wcount[BS / 64];
-  if (frozen(d)) return;
-  __shared__ int wtype[BS / 64][5];
-  __shared__ short perm[BS];
-  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+__device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5], short* perm, ActiveKin& a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // The block's entries are taken in the order of their contact type (a
   // counting sort in LDS; entries past the end last), so that a wavefront
@@ -392,7 +392,6 @@ __global__ void __launch_bounds__(BS) k_geom(Dev d) {
   }
   const int e = blk * BS + perm[threadIdx.x];
   bool act = false;
-  ActiveKin a;
   if (e < d.ne) {
     const int i = d.ei[e], j = d.ej[e];
     const uint32_t sub = d.esub[e];
@@ -426,6 +425,8 @@ __global__ void __launch_bounds__(BS) k_geom(Dev d) {
       a.prev_dn = k.prev_dn;
       a.e = e;
       a.deact = k.deact ? 1 : 0;
+      a.i = i;
+      a.j = j;
     } else {
       // activity byte: bit 0 active, bit 1 history maybe non-zero; the
       // history and the slot flags are written only when they change
@@ -468,16 +469,126 @@ __global__ void __launch_bounds__(BS) k_geom(Dev d) {
       }
     }
   }
+  return act;
+}
+
+// Rank of this thread's active entry among the block's (ballot + LDS
+// prefix, no atomics) and the block's count of active entries.
+template <int BS>
+__device__ __forceinline__ int active_rank(bool act, int* wcount, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long m = __ballot(act);
   if (lane == 0) wcount[wave] = __popcll(m);
   __syncthreads();
-  int base = 0, total = 0;
+  int base = 0, tot = 0;
   for (int w = 0; w < BS / 64; w++) {
     base += w < wave ? wcount[w] : 0;
-    total += wcount[w];
+    tot += wcount[w];
   }
-  if (act) d.akin[(size_t)blk * BS + base + __popcll(m & ((1ull << lane) - 1ull))] = a;
+  *total = tot;
+  return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+template <bool GEOM_ALL, int BS>
+__global__ void __launch_bounds__(BS) k_geom(Dev d) {
+  __shared__ int wcount[BS / 64];
+  __shared__ int wtype[BS / 64][5];
+  __shared__ short perm[BS];
+  if (frozen(d)) return;
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  ActiveKin a;
+  const bool act = geom_pass<GEOM_ALL, BS>(d, blk, wtype, perm, a);
+  int total = 0;
+  const int r = active_rank<BS>(act, wcount, &total);
+  if (act) d.akin[(size_t)blk * BS + r] = a;
   if (threadIdx.x == 0) d.acount[blk] = total;
+}
+
+// The force law of one active entry (kinematics in a): history, both
+// resultant slots, activity flags.
+template <int LAW>
+__device__ __forceinline__ void law_item(const Dev& d, const StepParams& sp, int write_geom_all, int pii,
+                                         const ActiveKin& a) {
+  const int e = a.e;
+  const int i = a.i, j = a.j;
+  const BodyKin Pi = d.kin[i];
+  const BodyKin Pj = d.kin[j];
+  const int stick = d.estick[e];
+  Kin k;
+  k.n = V(a.nx, a.ny, a.nz);
+  k.pos = V(a.px, a.py, a.pz);
+  k.dn = a.dn;
+  k.prev_dn = a.prev_dn;
+  k.deact = a.deact != 0;
+  Hist h;
+  if (k.deact) {  // a bond keeps its stale velocity; the history restarts
+    const EGeom& g = d.egeo[e];
+    k.vel = V(g.vx, g.vy, g.vz);
+    h.fn = 0.0;
+    h.ft = V(0, 0, 0);
+    h.mom = V(0, 0, 0);
+  } else {
+    k.vel = rel_vel(k.pos, Pi, Pj);
+    const EState s = d.est[e];
+    h.fn = s.fn;
+    h.ft = V(s.ftx, s.fty, s.ftz);
+    h.mom = V(s.momx, s.momy, s.momz);
+  }
+  const double damp = d.edamp[e];
+  if (LAW == LAW_AVALANCHE)
+    law_avalanche(d, sp, k, i, Pi, Pj, h);
+  else if (LAW == LAW_STICKED)
+    law_sticked(d, sp, k, damp, stick, i, j, Pi, Pj, h, pii != 0);
+  else
+    law_default(d, sp, k, damp, Pi, Pj, h);
+  EState o;
+  o.fn = h.fn;
+  o.ftx = h.ft.x;
+  o.fty = h.ft.y;
+  o.ftz = h.ft.z;
+  o.momx = h.mom.x;
+  o.momy = h.mom.y;
+  o.momz = h.mom.z;
+  d.est[e] = o;
+  d.edn[e] = k.dn;
+  // incrementResultants (Rockable.cpp:2756-2764), both sides
+  v3 f = vadd(vsc(h.fn, k.n), h.ft);
+  v3 Ci = vsub(k.pos, kpos(Pi));
+  v3 Cj = vsub(k.pos, vadd(kpos(Pj), V(0.0, 0.0, 0.0)));
+  v3 mi = vadd(vcross(Ci, f), h.mom);
+  v3 mj = vsub(vcross(Cj, vneg(f)), h.mom);
+  // the two resultant slots: body i adds (f, mi), body j adds (-f, mj);
+  // x + (-y) is x - y exactly, so the serial sums are unchanged
+  const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
+  double* ci = d.sval + 6 * (size_t)slots.x;
+  double* cj = d.sval + 6 * (size_t)slots.y;
+  st3(ci, f);
+  st3(ci + 3, mi);
+  st3(cj, vneg(f));
+  st3(cj + 3, mj);
+  // Geometry is persistent for bonds (a deactivated bond keeps its stale n,
+  // pos, vel); for other entries it is written when an output is due.
+  if (stick >= 0 || (write_geom_all && !k.deact)) {
+    EGeom g;
+    g.nx = k.n.x;
+    g.ny = k.n.y;
+    g.nz = k.n.z;
+    g.px = k.pos.x;
+    g.py = k.pos.y;
+    g.pz = k.pos.z;
+    g.vx = k.vel.x;
+    g.vy = k.vel.y;
+    g.vz = k.vel.z;
+    d.egeo[e] = g;
+  }
+  const uint8_t old = d.eact[e];
+  if (old != 3) {
+    d.eact[e] = 3;
+    if (!(old & 1)) {
+      d.sact[slots.x] = 1;
+      d.sact[slots.y] = 1;
+    }
+  }
 }
 
 // Each wavefront takes W (at most kLawGroup) consecutive geometry blocks
@@ -511,87 +622,29 @@ __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_
         before = pre[m];
       }
     const ActiveKin a = d.akin[(size_t)(g0 + gb) * d.geom_block + (t - before)];
-    const int e = a.e;
-    const int i = d.ei[e], j = d.ej[e];
-    const BodyKin Pi = d.kin[i];
-    const BodyKin Pj = d.kin[j];
-    const int stick = d.estick[e];
-    Kin k;
-    k.n = V(a.nx, a.ny, a.nz);
-    k.pos = V(a.px, a.py, a.pz);
-    k.dn = a.dn;
-    k.prev_dn = a.prev_dn;
-    k.deact = a.deact != 0;
-    Hist h;
-    if (k.deact) {  // a bond keeps its stale velocity; the history restarts
-      const EGeom& g = d.egeo[e];
-      k.vel = V(g.vx, g.vy, g.vz);
-      h.fn = 0.0;
-      h.ft = V(0, 0, 0);
-      h.mom = V(0, 0, 0);
-    } else {
-      k.vel = rel_vel(k.pos, Pi, Pj);
-      const EState s = d.est[e];
-      h.fn = s.fn;
-      h.ft = V(s.ftx, s.fty, s.ftz);
-      h.mom = V(s.momx, s.momy, s.momz);
-    }
-    const double damp = d.edamp[e];
-    if (LAW == LAW_AVALANCHE)
-      law_avalanche(d, sp, k, i, Pi, Pj, h);
-    else if (LAW == LAW_STICKED)
-      law_sticked(d, sp, k, damp, stick, i, j, Pi, Pj, h, pii != 0);
-    else
-      law_default(d, sp, k, damp, Pi, Pj, h);
-    EState o;
-    o.fn = h.fn;
-    o.ftx = h.ft.x;
-    o.fty = h.ft.y;
-    o.ftz = h.ft.z;
-    o.momx = h.mom.x;
-    o.momy = h.mom.y;
-    o.momz = h.mom.z;
-    d.est[e] = o;
-    d.edn[e] = k.dn;
-    // incrementResultants (Rockable.cpp:2756-2764), both sides
-    v3 f = vadd(vsc(h.fn, k.n), h.ft);
-    v3 Ci = vsub(k.pos, kpos(Pi));
-    v3 Cj = vsub(k.pos, vadd(kpos(Pj), V(0.0, 0.0, 0.0)));
-    v3 mi = vadd(vcross(Ci, f), h.mom);
-    v3 mj = vsub(vcross(Cj, vneg(f)), h.mom);
-    // the two resultant slots: body i adds (f, mi), body j adds (-f, mj);
-    // x + (-y) is x - y exactly, so the serial sums are unchanged
-    const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
-    double* ci = d.sval + 6 * (size_t)slots.x;
-    double* cj = d.sval + 6 * (size_t)slots.y;
-    st3(ci, f);
-    st3(ci + 3, mi);
-    st3(cj, vneg(f));
-    st3(cj + 3, mj);
-    // Geometry is persistent for bonds (a deactivated bond keeps its stale n,
-    // pos, vel); for other entries it is written when an output is due.
-    if (stick >= 0 || (write_geom_all && !k.deact)) {
-      EGeom g;
-      g.nx = k.n.x;
-      g.ny = k.n.y;
-      g.nz = k.n.z;
-      g.px = k.pos.x;
-      g.py = k.pos.y;
-      g.pz = k.pos.z;
-      g.vx = k.vel.x;
-      g.vy = k.vel.y;
-      g.vz = k.vel.z;
-      d.egeo[e] = g;
-    }
-    const uint8_t old = d.eact[e];
-    if (old != 3) {
-      d.eact[e] = 3;
-      if (!(old & 1)) {
-        d.sact[slots.x] = 1;
-        d.sact[slots.y] = 1;
-      }
-    }
+    law_item<LAW>(d, sp, write_geom_all, pii, a);
   }
+}
+
+// Geometry and force law fused: the block's active entries go to LDS instead
+// of the global active list, and the block's first threads apply the law to
+// them after the geometry pass (no second launch, no round trip of the
+// active kinematics through memory).
+template <int LAW, bool GEOM_ALL>
+__global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int write_geom_all, int pii) {
+  __shared__ int wcount[kBlock / 64];
+  __shared__ int wtype[kBlock / 64][5];
+  __shared__ short perm[kBlock];
+  __shared__ ActiveKin sa[kBlock];
+  if (frozen(d)) return;
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  ActiveKin a;
+  const bool act = geom_pass<GEOM_ALL, kBlock>(d, blk, wtype, perm, a);
+  int total = 0;
+  const int r = active_rank<kBlock>(act, wcount, &total);
+  if (act) sa[r] = a;
+  __syncthreads();
+  if ((int)threadIdx.x < total) law_item<LAW>(d, sp, write_geom_all, pii, sa[threadIdx.x]);
 }
 
 __device__ __forceinline__ v3 euler_domega(v3 I, v3 w, v3 M) {
@@ -644,18 +697,18 @@ __device__ __forceinline__ void init_resultants(const Dev& d, int b, const BodyK
   if (driven && d.nctrl > 0) {
     c0 = d.ctrl_off[b];
     c1 = d.ctrl_off[b + 1];
+    // selects rather than a switch: a switch over components compiles to
+    // an indexed store into a stack copy of (F, M), i.e. scratch memory
     for (int q = c0; q < c1; q++) {
       const DevControl C = d.ctrl[d.ctrl_idx[q]];
-      switch (C.type) {
-        case C_X_FOR: F.x = C.value; break;
-        case C_Y_FOR: F.y = C.value; break;
-        case C_Z_FOR: F.z = C.value; break;
-        case C_XROT_MOM: M.x = C.value; break;
-        case C_YROT_MOM: M.y = C.value; break;
-        case C_ZROT_MOM: M.z = C.value; break;
-        case C_XYZROT_MOM: M = V(C.vx, C.vy, C.vz); break;
-        default: break;
-      }
+      const int ty = C.type;
+      const double v = C.value;
+      F.x = ty == C_X_FOR ? v : F.x;
+      F.y = ty == C_Y_FOR ? v : F.y;
+      F.z = ty == C_Z_FOR ? v : F.z;
+      M.x = ty == C_XROT_MOM ? v : ty == C_XYZROT_MOM ? C.vx : M.x;
+      M.y = ty == C_YROT_MOM ? v : ty == C_XYZROT_MOM ? C.vy : M.y;
+      M.z = ty == C_ZROT_MOM ? v : ty == C_XYZROT_MOM ? C.vz : M.z;
     }
   }
 }
@@ -679,10 +732,11 @@ __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, 
     arot = V(0.0, 0.0, 0.0);
     for (int qq = c0; qq < c1; qq++) {
       const DevControl C = d.ctrl[d.ctrl_idx[qq]];
-      switch (C.type) {
-        case C_X_FOR: acc.x = F.x / m; break;
-        case C_Y_FOR: acc.y = F.y / m; break;
-        case C_Z_FOR: acc.z = F.z / m; break;
+      const int ty = C.type;
+      acc.x = ty == C_X_FOR ? F.x / m : acc.x;
+      acc.y = ty == C_Y_FOR ? F.y / m : acc.y;
+      acc.z = ty == C_Z_FOR ? F.z / m : acc.z;
+      switch (ty) {
         case C_XROT_MOM:
         case C_YROT_MOM:
         case C_ZROT_MOM:
@@ -725,21 +779,14 @@ __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, 
   if (driven) {
     for (int qq = c0; qq < c1; qq++) {
       const DevControl C = d.ctrl[d.ctrl_idx[qq]];
-      switch (C.type) {
-        case C_X_FOR: P.vx += sp.dt_2 * acc.x; break;
-        case C_Y_FOR: P.vy += sp.dt_2 * acc.y; break;
-        case C_Z_FOR: P.vz += sp.dt_2 * acc.z; break;
-        case C_XROT_MOM: P.wx += sp.dt_2 * arot.x; break;
-        case C_YROT_MOM: P.wy += sp.dt_2 * arot.y; break;
-        case C_ZROT_MOM: P.wz += sp.dt_2 * arot.z; break;
-        case C_XYZROT_MOM: {
-          v3 w = vadd(kvrot(P), vsc(sp.dt_2, arot));
-          P.wx = w.x;
-          P.wy = w.y;
-          P.wz = w.z;
-        } break;
-        default: break;
-      }
+      const int ty = C.type;
+      const v3 w = vadd(kvrot(P), vsc(sp.dt_2, arot));
+      P.vx = ty == C_X_FOR ? P.vx + sp.dt_2 * acc.x : P.vx;
+      P.vy = ty == C_Y_FOR ? P.vy + sp.dt_2 * acc.y : P.vy;
+      P.vz = ty == C_Z_FOR ? P.vz + sp.dt_2 * acc.z : P.vz;
+      P.wx = ty == C_XROT_MOM || ty == C_XYZROT_MOM ? w.x : P.wx;
+      P.wy = ty == C_YROT_MOM || ty == C_XYZROT_MOM ? w.y : P.wy;
+      P.wz = ty == C_ZROT_MOM || ty == C_XYZROT_MOM ? w.z : P.wz;
     }
   } else {
     v3 v = vadd(kvel(P), vsc(sp.dt_2, acc));
@@ -760,71 +807,131 @@ __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, 
   out->wz = P.wz;
 }
 
-// compute_resultants (Rockable.cpp:3267-3272) for bodies with at most
-// heavy_min slots.  Six lanes per body, one per component of (F, M); each
-// lane adds the body's active slots in slot order, which is the reference's
-// serial order, so every component is the reference's sum bit for bit.
-// Ten bodies per wavefront (lanes 60-63 idle).
-constexpr int kLanesPerBody = 6;
-constexpr int kBodiesPerWave = 10;
+// compute_resultants (Rockable.cpp:3267-3272).  Sixteen lanes per body,
+// four bodies per wavefront.  The lanes read the slot activity bytes 64 at a
+// time (four each) and join them into one 64-bit mask; lanes 0-5, one per
+// component of (F, M), then fetch the terms of the active slots eight at a
+// time (independent loads) and add them in slot order.  Called by every
+// lane of the wavefront; returns the updated sum in lanes gl < 6.
+constexpr int kLanesPerBody = 16;
+constexpr int kBodiesPerWave = 4;
 
+__device__ __forceinline__ double sum_slots(const Dev& d, bool run, int k0, int k1, int gl, double s) {
+  const double* val = d.sval + (gl < 6 ? gl : 0);
+  for (int it = 0;; it++) {
+    const int w = k0 + 64 * it;
+    const bool live = run && w < k1;
+    if (!__any(live)) break;
+    unsigned m4 = 0u;
+    if (live) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int k = w + 4 * gl + q;
+        if (k < k1 && d.sact[k]) m4 |= 1u << q;
+      }
+    }
+    unsigned long long rest = (unsigned long long)m4 << (4 * gl);
+    for (int o = 1; o < kLanesPerBody; o <<= 1) rest |= __shfl_xor(rest, o, kLanesPerBody);
+    if (gl < 6) {
+      while (rest) {
+        double x[8];
+        int n = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          x[t] = 0.0;
+          if (rest) {
+            const int pbit = __ffsll((long long)rest) - 1;
+            rest &= rest - 1ull;
+            x[t] = val[6 * (size_t)(w + pbit)];
+            n = t + 1;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+          if (t < n) s += x[t];
+      }
+    }
+  }
+  return s;
+}
+
+// Bodies with at most heavy_min slots: the serial sum, so every component is
+// the reference's bit for bit.  The wavefronts after the bodies' take the
+// 64-slot chunks of the heavy bodies and store each chunk's partial sums
+// (from -0.0, the identity of addition) for k_heavy_finish.
 __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
   if (frozen(d)) return;
   const int lane = threadIdx.x & 63;
   const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int wave = (int)((lb * (unsigned)blockDim.x + threadIdx.x) >> 6);
-  const int slot = lane / kLanesPerBody;
-  const int comp = lane - kLanesPerBody * slot;
-  const int b = wave * kBodiesPerWave + slot;
-  const bool valid = slot < kBodiesPerWave && b < d.nb;
+  const int gl = lane & (kLanesPerBody - 1), gbase = lane - gl;
+  const int body_waves = (d.nb + kBodiesPerWave - 1) / kBodiesPerWave;
+  if (wave >= body_waves) {
+    const int c = (wave - body_waves) * kBodiesPerWave + lane / kLanesPerBody;
+    const bool run = c < d.nhch;
+    int2 r = make_int2(0, 0);
+    if (run) r = d.hch[c];
+    const double s = sum_slots(d, run, r.x, r.y, gl, -0.0);
+    if (run && gl < 6) d.hpart[6 * (size_t)c + gl] = s;
+    return;
+  }
+  const int b = wave * kBodiesPerWave + lane / kLanesPerBody;
+  int k0 = 0, k1 = 0, c0 = 0, c1 = 0;
+  bool run = false, driven = false;
+  if (b < d.nb) {
+    k0 = d.soff[b];
+    k1 = d.soff[b + 1];
+    run = (k1 - k0) <= d.heavy_min;
+  }
   BodyKin P;
   v3 F = V(0.0, 0.0, 0.0), M = V(0.0, 0.0, 0.0);
-  int c0 = 0, c1 = 0;
-  bool mine = false, driven = false;
-  double s = 0.0;
-  if (valid) {
+  if (run) {
     P = d.kin[b];
     driven = b < d.nDriven;
     init_resultants(d, b, P, driven, F, M, c0, c1);
-    const int k0 = d.soff[b], k1 = d.soff[b + 1];
-    mine = (k1 - k0) <= d.heavy_min;
-    s = pick(F, M, comp);
-    if (mine) {
-      const uint8_t* act = d.sact;
-      const double* val = d.sval + comp;
-      int k = k0;
-      for (; k + 4 <= k1; k += 4) {
-        const uint8_t a0 = act[k], a1 = act[k + 1], a2 = act[k + 2], a3 = act[k + 3];
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3_ = 0.0;
-        if (a0) v0 = val[6 * (size_t)k];
-        if (a1) v1 = val[6 * (size_t)(k + 1)];
-        if (a2) v2 = val[6 * (size_t)(k + 2)];
-        if (a3) v3_ = val[6 * (size_t)(k + 3)];
-        if (a0) s += v0;
-        if (a1) s += v1;
-        if (a2) s += v2;
-        if (a3) s += v3_;
-      }
-      for (; k < k1; k++)
-        if (act[k]) s += val[6 * (size_t)k];
-    }
   }
-  const int base = kLanesPerBody * (slot < kBodiesPerWave ? slot : 0);
-  const double fx = __shfl(s, base), fy = __shfl(s, base + 1), fz = __shfl(s, base + 2);
-  const double mx = __shfl(s, base + 3), my = __shfl(s, base + 4), mz = __shfl(s, base + 5);
-  if (valid && mine && comp == 0) finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+  const double s = sum_slots(d, run, k0, k1, gl, pick(F, M, gl < 6 ? gl : 0));
+  const double fx = __shfl(s, gbase), fy = __shfl(s, gbase + 1), fz = __shfl(s, gbase + 2);
+  const double mx = __shfl(s, gbase + 3), my = __shfl(s, gbase + 4), mz = __shfl(s, gbase + 5);
+  if (run && gl == 0) finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
 }
 
-// compute_resultants for a body with more than heavy_min slots (a wall under
-// a packing), one workgroup per body.
-//  * passive body (its force and moment feed nothing back): a fixed-shape
-//    tree -- thread t sums slots t, t + 256, ... in order, then a pairwise
-//    tree over the threads.  Deterministic; it differs from the serial sum
-//    only by rounding (relative 1e-16 x log2 of the slot count).
-//  * otherwise (or with exact_sums): the serial sum.  Waves 1-3 stage chunks
-//    of kHeavyChunk slots into LDS (inactive slots as -0.0, the identity of
-//    addition: x + (-0.0) == x for every x), double-buffered, while lanes
-//    0-5 of wave 0 add them in order, one lane per component.
+// A passive heavy body (its force and moment feed nothing back into the
+// motion; exact_sums off): the initial force and moment plus its chunks'
+// partial sums, reduced by one wavefront in a fixed order (lane l adds chunks
+// l, l + 64, ... in turn, then a butterfly over the lanes; IEEE addition is
+// commutative, so every lane holds the same sum).  Deterministic; it differs
+// from the serial sum only by rounding.
+__global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int kick) {
+  if (frozen(d)) return;
+  const int h = blockIdx.x;
+  const int b = d.heavy[h];
+  if (!d.passive[b] || d.exact_sums) return;
+  const int lane = threadIdx.x;
+  double a[6] = {-0.0, -0.0, -0.0, -0.0, -0.0, -0.0};
+  for (int c = d.hch_off[h] + lane; c < d.hch_off[h + 1]; c += 64) {
+    const double* p = d.hpart + 6 * (size_t)c;
+#pragma unroll
+    for (int q = 0; q < 6; q++) a[q] += p[q];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < 6; q++) a[q] += __shfl_xor(a[q], off, 64);
+  if (lane != 0) return;
+  const BodyKin P = d.kin[b];
+  const bool driven = b < d.nDriven;
+  v3 F, M;
+  int c0, c1;
+  init_resultants(d, b, P, driven, F, M, c0, c1);
+  finish_body(d, sp, kick, b, P, driven, vadd(F, V(a[0], a[1], a[2])), vadd(M, V(a[3], a[4], a[5])), c0, c1);
+}
+
+// compute_resultants for a heavy body that is not passive (or any heavy
+// body with exact_sums): the serial sum, one workgroup per body.  Waves 1-3
+// stage chunks of kHeavyChunk slots into LDS (inactive slots as -0.0, the
+// identity of addition: x + (-0.0) == x for every x), double-buffered, while
+// lanes 0-5 of wave 0 add them in order, one lane per component.
 constexpr int kHeavyChunk = 512;
 
 __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, int kick) {
@@ -833,6 +940,7 @@ __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, i
   const int b = d.heavy[blockIdx.x];
   const int t = threadIdx.x;
   const int k0 = d.soff[b], k1 = d.soff[b + 1];
+  if (d.passive[b] && !d.exact_sums) return;  // summed by chunks
   const BodyKin P = d.kin[b];
   const bool driven = b < d.nDriven;
   v3 F, M;
@@ -840,31 +948,6 @@ __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, i
   init_resultants(d, b, P, driven, F, M, c0, c1);
   const uint8_t* act = d.sact;
   const double* val = d.sval;
-  if (d.passive[b] && !d.exact_sums) {
-    double a[6] = {-0.0, -0.0, -0.0, -0.0, -0.0, -0.0};
-    for (int k = k0 + t; k < k1; k += kBlock) {
-      if (act[k]) {
-        const double* v = val + 6 * (size_t)k;
-#pragma unroll
-        for (int c = 0; c < 6; c++) a[c] += v[c];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 6; c++) lds[c * kBlock + t] = a[c];
-    __syncthreads();
-    for (int h = kBlock / 2; h > 0; h >>= 1) {
-      if (t < h)
-#pragma unroll
-        for (int c = 0; c < 6; c++) lds[c * kBlock + t] += lds[c * kBlock + t + h];
-      __syncthreads();
-    }
-    if (t == 0) {
-      const v3 Fs = V(lds[0], lds[kBlock], lds[2 * kBlock]);
-      const v3 Ms = V(lds[3 * kBlock], lds[4 * kBlock], lds[5 * kBlock]);
-      finish_body(d, sp, kick, b, P, driven, vadd(F, Fs), vadd(M, Ms), c0, c1);
-    }
-    return;
-  }
   const int len = k1 - k0;
   const int nchunks = (len + kHeavyChunk - 1) / kHeavyChunk;
   const int wave = t >> 6, lane = t & 63;
@@ -1304,8 +1387,26 @@ void launch_geom(const Dev& d, int write_geom_all, hipStream_t s) {
     hipLaunchKernelGGL((k_geom<false, BS>), g, bl, 0, s, d);
 }
 
+template <bool GEOM_ALL>
+void launch_fused(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
+  const dim3 g((d.ne + kBlock - 1) / kBlock), bl(kBlock);
+  if (sp.law == LAW_AVALANCHE)
+    hipLaunchKernelGGL((k_interact<LAW_AVALANCHE, GEOM_ALL>), g, bl, 0, s, d, sp, write_geom_all, 0);
+  else if (sp.law == LAW_STICKED)
+    hipLaunchKernelGGL((k_interact<LAW_STICKED, GEOM_ALL>), g, bl, 0, s, d, sp, write_geom_all, d.paramsInInterfaces);
+  else
+    hipLaunchKernelGGL((k_interact<LAW_DEFAULT, GEOM_ALL>), g, bl, 0, s, d, sp, write_geom_all, 0);
+}
+
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
   if (d.ne <= 0) return;
+  if (d.fuse_law) {
+    if (write_geom_all)
+      launch_fused<true>(d, sp, write_geom_all, s);
+    else
+      launch_fused<false>(d, sp, write_geom_all, s);
+    return;
+  }
   switch (d.geom_block) {
     case 64: launch_geom<64>(d, write_geom_all, s); break;
     case 128: launch_geom<128>(d, write_geom_all, s); break;
@@ -1327,25 +1428,115 @@ void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hip
     hipLaunchKernelGGL(k_law<LAW_DEFAULT>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom, W);
 }
 
-void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s, hipStream_t side, hipEvent_t fork,
-                   hipEvent_t join) {
+void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) {
   if (d.nb <= 0) return;
-  const int waves = (d.nb + kBodiesPerWave - 1) / kBodiesPerWave;
+  const int waves = (d.nb + kBodiesPerWave - 1) / kBodiesPerWave + (d.nhch + kBodiesPerWave - 1) / kBodiesPerWave;
   const int wpb = d.gather_waves == 1 || d.gather_waves == 2 ? d.gather_waves : 4;
   const int blocks = (waves + wpb - 1) / wpb;
-  // the two kernels sum disjoint sets of bodies from the same slots
-  const bool split = d.nheavy > 0 && side && fork && join;
-  if (split) {
-    (void)hipEventRecord(fork, s);
-    (void)hipStreamWaitEvent(side, fork, 0);
-    hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, side, d, sp, kick);
-    (void)hipEventRecord(join, side);
-  }
   hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(64 * wpb), 0, s, d, sp, kick);
-  if (split)
-    (void)hipStreamWaitEvent(s, join, 0);
-  else if (d.nheavy > 0)
-    hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
+  if (d.nheavy > 0) {
+    hipLaunchKernelGGL(k_heavy_finish, dim3(d.nheavy), dim3(64), 0, s, d, sp, kick);
+    if (d.nheavy_exact > 0) hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
+  }
+}
+
+// --- statistics of the packing (DataExtractor-style quantities) -----------
+// Partial sums per workgroup (a fixed tree in LDS), then one thread adds the
+// partials in block order: the results do not depend on scheduling.
+constexpr int kStatBody = 4;   // KE translation, KE rotation, max |v|, free bodies
+constexpr int kStatEntry = 5;  // active, contacts (dn < 0), active pairs, sum dn, min dn
+
+__device__ __forceinline__ void stat_block_reduce(double (*red)[kBlock], int nval, const int* is_max, int t) {
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int k = 0; k < nval; k++)
+        red[k][t] = is_max[k] == 1   ? fmax(red[k][t], red[k][t + h])
+                    : is_max[k] == 2 ? fmin(red[k][t], red[k][t + h])
+                                     : red[k][t] + red[k][t + h];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_stat_bodies(Dev d, double* part) {
+  __shared__ double red[kStatBody][kBlock];
+  const int t = threadIdx.x;
+  const int b = blockIdx.x * kBlock + t;
+  double v[kStatBody] = {0.0, 0.0, 0.0, 0.0};
+  if (b < d.nb && b >= d.nDriven) {
+    const BodyKin P = d.kin[b];
+    const double m = d.mass[b];
+    const v3 I = ld3(d.inertia + 3 * (size_t)b);
+    const v3 vel = kvel(P);
+    const v3 wb = qrot(qconj(kq(P)), kvrot(P));
+    const double v2 = vdot(vel, vel);
+    v[0] = 0.5 * m * v2;
+    v[1] = 0.5 * (I.x * wb.x * wb.x + I.y * wb.y * wb.y + I.z * wb.z * wb.z);
+    v[2] = sqrt(v2);
+    v[3] = 1.0;
+  }
+  for (int k = 0; k < kStatBody; k++) red[k][t] = v[k];
+  const int ops[kStatBody] = {0, 0, 1, 0};
+  stat_block_reduce(red, kStatBody, ops, t);
+  if (t < kStatBody) part[(size_t)blockIdx.x * kStatBody + t] = red[t][0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_stat_entries(Dev d, double* part) {
+  __shared__ double red[kStatEntry][kBlock];
+  const int t = threadIdx.x;
+  const int e = blockIdx.x * kBlock + t;
+  double v[kStatEntry] = {0.0, 0.0, 0.0, 0.0, INFINITY};
+  if (e < d.ne && (d.eact[e] & 1)) {
+    const double dn = d.edn[e];
+    v[0] = 1.0;
+    if (dn < 0.0) {
+      v[1] = 1.0;
+      v[3] = dn;
+      v[4] = dn;
+    }
+    // the first active entry of its body pair (entries are sorted by pair)
+    bool head = true;
+    const int i = d.ei[e], j = d.ej[e];
+    for (int k = e - 1; k >= 0 && d.ei[k] == i && d.ej[k] == j; k--)
+      if (d.eact[k] & 1) {
+        head = false;
+        break;
+      }
+    v[2] = head ? 1.0 : 0.0;
+  }
+  for (int k = 0; k < kStatEntry; k++) red[k][t] = v[k];
+  const int ops[kStatEntry] = {0, 0, 0, 0, 2};
+  stat_block_reduce(red, kStatEntry, ops, t);
+  if (t < kStatEntry) part[(size_t)blockIdx.x * kStatEntry + t] = red[t][0];
+}
+
+// out[0..3] body statistics, out[4..8] entry statistics
+__global__ void k_stat_final(const double* pb, int nbp, const double* pe, int nep, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[kStatBody + kStatEntry] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
+  for (int q = 0; q < nbp; q++) {
+    s[0] += pb[kStatBody * q];
+    s[1] += pb[kStatBody * q + 1];
+    s[2] = fmax(s[2], pb[kStatBody * q + 2]);
+    s[3] += pb[kStatBody * q + 3];
+  }
+  for (int q = 0; q < nep; q++) {
+    for (int k = 0; k < 4; k++) s[kStatBody + k] += pe[kStatEntry * q + k];
+    s[kStatBody + 4] = fmin(s[kStatBody + 4], pe[kStatEntry * q + 4]);
+  }
+  for (int k = 0; k < kStatBody + kStatEntry; k++) out[k] = s[k];
+}
+
+int stat_parts(const Dev& d) {
+  return std::max(1, (d.nb + kBlock - 1) / kBlock) * kStatBody + std::max(1, (d.ne + kBlock - 1) / kBlock) * kStatEntry;
+}
+
+void launch_stats(const Dev& d, double* part, double* out, hipStream_t s) {
+  const int nbp = std::max(1, (d.nb + kBlock - 1) / kBlock), nep = std::max(1, (d.ne + kBlock - 1) / kBlock);
+  double* pe = part + (size_t)nbp * kStatBody;
+  hipLaunchKernelGGL(k_stat_bodies, dim3(nbp), dim3(kBlock), 0, s, d, part);
+  hipLaunchKernelGGL(k_stat_entries, dim3(nep), dim3(kBlock), 0, s, d, pe);
+  hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(64), 0, s, part, nbp, pe, nep, out);
 }
 
 __global__ void k_guard_set_step(Dev d, long long step) {

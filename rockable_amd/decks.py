@@ -594,6 +594,110 @@ def avalanche_deck(n_bodies=1000000, seed=3, n_shapes=64, size_range=(4e-3, 6e-3
     return wall_shapes + pool, deck
 
 
+def cube_shape(name, side, radius):
+    """A sphero-cube: cube of the given side (vertex to vertex) in its body
+    frame, 8 vertices, 12 edges, 6 square faces, Minkowski radius; volume of
+    the sphero-cube, inertia per unit mass of the cube of side side + 2 radius."""
+    h = 0.5 * side
+    v = np.array([[x, y, z] for x in (-h, h) for y in (-h, h) for z in (-h, h)])
+    edges = sorted({(a, b) for a in range(8) for b in range(a + 1, 8)
+                    if np.count_nonzero(v[a] != v[b]) == 1})
+    faces = [[0, 1, 3, 2], [4, 5, 7, 6], [0, 1, 5, 4], [2, 3, 7, 6], [0, 2, 6, 4], [1, 3, 7, 5]]
+    r = radius
+    vol = side ** 3 + 6 * side ** 2 * r + 3 * np.pi * side * r ** 2 + 4.0 / 3.0 * np.pi * r ** 3
+    im = (side + 2 * r) ** 2 / 6.0
+    return Shape(name, radius, v, edges, faces, volume=vol, inertia_mass=(im, im, im),
+                 obb_extent=(h + r, h + r, h + r))
+
+
+def bonded_clusters_deck(n_clusters=2000, cluster_dims=(5, 5, 4), side=5e-3, radius_frac=0.05, seed=3,
+                         strength_mean=50.0, strength_cv=0.3, speed=2.0, plane_deg=35.0, clearance=2e-3,
+                         kn=1e7, en2_bond=0.25, dt=1e-7, steps=10000, nl="linkCells", cluster_gap=2e-3):
+    """Breakable-bonds workload: n_clusters clusters of prod(cluster_dims)
+    sphero-cubes (side, Minkowski radius radius_frac side) stacked face to
+    face, each touching pair of a cluster bonded at its coincident vertex
+    pairs (the stickVerticesInClusters rule: vertices closer than
+    Ri + Rj + epsilon, here 1 % of the radius; one StickedLinks interface
+    per pair, its vertex-vertex bonds at rest), the interface strengths fn0
+    and ft0 drawn lognormal (mean strength_mean N, coefficient of variation
+    strength_cv) and kept in the interfaces (ParamsInInterfaces 1; kn, kt =
+    0.8 kn, no rotational stiffness, mom0 1e10, power 2).  The clusters sit
+    `clearance` above a rigid plane inclined at plane_deg (body 0, driven)
+    and fall onto it at `speed` under gravity; contacts follow the
+    StickedLinks contact branch (kn, kt = 0.8 kn, en2 0.25, mu 0.5).
+    Returns (shapes, deck)."""
+    rng = np.random.default_rng(seed)
+    r = radius_frac * side
+    pitch = side + 2 * r
+    cube = cube_shape("CUBE", side, r)
+    nx, ny, nz = cluster_dims
+    cl = np.array([nx, ny, nz], float) * pitch
+    ncx = int(np.ceil(np.sqrt(n_clusters)))
+    ncy = int(np.ceil(n_clusters / ncx))
+    span = np.array([ncx * (cl[0] + cluster_gap), ncy * (cl[1] + cluster_gap)])
+    th = np.radians(plane_deg)
+    half = 0.5 * float(max(span[0] / np.cos(th), span[1])) + 2 * cl.max()
+    plane = square_plane_shape("PLANE", half, r)
+    qp = (float(np.cos(th / 2)), 0.0, float(np.sin(th / 2)), 0.0)   # plane normal (sin th, 0, cos th)
+    nrm = np.array([np.sin(th), 0.0, np.cos(th)])
+    bodies = [Body("PLANE", group=1, pos=(0.0, 0.0, 0.0), Q=qp)]
+    # cube centres of one cluster, relative to its lowest corner
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    local = (np.column_stack([ix.ravel(), iy.ravel(), iz.ravel()]) + 0.5) * pitch
+    vtx = np.asarray(cube.vertex)
+    interactions, interfaces = [], []
+    meff = 0.5 * 2700.0 * cube.volume
+    logen = 0.5 * np.log(en2_bond)
+    damp = -logen / np.sqrt(logen * logen + np.pi ** 2) * 2.0 * np.sqrt(kn * meff)
+    s2 = np.log(1.0 + strength_cv ** 2)
+    mu_ln = np.log(strength_mean) - 0.5 * s2
+    eps = 0.01 * r
+    for c in range(n_clusters):
+        x0 = (c % ncx) * (cl[0] + cluster_gap) - 0.5 * span[0]
+        y0 = (c // ncx) * (cl[1] + cluster_gap) - 0.5 * span[1]
+        # lowest corner of the cluster's box just `clearance` above the plane
+        corners = np.array([[x0 + a * cl[0], y0 + b * cl[1], 0.0] for a in (0, 1) for b in (0, 1)])
+        z0 = max(-np.dot(p, nrm) / nrm[2] for p in corners) + (clearance + r) / nrm[2]
+        first = len(bodies)
+        pos = np.array([x0, y0, z0]) + local
+        for p in pos:
+            bodies.append(Body("CUBE", group=0, cluster=c, pos=tuple(p), vel=(0.0, 0.0, -speed)))
+        for a in range(len(pos)):
+            for b in range(a + 1, len(pos)):
+                if np.abs(pos[a] - pos[b]).sum() > pitch * 1.5:
+                    continue
+                gi, gj = pos[a] + vtx, pos[b] + vtx
+                bonds = []
+                for iv in range(8):
+                    d = np.linalg.norm(gj - gi[iv], axis=1)
+                    for jv in np.nonzero(d < eps + 2 * r)[0]:
+                        bonds.append((0, iv, int(jv)))
+                        n = gi[iv] - gj[jv]
+                        dist = np.linalg.norm(n)
+                        n = n / dist
+                        dn = dist - 2 * r
+                        cp = gi[iv] - n * (r + 0.5 * dn)
+                        interactions.append((first + a, first + b, 0, iv, int(jv)) + tuple(n) + (dn,) + tuple(cp)
+                                            + (0.0, 0.0, 0.0) + (0.0,) + (0.0,) * 6 + (damp,))
+                if bonds:
+                    fn0, ft0 = np.exp(mu_ln + np.sqrt(s2) * rng.standard_normal(2))
+                    par = (kn, 0.8 * kn, 0.0, float(fn0), float(ft0), 1e10, 2.0, 0.0)
+                    interfaces.append((first + a, first + b, 0.0, par, bonds))
+    deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=100 * dt, interConf=1e3,
+                DVerlet=0.1 * side, dVerlet=0.05 * side, density={0: 2700.0, 1: 2700.0},
+                gravity=(0.0, 0.0, -9.81), update_nl=nl, nDriven=1, force_law="StickedLinks",
+                params=contact_params(kn, 0.8 * kn, 0.25, 0.5, groups=((0, 0), (0, 1))))
+    cell = 2.0 * pitch
+    deck.cell_min_sizes = (cell, cell, cell)
+    deck.box_for_linkcells_opt = 1
+    deck.bodies = bodies
+    interactions.sort(key=lambda t: t[:5])
+    deck.interactions = interactions
+    deck.interfaces = interfaces
+    deck.params_in_interfaces = 1
+    return [plane, cube], deck
+
+
 def write_deck(dirpath, shapes, deck, name="deck.conf"):
     import os
     os.makedirs(dirpath, exist_ok=True)

@@ -30,13 +30,22 @@ class RkCounts(ctypes.Structure):
                 ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64)]
 
 
+class RkStats(ctypes.Structure):
+    _fields_ = [("kinetic_translational", ctypes.c_double), ("kinetic_rotational", ctypes.c_double),
+                ("vmax", ctypes.c_double), ("nfree", ctypes.c_int64), ("nactive", ctypes.c_int64),
+                ("ncontacts", ctypes.c_int64), ("active_pairs", ctypes.c_int64), ("coordination", ctypes.c_double),
+                ("dn_mean", ctypes.c_double), ("dn_min", ctypes.c_double), ("ninterfaces", ctypes.c_int64),
+                ("broken_interfaces", ctypes.c_int64), ("total_broken_area", ctypes.c_double)]
+
+
 EXPORTED = ["rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_conf", "rk_load_driving_system",
             "rk_set_force_law", "rk_set_body_force", "rk_set_option", "rk_update_nl", "rk_accelerations",
             "rk_step", "rk_integrate", "rk_save_conf", "rk_synchronize", "rk_get_counts", "rk_get_bodies",
             "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time",
             "rk_mlp_create", "rk_mlp_destroy", "rk_mlp_last_error", "rk_mlp_nparams", "rk_mlp_get_params",
             "rk_mlp_set_params", "rk_mlp_forward", "rk_mlp_loss_grad", "rk_mlp_train", "rk_set_rebuild_model",
-            "rk_guard_features", "rk_get_option", "rk_autotune"]
+            "rk_guard_features", "rk_get_option", "rk_autotune", "rk_get_stats", "rk_load_batch",
+            "rk_batch_packing"]
 
 _lib = None
 
@@ -88,6 +97,9 @@ def load_library(path=LIB_PATH):
     L.rk_guard_features.argtypes = [P, F]
     L.rk_get_option.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
     L.rk_autotune.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.rk_get_stats.argtypes = [P, ctypes.POINTER(RkStats)]
+    L.rk_load_batch.argtypes = [P, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]
+    L.rk_batch_packing.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     _lib = L
     return L
 
@@ -127,6 +139,25 @@ class Engine:
         except Exception:
             pass
 
+    def load_batch(self, conf_paths, driving_paths=None):
+        """Independent packings (decks lying apart in space) as one batch."""
+        n = len(conf_paths)
+        cp = (ctypes.c_char_p * n)(*[os.fspath(x).encode() for x in conf_paths])
+        dp = None
+        if driving_paths is not None:
+            dp = (ctypes.c_char_p * n)(*[None if x is None else os.fspath(x).encode() for x in driving_paths])
+        self._check(self.L.rk_load_batch(self.h, cp, dp, n))
+
+    def packing(self, p):
+        """(first driven, driven count, first free, free count) of packing p."""
+        out = (ctypes.c_int64 * 4)()
+        self._check(self.L.rk_batch_packing(self.h, int(p), out))
+        return tuple(int(x) for x in out)
+
+    def packing_rows(self, p):
+        fd, nd, ff, nf = self.packing(p)
+        return np.concatenate([np.arange(fd, fd + nd), np.arange(ff, ff + nf)])
+
     def load_conf(self, path):
         self._check(self.L.rk_load_conf(self.h, os.fspath(path).encode()))
 
@@ -143,12 +174,18 @@ class Engine:
     def set_option(self, name, value):
         self._check(self.L.rk_set_option(self.h, name.encode(), float(value)))
 
+    def stats(self):
+        """Kinetic energies, contact and bond statistics reduced on the device."""
+        st = RkStats()
+        self._check(self.L.rk_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in RkStats._fields_}
+
     def get_option(self, name):
         v = ctypes.c_double(0.0)
         self._check(self.L.rk_get_option(self.h, name.encode(), ctypes.byref(v)))
         return v.value
 
-    TUNED = ("law_group", "geom_block", "gather_waves", "body_block", "graph_steps")
+    TUNED = ("fuse_law", "law_group", "geom_block", "gather_waves", "body_block", "graph_steps")
 
     def autotune(self, steps_per_trial=100):
         """Pick the launch shapes by timing them on the running simulation

@@ -1,5 +1,7 @@
-"""Launch shapes and the autotuner: every choice of law_group, geom_block,
-gather_waves and body_block computes the same state bit for bit, and the
+"""Launch shapes and the autotuner: every choice of fuse_law, law_group,
+geom_block, gather_waves and body_block computes the same state bit for
+bit (the default engine, against which they are compared, fuses the
+geometry and law kernels; the first three shapes use them apart), and the
 steps the autotuner takes while timing candidates are the steps rk_step
 takes."""
 import pytest
@@ -10,9 +12,10 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("dn", "fn", "ft", "mom", "damp", "stick")
 
-SHAPES = [dict(law_group=1, geom_block=64, gather_waves=1, body_block=64),
-          dict(law_group=8, geom_block=512, gather_waves=2, body_block=128),
-          dict(law_group=2, geom_block=128, gather_waves=4, body_block=256, graph_steps=8)]
+SHAPES = [dict(fuse_law=0, law_group=1, geom_block=64, gather_waves=1, body_block=64),
+          dict(fuse_law=0, law_group=8, geom_block=512, gather_waves=2, body_block=128),
+          dict(fuse_law=0, law_group=2, geom_block=128, gather_waves=4, body_block=256, graph_steps=8),
+          dict(fuse_law=1, gather_waves=2, body_block=64, graph_steps=4)]
 
 
 def _hull_deck(tmp_path):
@@ -55,7 +58,7 @@ def test_autotune_steps_are_rk_steps(tmp_path, dynamic):
     a.update_nl()
     b.update_nl()
     n, choice = b.autotune(steps_per_trial=20)
-    assert n > 20 * 17
+    assert n > 20 * 19
     assert choice["law_group"] in (1, 2, 4, 8) and choice["geom_block"] in (64, 128, 256, 512)
     a.step(n)
     assert_bodies_equal(b.bodies(), a.bodies())
