@@ -297,6 +297,7 @@ def avalanche_forms(a, log):
                        "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
                        "guard_reads": c1["guard_reads"] - c0["guard_reads"],
                        "frozen_steps": c1["guard_wasted"] - c0["guard_wasted"],
+                       "guard_polls": c1["guard_polls"] - c0["guard_polls"],
                        "graph_launches": c1["graph_launches"] - c0["graph_launches"]}
         b = e.bodies()
         finals[name] = {k: b[k] for k in ("pos", "vel", "Q", "vrot")}

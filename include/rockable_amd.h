@@ -58,6 +58,7 @@ typedef struct {
   int64_t guard_wasted;    /* iterations enqueued after a rebuild fell due (done nothing) */
   int64_t graph_launches;  /* runs of graph_steps steps launched as one captured graph */
   int64_t graph_captures;  /* graphs captured (one per device view, cached) */
+  int64_t guard_polls;     /* asynchronous polls of the guard (with a rebuild network) */
 } rk_counts;
 
 /* Statistics of the packing, reduced on the device in a fixed order
@@ -122,15 +123,21 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * "graph_steps" (n > 1: runs of n steps with no host action between them go
  * out as one HIP graph, captured once per device view and replayed; 0 or 1,
  * the default: every kernel launched on its own),
- * launch shapes: "law_group" (geometry blocks per force-law wavefront: 1, 2,
- * 4, 8; default 1), "geom_block" (contact-kinematics workgroup: 64, 128,
- * 256, 512; default 256), "gather_waves" (wavefronts per resultant
- * workgroup: 1, 2, 4; default 4), "body_block" (per-body kernels: 64, 128,
- * 256; default 256).  No option changes results. */
+ * "model_scale" (the rebuild-interval network's proposals are multiplied by
+ * it; default 0.75),
+ * launch shapes: "fuse_law" (1, the default: contact kinematics and force law
+ * in one kernel; 0: two), "law_group" (geometry blocks per force-law
+ * wavefront when not fused: 1, 2, 4, 8; default 1), "geom_block"
+ * (contact-kinematics workgroup when not fused: 64, 128, 256, 512; default
+ * 256), "gather_lanes" (lanes per body in the resultant sums: 8, 16; default
+ * 16), "gather_waves" (wavefronts per resultant workgroup: 1, 2, 4; default
+ * 4), "body_block" (per-body kernels: 64, 128, 256; default 256).  No option
+ * changes results. */
 int rk_set_option(rk_engine* eng, const char* name, double value);
 int rk_get_option(rk_engine* eng, const char* name, double* value);
-/* Autotuner (no counterpart in the reference) of law_group, geom_block,
- * gather_waves, body_block and graph_steps, one after the other: the
+/* Autotuner (no counterpart in the reference) of fuse_law, law_group,
+ * geom_block, gather_lanes, gather_waves, body_block and graph_steps, one
+ * after the other: the
  * simulation runs on, trial after trial of steps_per_trial steps, each
  * candidate value twice, and the fastest (wall time per step, rebuilds
  * excluded) is kept.  Every candidate computes the same state, so the steps

@@ -30,6 +30,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "rk_math.hpp"
@@ -187,8 +188,10 @@ struct Dev {
   int law_group;          // geometry blocks per k_law wavefront (1..8)
   int geom_block;         // k_geom workgroup size (64, 128, 256, 512)
   int gather_waves;       // k_gather wavefronts per workgroup (1, 2, 4)
+  int gather_lanes;       // k_gather lanes per body (8, 16)
   int body_block;         // workgroup size of the per-body kernels (64, 128, 256)
   int fuse_law;           // 1: geometry and law in one kernel (k_interact)
+  int persist;            // step kernels' grid cap (0: one tile per workgroup)
   // device displacement guard (dynamicUpdateNL)
   int guard_on;
   DevGuard* guard;
@@ -223,6 +226,42 @@ struct StepParams {
 
 RK_HD uint32_t pack_sub(int type, int isub, int jsub) {
   return ((uint32_t)type << 30) | ((uint32_t)isub << 15) | (uint32_t)jsub;
+}
+
+// Workgroups are handed to the 8 XCDs round-robin in dispatch order (XCD =
+// block mod 8), and each XCD has its own L2.  xcd_block maps the blocks one
+// XCD runs onto one contiguous range of [0, n) (a bijection), so that the
+// bodies a block's work reads are mostly those its XCD's L2 already holds
+// (entries and candidate pairs are sorted by body, and neighbouring indices
+// are near in space).
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int x = b & 7, k = b >> 3;
+  const int q = n >> 3, r = n & 7;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Persistent launches: a grid of G workgroups covers T tiles.  The workgroups
+// on XCD x (block mod 8) take XCD x's contiguous share of [0, T) -- the tile
+// xcd_block gives them when G == T -- each every (workgroups on x)-th tile of
+// it, so the locality is the same whatever G.  G is T, or a multiple of 8
+// below it (persistent_grid).
+struct TileLoop {
+  int t, end, step;
+};
+__device__ __forceinline__ TileLoop xcd_tiles(int T) {
+  const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b & 7, k = b >> 3;
+  const int q = T >> 3, r = T & 7;
+  const int start = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  TileLoop L;
+  L.t = start + k;
+  L.end = start + q + (x < r ? 1 : 0);
+  L.step = (G - x + 7) >> 3;
+  return L;
+}
+inline int persistent_grid(int T, int persist) {
+  if (persist <= 0 || T <= persist) return T;
+  return std::max(8, persist & ~7);
 }
 
 // rk_step.hip

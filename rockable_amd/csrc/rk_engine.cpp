@@ -40,7 +40,8 @@ struct rk_engine {
   int profile = 0;
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int law_group = 1, geom_block = 256, gather_waves = 4, body_block = 256, fuse_law = 1;
+  int law_group = 1, geom_block = 256, gather_waves = 4, body_block = 256, fuse_law = 1, gather_lanes = 16;
+  int persist = 0;
   // host wall time of the rebuilds while the autotuner runs (it synchronises
   // after each rebuild then)
   double* stat_part = nullptr;
@@ -91,7 +92,16 @@ struct rk_engine {
   double* feat_part = nullptr;
   size_t feat_part_cap = 0;
   int64_t pred_batch = 16;
-  int64_t guard_reads = 0, guard_wasted = 0;
+  // proposals are scaled by this: a step enqueued past a rebuild (frozen)
+  // costs more than an extra read of the check on large packings
+  double model_scale = 0.75;
+  // calibration of the network on the running packing: shrinks after a batch
+  // that ran past a rebuild, recovers slowly after batches that did not
+  double model_trust = 1.0;
+  int64_t guard_reads = 0, guard_wasted = 0, guard_polls = 0;
+  // asynchronous polls of the device check: pinned flags and their events
+  int* poll_host = nullptr;
+  hipEvent_t poll_ev[8] = {};
   int64_t last_rebuild_step = 0, last_interval = 0;
   int exact_sums = 0;
   // steps enqueued but not yet launched (deferred so that runs of graph_steps
@@ -104,6 +114,8 @@ struct rk_engine {
     std::vector<unsigned char> key;
     hipGraphExec_t exec = nullptr;
     int64_t used = 0;
+    int n = 0;
+    bool with_guard = false;
   };
   std::vector<StepGraph> graphs;
   int64_t graph_captures = 0, graph_launches = 0, graph_clock = 0;
@@ -118,6 +130,9 @@ struct rk_engine {
 };
 
 namespace {
+
+constexpr int kPollSlots = 8;  // ring of pinned flags and events
+constexpr int kPollAhead = 2;  // polls in flight before the host waits
 
 int fail(rk_engine* e, const std::string& msg) {
   e->err = msg;
@@ -292,6 +307,8 @@ void assemble_dev(rk_engine* eng) {
   d.gather_waves = eng->gather_waves;
   d.body_block = eng->body_block;
   d.fuse_law = eng->fuse_law;
+  d.gather_lanes = eng->gather_lanes;
+  d.persist = eng->persist;
   d.istart = E.istart;
   d.jstart = E.jstart;
   d.jlist = E.jlist;
@@ -801,11 +818,32 @@ int step_graph(rk_engine* eng, int n, bool with_guard, hipGraphExec_t* out) {
     if (graph) (void)hipGraphDestroy(graph);
     return fail(eng, rc != 0 ? eng->err : std::string("step capture: ") + hipGetErrorString(ec));
   }
+  eng->graph_captures++;
+  // after a rebuild only kernel arguments and grid sizes change: update the
+  // least recently used graph of the same length in place, which is much
+  // cheaper than instantiating a new one
+  int lru = -1;
+  for (size_t q = 0; q < eng->graphs.size(); q++)
+    if (eng->graphs[q].n == n && eng->graphs[q].with_guard == with_guard &&
+        (lru < 0 || eng->graphs[q].used < eng->graphs[lru].used))
+      lru = (int)q;
+  if (lru >= 0 && eng->graphs.size() >= 2) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(eng->graphs[lru].exec, graph, &bad, &res) == hipSuccess &&
+        res == hipGraphExecUpdateSuccess) {
+      (void)hipGraphDestroy(graph);
+      eng->graphs[lru].key = std::move(key);
+      eng->graphs[lru].used = eng->graph_clock;
+      *out = eng->graphs[lru].exec;
+      return 0;
+    }
+    (void)hipGetLastError();
+  }
   hipGraphExec_t exec = nullptr;
   const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ei != hipSuccess) return fail(eng, std::string("step graph: ") + hipGetErrorString(ei));
-  eng->graph_captures++;
   if (eng->graphs.size() >= 4) {
     size_t old = 0;
     for (size_t q = 1; q < eng->graphs.size(); q++)
@@ -817,6 +855,8 @@ int step_graph(rk_engine* eng, int n, bool with_guard, hipGraphExec_t* out) {
   g.key = std::move(key);
   g.exec = exec;
   g.used = eng->graph_clock;
+  g.n = n;
+  g.with_guard = with_guard;
   eng->graphs.push_back(std::move(g));
   *out = exec;
   return 0;
@@ -1043,7 +1083,7 @@ int propose_batch(rk_engine* eng) {
   float y = 0.0f;
   HIPCHK(hipMemcpyAsync(&y, eng->pred, sizeof y, hipMemcpyDeviceToHost, eng->stream));
   HIPCHK(hipStreamSynchronize(eng->stream));
-  double b = std::expm1((double)y);
+  double b = eng->model_scale * eng->model_trust * std::expm1((double)y);
   if (!(b == b)) b = 1.0;
   eng->pred_batch = (int64_t)std::max(1.0, std::min(b, 65536.0));
   return 0;
@@ -1052,11 +1092,12 @@ int propose_batch(rk_engine* eng) {
 // The integrate() loop with dynamicUpdateNL, its displacement check on the
 // device.  Iterations are enqueued in batches without host synchronisation;
 // a batch ends where the host has work (a conf file, a scheduled rebuild,
-// bond breakage accounting) or after guard_batch iterations.  When the device
-// check finds a rebuild due after iteration s, the step kernels of the later
-// iterations do nothing: the host reads the guard once per batch, keeps the
-// loop scalars of iteration s, rebuilds and continues.  The schedule of
-// rebuilds is exactly the reference's.
+// bond breakage accounting), after guard_batch iterations (no network), or
+// once an asynchronous poll has seen the check fire (with a network).  When
+// the device check finds a rebuild due after iteration s, the step kernels of
+// the later iterations do nothing: the host reads the guard at the end of the
+// batch, keeps the loop scalars of iteration s, rebuilds and continues.  The
+// schedule of rebuilds is exactly the reference's.
 int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tmax, int64_t* ndone) {
   Model& m = eng->m;
   int64_t done = 0;
@@ -1064,12 +1105,26 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
   std::vector<char> conf_at;
   while (done < nmax && (!until_tmax || m.t <= m.tmax)) {
     const bool bonds = m.law == LAW_STICKED && eng->live_interfaces > 0;
-    const int64_t want = eng->model ? eng->pred_batch : eng->guard_batch;
+    // With the network the batch runs on until a host action or a rebuild is
+    // seen, the check polled asynchronously after each chunk of steps (a
+    // copy of the frozen flag to pinned memory and an event; at most
+    // kPollAhead polls in flight, so the host never waits on the GPU's
+    // current chunk).  Chunks follow the network's estimate of the steps to
+    // the next rebuild: long far from it, short near it.
+    const bool pipe = eng->model != nullptr && !bonds && eng->poll_host != nullptr;
+    const int64_t want = pipe ? nmax - done : eng->model ? eng->pred_batch : eng->guard_batch;
     const int64_t B = std::max<int64_t>(1, std::min<int64_t>(bonds ? 1 : want, nmax - done));
     hist.assign(1, loop_scalars(eng));
     conf_at.assign(1, 0);
     launch_guard_set_step(eng->d, (long long)eng->steps, eng->stream);
     int64_t k = 0;
+    int head = 0, inflight = 0;
+    bool seen = false;
+    auto chunk_for = [&](int64_t taken) -> int64_t {
+      const int64_t r = eng->pred_batch - taken;
+      return std::max<int64_t>(2, std::min<int64_t>(32, r / 2));
+    };
+    int64_t chunk_left = pipe ? chunk_for(0) : B;
     while (k < B) {
       m.t += m.dt;
       m.interConfC += m.dt;
@@ -1087,6 +1142,26 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       if ((conf_due && outdir) || eng->needUpdate || m.interVerletC >= m.interVerlet - m.dt_2 ||
           (until_tmax && m.t > m.tmax))
         break;
+      if (pipe && --chunk_left == 0) {
+        if (flush_steps(eng) != 0) return -1;
+        const int slot = (head + inflight) % kPollSlots;
+        HIPCHK(hipMemcpyAsync(eng->poll_host + slot, &eng->guard->frozen, sizeof(int), hipMemcpyDeviceToHost,
+                              eng->stream));
+        HIPCHK(hipEventRecord(eng->poll_ev[slot], eng->stream));
+        inflight++;
+        while (inflight > 0) {
+          const hipError_t q =
+              inflight > kPollAhead ? hipEventSynchronize(eng->poll_ev[head]) : hipEventQuery(eng->poll_ev[head]);
+          if (q == hipErrorNotReady) break;
+          HIPCHK(q);
+          if (eng->poll_host[head]) seen = true;
+          head = (head + 1) % kPollSlots;
+          inflight--;
+        }
+        eng->guard_polls++;
+        if (seen) break;
+        chunk_left = chunk_for(k);
+      }
     }
     if (flush_steps(eng) != 0) return -1;
     DevGuard g;
@@ -1100,6 +1175,7 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       if (executed < 1 || executed > k) return fail(eng, "device guard: inconsistent frozen step");
       set_loop_scalars(eng, hist[executed]);
       eng->guard_wasted += k - executed;
+      if (eng->model && k - executed > 1) eng->model_trust = std::max(0.05, eng->model_trust * 0.7);
       need = true;
       const int zero = 0;
       HIPCHK(hipMemcpy(&eng->guard->frozen, &zero, sizeof zero, hipMemcpyHostToDevice));
@@ -1115,6 +1191,7 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       eng->needUpdate = false;
     }
     done += executed;
+    if (eng->model && !g.frozen) eng->model_trust = std::min(1.0, eng->model_trust * 1.1);
     if (eng->model && propose_batch(eng) != 0) return -1;
   }
   if (ndone) *ndone = done;
@@ -1138,6 +1215,18 @@ int rk_engine_create(rk_engine** out, int device) {
     delete eng;
     return -2;
   }
+  // pinned flags and events of the asynchronous guard polls (without them
+  // the guarded loop reads the check synchronously)
+  if (hipHostMalloc((void**)&eng->poll_host, sizeof(int) * kPollSlots, hipHostMallocDefault) == hipSuccess) {
+    for (int k = 0; k < kPollSlots; k++)
+      if (hipEventCreateWithFlags(&eng->poll_ev[k], hipEventDisableTiming) != hipSuccess) {
+        (void)hipHostFree(eng->poll_host);
+        eng->poll_host = nullptr;
+        break;
+      }
+  } else {
+    eng->poll_host = nullptr;
+  }
   *out = eng;
   return 0;
 }
@@ -1150,6 +1239,9 @@ void rk_engine_destroy(rk_engine* eng) {
   for (hipEvent_t e : eng->ev_pool) (void)hipEventDestroy(e);
   if (eng->ev0) (void)hipEventDestroy(eng->ev0);
   if (eng->ev1) (void)hipEventDestroy(eng->ev1);
+  for (hipEvent_t e : eng->poll_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (eng->poll_host) (void)hipHostFree(eng->poll_host);
   if (eng->stream) (void)hipStreamDestroy(eng->stream);
   delete eng;
 }
@@ -1319,6 +1411,18 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     // the conf keyword "gravity" (Rockable.cpp:978-980), changed between steps
     if (check_loaded(eng)) return -1;
     (n == "gravity_x" ? eng->m.gravity.x : n == "gravity_y" ? eng->m.gravity.y : eng->m.gravity.z) = value;
+  } else if (n == "model_scale") {
+    if (!(value > 0.0 && value <= 4.0)) return fail(eng, "model_scale must be in (0, 4]");
+    eng->model_scale = value;
+  } else if (n == "persistent") {
+    if (!(value == 0.0 || (value >= 8.0 && value <= 1048576.0 && value == (double)(int)value)))
+      return fail(eng, "persistent must be 0 or an integer in [8, 1048576]");
+    eng->persist = (int)value;
+    if (eng->loaded) assemble_dev(eng);
+  } else if (n == "gather_lanes") {
+    if (!(value == 8.0 || value == 16.0)) return fail(eng, "gather_lanes must be 8 or 16");
+    eng->gather_lanes = (int)value;
+    if (eng->loaded) assemble_dev(eng);
   } else if (n == "fuse_law") {
     if (!(value == 0.0 || value == 1.0)) return fail(eng, "fuse_law must be 0 or 1");
     eng->fuse_law = (int)value;
@@ -1397,6 +1501,8 @@ int* tuned_option(rk_engine* eng, const std::string& n) {
   if (n == "body_block") return &eng->body_block;
   if (n == "graph_steps") return &eng->graph_steps;
   if (n == "fuse_law") return &eng->fuse_law;
+  if (n == "gather_lanes") return &eng->gather_lanes;
+  if (n == "persistent") return &eng->persist;
   return nullptr;
 }
 }  // namespace
@@ -1413,8 +1519,10 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
   const Knob knobs[] = {{"fuse_law", {0, 1}},
                         {"law_group", {1, 2, 4, 8}},
                         {"geom_block", {64, 128, 256, 512}},
+                        {"gather_lanes", {8, 16}},
                         {"gather_waves", {1, 2, 4}},
                         {"body_block", {64, 128, 256}},
+                        {"persistent", {0, 2048, 8192}},
                         {"graph_steps", {0, 8, 32}}};
   int64_t done = 0;
   eng->tuning = true;
@@ -1478,6 +1586,8 @@ int rk_get_option(rk_engine* eng, const char* name, double* value) {
     *value = eng->w.heavy_min;
   else if (n == "guard_batch")
     *value = eng->guard_batch;
+  else if (n == "model_scale")
+    *value = eng->model_scale;
   else if (n == "gravity_x")
     *value = eng->m.gravity.x;
   else if (n == "gravity_y")
@@ -1581,6 +1691,7 @@ int rk_get_counts(rk_engine* eng, rk_counts* out) {
   out->guard_wasted = eng->guard_wasted;
   out->graph_launches = eng->graph_launches;
   out->graph_captures = eng->graph_captures;
+  out->guard_polls = eng->guard_polls;
   return 0;
 }
 
@@ -1711,6 +1822,7 @@ int rk_set_rebuild_model(rk_engine* eng, rk_mlp* mlp) {
   if (!eng) return -1;
   eng->model = mlp;
   eng->pred_batch = eng->guard_batch;
+  eng->model_trust = 1.0;
   return 0;
 }
 

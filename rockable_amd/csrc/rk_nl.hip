@@ -664,7 +664,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
                                                const uint64_t* cand, uint32_t* nearbuf, int* near_cnt,
                                                int near_rcap, int32_t* cbase, int32_t* cnf, int32_t* cnb) {
   extern __shared__ double lds[];
-  const int c = blockIdx.x;
+  const int c = xcd_block((int)blockIdx.x, (int)gridDim.x);
   if (c >= nc) return;
   const int lane = threadIdx.x;
   const uint64_t key = cand[c];
@@ -1037,6 +1037,24 @@ __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
 }
 
 // --- CSR -----------------------------------------------------------------
+// Counts of a sorted key array without atomics (a wall owning thousands of
+// entries would serialise them): the last entry of each run writes the run's
+// length, its first index found by binary search.
+__global__ void k_count_sorted(int n, const int32_t* key, int32_t* cnt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int k = key[e];
+  if (e + 1 < n && key[e + 1] == k) return;
+  int lo = 0, hi = e;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (key[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  cnt[k] = e - lo + 1;
+}
 __global__ void k_count_by(int n, const int32_t* key, int32_t* cnt) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
@@ -1183,7 +1201,7 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipS
     return -1;
   // istart: counts by owner, exclusive scan over nb + 1 (trailing zero)
   RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));
-  if (n > 0) hipLaunchKernelGGL(k_count_by, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ei, w.i32a);
+  if (n > 0) hipLaunchKernelGGL(k_count_sorted, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ei, w.i32a);
   if (scan_exclusive(w, w.i32a, E.istart, nb + 1, s)) return -1;
   // jstart
   RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));
@@ -1193,14 +1211,10 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipS
   if (n > 0) {
     hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(kBlock), 0, s, n, w.i32b);
     RK_CHECK(hipMemcpyAsync(w.i32c, E.ej, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
-    int32_t* kout = nullptr;
-    size_t kcap = 0;
-    if (ensure(w, kout, kcap, (size_t)n)) return -1;
-    int rc = sort_pairs(w, (const uint32_t*)w.i32c, (uint32_t*)kout, w.i32b, E.jlist, n,
-                        bits_for((uint64_t)(nb > 1 ? nb - 1 : 1)), s);
-    RK_CHECK(hipStreamSynchronize(s));
-    (void)hipFree(kout);
-    if (rc) return -1;
+    if (ensure(w, w.kout, w.kout_cap, (size_t)n)) return -1;
+    if (sort_pairs(w, (const uint32_t*)w.i32c, (uint32_t*)w.kout, w.i32b, E.jlist, n,
+                   bits_for((uint64_t)(nb > 1 ? nb - 1 : 1)), s))
+      return -1;
   }
   // resultant slots
   if (ensure(w, E.soff, E.soff_cap, (size_t)nb + 1)) return -1;
@@ -1552,7 +1566,7 @@ void nl_work_free(NLWork& w) {
   void* ps[] = {w.cstripe, w.scnt, w.soff, w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
                 w.gk,    w.gk2,   w.gv,      w.gv2,   w.gflag, w.gpos,  w.ukey,  w.ucand,   w.uold,  w.gcount,
                 w.goff,  w.tmp,   w.i32a,    w.i32b,  w.i32c,  w.f64a,  w.cell_of, w.ccount, w.cstart, w.cfill,
-                w.cbody, w.over, w.gedge, w.gface};
+                w.cbody, w.over, w.gedge, w.gface, w.kout};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   NLWork z;

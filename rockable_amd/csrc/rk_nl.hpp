@@ -79,6 +79,7 @@ struct NLWork {
   RK_BUF(double, baabb)
   RK_BUF(double, gedge)
   RK_BUF(double, gface)
+  RK_BUF(int32_t, kout)
   RK_BUF(int, counter)
   RK_BUF(uint64_t, cand)
   RK_BUF(uint64_t, cstripe)

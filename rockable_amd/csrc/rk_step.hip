@@ -52,17 +52,6 @@ __device__ __forceinline__ void st3(double* p, v3 v) {
 
 // A rebuild found due by the device guard suspends the step kernels until
 // the host has rebuilt the list (every thread reads the same flag).
-// Workgroups are handed to the 8 XCDs round-robin in dispatch order (XCD =
-// block mod 8), and each XCD has its own L2.  xcd_block maps the blocks one
-// XCD runs onto one contiguous range of [0, n) (a bijection), so that the
-// bodies a block's entries read are mostly those its XCD's L2 already holds
-// (entries are sorted by body, and neighbouring indices are near in space).
-__device__ __forceinline__ int xcd_block(int b, int n) {
-  const int x = b & 7, k = b >> 3;
-  const int q = n >> 3, r = n & 7;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
 __device__ __forceinline__ bool frozen(const Dev& d) {
   return d.guard_on && *(volatile const int*)&d.guard->frozen;
 }
@@ -495,13 +484,16 @@ __global__ void __launch_bounds__(BS) k_geom(Dev d) {
   __shared__ int wtype[BS / 64][5];
   __shared__ short perm[BS];
   if (frozen(d)) return;
-  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  ActiveKin a;
-  const bool act = geom_pass<GEOM_ALL, BS>(d, blk, wtype, perm, a);
-  int total = 0;
-  const int r = active_rank<BS>(act, wcount, &total);
-  if (act) d.akin[(size_t)blk * BS + r] = a;
-  if (threadIdx.x == 0) d.acount[blk] = total;
+  for (TileLoop L = xcd_tiles((d.ne + BS - 1) / BS); L.t < L.end; L.t += L.step) {
+    const int blk = L.t;
+    ActiveKin a;
+    const bool act = geom_pass<GEOM_ALL, BS>(d, blk, wtype, perm, a);
+    int total = 0;
+    const int r = active_rank<BS>(act, wcount, &total);
+    if (act) d.akin[(size_t)blk * BS + r] = a;
+    if (threadIdx.x == 0) d.acount[blk] = total;
+    __syncthreads();
+  }
 }
 
 // The force law of one active entry (kinematics in a): history, both
@@ -599,9 +591,10 @@ constexpr int kLawGroup = 8;
 template <int LAW>
 __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_geom_all, int pii, int ngeom, int W) {
   if (frozen(d)) return;
-  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int g0 = (int)((lb * (unsigned)blockDim.x + threadIdx.x) >> 6) * W;  // first geometry block
-  if (g0 >= ngeom) return;
+  const int waves = (ngeom + W - 1) / W;
+  for (TileLoop L = xcd_tiles((waves + kBlock / 64 - 1) / (kBlock / 64)); L.t < L.end; L.t += L.step) {
+  const int g0 = (int)((L.t * (unsigned)blockDim.x + threadIdx.x) >> 6) * W;  // first geometry block
+  if (g0 >= ngeom) continue;
   const int lane = threadIdx.x & 63;
   // inclusive prefix sums of the active counts of the W blocks (lanes < W)
   int incl = (lane < W && g0 + lane < ngeom) ? d.acount[g0 + lane] : 0;
@@ -624,6 +617,7 @@ __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_
     const ActiveKin a = d.akin[(size_t)(g0 + gb) * d.geom_block + (t - before)];
     law_item<LAW>(d, sp, write_geom_all, pii, a);
   }
+  }
 }
 
 // Geometry and force law fused: the block's active entries go to LDS instead
@@ -637,14 +631,16 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
   __shared__ short perm[kBlock];
   __shared__ ActiveKin sa[kBlock];
   if (frozen(d)) return;
-  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  ActiveKin a;
-  const bool act = geom_pass<GEOM_ALL, kBlock>(d, blk, wtype, perm, a);
-  int total = 0;
-  const int r = active_rank<kBlock>(act, wcount, &total);
-  if (act) sa[r] = a;
-  __syncthreads();
-  if ((int)threadIdx.x < total) law_item<LAW>(d, sp, write_geom_all, pii, sa[threadIdx.x]);
+  for (TileLoop L = xcd_tiles((d.ne + kBlock - 1) / kBlock); L.t < L.end; L.t += L.step) {
+    ActiveKin a;
+    const bool act = geom_pass<GEOM_ALL, kBlock>(d, L.t, wtype, perm, a);
+    int total = 0;
+    const int r = active_rank<kBlock>(act, wcount, &total);
+    if (act) sa[r] = a;
+    __syncthreads();
+    if ((int)threadIdx.x < total) law_item<LAW>(d, sp, write_geom_all, pii, sa[threadIdx.x]);
+    __syncthreads();
+  }
 }
 
 __device__ __forceinline__ v3 euler_domega(v3 I, v3 w, v3 M) {
@@ -807,19 +803,17 @@ __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, 
   out->wz = P.wz;
 }
 
-// compute_resultants (Rockable.cpp:3267-3272).  Sixteen lanes per body,
-// four bodies per wavefront.  The lanes read the slot activity bytes 64 at a
-// time (four each) and join them into one 64-bit mask; lanes 0-5, one per
+// compute_resultants (Rockable.cpp:3267-3272).  L lanes per body (8 or 16;
+// 64 / L bodies per wavefront).  The lanes read the slot activity bytes 4 L
+// at a time (four each) and join them into one mask; lanes 0-5, one per
 // component of (F, M), then fetch the terms of the active slots eight at a
 // time (independent loads) and add them in slot order.  Called by every
 // lane of the wavefront; returns the updated sum in lanes gl < 6.
-constexpr int kLanesPerBody = 16;
-constexpr int kBodiesPerWave = 4;
-
+template <int L>  // lanes per body: 8 or 16 (windows of 4 L slots)
 __device__ __forceinline__ double sum_slots(const Dev& d, bool run, int k0, int k1, int gl, double s) {
   const double* val = d.sval + (gl < 6 ? gl : 0);
   for (int it = 0;; it++) {
-    const int w = k0 + 64 * it;
+    const int w = k0 + 4 * L * it;
     const bool live = run && w < k1;
     if (!__any(live)) break;
     unsigned m4 = 0u;
@@ -831,7 +825,7 @@ __device__ __forceinline__ double sum_slots(const Dev& d, bool run, int k0, int 
       }
     }
     unsigned long long rest = (unsigned long long)m4 << (4 * gl);
-    for (int o = 1; o < kLanesPerBody; o <<= 1) rest |= __shfl_xor(rest, o, kLanesPerBody);
+    for (int o = 1; o < L; o <<= 1) rest |= __shfl_xor(rest, o, L);
     if (gl < 6) {
       while (rest) {
         double x[8];
@@ -859,23 +853,27 @@ __device__ __forceinline__ double sum_slots(const Dev& d, bool run, int k0, int 
 // the reference's bit for bit.  The wavefronts after the bodies' take the
 // 64-slot chunks of the heavy bodies and store each chunk's partial sums
 // (from -0.0, the identity of addition) for k_heavy_finish.
+template <int L>
 __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
+  constexpr int kPer = 64 / L;  // bodies (or chunks) per wavefront
   if (frozen(d)) return;
   const int lane = threadIdx.x & 63;
-  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int wave = (int)((lb * (unsigned)blockDim.x + threadIdx.x) >> 6);
-  const int gl = lane & (kLanesPerBody - 1), gbase = lane - gl;
-  const int body_waves = (d.nb + kBodiesPerWave - 1) / kBodiesPerWave;
+  const int gl = lane & (L - 1), gbase = lane - gl;
+  const int wpb = (int)(blockDim.x >> 6);
+  const int body_waves = (d.nb + kPer - 1) / kPer;
+  const int waves = body_waves + (d.nhch + kPer - 1) / kPer;
+  for (TileLoop T = xcd_tiles((waves + wpb - 1) / wpb); T.t < T.end; T.t += T.step) {
+  const int wave = (int)((T.t * (unsigned)blockDim.x + threadIdx.x) >> 6);
   if (wave >= body_waves) {
-    const int c = (wave - body_waves) * kBodiesPerWave + lane / kLanesPerBody;
+    const int c = (wave - body_waves) * kPer + lane / L;
     const bool run = c < d.nhch;
     int2 r = make_int2(0, 0);
     if (run) r = d.hch[c];
-    const double s = sum_slots(d, run, r.x, r.y, gl, -0.0);
+    const double s = sum_slots<L>(d, run, r.x, r.y, gl, -0.0);
     if (run && gl < 6) d.hpart[6 * (size_t)c + gl] = s;
-    return;
+    continue;
   }
-  const int b = wave * kBodiesPerWave + lane / kLanesPerBody;
+  const int b = wave * kPer + lane / L;
   int k0 = 0, k1 = 0, c0 = 0, c1 = 0;
   bool run = false, driven = false;
   if (b < d.nb) {
@@ -890,10 +888,11 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
     driven = b < d.nDriven;
     init_resultants(d, b, P, driven, F, M, c0, c1);
   }
-  const double s = sum_slots(d, run, k0, k1, gl, pick(F, M, gl < 6 ? gl : 0));
+  const double s = sum_slots<L>(d, run, k0, k1, gl, pick(F, M, gl < 6 ? gl : 0));
   const double fx = __shfl(s, gbase), fy = __shfl(s, gbase + 1), fz = __shfl(s, gbase + 2);
   const double mx = __shfl(s, gbase + 3), my = __shfl(s, gbase + 4), mz = __shfl(s, gbase + 5);
   if (run && gl == 0) finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+  }
 }
 
 // A passive heavy body (its force and moment feed nothing back into the
@@ -904,9 +903,10 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
 // from the serial sum only by rounding.
 __global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int kick) {
   if (frozen(d)) return;
-  const int h = blockIdx.x;
+  for (TileLoop T = xcd_tiles(d.nheavy); T.t < T.end; T.t += T.step) {
+  const int h = T.t;
   const int b = d.heavy[h];
-  if (!d.passive[b] || d.exact_sums) return;
+  if (!d.passive[b] || d.exact_sums) continue;
   const int lane = threadIdx.x;
   double a[6] = {-0.0, -0.0, -0.0, -0.0, -0.0, -0.0};
   for (int c = d.hch_off[h] + lane; c < d.hch_off[h + 1]; c += 64) {
@@ -918,13 +918,14 @@ __global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int k
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
     for (int q = 0; q < 6; q++) a[q] += __shfl_xor(a[q], off, 64);
-  if (lane != 0) return;
+  if (lane != 0) continue;
   const BodyKin P = d.kin[b];
   const bool driven = b < d.nDriven;
   v3 F, M;
   int c0, c1;
   init_resultants(d, b, P, driven, F, M, c0, c1);
   finish_body(d, sp, kick, b, P, driven, vadd(F, V(a[0], a[1], a[2])), vadd(M, V(a[3], a[4], a[5])), c0, c1);
+  }
 }
 
 // compute_resultants for a heavy body that is not passive (or any heavy
@@ -994,16 +995,18 @@ __device__ __forceinline__ unsigned long long wkey(double w) {
 __global__ void __launch_bounds__(kBlock) k_guard_acc(Dev d, StepParams sp) {
   __shared__ unsigned long long rp[kBlock / 64], rw[kBlock / 64];
   if (frozen(d)) return;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
   const bool r = *(volatile const int*)&d.guard->reset != 0;
   unsigned long long kp = 0ull, kw = ~0ull;
-  if (b < d.nb) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < d.nb; b += gridDim.x * blockDim.x) {
     const BodyKin& P = d.kin[b];
     v3 dp = r ? V(0.0, 0.0, 0.0) : ld3(d.gdpos + 3 * (size_t)b);
     dp = vadd(dp, vsc(sp.dt, kvel(P)));
     st3(d.gdpos + 3 * (size_t)b, dp);
     const double tmp1 = vnorm2(dp);
-    if (tmp1 == tmp1) kp = (unsigned long long)__double_as_longlong(tmp1);
+    if (tmp1 == tmp1) {
+      const unsigned long long k1 = (unsigned long long)__double_as_longlong(tmp1);
+      kp = k1 > kp ? k1 : kp;
+    }
     const qt dq = qdot(kq(P), kvrot(P));
     double* g = d.gdq + 4 * (size_t)b;
     qt q;
@@ -1027,7 +1030,10 @@ __global__ void __launch_bounds__(kBlock) k_guard_acc(Dev d, StepParams sp) {
     double w = q.w;
     if (w > 1.0) w = 1.0;
     if (w < -1.0) w = -1.0;
-    if (w == w) kw = wkey(w);
+    if (w == w) {
+      const unsigned long long k2 = wkey(w);
+      kw = k2 < kw ? k2 : kw;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long op = __shfl_xor(kp, off, 64), ow = __shfl_xor(kw, off, 64);
@@ -1164,17 +1170,17 @@ __global__ void __launch_bounds__(64) k_guard_feat_final(Dev d, StepParams sp, c
 // once per step: one thread per vertex.
 __global__ void __launch_bounds__(kBlock) k_glob(Dev d) {
   if (frozen(d)) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.nvtot) return;
-  const int b = d.bvbody[k];
-  const BodyKin& P = d.kin[b];
-  st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, k - d.bvoff[b]));
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.nvtot; k += gridDim.x * blockDim.x) {
+    const int b = d.bvbody[k];
+    const BodyKin& P = d.kin[b];
+    st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, k - d.bvoff[b]));
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
   if (frozen(d)) return;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b == 0 && d.nctrl > 0) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid == 0 && d.nctrl > 0) {
     // ServoFunction (DrivingSystem.cpp:242-268), then the controlled bodies in
     // list order (Rockable.cpp:1856-1900, 1988-2033), all by one thread.
     if (d.servo != 0) {
@@ -1286,33 +1292,35 @@ __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
       }
     }
   }
-  if (b < d.nDriven || b >= d.nb) return;
-  // free bodies (Rockable.cpp:2036-2055)
-  BodyKin P = d.kin[b];
-  const v3 a = ld3(d.acc + 3 * (size_t)b);
-  const v3 ar = ld3(d.arot + 3 * (size_t)b);
-  v3 pos = vadd(kpos(P), vadd(vsc(sp.dt, kvel(P)), vsc(sp.dt2_2, a)));
-  v3 vel = vadd(kvel(P), vsc(sp.dt_2, a));
-  qt Q = kq(P);
-  const v3 w = kvrot(P);
-  Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
-  Q = qadd_scaled(Q, qddot(Q, w, ar), sp.dt2_2);
-  qnormalize(&Q);
-  v3 w2 = vadd(w, vsc(sp.dt_2, ar));
-  BodyKin* out = d.kin + b;
-  out->px = pos.x;
-  out->py = pos.y;
-  out->pz = pos.z;
-  out->qw = Q.w;
-  out->qx = Q.x;
-  out->qy = Q.y;
-  out->qz = Q.z;
-  out->vx = vel.x;
-  out->vy = vel.y;
-  out->vz = vel.z;
-  out->wx = w2.x;
-  out->wy = w2.y;
-  out->wz = w2.z;
+  // free bodies (Rockable.cpp:2036-2055), a grid-stride loop
+  for (int b = gid; b < d.nb; b += gridDim.x * blockDim.x) {
+    if (b < d.nDriven) continue;
+    BodyKin P = d.kin[b];
+    const v3 a = ld3(d.acc + 3 * (size_t)b);
+    const v3 ar = ld3(d.arot + 3 * (size_t)b);
+    v3 pos = vadd(kpos(P), vadd(vsc(sp.dt, kvel(P)), vsc(sp.dt2_2, a)));
+    v3 vel = vadd(kvel(P), vsc(sp.dt_2, a));
+    qt Q = kq(P);
+    const v3 w = kvrot(P);
+    Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+    Q = qadd_scaled(Q, qddot(Q, w, ar), sp.dt2_2);
+    qnormalize(&Q);
+    v3 w2 = vadd(w, vsc(sp.dt_2, ar));
+    BodyKin* out = d.kin + b;
+    out->px = pos.x;
+    out->py = pos.y;
+    out->pz = pos.z;
+    out->qw = Q.w;
+    out->qx = Q.x;
+    out->qy = Q.y;
+    out->qz = Q.z;
+    out->vx = vel.x;
+    out->vy = vel.y;
+    out->vz = vel.z;
+    out->wx = w2.x;
+    out->wy = w2.y;
+    out->wz = w2.z;
+  }
 }
 
 // locate entry (owner, other, sub) in the owner's sorted run
@@ -1367,20 +1375,23 @@ __global__ void k_breakage(Dev d) {
 
 int body_block(const Dev& d) { return d.body_block == 64 || d.body_block == 128 ? d.body_block : 256; }
 
+// workgroups for T tiles (persistent kernels loop over their tiles)
+dim3 grid_for(const Dev& d, long T) { return dim3(persistent_grid((int)std::max(1L, T), d.persist)); }
+
 void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s) {
   const int n = d.nb > 0 ? d.nb : 1, bs = body_block(d);
-  hipLaunchKernelGGL(k_predict, dim3((n + bs - 1) / bs), dim3(bs), 0, s, d, sp);
+  hipLaunchKernelGGL(k_predict, grid_for(d, (n + bs - 1) / bs), dim3(bs), 0, s, d, sp);
 }
 
 void launch_glob(const Dev& d, hipStream_t s) {
   if (d.nvtot <= 0) return;
   const int bs = body_block(d);
-  hipLaunchKernelGGL(k_glob, dim3((d.nvtot + bs - 1) / bs), dim3(bs), 0, s, d);
+  hipLaunchKernelGGL(k_glob, grid_for(d, (d.nvtot + bs - 1) / bs), dim3(bs), 0, s, d);
 }
 
 template <int BS>
 void launch_geom(const Dev& d, int write_geom_all, hipStream_t s) {
-  const dim3 g((d.ne + BS - 1) / BS), bl(BS);
+  const dim3 g = grid_for(d, (d.ne + BS - 1) / BS), bl(BS);
   if (write_geom_all)
     hipLaunchKernelGGL((k_geom<true, BS>), g, bl, 0, s, d);
   else
@@ -1389,7 +1400,7 @@ void launch_geom(const Dev& d, int write_geom_all, hipStream_t s) {
 
 template <bool GEOM_ALL>
 void launch_fused(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
-  const dim3 g((d.ne + kBlock - 1) / kBlock), bl(kBlock);
+  const dim3 g = grid_for(d, (d.ne + kBlock - 1) / kBlock), bl(kBlock);
   if (sp.law == LAW_AVALANCHE)
     hipLaunchKernelGGL((k_interact<LAW_AVALANCHE, GEOM_ALL>), g, bl, 0, s, d, sp, write_geom_all, 0);
   else if (sp.law == LAW_STICKED)
@@ -1419,7 +1430,7 @@ void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hip
   const int ngeom = (d.ne + gbs - 1) / gbs;
   const int W = std::min(std::max(d.law_group, 1), kLawGroup);
   const int waves = (ngeom + W - 1) / W;
-  dim3 gl((waves + kBlock / 64 - 1) / (kBlock / 64));
+  const dim3 gl = grid_for(d, (waves + kBlock / 64 - 1) / (kBlock / 64));
   if (sp.law == LAW_AVALANCHE)
     hipLaunchKernelGGL(k_law<LAW_AVALANCHE>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom, W);
   else if (sp.law == LAW_STICKED)
@@ -1430,12 +1441,16 @@ void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hip
 
 void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) {
   if (d.nb <= 0) return;
-  const int waves = (d.nb + kBodiesPerWave - 1) / kBodiesPerWave + (d.nhch + kBodiesPerWave - 1) / kBodiesPerWave;
+  const int per = d.gather_lanes == 8 ? 8 : 4;  // bodies per wavefront
+  const int waves = (d.nb + per - 1) / per + (d.nhch + per - 1) / per;
   const int wpb = d.gather_waves == 1 || d.gather_waves == 2 ? d.gather_waves : 4;
-  const int blocks = (waves + wpb - 1) / wpb;
-  hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(64 * wpb), 0, s, d, sp, kick);
+  const dim3 blocks = grid_for(d, (waves + wpb - 1) / wpb);
+  if (d.gather_lanes == 8)
+    hipLaunchKernelGGL(k_gather<8>, blocks, dim3(64 * wpb), 0, s, d, sp, kick);
+  else
+    hipLaunchKernelGGL(k_gather<16>, blocks, dim3(64 * wpb), 0, s, d, sp, kick);
   if (d.nheavy > 0) {
-    hipLaunchKernelGGL(k_heavy_finish, dim3(d.nheavy), dim3(64), 0, s, d, sp, kick);
+    hipLaunchKernelGGL(k_heavy_finish, grid_for(d, d.nheavy), dim3(64), 0, s, d, sp, kick);
     if (d.nheavy_exact > 0) hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
   }
 }
@@ -1545,7 +1560,7 @@ __global__ void k_guard_set_step(Dev d, long long step) {
 
 void launch_guard(const Dev& d, const StepParams& sp, hipStream_t s) {
   const int bs = body_block(d);
-  if (d.nb > 0) hipLaunchKernelGGL(k_guard_acc, dim3((d.nb + bs - 1) / bs), dim3(bs), 0, s, d, sp);
+  if (d.nb > 0) hipLaunchKernelGGL(k_guard_acc, grid_for(d, (d.nb + bs - 1) / bs), dim3(bs), 0, s, d, sp);
   hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(64), 0, s, d);
 }
 

@@ -27,7 +27,8 @@ class RkCounts(ctypes.Structure):
                 ("steps", ctypes.c_int64), ("t", ctypes.c_double), ("dt", ctypes.c_double),
                 ("total_broken_area", ctypes.c_double), ("last_candidates", ctypes.c_int64),
                 ("guard_reads", ctypes.c_int64), ("guard_wasted", ctypes.c_int64),
-                ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64)]
+                ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
+                ("guard_polls", ctypes.c_int64)]
 
 
 class RkStats(ctypes.Structure):
@@ -185,7 +186,8 @@ class Engine:
         self._check(self.L.rk_get_option(self.h, name.encode(), ctypes.byref(v)))
         return v.value
 
-    TUNED = ("fuse_law", "law_group", "geom_block", "gather_waves", "body_block", "graph_steps")
+    TUNED = ("fuse_law", "law_group", "geom_block", "gather_lanes", "gather_waves", "body_block", "persistent",
+             "graph_steps")
 
     def autotune(self, steps_per_trial=100):
         """Pick the launch shapes by timing them on the running simulation

@@ -46,11 +46,16 @@ def test_graph_steps_periodic_rebuild(tmp_path, graph_steps):
 
 @pytest.mark.parametrize("batch,graph_steps", [(7, 4), (64, 8)])
 def test_graph_steps_device_guard(tmp_path, batch, graph_steps):
-    """dynamicUpdateNL: batches of guarded steps as graphs; the device keeps
-    the step index, so a rebuild found inside a graph freezes the later
-    steps and the host resumes from the right iteration."""
+    """dynamicUpdateNL: batches of guarded steps as graphs (and persistent
+    kernels); the device keeps the step index, so a rebuild found inside a
+    graph freezes the later steps and the host resumes from the right
+    iteration."""
     path = _dynamic_hull_deck(tmp_path)
-    a, b = _pair(path, (150, 350, 500), graph_steps, setup=lambda e: e.set_option("guard_batch", batch))
+
+    def setup(e):
+        e.set_option("guard_batch", batch)
+        e.set_option("persistent", 16)
+    a, b = _pair(path, (150, 350, 500), graph_steps, setup=setup)
     assert a.counts()["nl_updates"] > 10
     assert b.counts()["graph_launches"] > 0
 

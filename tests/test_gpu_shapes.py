@@ -15,7 +15,10 @@ FIELDS = ("dn", "fn", "ft", "mom", "damp", "stick")
 SHAPES = [dict(fuse_law=0, law_group=1, geom_block=64, gather_waves=1, body_block=64),
           dict(fuse_law=0, law_group=8, geom_block=512, gather_waves=2, body_block=128),
           dict(fuse_law=0, law_group=2, geom_block=128, gather_waves=4, body_block=256, graph_steps=8),
-          dict(fuse_law=1, gather_waves=2, body_block=64, graph_steps=4)]
+          dict(fuse_law=1, gather_waves=2, body_block=64, graph_steps=4),
+          dict(fuse_law=1, gather_lanes=8, gather_waves=1),
+          dict(fuse_law=1, persistent=8, gather_lanes=8),
+          dict(fuse_law=0, persistent=16, law_group=2, geom_block=128, graph_steps=4)]
 
 
 def _hull_deck(tmp_path):
@@ -45,7 +48,8 @@ def test_launch_shapes_bit_exact(tmp_path, k):
 def test_bad_launch_shape_rejected(tmp_path):
     from rockable_amd.engine import Engine
     e = Engine(_hull_deck(tmp_path))
-    for name, v in (("geom_block", 96), ("law_group", 3), ("gather_waves", 8), ("body_block", 512)):
+    for name, v in (("geom_block", 96), ("law_group", 3), ("gather_waves", 8), ("body_block", 512),
+                    ("gather_lanes", 4), ("fuse_law", 2), ("model_scale", 0.0), ("persistent", 7)):
         with pytest.raises(RuntimeError):
             e.set_option(name, v)
 
