@@ -130,14 +130,20 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * wavefront when not fused: 1, 2, 4, 8; default 1), "geom_block"
  * (contact-kinematics workgroup when not fused: 64, 128, 256, 512; default
  * 256), "gather_lanes" (lanes per body in the resultant sums: 8, 16; default
- * 16), "gather_waves" (wavefronts per resultant workgroup: 1, 2, 4; default
- * 4), "body_block" (per-body kernels: 64, 128, 256; default 256).  No option
- * changes results. */
+ * 16), "gather_groups" (groups of 64 / gather_lanes bodies per resultant
+ * wavefront: 1, 2, 4, 8, 16, at most gather_lanes; 0, the default: 1 below
+ * 262144 bodies, gather_lanes from there), "fuse_glob" (1: the predictor
+ * writes the free bodies' global vertices; 0: a kernel of its own; -1, the
+ * default: 1 from 262144 bodies), "gather_waves" (wavefronts per resultant
+ * workgroup: 1, 2, 4; default 4), "body_block" (per-body kernels: 64, 128,
+ * 256; default 256), "persistent" (0, the default: one tile per workgroup;
+ * n in [8, 1048576]: the step kernels' grids capped at n workgroups, each
+ * looping over its tiles).  No option changes results. */
 int rk_set_option(rk_engine* eng, const char* name, double value);
 int rk_get_option(rk_engine* eng, const char* name, double* value);
 /* Autotuner (no counterpart in the reference) of fuse_law, law_group,
- * geom_block, gather_lanes, gather_waves, body_block and graph_steps, one
- * after the other: the
+ * geom_block, gather_lanes, gather_groups, fuse_glob, gather_waves,
+ * body_block, persistent and graph_steps, one after the other: the
  * simulation runs on, trial after trial of steps_per_trial steps, each
  * candidate value twice, and the fastest (wall time per step, rebuilds
  * excluded) is kept.  Every candidate computes the same state, so the steps

@@ -142,6 +142,7 @@ struct Dev {
   int netot, nftot;       // edges and faces of all bodies
   const int32_t* bvbody;
   int nvtot;
+  int nvdriven;           // vertices of the driven bodies (the first ones)
   double* acc;
   double* arot;
   double* force;
@@ -189,6 +190,8 @@ struct Dev {
   int geom_block;         // k_geom workgroup size (64, 128, 256, 512)
   int gather_waves;       // k_gather wavefronts per workgroup (1, 2, 4)
   int gather_lanes;       // k_gather lanes per body (8, 16)
+  int gather_groups;      // k_gather slot groups per body wavefront (1 .. gather_lanes)
+  int fuse_glob;          // 1: k_predict writes the free bodies' global vertices
   int body_block;         // workgroup size of the per-body kernels (64, 128, 256)
   int fuse_law;           // 1: geometry and law in one kernel (k_interact)
   int persist;            // step kernels' grid cap (0: one tile per workgroup)
@@ -197,6 +200,7 @@ struct Dev {
   DevGuard* guard;
   double* gdpos;          // 3 per body
   double* gdq;            // 4 per body
+  unsigned long long* gpart;  // per gather workgroup: max |dx|^2 key, min w key
   double disp, wstar;
   // active entries of the current force phase: geometry block b's in
   // akin[b * geom_block, b * geom_block + acount[b])
@@ -265,17 +269,24 @@ inline int persistent_grid(int T, int persist) {
 }
 
 // rk_step.hip
-void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s);
-void launch_glob(const Dev& d, hipStream_t s);
+// the predictor; then the global vertices of the new poses: of the listed
+// bodies (glob), else of every body
+void launch_predict(const Dev& d, const StepParams& sp, int glob, hipStream_t s);
+// all: every body's vertices; else those of the bodies with listed slots
+void launch_glob(const Dev& d, int all, hipStream_t s);
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s);
-void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s);
+// kick: the closing half kick; guard: also the displacement check's
+// accumulation (k_guard_check then follows)
+void launch_gather(const Dev& d, const StepParams& sp, int kick, int guard, hipStream_t s);
+// capacity of gpart for any launch shape
+int gather_blocks_max(int nb, int nhch);
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);
 // statistics: part holds stat_parts(d) doubles, out 9 doubles (KE translation,
 // KE rotation, max |v|, free bodies, active, contacts, active pairs, sum dn,
 // min dn over contacts)
 int stat_parts(const Dev& d);
 void launch_stats(const Dev& d, double* part, double* out, hipStream_t s);
-void launch_guard(const Dev& d, const StepParams& sp, hipStream_t s);
+void launch_guard(const Dev& d, hipStream_t s);
 // the step index the next guard check ends (the checks count on from it)
 void launch_guard_set_step(const Dev& d, long long step, hipStream_t s);
 // part: (nb / 256 + 1) * 9 doubles; feat: 16 floats

@@ -41,6 +41,9 @@ struct rk_engine {
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int law_group = 1, geom_block = 256, gather_waves = 4, body_block = 256, fuse_law = 1, gather_lanes = 16;
+  // 0 / -1: chosen from the packing's size (large packings: 64 bodies per
+  // resultant wavefront, global vertices written by the predictor)
+  int gather_groups = 0, fuse_glob = -1;
   int persist = 0;
   // host wall time of the rebuilds while the autotuner runs (it synchronises
   // after each rebuild then)
@@ -78,11 +81,15 @@ struct rk_engine {
   double* gv = nullptr;
   int32_t *bvoff = nullptr, *bvbody = nullptr, *beoff = nullptr, *bfoff = nullptr;
   int netot = 0, nftot = 0;
-  int nvtot = 0;
-  bool gv_dirty = true;  // body poses changed since the last k_glob
+  int nvtot = 0, nvdriven = 0;
+  // global vertices: of no body, of the listed bodies (those the force phase
+  // reads), or of every body (the neighbour list reads them) current
+  enum { GV_STALE, GV_LISTED, GV_ALL } gv_state = GV_STALE;
   // device displacement guard (dynamicUpdateNL)
   DevGuard* guard = nullptr;
   double *gdpos = nullptr, *gdq = nullptr;
+  unsigned long long* gpart = nullptr;  // the gather workgroups' guard maxima
+  int gpart_cap = 0;                    // pairs
   double wstar = -2.0;
   int guard_batch = 16;  // steps enqueued between two reads of the guard
   // rebuild-interval network (not owned) and its inputs / output on the device
@@ -132,6 +139,9 @@ struct rk_engine {
 namespace {
 
 constexpr int kPollSlots = 8;  // ring of pinned flags and events
+// bodies from which the per-body kernels favour whole wavefronts of bodies
+// over more wavefronts in flight (about 4 wavefronts per SIMD of 64 bodies)
+constexpr int kLargePacking = 1 << 18;
 constexpr int kPollAhead = 2;  // polls in flight before the host waits
 
 int fail(rk_engine* e, const std::string& msg) {
@@ -169,7 +179,7 @@ void free_device(rk_engine* eng) {
                 eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->ptab,      eng->damp_rate, eng->damp_kind,
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
                 eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
-                eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,
+                eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,      eng->gpart,
                 eng->beoff,    eng->bfoff,
                 eng->feat,     eng->pred,      eng->feat_part, eng->stat_part, eng->stat_out};
   for (void* p : ps)
@@ -195,12 +205,14 @@ void free_device(rk_engine* eng) {
   eng->netot = eng->nftot = 0;
   eng->guard = nullptr;
   eng->gdpos = eng->gdq = nullptr;
+  eng->gpart = nullptr;
+  eng->gpart_cap = 0;
   eng->feat = eng->pred = nullptr;
   eng->feat_part = nullptr;
   eng->feat_part_cap = 0;
   eng->stat_part = eng->stat_out = nullptr;
   eng->stat_cap = 0;
-  eng->nvtot = 0;
+  eng->nvtot = eng->nvdriven = 0;
   for (auto& E : eng->E) {
     entries_free(E);
     if (E.istart) (void)hipFree(E.istart);
@@ -231,6 +243,20 @@ std::vector<uint8_t> passive_flags(const Model& m) {
   return p;
 }
 
+// Room in gpart for the gather's workgroups under any launch shape (their
+// count grows with the heavy bodies' chunks, so after each list change).
+int ensure_gpart(rk_engine* eng) {
+  if (!eng->guard) return 0;
+  const int need = gather_blocks_max((int)eng->m.bodies.size(), eng->E[eng->cur].nhch);
+  if (need <= eng->gpart_cap) return 0;
+  if (eng->gpart) (void)hipFree(eng->gpart);
+  eng->gpart = nullptr;
+  eng->gpart_cap = 0;
+  HIPCHK(hipMalloc(&eng->gpart, 2 * sizeof(unsigned long long) * (size_t)need));
+  eng->gpart_cap = need;
+  return 0;
+}
+
 void assemble_dev(rk_engine* eng) {
   Dev& d = eng->d;
   memset(&d, 0, sizeof d);  // the captured-step cache compares Dev bytewise
@@ -247,6 +273,7 @@ void assemble_dev(rk_engine* eng) {
   d.nftot = eng->nftot;
   d.bvbody = eng->bvbody;
   d.nvtot = eng->nvtot;
+  d.nvdriven = eng->nvdriven;
   d.acc = eng->acc;
   d.arot = eng->arot;
   d.force = eng->force;
@@ -298,6 +325,7 @@ void assemble_dev(rk_engine* eng) {
   d.guard = eng->guard;
   d.gdpos = eng->gdpos;
   d.gdq = eng->gdq;
+  d.gpart = eng->gpart;
   d.disp = eng->m.dispUpdateNL;
   d.wstar = eng->wstar;
   d.acount = E.acount;
@@ -308,6 +336,12 @@ void assemble_dev(rk_engine* eng) {
   d.body_block = eng->body_block;
   d.fuse_law = eng->fuse_law;
   d.gather_lanes = eng->gather_lanes;
+  {
+    const bool large = d.nb >= kLargePacking;
+    const int g = eng->gather_groups > 0 ? eng->gather_groups : large ? eng->gather_lanes : 1;
+    d.gather_groups = std::min(g, eng->gather_lanes);
+    d.fuse_glob = eng->fuse_glob >= 0 ? eng->fuse_glob : large ? 1 : 0;
+  }
   d.persist = eng->persist;
   d.istart = E.istart;
   d.jstart = E.jstart;
@@ -486,10 +520,11 @@ int upload_model(rk_engine* eng) {
       for (int v = 0; v < nvb; v++) bvbody.push_back(b);
     }
     eng->nvtot = bvoff[nb];
+    eng->nvdriven = bvoff[m.nDriven];
     if (upload(eng, eng->bvoff, bvoff) || upload(eng, eng->bvbody, bvbody) ||
         upload(eng, eng->gv, std::vector<double>(3 * (size_t)std::max(1, eng->nvtot), 0.0)))
       return -1;
-    eng->gv_dirty = true;
+    eng->gv_state = rk_engine::GV_STALE;
     // each body's edges and faces (for the narrow phase's cull boxes)
     std::vector<int32_t> beoff(nb + 1, 0), bfoff(nb + 1, 0);
     for (int b = 0; b < nb; b++) {
@@ -513,11 +548,13 @@ int upload_model(rk_engine* eng) {
     if (upload(eng, eng->guard, std::vector<DevGuard>(1, g0))) return -1;
     eng->wstar = wstar_for(m.angleUpdateNL);
   } else {
-    void* ps[] = {eng->guard, eng->gdpos, eng->gdq};
+    void* ps[] = {eng->guard, eng->gdpos, eng->gdq, eng->gpart};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     eng->guard = nullptr;
     eng->gdpos = eng->gdq = nullptr;
+    eng->gpart = nullptr;
+    eng->gpart_cap = 0;
   }
   // parameter tables (compact G = ngroup)
   const int G = std::max(1, m.ngroup);
@@ -650,6 +687,7 @@ int upload_model(rk_engine* eng) {
   eng->nlp.cellMinSizes[1] = m.cellMinSizes.y;
   eng->nlp.cellMinSizes[2] = m.cellMinSizes.z;
   eng->nlp.boxForLinkCellsOpt = m.boxForLinkCellsOpt;
+  if (ensure_gpart(eng) != 0) return -1;
   assemble_dev(eng);
   HIPCHK(hipStreamSynchronize(eng->stream));
   return 0;
@@ -697,24 +735,25 @@ void toc(rk_engine* eng, const char* name, int level) {
   eng->tic_index = -1;
 }
 
-// Global vertices of the current poses (k_glob), before a kernel reads them.
-void ensure_gv(rk_engine* eng) {
-  if (!eng->gv_dirty) return;
-  launch_glob(eng->d, eng->stream);
-  eng->gv_dirty = false;
+// Global vertices of the current poses (k_glob), before a kernel reads them:
+// of every body (all) or of the bodies with a listed interaction.
+void ensure_gv(rk_engine* eng, bool all) {
+  if (eng->gv_state == rk_engine::GV_ALL || (!all && eng->gv_state == rk_engine::GV_LISTED)) return;
+  launch_glob(eng->d, all ? 1 : 0, eng->stream);
+  eng->gv_state = all ? rk_engine::GV_ALL : rk_engine::GV_LISTED;
 }
 
 // Rockable::accelerations (Rockable.cpp:3562-3719); kick fuses the closing
 // half kick of velocityVerletStep.
-int run_accelerations(rk_engine* eng, int kick, int write_geom) {
+int run_accelerations(rk_engine* eng, int kick, int write_geom, bool guard = false) {
   const StepParams sp = step_params(eng);
-  ensure_gv(eng);
+  ensure_gv(eng, false);
   tic(eng, 1);
   tic(eng, 2);
   launch_interact(eng->d, sp, write_geom || eng->record_geometry, eng->stream);
   toc(eng, "interact", 2);
   tic(eng, 2);
-  launch_gather(eng->d, sp, kick, eng->stream);
+  launch_gather(eng->d, sp, kick, guard ? 1 : 0, eng->stream);
   toc(eng, "gather", 2);
   toc(eng, "force", 1);
   if (eng->m.law == LAW_STICKED && eng->live_interfaces > 0) {
@@ -748,12 +787,13 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom) {
 int do_update_nl(rk_engine* eng) {
   if (eng->tuning) HIPCHK(hipStreamSynchronize(eng->stream));
   const auto t0 = std::chrono::steady_clock::now();
-  ensure_gv(eng);
+  ensure_gv(eng, true);
   tic(eng, eng->profile);
   const int nxt = 1 - eng->cur;
   if (nl_rebuild(eng->w, eng->d, eng->nlp, eng->E[eng->cur], eng->E[nxt], eng->stream) != 0)
     return fail(eng, std::string("neighbour list: ") + eng->w.err);
   eng->cur = nxt;
+  if (ensure_gpart(eng) != 0) return -1;
   assemble_dev(eng);
   eng->nl_updates++;
   eng->last_interval = eng->steps - eng->last_rebuild_step;
@@ -766,21 +806,31 @@ int do_update_nl(rk_engine* eng) {
   return 0;
 }
 
-int vv_step(rk_engine* eng, int write_geom) {
+// guard: the step ends with the displacement check (dynamicUpdateNL), its
+// accumulation fused into the resultant kernels
+int vv_step(rk_engine* eng, int write_geom, bool guard) {
   const StepParams sp = step_params(eng);
+  if (guard && (!eng->guard || eng->gpart_cap < gather_blocks_max(eng->d.nb, eng->d.nhch)))
+    return fail(eng, "device guard: no room for the gather's maxima");
   tic(eng, 2);
-  launch_predict(eng->d, sp, eng->stream);
+  launch_predict(eng->d, sp, 1, eng->stream);
   toc(eng, "predict", 2);
-  eng->gv_dirty = true;
-  return run_accelerations(eng, 1, write_geom);
+  // the predictor leaves the listed bodies' global vertices current (every
+  // body's while there is no list)
+  eng->gv_state = eng->d.soff ? rk_engine::GV_LISTED : rk_engine::GV_ALL;
+  if (run_accelerations(eng, 1, write_geom, guard) != 0) return -1;
+  if (guard) {
+    tic(eng, 2);
+    launch_guard(eng->d, eng->stream);
+    toc(eng, "guard", 2);
+  }
+  return 0;
 }
 
 // One step of the loop as the device runs it (write_geom 0): the
 // velocity-Verlet step, and the displacement check with dynamicUpdateNL.
 int launch_one_step(rk_engine* eng, bool with_guard) {
-  if (vv_step(eng, 0) != 0) return -1;
-  if (with_guard) launch_guard(eng->d, step_params(eng), eng->stream);
-  return 0;
+  return vv_step(eng, 0, with_guard);
 }
 
 // What a captured run of steps depends on: the device view (pointers and
@@ -813,7 +863,7 @@ int step_graph(rk_engine* eng, int n, bool with_guard, hipGraphExec_t* out) {
   for (int k = 0; k < n && rc == 0; k++) rc = launch_one_step(eng, with_guard);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(eng->stream, &graph);
-  eng->gv_dirty = false;  // each step of the graph ends with fresh global vertices
+  eng->gv_state = rk_engine::GV_LISTED;  // each step of the graph ends with the listed bodies' fresh
   if (rc != 0 || ec != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
     return fail(eng, rc != 0 ? eng->err : std::string("step capture: ") + hipGetErrorString(ec));
@@ -877,6 +927,8 @@ int flush_steps(rk_engine* eng) {
       HIPCHK(hipGraphLaunch(exec, eng->stream));
       eng->graph_launches++;
     }
+    // a replayed step leaves only the listed bodies' global vertices current
+    eng->gv_state = rk_engine::GV_LISTED;
   }
   for (; n > 0; n--)
     if (launch_one_step(eng, with_guard) != 0) return -1;
@@ -891,9 +943,7 @@ int enqueue_step(rk_engine* eng, bool write_geom, bool with_guard) {
   if (eng->pending > 0 && eng->pending_guard != with_guard && flush_steps(eng) != 0) return -1;
   if (write_geom || bonds || eng->graph_steps <= 1) {
     if (flush_steps(eng) != 0) return -1;
-    if (vv_step(eng, write_geom ? 1 : 0) != 0) return -1;
-    if (with_guard) launch_guard(eng->d, step_params(eng), eng->stream);
-    return 0;
+    return vv_step(eng, write_geom ? 1 : 0, with_guard);
   }
   eng->pending_guard = with_guard;
   eng->pending++;
@@ -1340,6 +1390,7 @@ int rk_load_driving_system(rk_engine* eng, const char* path) {
   // the heavy bodies' summation paths depend on the passive flags
   if (nl_build_heavy(eng->w, (int)eng->m.bodies.size(), eng->E[eng->cur], eng->passive, eng->stream) != 0)
     return fail(eng, eng->w.err);
+  if (ensure_gpart(eng) != 0) return -1;
   assemble_dev(eng);
   HIPCHK(hipStreamSynchronize(eng->stream));
   return 0;
@@ -1423,6 +1474,15 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     if (!(value == 8.0 || value == 16.0)) return fail(eng, "gather_lanes must be 8 or 16");
     eng->gather_lanes = (int)value;
     if (eng->loaded) assemble_dev(eng);
+  } else if (n == "gather_groups") {
+    if (!(value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0 || value == 8.0 || value == 16.0))
+      return fail(eng, "gather_groups must be 0 (by size), 1, 2, 4, 8 or 16");
+    eng->gather_groups = (int)value;
+    if (eng->loaded) assemble_dev(eng);
+  } else if (n == "fuse_glob") {
+    if (!(value == -1.0 || value == 0.0 || value == 1.0)) return fail(eng, "fuse_glob must be -1 (by size), 0 or 1");
+    eng->fuse_glob = (int)value;
+    if (eng->loaded) assemble_dev(eng);
   } else if (n == "fuse_law") {
     if (!(value == 0.0 || value == 1.0)) return fail(eng, "fuse_law must be 0 or 1");
     eng->fuse_law = (int)value;
@@ -1462,6 +1522,7 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
       (void)hipSetDevice(eng->device);
       if (nl_build_heavy(eng->w, (int)eng->m.bodies.size(), eng->E[eng->cur], eng->passive, eng->stream) != 0)
         return fail(eng, eng->w.err);
+      if (ensure_gpart(eng) != 0) return -1;
       assemble_dev(eng);
     }
   } else
@@ -1502,6 +1563,8 @@ int* tuned_option(rk_engine* eng, const std::string& n) {
   if (n == "graph_steps") return &eng->graph_steps;
   if (n == "fuse_law") return &eng->fuse_law;
   if (n == "gather_lanes") return &eng->gather_lanes;
+  if (n == "gather_groups") return &eng->gather_groups;
+  if (n == "fuse_glob") return &eng->fuse_glob;
   if (n == "persistent") return &eng->persist;
   return nullptr;
 }
@@ -1520,6 +1583,8 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
                         {"law_group", {1, 2, 4, 8}},
                         {"geom_block", {64, 128, 256, 512}},
                         {"gather_lanes", {8, 16}},
+                        {"gather_groups", {1, 4, 16}},
+                        {"fuse_glob", {0, 1}},
                         {"gather_waves", {1, 2, 4}},
                         {"body_block", {64, 128, 256}},
                         {"persistent", {0, 2048, 8192}},
@@ -1759,7 +1824,7 @@ int rk_set_bodies(rk_engine* eng, const double* pos, const double* Q, const doub
     }
   }
   HIPCHK(hipMemcpy(eng->kin, kin.data(), sizeof(BodyKin) * nb, hipMemcpyHostToDevice));
-  eng->gv_dirty = true;
+  eng->gv_state = rk_engine::GV_STALE;
   return 0;
 }
 
@@ -1829,7 +1894,7 @@ int rk_set_rebuild_model(rk_engine* eng, rk_mlp* mlp) {
 int rk_guard_features(rk_engine* eng, float* out16) {
   if (check_loaded(eng) || !out16) return -1;
   (void)hipSetDevice(eng->device);
-  ensure_gv(eng);
+  ensure_gv(eng, true);
   if (compute_features(eng) != 0) return -1;
   HIPCHK(hipMemcpyAsync(out16, eng->feat, 16 * sizeof(float), hipMemcpyDeviceToHost, eng->stream));
   HIPCHK(hipStreamSynchronize(eng->stream));

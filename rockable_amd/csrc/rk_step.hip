@@ -648,14 +648,22 @@ __device__ __forceinline__ v3 euler_domega(v3 I, v3 w, v3 M) {
            (M.z - (I.y - I.x) * w.x * w.y) / I.z);
 }
 
-__device__ void body_force(const Dev& d, int b, const BodyKin& P, v3& F, v3& M) {
+// The body force of a free body as (F, M), returned by value: reference
+// outputs merged across the branches compile to a stack copy (scratch).
+struct FM {
+  v3 F, M;
+};
+__device__ __forceinline__ FM body_force(const Dev& d, int b, const BodyKin& P) {
+  FM r;
+  r.F = V(0.0, 0.0, 0.0);
+  r.M = V(0.0, 0.0, 0.0);
   if (d.bf == BF_ATTRACTING) {  // BodyForce_AttractingPoint.cpp:51-59
     v3 dir = vsub(V(d.bf_point[0], d.bf_point[1], d.bf_point[2]), kpos(P));
     double l2 = vnorm2(dir);
     if (l2 > 0.0) {
       double inv = 1.0 / sqrt(l2);
       dir = V(dir.x * inv, dir.y * inv, dir.z * inv);
-      F = vsc(d.mass[b] * d.bf_acc, dir);
+      r.F = vsc(d.mass[b] * d.bf_acc, dir);
     }
   } else if (d.bf == BF_PREFDIR) {  // BodyForce_PreferredDirection.cpp:54-75
     v3 axis = V(d.bf_axis[0], d.bf_axis[1], d.bf_axis[2]);
@@ -670,25 +678,27 @@ __device__ void body_force(const Dev& d, int b, const BodyKin& P, v3& F, v3& M) 
       sgn = 1.0;
       angle = M_PI - angle;
     }
-    M = vsc(angle, vsc(d.bf_Kr * sgn, u));
+    r.M = vsc(angle, vsc(d.bf_Kr * sgn, u));
   } else if (d.bf == BF_VISCOUS) {  // BodyForce_ViscousFluid.cpp:51-62
     v3 velSqr = V(P.vx * P.vx, P.vy * P.vy, P.vz * P.vz);
-    F = vsc(0.5 * 0.47 * d.bf_fluidDensity * d.bf_surface[b], velSqr);
+    r.F = vsc(0.5 * 0.47 * d.bf_fluidDensity * d.bf_surface[b], velSqr);
   }
+  return r;
 }
 
 // initialise_particle_forces_and_moments (Rockable.cpp:2869-2929): body force
 // for free bodies, imposed force and moment values for driven ones.
+// BF false: the deck has no body force (the kernel then holds none of its code).
+template <bool BF = true>
 __device__ __forceinline__ void init_resultants(const Dev& d, int b, const BodyKin& P, bool driven, v3& F, v3& M,
                                                 int& c0, int& c1) {
   F = V(0.0, 0.0, 0.0);
   M = V(0.0, 0.0, 0.0);
   c0 = c1 = 0;
-  if (!driven && d.bf != BF_NONE) {
-    v3 fm = V(0, 0, 0), mm = V(0, 0, 0);
-    body_force(d, b, P, fm, mm);
-    F = vadd(F, fm);
-    M = vadd(M, mm);
+  if (BF && !driven && d.bf != BF_NONE) {
+    const FM bf = body_force(d, b, P);
+    F = vadd(F, bf.F);
+    M = vadd(M, bf.M);
   }
   if (driven && d.nctrl > 0) {
     c0 = d.ctrl_off[b];
@@ -696,7 +706,7 @@ __device__ __forceinline__ void init_resultants(const Dev& d, int b, const BodyK
     // selects rather than a switch: a switch over components compiles to
     // an indexed store into a stack copy of (F, M), i.e. scratch memory
     for (int q = c0; q < c1; q++) {
-      const DevControl C = d.ctrl[d.ctrl_idx[q]];
+      const DevControl& C = d.ctrl[d.ctrl_idx[q]];
       const int ty = C.type;
       const double v = C.value;
       F.x = ty == C_X_FOR ? v : F.x;
@@ -715,7 +725,7 @@ __device__ __forceinline__ double pick(const v3& F, const v3& M, int c) {
 
 // Stores the resultants, compute_accelerations_from_resultants
 // (Rockable.cpp:3419-3552) and the closing half kick (2120-2155).
-__device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, int kick, int b, BodyKin P,
+__device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, int kick, int b, BodyKin& P,
                                             bool driven, v3 F, v3 M, int c0, int c1) {
   st3(d.force + 3 * (size_t)b, F);
   st3(d.moment + 3 * (size_t)b, M);
@@ -727,39 +737,24 @@ __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, 
     acc = V(0.0, 0.0, 0.0);
     arot = V(0.0, 0.0, 0.0);
     for (int qq = c0; qq < c1; qq++) {
-      const DevControl C = d.ctrl[d.ctrl_idx[qq]];
+      const DevControl& C = d.ctrl[d.ctrl_idx[qq]];
       const int ty = C.type;
       acc.x = ty == C_X_FOR ? F.x / m : acc.x;
       acc.y = ty == C_Y_FOR ? F.y / m : acc.y;
       acc.z = ty == C_Z_FOR ? F.z / m : acc.z;
-      switch (ty) {
-        case C_XROT_MOM:
-        case C_YROT_MOM:
-        case C_ZROT_MOM:
-        case C_XYZROT_MOM: {
-          qt Qinv = qconj(Q);
-          v3 w;
-          if (C.type == C_XROT_MOM)
-            w = V(P.wx, 0.0, 0.0);
-          else if (C.type == C_YROT_MOM)
-            w = V(0.0, P.wy, 0.0);
-          else if (C.type == C_ZROT_MOM)
-            w = V(0.0, 0.0, P.wz);
-          else
-            w = kvrot(P);
-          v3 omega = qrot(Qinv, w);
-          v3 Mb = qrot(Qinv, M);
-          v3 dw = qrot(Q, euler_domega(I, omega, Mb));
-          if (C.type == C_XROT_MOM)
-            arot.x = dw.x;
-          else if (C.type == C_YROT_MOM)
-            arot.y = dw.y;
-          else if (C.type == C_ZROT_MOM)
-            arot.z = dw.z;
-          else
-            arot = dw;
-        } break;
-        default: break;
+      // selects rather than branches on the component (branches compile to
+      // an indexed store into a stack copy of arot, i.e. scratch memory)
+      const bool all = ty == C_XYZROT_MOM;
+      const bool rx = all || ty == C_XROT_MOM, ry = all || ty == C_YROT_MOM, rz = all || ty == C_ZROT_MOM;
+      if (rx || ry || rz) {
+        const qt Qinv = qconj(Q);
+        const v3 w = V(rx ? P.wx : 0.0, ry ? P.wy : 0.0, rz ? P.wz : 0.0);
+        const v3 omega = qrot(Qinv, w);
+        const v3 Mb = qrot(Qinv, M);
+        const v3 dw = qrot(Q, euler_domega(I, omega, Mb));
+        arot.x = rx ? dw.x : arot.x;
+        arot.y = ry ? dw.y : arot.y;
+        arot.z = rz ? dw.z : arot.z;
       }
     }
   } else {
@@ -774,7 +769,7 @@ __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, 
   if (!kick) return;
   if (driven) {
     for (int qq = c0; qq < c1; qq++) {
-      const DevControl C = d.ctrl[d.ctrl_idx[qq]];
+      const DevControl& C = d.ctrl[d.ctrl_idx[qq]];
       const int ty = C.type;
       const v3 w = vadd(kvrot(P), vsc(sp.dt_2, arot));
       P.vx = ty == C_X_FOR ? P.vx + sp.dt_2 * acc.x : P.vx;
@@ -802,6 +797,58 @@ __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, 
   out->wy = P.wy;
   out->wz = P.wz;
 }
+
+// dynamicCheckUpdateNL (Rockable.cpp:1571-1596), accumulation for one body
+// at the end of its step: deltaPos += vel dt, deltaQ += Q.dot(vrot) dt, then
+// the running max of |deltaPos|^2 and min of deltaQ.w as order keys (NaNs are
+// ignored as the serial comparisons ignore them).  rst: the accumulations
+// restart (a rebuild at the previous step).
+__device__ __forceinline__ unsigned long long wkey(double w) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(w);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+struct GuardAcc {
+  unsigned long long kp, kw;  // max key of |deltaPos|^2, min key of deltaQ.w
+};
+__device__ __forceinline__ void guard_body(const Dev& d, const StepParams& sp, int b, const BodyKin& P, bool rst,
+                                           GuardAcc& a) {
+  v3 dp = rst ? V(0.0, 0.0, 0.0) : ld3(d.gdpos + 3 * (size_t)b);
+  dp = vadd(dp, vsc(sp.dt, kvel(P)));
+  st3(d.gdpos + 3 * (size_t)b, dp);
+  const double tmp1 = vnorm2(dp);
+  if (tmp1 == tmp1) {
+    const unsigned long long k1 = (unsigned long long)__double_as_longlong(tmp1);
+    a.kp = k1 > a.kp ? k1 : a.kp;
+  }
+  const qt dq = qdot(kq(P), kvrot(P));
+  double* g = d.gdq + 4 * (size_t)b;
+  qt q;
+  if (rst) {
+    q.w = 1.0;
+    q.x = q.y = q.z = 0.0;
+  } else {
+    q.w = g[0];
+    q.x = g[1];
+    q.y = g[2];
+    q.z = g[3];
+  }
+  q.w += dq.w * sp.dt;
+  q.x += dq.x * sp.dt;
+  q.y += dq.y * sp.dt;
+  q.z += dq.z * sp.dt;
+  g[0] = q.w;
+  g[1] = q.x;
+  g[2] = q.y;
+  g[3] = q.z;
+  double w = q.w;
+  if (w > 1.0) w = 1.0;
+  if (w < -1.0) w = -1.0;
+  if (w == w) {
+    const unsigned long long k2 = wkey(w);
+    a.kw = k2 < a.kw ? k2 : a.kw;
+  }
+}
+__device__ __forceinline__ bool guard_reset(const Dev& d) { return *(volatile const int*)&d.guard->reset != 0; }
 
 // compute_resultants (Rockable.cpp:3267-3272).  L lanes per body (8 or 16;
 // 64 / L bodies per wavefront).  The lanes read the slot activity bytes 4 L
@@ -850,48 +897,104 @@ __device__ __forceinline__ double sum_slots(const Dev& d, bool run, int k0, int 
 }
 
 // Bodies with at most heavy_min slots: the serial sum, so every component is
-// the reference's bit for bit.  The wavefronts after the bodies' take the
-// 64-slot chunks of the heavy bodies and store each chunk's partial sums
-// (from -0.0, the identity of addition) for k_heavy_finish.
-template <int L>
-__global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick) {
-  constexpr int kPer = 64 / L;  // bodies (or chunks) per wavefront
+// the reference's bit for bit.  A body wavefront takes G groups of 64 / L
+// bodies (G from 1 to L, so 64 / L to 64 bodies), one per lane: each lane
+// loads its body's slot range and initial force and moment, the slots are
+// summed group after group (L lanes per body; groups with no slot are
+// skipped), and each lane then finishes its own body.  Few groups keep more
+// wavefronts in flight (small packings), many use every lane for the
+// per-body work (large ones).  The
+// wavefronts after the bodies' take the 64-slot chunks of the heavy bodies
+// and store each chunk's partial sums (from -0.0, the identity of addition)
+// for k_heavy_finish.  With guard, the displacement check accumulates each
+// finished body and every workgroup stores its maxima in gpart[2 blockIdx.x].
+template <int L, bool BF>  // BF: the deck has a body force
+__global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick, int guard) {
+  constexpr int kPer = 64 / L;  // bodies per slot group (chunks per chunk wavefront)
+  __shared__ unsigned long long red[2][kBlock / 64];
   if (frozen(d)) return;
   const int lane = threadIdx.x & 63;
-  const int gl = lane & (L - 1), gbase = lane - gl;
+  const int gl = lane & (L - 1), sub = lane / L;
   const int wpb = (int)(blockDim.x >> 6);
-  const int body_waves = (d.nb + kPer - 1) / kPer;
+  const int G = d.gather_groups;  // slot groups per body wavefront (1..L)
+  const int NB = kPer * G;        // bodies per body wavefront
+  const int body_waves = (d.nb + NB - 1) / NB;
   const int waves = body_waves + (d.nhch + kPer - 1) / kPer;
+  const bool rst = guard && guard_reset(d);
+  GuardAcc ga;
+  ga.kp = 0ull;
+  ga.kw = ~0ull;
   for (TileLoop T = xcd_tiles((waves + wpb - 1) / wpb); T.t < T.end; T.t += T.step) {
-  const int wave = (int)((T.t * (unsigned)blockDim.x + threadIdx.x) >> 6);
-  if (wave >= body_waves) {
-    const int c = (wave - body_waves) * kPer + lane / L;
-    const bool run = c < d.nhch;
-    int2 r = make_int2(0, 0);
-    if (run) r = d.hch[c];
-    const double s = sum_slots<L>(d, run, r.x, r.y, gl, -0.0);
-    if (run && gl < 6) d.hpart[6 * (size_t)c + gl] = s;
-    continue;
+    const int wave = (int)((T.t * (unsigned)blockDim.x + threadIdx.x) >> 6);
+    if (wave >= body_waves) {
+      const int c = (wave - body_waves) * kPer + sub;
+      const bool run = c < d.nhch;
+      int2 r = make_int2(0, 0);
+      if (run) r = d.hch[c];
+      const double sc = sum_slots<L>(d, run, r.x, r.y, gl, -0.0);
+      if (run && gl < 6) d.hpart[6 * (size_t)c + gl] = sc;
+      continue;
+    }
+    const int b = wave * NB + lane;
+    int k0 = 0, k1 = 0, c0 = 0, c1 = 0;
+    bool run = false, driven = false;
+    if (lane < NB && b < d.nb) {
+      k0 = d.soff[b];
+      k1 = d.soff[b + 1];
+      run = (k1 - k0) <= d.heavy_min;
+    }
+    v3 F = V(0.0, 0.0, 0.0), M = V(0.0, 0.0, 0.0);
+    if (run) {
+      // the body's state is read here only where a body force needs it, and
+      // in full after the sums (fewer registers live across them)
+      driven = b < d.nDriven;
+      init_resultants<BF>(d, b, d.kin[b], driven, F, M, c0, c1);
+    }
+    const unsigned long long busy = __ballot(run && k1 > k0);
+    for (int g = 0; g < G; g++) {
+      if (((busy >> (g * kPer)) & ((1ull << kPer) - 1ull)) == 0ull) continue;  // uniform
+      // lane (sub, gl) sums component gl of body lane g kPer + sub
+      const int src = g * kPer + sub;
+      const int s0 = __shfl(k0, src), s1 = __shfl(k1, src);
+      const bool r2 = __shfl(run ? 1 : 0, src) != 0;
+      const v3 Fs = V(__shfl(F.x, src), __shfl(F.y, src), __shfl(F.z, src));
+      const v3 Ms = V(__shfl(M.x, src), __shfl(M.y, src), __shfl(M.z, src));
+      const double sg = sum_slots<L>(d, r2, s0, s1, gl, pick(Fs, Ms, gl < 6 ? gl : 0));
+      // back to the body lanes of group g: component c of body lane l is in
+      // lane (l - g kPer) L + c
+      const int base = (lane & (kPer - 1)) * L;
+      const double fx = __shfl(sg, base), fy = __shfl(sg, base + 1), fz = __shfl(sg, base + 2);
+      const double mx = __shfl(sg, base + 3), my = __shfl(sg, base + 4), mz = __shfl(sg, base + 5);
+      if (lane / kPer == g) {
+        F = V(fx, fy, fz);
+        M = V(mx, my, mz);
+      }
+    }
+    if (run) {
+      BodyKin P = d.kin[b];
+      finish_body(d, sp, kick, b, P, driven, F, M, c0, c1);
+      if (guard) guard_body(d, sp, b, P, rst, ga);
+    }
   }
-  const int b = wave * kPer + lane / L;
-  int k0 = 0, k1 = 0, c0 = 0, c1 = 0;
-  bool run = false, driven = false;
-  if (b < d.nb) {
-    k0 = d.soff[b];
-    k1 = d.soff[b + 1];
-    run = (k1 - k0) <= d.heavy_min;
+  if (!guard) return;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long op = __shfl_xor(ga.kp, off, 64), ow = __shfl_xor(ga.kw, off, 64);
+    ga.kp = op > ga.kp ? op : ga.kp;
+    ga.kw = ow < ga.kw ? ow : ga.kw;
   }
-  BodyKin P;
-  v3 F = V(0.0, 0.0, 0.0), M = V(0.0, 0.0, 0.0);
-  if (run) {
-    P = d.kin[b];
-    driven = b < d.nDriven;
-    init_resultants(d, b, P, driven, F, M, c0, c1);
+  const int wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wv] = ga.kp;
+    red[1][wv] = ga.kw;
   }
-  const double s = sum_slots<L>(d, run, k0, k1, gl, pick(F, M, gl < 6 ? gl : 0));
-  const double fx = __shfl(s, gbase), fy = __shfl(s, gbase + 1), fz = __shfl(s, gbase + 2);
-  const double mx = __shfl(s, gbase + 3), my = __shfl(s, gbase + 4), mz = __shfl(s, gbase + 5);
-  if (run && gl == 0) finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < wpb; k++) {
+      ga.kp = red[0][k] > ga.kp ? red[0][k] : ga.kp;
+      ga.kw = red[1][k] < ga.kw ? red[1][k] : ga.kw;
+    }
+    d.gpart[2 * (size_t)blockIdx.x] = ga.kp;
+    d.gpart[2 * (size_t)blockIdx.x + 1] = ga.kw;
   }
 }
 
@@ -901,7 +1004,16 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
 // l, l + 64, ... in turn, then a butterfly over the lanes; IEEE addition is
 // commutative, so every lane holds the same sum).  Deterministic; it differs
 // from the serial sum only by rounding.
-__global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int kick) {
+__device__ __forceinline__ void guard_heavy(const Dev& d, const StepParams& sp, int b, const BodyKin& P) {
+  GuardAcc a;
+  a.kp = 0ull;
+  a.kw = ~0ull;
+  guard_body(d, sp, b, P, guard_reset(d), a);
+  if (a.kp) atomicMax(&d.guard->maxpos2, a.kp);
+  if (a.kw != ~0ull) atomicMin(&d.guard->minw, a.kw);
+}
+
+__global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int kick, int guard) {
   if (frozen(d)) return;
   for (TileLoop T = xcd_tiles(d.nheavy); T.t < T.end; T.t += T.step) {
   const int h = T.t;
@@ -919,12 +1031,13 @@ __global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int k
 #pragma unroll
     for (int q = 0; q < 6; q++) a[q] += __shfl_xor(a[q], off, 64);
   if (lane != 0) continue;
-  const BodyKin P = d.kin[b];
+  BodyKin P = d.kin[b];
   const bool driven = b < d.nDriven;
   v3 F, M;
   int c0, c1;
   init_resultants(d, b, P, driven, F, M, c0, c1);
   finish_body(d, sp, kick, b, P, driven, vadd(F, V(a[0], a[1], a[2])), vadd(M, V(a[3], a[4], a[5])), c0, c1);
+  if (guard) guard_heavy(d, sp, b, P);
   }
 }
 
@@ -935,14 +1048,14 @@ __global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int k
 // lanes 0-5 of wave 0 add them in order, one lane per component.
 constexpr int kHeavyChunk = 512;
 
-__global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, int kick) {
+__global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, int kick, int guard) {
   __shared__ double lds[2 * kHeavyChunk * 6];
   if (frozen(d)) return;
   const int b = d.heavy[blockIdx.x];
   const int t = threadIdx.x;
   const int k0 = d.soff[b], k1 = d.soff[b + 1];
   if (d.passive[b] && !d.exact_sums) return;  // summed by chunks
-  const BodyKin P = d.kin[b];
+  BodyKin P = d.kin[b];
   const bool driven = b < d.nDriven;
   v3 F, M;
   int c0, c1;
@@ -980,89 +1093,47 @@ __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, i
   if (wave == 0) {
     const double fx = __shfl(s, 0), fy = __shfl(s, 1), fz = __shfl(s, 2);
     const double mx = __shfl(s, 3), my = __shfl(s, 4), mz = __shfl(s, 5);
-    if (lane == 0) finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+    if (lane == 0) {
+      finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+      if (guard) guard_heavy(d, sp, b, P);
+    }
   }
 }
 
-// dynamicCheckUpdateNL (Rockable.cpp:1571-1596), accumulation: deltaPos +=
-// vel dt, deltaQ += Q.dot(vrot) dt, then the running max of |deltaPos|^2 and
-// min of deltaQ.w (integer atomics on order keys; NaNs are ignored as the
-// serial comparisons ignore them).
-__device__ __forceinline__ unsigned long long wkey(double w) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(w);
-  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-}
-__global__ void __launch_bounds__(kBlock) k_guard_acc(Dev d, StepParams sp) {
-  __shared__ unsigned long long rp[kBlock / 64], rw[kBlock / 64];
+// the end of a step of the integrate() loop (Rockable.cpp:2690-2740): a
+// needUpdate left by the previous check makes the rebuild due now (the step
+// kernels then wait for the host); then the check itself, on the maxima of
+// the gather's workgroups (nparts pairs in gpart; max and min do not depend
+// on the order).  The step index lives on the device, so that a captured
+// sequence of steps replays as is.
+constexpr int kCheckBlock = 1024;
+__global__ void __launch_bounds__(kCheckBlock) k_guard_check(Dev d, int nparts) {
+  __shared__ unsigned long long red[2][kCheckBlock / 64];
   if (frozen(d)) return;
-  const bool r = *(volatile const int*)&d.guard->reset != 0;
   unsigned long long kp = 0ull, kw = ~0ull;
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < d.nb; b += gridDim.x * blockDim.x) {
-    const BodyKin& P = d.kin[b];
-    v3 dp = r ? V(0.0, 0.0, 0.0) : ld3(d.gdpos + 3 * (size_t)b);
-    dp = vadd(dp, vsc(sp.dt, kvel(P)));
-    st3(d.gdpos + 3 * (size_t)b, dp);
-    const double tmp1 = vnorm2(dp);
-    if (tmp1 == tmp1) {
-      const unsigned long long k1 = (unsigned long long)__double_as_longlong(tmp1);
-      kp = k1 > kp ? k1 : kp;
-    }
-    const qt dq = qdot(kq(P), kvrot(P));
-    double* g = d.gdq + 4 * (size_t)b;
-    qt q;
-    if (r) {
-      q.w = 1.0;
-      q.x = q.y = q.z = 0.0;
-    } else {
-      q.w = g[0];
-      q.x = g[1];
-      q.y = g[2];
-      q.z = g[3];
-    }
-    q.w += dq.w * sp.dt;
-    q.x += dq.x * sp.dt;
-    q.y += dq.y * sp.dt;
-    q.z += dq.z * sp.dt;
-    g[0] = q.w;
-    g[1] = q.x;
-    g[2] = q.y;
-    g[3] = q.z;
-    double w = q.w;
-    if (w > 1.0) w = 1.0;
-    if (w < -1.0) w = -1.0;
-    if (w == w) {
-      const unsigned long long k2 = wkey(w);
-      kw = k2 < kw ? k2 : kw;
-    }
+  for (int k = threadIdx.x; k < nparts; k += kCheckBlock) {
+    const unsigned long long p = d.gpart[2 * (size_t)k], w = d.gpart[2 * (size_t)k + 1];
+    kp = p > kp ? p : kp;
+    kw = w < kw ? w : kw;
   }
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long op = __shfl_xor(kp, off, 64), ow = __shfl_xor(kw, off, 64);
     kp = op > kp ? op : kp;
     kw = ow < kw ? ow : kw;
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    rp[wave] = kp;
-    rw[wave] = kw;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = kp;
+    red[1][threadIdx.x >> 6] = kw;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
-      kp = rp[k] > kp ? rp[k] : kp;
-      kw = rw[k] < kw ? rw[k] : kw;
-    }
-    if (kp) atomicMax(&d.guard->maxpos2, kp);
-    if (kw != ~0ull) atomicMin(&d.guard->minw, kw);
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < kCheckBlock / 64; k++) {
+    kp = red[0][k] > kp ? red[0][k] : kp;
+    kw = red[1][k] < kw ? red[1][k] : kw;
   }
-}
-
-// the end of a step of the integrate() loop (Rockable.cpp:2690-2740): a
-// needUpdate left by the previous check makes the rebuild due now (the step
-// kernels then wait for the host); then the check itself.  The step index
-// lives on the device, so that a captured sequence of steps replays as is.
-__global__ void k_guard_check(Dev d) {
-  if (threadIdx.x != 0 || frozen(d)) return;
   DevGuard* G = d.guard;
+  if (kp > G->maxpos2) G->maxpos2 = kp;
+  if (kw < G->minw) G->minw = kw;
   const long long step = G->step;
   G->step = step + 1;
   const int rebuild_now = G->nu;
@@ -1166,18 +1237,23 @@ __global__ void __launch_bounds__(64) k_guard_feat_final(Dev d, StepParams sp, c
   }
 }
 
-// Particle::GlobVertex (Particle.cpp:55-57) of every vertex of every body,
-// once per step: one thread per vertex.
-__global__ void __launch_bounds__(kBlock) k_glob(Dev d) {
+// Particle::GlobVertex (Particle.cpp:55-57): one thread per vertex of the
+// first nv vertices (every body's, or the driven bodies', which come first),
+// of every body (all) or of the bodies with slots in the current list (the
+// only ones the force phase reads; the list builder reads all).
+__global__ void __launch_bounds__(kBlock) k_glob(Dev d, int all, int nv) {
   if (frozen(d)) return;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.nvtot; k += gridDim.x * blockDim.x) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nv; k += gridDim.x * blockDim.x) {
     const int b = d.bvbody[k];
+    if (!all && d.soff[b + 1] == d.soff[b]) continue;
     const BodyKin& P = d.kin[b];
     st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, k - d.bvoff[b]));
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
+// glob: also the global vertices of the free bodies with listed slots, from
+// the new pose (the driven bodies' follow in k_glob)
+__global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp, int glob) {
   if (frozen(d)) return;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid == 0 && d.nctrl > 0) {
@@ -1320,6 +1396,17 @@ __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp) {
     out->wx = w2.x;
     out->wy = w2.y;
     out->wz = w2.z;
+    if (glob && d.soff[b + 1] != d.soff[b]) {
+      P.px = pos.x;
+      P.py = pos.y;
+      P.pz = pos.z;
+      P.qw = Q.w;
+      P.qx = Q.x;
+      P.qy = Q.y;
+      P.qz = Q.z;
+      const int v0 = d.bvoff[b], nv = d.bvoff[b + 1] - v0;
+      for (int v = 0; v < nv; v++) st3(d.gv + 3 * (size_t)(v0 + v), glob_vertex(d.sh, P, v));
+    }
   }
 }
 
@@ -1378,15 +1465,20 @@ int body_block(const Dev& d) { return d.body_block == 64 || d.body_block == 128 
 // workgroups for T tiles (persistent kernels loop over their tiles)
 dim3 grid_for(const Dev& d, long T) { return dim3(persistent_grid((int)std::max(1L, T), d.persist)); }
 
-void launch_predict(const Dev& d, const StepParams& sp, hipStream_t s) {
-  const int n = d.nb > 0 ? d.nb : 1, bs = body_block(d);
-  hipLaunchKernelGGL(k_predict, grid_for(d, (n + bs - 1) / bs), dim3(bs), 0, s, d, sp);
+void launch_glob(const Dev& d, int all, int nv, hipStream_t s) {
+  if (nv <= 0) return;
+  const int bs = body_block(d);
+  hipLaunchKernelGGL(k_glob, grid_for(d, (nv + bs - 1) / bs), dim3(bs), 0, s, d, all || !d.soff ? 1 : 0, nv);
 }
 
-void launch_glob(const Dev& d, hipStream_t s) {
-  if (d.nvtot <= 0) return;
-  const int bs = body_block(d);
-  hipLaunchKernelGGL(k_glob, grid_for(d, (d.nvtot + bs - 1) / bs), dim3(bs), 0, s, d);
+void launch_glob(const Dev& d, int all, hipStream_t s) { launch_glob(d, all, d.nvtot, s); }
+
+void launch_predict(const Dev& d, const StepParams& sp, int glob, hipStream_t s) {
+  const int n = d.nb > 0 ? d.nb : 1, bs = body_block(d);
+  const bool listed = glob && d.soff;     // else every body's vertices
+  const int fused = listed && d.fuse_glob;  // the free bodies' in k_predict
+  hipLaunchKernelGGL(k_predict, grid_for(d, (n + bs - 1) / bs), dim3(bs), 0, s, d, sp, fused);
+  launch_glob(d, listed ? 0 : 1, fused ? d.nvdriven : d.nvtot, s);
 }
 
 template <int BS>
@@ -1439,19 +1531,37 @@ void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hip
     hipLaunchKernelGGL(k_law<LAW_DEFAULT>, gl, bl, 0, s, d, sp, write_geom_all, 0, ngeom, W);
 }
 
-void launch_gather(const Dev& d, const StepParams& sp, int kick, hipStream_t s) {
-  if (d.nb <= 0) return;
-  const int per = d.gather_lanes == 8 ? 8 : 4;  // bodies per wavefront
-  const int waves = (d.nb + per - 1) / per + (d.nhch + per - 1) / per;
+// k_gather's workgroups: 64 / lanes x gather_groups bodies per body
+// wavefront, 64 / lanes heavy chunks per chunk wavefront, gather_waves
+// wavefronts per workgroup
+int gather_blocks(const Dev& d) {
+  const int per = d.gather_lanes == 8 ? 8 : 4;
+  const long nbw = (long)per * d.gather_groups;
+  const long waves = ((long)d.nb + nbw - 1) / nbw + ((long)d.nhch + per - 1) / per;
   const int wpb = d.gather_waves == 1 || d.gather_waves == 2 ? d.gather_waves : 4;
-  const dim3 blocks = grid_for(d, (waves + wpb - 1) / wpb);
-  if (d.gather_lanes == 8)
-    hipLaunchKernelGGL(k_gather<8>, blocks, dim3(64 * wpb), 0, s, d, sp, kick);
+  return (int)grid_for(d, (waves + wpb - 1) / wpb).x;
+}
+
+int gather_blocks_max(int nb, int nhch) { return (nb + 3) / 4 + (nhch + 3) / 4 + 8; }
+
+void launch_gather(const Dev& d, const StepParams& sp, int kick, int guard, hipStream_t s) {
+  if (d.nb <= 0) return;
+  guard = guard && kick;
+  const int wpb = d.gather_waves == 1 || d.gather_waves == 2 ? d.gather_waves : 4;
+  const dim3 blocks(gather_blocks(d));
+  const bool bf = d.bf != BF_NONE;
+  if (d.gather_lanes == 8 && bf)
+    hipLaunchKernelGGL((k_gather<8, true>), blocks, dim3(64 * wpb), 0, s, d, sp, kick, guard);
+  else if (d.gather_lanes == 8)
+    hipLaunchKernelGGL((k_gather<8, false>), blocks, dim3(64 * wpb), 0, s, d, sp, kick, guard);
+  else if (bf)
+    hipLaunchKernelGGL((k_gather<16, true>), blocks, dim3(64 * wpb), 0, s, d, sp, kick, guard);
   else
-    hipLaunchKernelGGL(k_gather<16>, blocks, dim3(64 * wpb), 0, s, d, sp, kick);
+    hipLaunchKernelGGL((k_gather<16, false>), blocks, dim3(64 * wpb), 0, s, d, sp, kick, guard);
   if (d.nheavy > 0) {
-    hipLaunchKernelGGL(k_heavy_finish, grid_for(d, d.nheavy), dim3(64), 0, s, d, sp, kick);
-    if (d.nheavy_exact > 0) hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick);
+    hipLaunchKernelGGL(k_heavy_finish, grid_for(d, d.nheavy), dim3(64), 0, s, d, sp, kick, guard);
+    if (d.nheavy_exact > 0)
+      hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick, guard);
   }
 }
 
@@ -1558,10 +1668,8 @@ __global__ void k_guard_set_step(Dev d, long long step) {
   if (threadIdx.x == 0) d.guard->step = step;
 }
 
-void launch_guard(const Dev& d, const StepParams& sp, hipStream_t s) {
-  const int bs = body_block(d);
-  if (d.nb > 0) hipLaunchKernelGGL(k_guard_acc, grid_for(d, (d.nb + bs - 1) / bs), dim3(bs), 0, s, d, sp);
-  hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(64), 0, s, d);
+void launch_guard(const Dev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(kCheckBlock), 0, s, d, d.nb > 0 ? gather_blocks(d) : 0);
 }
 
 void launch_guard_set_step(const Dev& d, long long step, hipStream_t s) {

@@ -186,7 +186,8 @@ class Engine:
         self._check(self.L.rk_get_option(self.h, name.encode(), ctypes.byref(v)))
         return v.value
 
-    TUNED = ("fuse_law", "law_group", "geom_block", "gather_lanes", "gather_waves", "body_block", "persistent",
+    TUNED = ("fuse_law", "law_group", "geom_block", "gather_lanes", "gather_groups", "fuse_glob", "gather_waves",
+             "body_block", "persistent",
              "graph_steps")
 
     def autotune(self, steps_per_trial=100):

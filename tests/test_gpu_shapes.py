@@ -18,7 +18,10 @@ SHAPES = [dict(fuse_law=0, law_group=1, geom_block=64, gather_waves=1, body_bloc
           dict(fuse_law=1, gather_waves=2, body_block=64, graph_steps=4),
           dict(fuse_law=1, gather_lanes=8, gather_waves=1),
           dict(fuse_law=1, persistent=8, gather_lanes=8),
-          dict(fuse_law=0, persistent=16, law_group=2, geom_block=128, graph_steps=4)]
+          dict(fuse_law=0, persistent=16, law_group=2, geom_block=128, graph_steps=4),
+          dict(gather_lanes=16, gather_groups=16, fuse_glob=1, gather_waves=1),
+          dict(gather_lanes=8, gather_groups=4, fuse_glob=1, persistent=8, graph_steps=4),
+          dict(gather_lanes=8, gather_groups=16, fuse_glob=0, gather_waves=2)]
 
 
 def _hull_deck(tmp_path):
@@ -49,7 +52,8 @@ def test_bad_launch_shape_rejected(tmp_path):
     from rockable_amd.engine import Engine
     e = Engine(_hull_deck(tmp_path))
     for name, v in (("geom_block", 96), ("law_group", 3), ("gather_waves", 8), ("body_block", 512),
-                    ("gather_lanes", 4), ("fuse_law", 2), ("model_scale", 0.0), ("persistent", 7)):
+                    ("gather_lanes", 4), ("fuse_law", 2), ("model_scale", 0.0), ("persistent", 7),
+                    ("gather_groups", 3), ("fuse_glob", 2)):
         with pytest.raises(RuntimeError):
             e.set_option(name, v)
 
