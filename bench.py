@@ -48,9 +48,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per force-phase launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
-# (profiles/): None until measured for the current kernels
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per force-phase launch (k_interact + k_gather + k_heavy_finish on
+# the 50 005-body settled pour): rocprofv3 FETCH_SIZE and WRITE_SIZE, each
+# scaled by what it reads for a plain 256 MiB device copy (factors 2.0 and
+# 1.0), 152.9 MB read + 36.4 MB written; profiles/c2_force_phase_traffic.md
+TRAFFIC_BYTES_PER_LAUNCH = 189.3e6
 
 
 def make_settled_deck(n_bodies, seed, dirpath, settle_steps, relax_steps, log):
@@ -461,7 +463,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "kernel": "force phase (k_geom + k_law + k_gather + k_gather_heavy)",
+                     "kernel": "force phase (k_interact, or k_geom + k_law, then k_gather + k_heavy_finish)",
                      "algorithmic_bytes_per_launch": bytes_per},
     }
     e.close()

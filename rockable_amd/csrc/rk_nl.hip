@@ -443,8 +443,8 @@ struct NarrowCfg {
   int maxwords;   // max 64-bit bitmask words per pair
   double dmax;    // dVerlet
   // the culls' boxes of every body's edges and faces (k_gate_boxes)
-  const double* gedge;
-  const double* gface;
+  const float* gedge;  // kGate floats per edge / face
+  const float* gface;
 };
 
 __device__ __forceinline__ v3 lds_v(const double* a, int k) { return V(a[3 * k], a[3 * k + 1], a[3 * k + 2]); }
@@ -568,8 +568,25 @@ __device__ __forceinline__ bool box_meets(const double lo[3], const double hi[3]
 // per body and rebuild instead of once per candidate pair: an edge's box and
 // the L1 length of its box; a face's box and its thickness along its normal
 // (the face normal from its first two edges, as the predicates take it).
-// 7 doubles each, body b's edges from beoff[b], its faces from bfoff[b].
-__global__ void k_gate_boxes(Dev d, double* gedge, double* gface) {
+// Stored as 8 floats each (lo[3], hi[3], length or thickness, unused),
+// rounded outwards: lo down, hi and the growth terms up, so that the culls
+// stay conservative at half the bytes (the narrow phase reads them once per
+// candidate pair).  Body b's edges from beoff[b], its faces from bfoff[b].
+constexpr int kGate = 8;
+__device__ __forceinline__ void store_gate(float* o, const double lo[3], const double hi[3], double x) {
+  float4 a, b;
+  a.x = __double2float_rd(lo[0]);
+  a.y = __double2float_rd(lo[1]);
+  a.z = __double2float_rd(lo[2]);
+  a.w = __double2float_ru(hi[0]);
+  b.x = __double2float_ru(hi[1]);
+  b.y = __double2float_ru(hi[2]);
+  b.z = __double2float_ru(x);
+  b.w = 0.0f;
+  reinterpret_cast<float4*>(o)[0] = a;
+  reinterpret_cast<float4*>(o)[1] = b;
+}
+__global__ void k_gate_boxes(Dev d, float* gedge, float* gface) {
   const int b = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   if (b >= d.nb) return;
@@ -613,14 +630,8 @@ __global__ void k_gate_boxes(Dev d, double* gedge, double* gface) {
         x = fmax(x, fabs(vdot(vsub(p, A), n)));
       }
     }
-    double* o = k < ne ? gedge + 7 * ((size_t)d.beoff[b] + k) : gface + 7 * ((size_t)d.bfoff[b] + (k - ne));
-    o[0] = lo[0];
-    o[1] = lo[1];
-    o[2] = lo[2];
-    o[3] = hi[0];
-    o[4] = hi[1];
-    o[5] = hi[2];
-    o[6] = x;
+    store_gate(k < ne ? gedge + kGate * ((size_t)d.beoff[b] + k) : gface + kGate * ((size_t)d.bfoff[b] + (k - ne)),
+               lo, hi, x);
   }
 }
 
@@ -630,16 +641,18 @@ __global__ void k_gate_boxes(Dev d, double* gedge, double* gface) {
 // VertexIsNearEdge's r in [-dmax, 1 + dmax]) misses the other body's box has
 // no test to pass; VertexIsNearFace needs the projection inside the face's
 // prism (box grown by twice the face's thickness).
-__device__ __forceinline__ bool edge_gate_pre(const double* g, double sum, double dmax, double margin,
+__device__ __forceinline__ bool edge_gate_pre(const float* g, double sum, double dmax, double margin,
                                               const double* blo, const double* bhi) {
-  const double lo[3] = {g[0], g[1], g[2]}, hi[3] = {g[3], g[4], g[5]};
-  return box_meets(lo, hi, sum + dmax * g[6] + margin, blo, bhi);
+  const float4 a = reinterpret_cast<const float4*>(g)[0], b = reinterpret_cast<const float4*>(g)[1];
+  const double lo[3] = {a.x, a.y, a.z}, hi[3] = {a.w, b.x, b.y};
+  return box_meets(lo, hi, sum + dmax * (double)b.z + margin, blo, bhi);
 }
-__device__ __forceinline__ bool face_gate_pre(const double* g, double sum, double margin, const double* blo,
+__device__ __forceinline__ bool face_gate_pre(const float* g, double sum, double margin, const double* blo,
                                               const double* bhi) {
-  const double thick = g[6];
+  const float4 a = reinterpret_cast<const float4*>(g)[0], b = reinterpret_cast<const float4*>(g)[1];
+  const double thick = b.z;
   if (!(thick == thick)) return true;  // degenerate face: no cull
-  const double lo[3] = {g[0], g[1], g[2]}, hi[3] = {g[3], g[4], g[5]};
+  const double lo[3] = {a.x, a.y, a.z}, hi[3] = {a.w, b.x, b.y};
   return box_meets(lo, hi, sum + 2.0 * thick + margin, blo, bhi);
 }
 
@@ -721,15 +734,15 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
   // edges and faces of both bodies in one pass; tag = kind * 2^20 + index,
   // kinds 0 edge of i, 1 edge of j, 2 face of i, 3 face of j
   const int ne2 = nei + nej, nef = ne2 + nfi + nfj;
-  const double* gei = cfg.gedge + 7 * (size_t)d.beoff[i];
-  const double* gej = cfg.gedge + 7 * (size_t)d.beoff[j];
-  const double* gfi = cfg.gface + 7 * (size_t)d.bfoff[i];
-  const double* gfj = cfg.gface + 7 * (size_t)d.bfoff[j];
+  const float* gei = cfg.gedge + kGate * (size_t)d.beoff[i];
+  const float* gej = cfg.gedge + kGate * (size_t)d.beoff[j];
+  const float* gfi = cfg.gface + kGate * (size_t)d.bfoff[i];
+  const float* gfj = cfg.gface + kGate * (size_t)d.bfoff[j];
   const int nefg = wave_compact(nef, lane, ef_l, [&](int q) {
-    if (q < nei) return edge_gate_pre(gei + 7 * q, sum, dmax, margin, bx + 6, bx + 9);
-    if (q < ne2) return edge_gate_pre(gej + 7 * (q - nei), sum, dmax, margin, bx, bx + 3);
-    if (q < ne2 + nfi) return face_gate_pre(gfi + 7 * (q - ne2), sum, margin, bx + 6, bx + 9);
-    return face_gate_pre(gfj + 7 * (q - ne2 - nfi), sum, margin, bx, bx + 3);
+    if (q < nei) return edge_gate_pre(gei + kGate * q, sum, dmax, margin, bx + 6, bx + 9);
+    if (q < ne2) return edge_gate_pre(gej + kGate * (q - nei), sum, dmax, margin, bx, bx + 3);
+    if (q < ne2 + nfi) return face_gate_pre(gfi + kGate * (q - ne2), sum, margin, bx + 6, bx + 9);
+    return face_gate_pre(gfj + kGate * (q - ne2 - nfi), sum, margin, bx, bx + 3);
   });
   // vertex-vertex partners: vertices of j within sum of i's box
   const double gvv = sum + margin;
@@ -1388,8 +1401,8 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     cfg.maxwords = p.maxwords;
     cfg.dmax = p.dVerlet;
     // the bodies' edge and face boxes for the culls
-    if (ensure(w, w.gedge, w.gedge_cap, (size_t)7 * std::max(d.netot, 1)) ||
-        ensure(w, w.gface, w.gface_cap, (size_t)7 * std::max(d.nftot, 1)))
+    if (ensure(w, w.gedge, w.gedge_cap, (size_t)kGate * std::max(d.netot, 1)) ||
+        ensure(w, w.gface, w.gface_cap, (size_t)kGate * std::max(d.nftot, 1)))
       return -1;
     hipLaunchKernelGGL(k_gate_boxes, dim3((d.nb + 3) / 4), dim3(kBlock), 0, s, d, w.gedge, w.gface);
     cfg.gedge = w.gedge;

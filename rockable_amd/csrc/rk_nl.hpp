@@ -77,8 +77,8 @@ struct NLWork {
   size_t name##_cap = 0;
   RK_BUF(double, bobb)
   RK_BUF(double, baabb)
-  RK_BUF(double, gedge)
-  RK_BUF(double, gface)
+  RK_BUF(float, gedge)
+  RK_BUF(float, gface)
   RK_BUF(int32_t, kout)
   RK_BUF(int, counter)
   RK_BUF(uint64_t, cand)
