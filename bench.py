@@ -34,8 +34,9 @@ independent sphere packings in one engine, autotuned launch shapes and
 graph-captured steps, with a sample of the same packings run one engine
 after another) and "avalanche" (a 1,000,000-body polyhedral avalanche with
 the displacement-check rebuild: the plain GPU form against the batched +
-learned-rebuild form, timed on the same steps of the same trajectory).  --forms ""
-leaves them out.
+learned-rebuild form, timed on the same steps of the same trajectory) and
+"bonds" (200,000 sphero-cubes in 2,000 bonded clusters hitting a plane, bonds
+breaking on impact).  --forms "" leaves them out.
 """
 import argparse
 import json
@@ -241,6 +242,49 @@ def batched_form(a, log):
                                  "note": "the same packings, one engine after another, default launch shapes"}}
 
 
+def bonds_form(a, log):
+    """Breakable-bonds workload: a.bond_clusters clusters of 5 x 5 x 4
+    sphero-cubes (5 mm, Minkowski radius 0.25 mm) bonded face to face at
+    their coincident vertices (one StickedLinks interface per touching pair,
+    fn0 and ft0 lognormal with mean 50 N and cv 0.3, ParamsInInterfaces 1),
+    0.1 mm above a rigid plane inclined at 35 deg and falling onto it at
+    2 m/s, dt 1e-7 s, linkCells.  The launch shapes are autotuned while the
+    clusters fall, then a.bond_steps steps through the impact are timed;
+    the bond breakage is accounted on the host at the step it happens, as
+    the reference does."""
+    from rockable_amd import decks as D
+    from rockable_amd.engine import Engine
+    d = tempfile.mkdtemp(prefix="rk_bonds_")
+    t0 = time.perf_counter()
+    shapes, deck = D.bonded_clusters_deck(n_clusters=a.bond_clusters, clearance=1e-4)
+    path = D.write_deck(d, shapes, deck)
+    log("bonded-cluster deck: %d clusters, written in %.1f s" % (a.bond_clusters, time.perf_counter() - t0))
+    e = Engine(path, device=0)
+    e.set_option("record_geometry", 0)
+    e.update_nl()
+    ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
+    log("autotune: %d steps, %s" % (ntune, choice))
+    c0 = e.counts()
+    e.synchronize()
+    t1 = time.perf_counter()
+    e.step(a.bond_steps)
+    e.synchronize()
+    wall = time.perf_counter() - t1
+    c1 = e.counts()
+    st = e.stats()
+    nb = c1["nbodies"]
+    e.close()
+    return {"value": nb * a.bond_steps / wall, "unit": "body-steps/s", "ms_per_step": 1e3 * wall / a.bond_steps,
+            "steps": a.bond_steps, "bodies": nb, "autotuned": choice, "autotune_steps": ntune,
+            "interfaces_before": c0["ninterfaces"], "interfaces_after": c1["ninterfaces"],
+            "broken_area": c1["total_broken_area"], "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
+            "active_per_body": c1["nactive"] / nb, "kinetic_energy": st["kinetic_translational"] +
+            st["kinetic_rotational"],
+            "workload": "%d clusters of 5x5x4 bonded sphero-cubes (%d bodies) falling at 2 m/s onto a 35 deg "
+                        "plane, StickedLinks, dt 1e-7 s; %d steps (autotuning) untimed, then %d steps timed "
+                        "through the impact" % (a.bond_clusters, nb, ntune, a.bond_steps)}
+
+
 def avalanche_forms(a, log):
     """Learned-rebuild workload: a.avalanche_bodies of the plain GPU form's
     hull polyhedra packed on the floor of a 400 x 100-size box and sent down
@@ -346,7 +390,9 @@ def main():
     ap.add_argument("--avalanche-steps", type=int, default=3000, help="timed steps per form")
     ap.add_argument("--avalanche-warmup", type=int, default=2000)
     ap.add_argument("--tune-trial", type=int, default=50)
-    ap.add_argument("--forms", default="batched,avalanche",
+    ap.add_argument("--bond-clusters", type=int, default=2000)
+    ap.add_argument("--bond-steps", type=int, default=2000, help="timed steps of the bonds form")
+    ap.add_argument("--forms", default="batched,avalanche,bonds",
                     help="extra forms reported beside the plain one (comma list; empty for none)")
     ap.add_argument("--prepare-only", action="store_true",
                     help="settle the packing into --deck-dir and stop (later runs reuse it)")
@@ -481,6 +527,8 @@ def main():
         out["forms"]["batched"] = batched_form(a, log)
     if "avalanche" in forms:
         out["forms"]["avalanche"] = avalanche_forms(a, log)
+    if "bonds" in forms:
+        out["forms"]["bonds"] = bonds_form(a, log)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(path, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(out), flush=True)

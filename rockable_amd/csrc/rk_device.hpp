@@ -157,6 +157,7 @@ struct Dev {
   int32_t* ei;
   int32_t* ej;
   uint32_t* esub;
+  int4* edesc;            // per entry: resolved vertex indices of its geometry (k_entry_desc)
   EState* est;
   double* edn;
   EGeom* egeo;
@@ -269,6 +270,8 @@ inline int persistent_grid(int T, int persist) {
 }
 
 // rk_step.hip
+// the entries' resolved vertex indices, after every change of the list
+void launch_entry_desc(const Dev& d, hipStream_t s);
 // the predictor; then the global vertices of the new poses: of the listed
 // bodies (glob), else of every body
 void launch_predict(const Dev& d, const StepParams& sp, int glob, hipStream_t s);

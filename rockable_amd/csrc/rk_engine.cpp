@@ -223,6 +223,7 @@ void free_device(rk_engine* eng) {
     if (E.hch) (void)hipFree(E.hch);
     if (E.hch_off) (void)hipFree(E.hch_off);
     if (E.hpart) (void)hipFree(E.hpart);
+    if (E.edesc) (void)hipFree(E.edesc);
     E = EntryArrays();
   }
   nl_work_free(eng->w);
@@ -301,6 +302,7 @@ void assemble_dev(rk_engine* eng) {
   d.ei = E.ei;
   d.ej = E.ej;
   d.esub = E.esub;
+  d.edesc = E.edesc;
   d.est = E.est;
   d.edn = E.edn;
   d.egeo = E.egeo;
@@ -689,6 +691,7 @@ int upload_model(rk_engine* eng) {
   eng->nlp.boxForLinkCellsOpt = m.boxForLinkCellsOpt;
   if (ensure_gpart(eng) != 0) return -1;
   assemble_dev(eng);
+  launch_entry_desc(eng->d, eng->stream);
   HIPCHK(hipStreamSynchronize(eng->stream));
   return 0;
 }
@@ -795,6 +798,7 @@ int do_update_nl(rk_engine* eng) {
   eng->cur = nxt;
   if (ensure_gpart(eng) != 0) return -1;
   assemble_dev(eng);
+  launch_entry_desc(eng->d, eng->stream);
   eng->nl_updates++;
   eng->last_interval = eng->steps - eng->last_rebuild_step;
   eng->last_rebuild_step = eng->steps;

@@ -1210,7 +1210,8 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipS
   const int n = E.n;
   if (ensure(w, E.istart, E.istart_cap, (size_t)nb + 1) || ensure(w, E.jstart, E.jstart_cap, (size_t)nb + 1) ||
       ensure(w, E.jlist, E.jlist_cap, (size_t)(n > 0 ? n : 1)) || ensure(w, w.i32a, w.i32a_cap, (size_t)nb + 1) ||
-      ensure(w, w.i32b, w.i32b_cap, (size_t)(n > 0 ? n : 1)) || ensure(w, w.i32c, w.i32c_cap, (size_t)(n > 0 ? n : 1)))
+      ensure(w, w.i32b, w.i32b_cap, (size_t)(n > 0 ? n : 1)) || ensure(w, w.i32c, w.i32c_cap, (size_t)(n > 0 ? n : 1)) ||
+      ensure(w, E.edesc, E.edesc_cap, (size_t)(n > 0 ? n : 1)))
     return -1;
   // istart: counts by owner, exclusive scan over nb + 1 (trailing zero)
   RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));

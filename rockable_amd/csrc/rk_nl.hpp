@@ -28,6 +28,8 @@ struct EntryArrays {
   double* sval = nullptr;    // 12 per entry
   ActiveKin* akin = nullptr; // 1 per entry (+512)
   int32_t* acount = nullptr; // 1 per geometry block (64 entries or more)
+  int4* edesc = nullptr;     // 1 per entry: vertex indices of its geometry
+  size_t edesc_cap = 0;
   int32_t* istart = nullptr;
   size_t istart_cap = 0;
   int32_t* jstart = nullptr;

@@ -84,16 +84,17 @@ struct Kin {
 // The global vertices of a body, computed once per step by k_glob.
 __device__ __forceinline__ v3 gvtx(const double* g, int v) { return V(g[3 * v], g[3 * v + 1], g[3 * v + 2]); }
 
-// even-odd crossing test of P against face f of body j (Interaction.cpp:236-253)
-__device__ bool face_crossing(const DevShapes& sh, const double* gj, int gface, int nbv, v3 P, v3 v1, v3 v2) {
+// even-odd crossing test of P against a face of body j (Interaction.cpp:236-253):
+// its nbv vertices are gv[bj + fidx[fs + k]]
+__device__ bool face_crossing(const Dev& d, int fs, int nbv, int bj, v3 P, v3 v1, v3 v2) {
   int odd = 0;
   double ori1 = vdot(P, v1), ori2 = vdot(P, v2);
-  const int* fv = sh.fidx + sh.fstart[gface];
-  v3 A = gvtx(gj, fv[0]);
+  const int* fv = d.sh.fidx + fs;
+  v3 A = gvtx(d.gv, bj + fv[0]);
   for (int iva = 0; iva < nbv; iva++) {
     int ivb = iva + 1;
     if (ivb == nbv) ivb = 0;
-    v3 B = gvtx(gj, fv[ivb]);
+    v3 B = gvtx(d.gv, bj + fv[ivb]);
     double pa1 = vdot(A, v1), pb1 = vdot(B, v1);
     double pa2 = vdot(A, v2), pb2 = vdot(B, v2);
     if ((pa2 < ori2 && pb2 >= ori2) || (pb2 < ori2 && pa2 >= ori2)) {
@@ -104,24 +105,56 @@ __device__ bool face_crossing(const DevShapes& sh, const double* gj, int gface, 
   return odd != 0;
 }
 
-// Interaction::UpdateDispatcher (Interaction.cpp:147-326)
+// Per entry, the global-vertex indices (and face range) its contact geometry
+// reads, resolved once per list change rather than through the bodies'
+// shapes at every step (which costs a chain of dependent loads):
+//   VV (vi, vj, -, -); VE (vi, j's edge ends); EE (i's edge ends, j's edge
+//   ends); VF (vi, start of j's face in sh.fidx, its vertex count, bvoff[j]).
+__global__ void __launch_bounds__(kBlock) k_entry_desc(Dev d) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.ne; e += gridDim.x * blockDim.x) {
+    const int i = d.ei[e], j = d.ej[e];
+    const uint32_t sub = d.esub[e];
+    const int type = (int)(sub >> 30), isub = (int)((sub >> 15) & 0x7fffu), jsub = (int)(sub & 0x7fffu);
+    const int bi = d.bvoff[i], bj = d.bvoff[j];
+    const int shi = d.kin[i].shape, shj = d.kin[j].shape;
+    int4 q = make_int4(bi + isub, 0, 0, 0);
+    if (type == VV) {
+      q.y = bj + jsub;
+    } else if (type == VE) {
+      const int* ed = d.sh.edge + 2 * (d.sh.eoff[shj] + jsub);
+      q.y = bj + ed[0];
+      q.z = bj + ed[1];
+    } else if (type == VF) {
+      const int gface = d.sh.foff[shj] + jsub;
+      q.y = d.sh.fstart[gface];
+      q.z = d.sh.fstart[gface + 1] - q.y;
+      q.w = bj;
+    } else {
+      const int* edi = d.sh.edge + 2 * (d.sh.eoff[shi] + isub);
+      const int* edj = d.sh.edge + 2 * (d.sh.eoff[shj] + jsub);
+      q = make_int4(bi + edi[0], bi + edi[1], bj + edj[0], bj + edj[1]);
+    }
+    d.edesc[e] = q;
+  }
+}
+
+// Interaction::UpdateDispatcher (Interaction.cpp:147-326), on the entry's
+// resolved vertex indices q (k_entry_desc).
 // WITH_VEL: also the relative velocity at the contact point (Interaction.cpp:162).
 template <bool WITH_VEL>
-__device__ void update_kinematics(const DevShapes& sh, int type, int isub, int jsub, const BodyKin& Pi,
-                                  const BodyKin& Pj, const double* gi, const double* gj, Kin& k) {
+__device__ void update_kinematics(const Dev& d, int type, int4 q, const BodyKin& Pi, const BodyKin& Pj, Kin& k) {
   const double Ri = Pi.R, Rj = Pj.R;
   if (type == VV) {
-    v3 posi = gvtx(gi, isub);
-    v3 posj = gvtx(gj, jsub);
+    v3 posi = gvtx(d.gv, q.x);
+    v3 posj = gvtx(d.gv, q.y);
     k.n = vsub(posi, posj);
     k.dn = vnormalize(&k.n) - (Ri + Rj);
     k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
     if (WITH_VEL) k.vel = rel_vel(k.pos, Pi, Pj);
   } else if (type == VE) {
-    const int* ed = sh.edge + 2 * (sh.eoff[Pj.shape] + jsub);
-    v3 posj1 = gvtx(gj, ed[0]);
-    v3 posj2 = gvtx(gj, ed[1]);
-    v3 posi = gvtx(gi, isub);
+    v3 posj1 = gvtx(d.gv, q.y);
+    v3 posj2 = gvtx(d.gv, q.z);
+    v3 posi = gvtx(d.gv, q.x);
     v3 E = vsub(posj2, posj1);
     v3 v = vsub(posi, posj1);
     double r = vdot(v, E) / vdot(E, E);
@@ -134,14 +167,12 @@ __device__ void update_kinematics(const DevShapes& sh, int type, int isub, int j
     k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
     if (WITH_VEL) k.vel = rel_vel(k.pos, Pi, Pj);
   } else if (type == VF) {
-    const int gface = sh.foff[Pj.shape] + jsub;
-    const int fs = sh.fstart[gface];
-    const int nbv = sh.fstart[gface + 1] - fs;
-    const int* fv = sh.fidx + fs;
-    v3 A = gvtx(gj, fv[0]);
-    v3 B = gvtx(gj, fv[1]);
-    v3 C = gvtx(gj, fv[nbv - 1]);
-    v3 posi = gvtx(gi, isub);
+    const int fs = q.y, nbv = q.z, bj = q.w;
+    const int* fv = d.sh.fidx + fs;
+    v3 A = gvtx(d.gv, bj + fv[0]);
+    v3 B = gvtx(d.gv, bj + fv[1]);
+    v3 C = gvtx(d.gv, bj + fv[nbv - 1]);
+    v3 posi = gvtx(d.gv, q.x);
     v3 v = vsub(posi, A);
     v3 v1 = vsub(B, A);
     vnormalize(&v1);
@@ -155,7 +186,7 @@ __device__ void update_kinematics(const DevShapes& sh, int type, int isub, int j
     }
     v3 P = vsub(posi, vsc(dist, k.n));
     v2 = vcross(k.n, v1);
-    if (face_crossing(sh, gj, gface, nbv, P, v1, v2)) {
+    if (face_crossing(d, fs, nbv, bj, P, v1, v2)) {
       k.dn = dist - (Ri + Rj);
       k.pos = vsub(posi, vsc(Ri + 0.5 * k.dn, k.n));
       if (WITH_VEL) k.vel = rel_vel(k.pos, Pi, Pj);
@@ -163,21 +194,19 @@ __device__ void update_kinematics(const DevShapes& sh, int type, int isub, int j
       k.deact = true;  // n keeps the new face normal, pos and vel stay stale
     }
   } else {
-    const int* edi = sh.edge + 2 * (sh.eoff[Pi.shape] + isub);
-    const int* edj = sh.edge + 2 * (sh.eoff[Pj.shape] + jsub);
-    v3 posi1 = gvtx(gi, edi[0]);
-    v3 posi2 = gvtx(gi, edi[1]);
-    v3 posj1 = gvtx(gj, edj[0]);
-    v3 posj2 = gvtx(gj, edj[1]);
+    v3 posi1 = gvtx(d.gv, q.x);
+    v3 posi2 = gvtx(d.gv, q.y);
+    v3 posj1 = gvtx(d.gv, q.z);
+    v3 posj2 = gvtx(d.gv, q.w);
     v3 Ei = vsub(posi2, posi1);
     v3 Ej = vsub(posj2, posj1);
     v3 v = vsub(posi1, posj1);
-    double c = vdot(Ei, Ei), d = vdot(Ej, Ej), e = vdot(Ei, Ej);
-    double f = (c * d) - (e * e);
+    double c = vdot(Ei, Ei), dd = vdot(Ej, Ej), e = vdot(Ei, Ej);
+    double f = (c * dd) - (e * e);
     if (fabs(f) > 1.0e-12) {
       f = 1.0 / f;
       double a = vdot(Ei, v), b = vdot(Ej, v);
-      double s = (e * b - a * d) * f;
+      double s = (e * b - a * dd) * f;
       double t = (c * b - e * a) * f;
       if (s <= 0.0 || s >= 1.0 || t <= 0.0 || t >= 1.0) {
         k.deact = true;
@@ -383,8 +412,7 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
   bool act = false;
   if (e < d.ne) {
     const int i = d.ei[e], j = d.ej[e];
-    const uint32_t sub = d.esub[e];
-    const int type = (int)(sub >> 30), isub = (int)((sub >> 15) & 0x7fffu), jsub = (int)(sub & 0x7fffu);
+    const int type = (int)(d.esub[e] >> 30);
     const BodyKin& Pi = d.kin[i];
     const BodyKin& Pj = d.kin[j];
     const int stick = d.ifc.n > 0 ? d.estick[e] : -1;  // bonds exist only with interfaces
@@ -396,8 +424,7 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
       k.n = V(g.nx, g.ny, g.nz);
       k.pos = V(g.px, g.py, g.pz);
     }
-    update_kinematics<false>(d.sh, type, isub, jsub, Pi, Pj, d.gv + 3 * (size_t)d.bvoff[i],
-                             d.gv + 3 * (size_t)d.bvoff[j], k);
+    update_kinematics<false>(d, type, d.edesc[e], Pi, Pj, k);
     if (k.deact) {  // Interaction::deactivate (Interaction.cpp:123-128)
       k.dn = 0.0;
       k.prev_dn = 0.0;
@@ -725,12 +752,11 @@ __device__ __forceinline__ double pick(const v3& F, const v3& M, int c) {
 
 // Stores the resultants, compute_accelerations_from_resultants
 // (Rockable.cpp:3419-3552) and the closing half kick (2120-2155).
+// m, I: the body's mass and principal inertia (loaded by the caller, early)
 __device__ __forceinline__ void finish_body(const Dev& d, const StepParams& sp, int kick, int b, BodyKin& P,
-                                            bool driven, v3 F, v3 M, int c0, int c1) {
+                                            bool driven, v3 F, v3 M, int c0, int c1, double m, v3 I) {
   st3(d.force + 3 * (size_t)b, F);
   st3(d.moment + 3 * (size_t)b, M);
-  const double m = d.mass[b];
-  const v3 I = ld3(d.inertia + 3 * (size_t)b);
   const qt Q = kq(P);
   v3 acc, arot;
   if (driven) {
@@ -944,11 +970,17 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
       run = (k1 - k0) <= d.heavy_min;
     }
     v3 F = V(0.0, 0.0, 0.0), M = V(0.0, 0.0, 0.0);
+    BodyKin P;
+    double mb = 0.0;
+    v3 Ib = V(0.0, 0.0, 0.0);
     if (run) {
-      // the body's state is read here only where a body force needs it, and
-      // in full after the sums (fewer registers live across them)
+      // the body's state, mass and inertia are loaded before the sums, so
+      // that their latency overlaps the slot loads
+      P = d.kin[b];
+      mb = d.mass[b];
+      Ib = ld3(d.inertia + 3 * (size_t)b);
       driven = b < d.nDriven;
-      init_resultants<BF>(d, b, d.kin[b], driven, F, M, c0, c1);
+      init_resultants<BF>(d, b, P, driven, F, M, c0, c1);
     }
     const unsigned long long busy = __ballot(run && k1 > k0);
     for (int g = 0; g < G; g++) {
@@ -971,8 +1003,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kic
       }
     }
     if (run) {
-      BodyKin P = d.kin[b];
-      finish_body(d, sp, kick, b, P, driven, F, M, c0, c1);
+      finish_body(d, sp, kick, b, P, driven, F, M, c0, c1, mb, Ib);
       if (guard) guard_body(d, sp, b, P, rst, ga);
     }
   }
@@ -1036,7 +1067,8 @@ __global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int k
   v3 F, M;
   int c0, c1;
   init_resultants(d, b, P, driven, F, M, c0, c1);
-  finish_body(d, sp, kick, b, P, driven, vadd(F, V(a[0], a[1], a[2])), vadd(M, V(a[3], a[4], a[5])), c0, c1);
+  finish_body(d, sp, kick, b, P, driven, vadd(F, V(a[0], a[1], a[2])), vadd(M, V(a[3], a[4], a[5])), c0, c1,
+              d.mass[b], ld3(d.inertia + 3 * (size_t)b));
   if (guard) guard_heavy(d, sp, b, P);
   }
 }
@@ -1094,7 +1126,8 @@ __global__ void __launch_bounds__(kBlock) k_gather_heavy(Dev d, StepParams sp, i
     const double fx = __shfl(s, 0), fy = __shfl(s, 1), fz = __shfl(s, 2);
     const double mx = __shfl(s, 3), my = __shfl(s, 4), mz = __shfl(s, 5);
     if (lane == 0) {
-      finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1);
+      finish_body(d, sp, kick, b, P, driven, V(fx, fy, fz), V(mx, my, mz), c0, c1, d.mass[b],
+                  ld3(d.inertia + 3 * (size_t)b));
       if (guard) guard_heavy(d, sp, b, P);
     }
   }
@@ -1472,6 +1505,11 @@ void launch_glob(const Dev& d, int all, int nv, hipStream_t s) {
 }
 
 void launch_glob(const Dev& d, int all, hipStream_t s) { launch_glob(d, all, d.nvtot, s); }
+
+void launch_entry_desc(const Dev& d, hipStream_t s) {
+  if (d.ne <= 0 || !d.edesc) return;
+  hipLaunchKernelGGL(k_entry_desc, dim3((d.ne + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d);
+}
 
 void launch_predict(const Dev& d, const StepParams& sp, int glob, hipStream_t s) {
   const int n = d.nb > 0 ? d.nb : 1, bs = body_block(d);

@@ -652,6 +652,25 @@ def bonded_clusters_deck(n_clusters=2000, cluster_dims=(5, 5, 4), side=5e-3, rad
     s2 = np.log(1.0 + strength_cv ** 2)
     mu_ln = np.log(strength_mean) - 0.5 * s2
     eps = 0.01 * r
+    # the bonds of one cluster (every cluster has the same): touching pairs
+    # (a, b) in order, and per pair its vertex pairs (iv, jv) closer than
+    # 2 r + eps
+    tpl = []
+    for a in range(len(local)):
+        for b in range(a + 1, len(local)):
+            if np.abs(local[a] - local[b]).sum() > pitch * 1.5:
+                continue
+            gi, gj = local[a] + vtx, local[b] + vtx
+            vv = [(iv, int(jv)) for iv in range(8)
+                  for jv in np.nonzero(np.linalg.norm(gj - gi[iv], axis=1) < eps + 2 * r)[0]]
+            if vv:
+                tpl.append((a, b, vv))
+    ta = np.array([a for a, b, vv in tpl for _ in vv])
+    tb = np.array([b for a, b, vv in tpl for _ in vv])
+    tiv = np.array([iv for a, b, vv in tpl for iv, jv in vv])
+    tjv = np.array([jv for a, b, vv in tpl for iv, jv in vv])
+    tbonds = [[(0, iv, jv) for iv, jv in vv] for a, b, vv in tpl]
+    zeros = (0.0, 0.0, 0.0) + (0.0,) + (0.0,) * 6 + (damp,)
     for c in range(n_clusters):
         x0 = (c % ncx) * (cl[0] + cluster_gap) - 0.5 * span[0]
         y0 = (c // ncx) * (cl[1] + cluster_gap) - 0.5 * span[1]
@@ -662,27 +681,20 @@ def bonded_clusters_deck(n_clusters=2000, cluster_dims=(5, 5, 4), side=5e-3, rad
         pos = np.array([x0, y0, z0]) + local
         for p in pos:
             bodies.append(Body("CUBE", group=0, cluster=c, pos=tuple(p), vel=(0.0, 0.0, -speed)))
-        for a in range(len(pos)):
-            for b in range(a + 1, len(pos)):
-                if np.abs(pos[a] - pos[b]).sum() > pitch * 1.5:
-                    continue
-                gi, gj = pos[a] + vtx, pos[b] + vtx
-                bonds = []
-                for iv in range(8):
-                    d = np.linalg.norm(gj - gi[iv], axis=1)
-                    for jv in np.nonzero(d < eps + 2 * r)[0]:
-                        bonds.append((0, iv, int(jv)))
-                        n = gi[iv] - gj[jv]
-                        dist = np.linalg.norm(n)
-                        n = n / dist
-                        dn = dist - 2 * r
-                        cp = gi[iv] - n * (r + 0.5 * dn)
-                        interactions.append((first + a, first + b, 0, iv, int(jv)) + tuple(n) + (dn,) + tuple(cp)
-                                            + (0.0, 0.0, 0.0) + (0.0,) + (0.0,) * 6 + (damp,))
-                if bonds:
-                    fn0, ft0 = np.exp(mu_ln + np.sqrt(s2) * rng.standard_normal(2))
-                    par = (kn, 0.8 * kn, 0.0, float(fn0), float(ft0), 1e10, 2.0, 0.0)
-                    interfaces.append((first + a, first + b, 0.0, par, bonds))
+        # the bonded vertex pairs at rest: normal, gap and contact point
+        gi = pos[ta] + vtx[tiv]
+        n = gi - (pos[tb] + vtx[tjv])
+        dist = np.sqrt((n * n).sum(axis=1))
+        n = n / dist[:, None]
+        dn = dist - 2 * r
+        cp = gi - n * (r + 0.5 * dn)[:, None]
+        for k in range(len(ta)):
+            interactions.append((first + int(ta[k]), first + int(tb[k]), 0, int(tiv[k]), int(tjv[k]))
+                                + tuple(n[k].tolist()) + (float(dn[k]),) + tuple(cp[k].tolist()) + zeros)
+        for (a, b, vv), bonds in zip(tpl, tbonds):
+            fn0, ft0 = np.exp(mu_ln + np.sqrt(s2) * rng.standard_normal(2))
+            par = (kn, 0.8 * kn, 0.0, float(fn0), float(ft0), 1e10, 2.0, 0.0)
+            interfaces.append((first + a, first + b, 0.0, par, list(bonds)))
     deck = Deck(shape_file="shapes.shp", dt=dt, tmax=steps * dt * (1 - 1e-9), interVerlet=100 * dt, interConf=1e3,
                 DVerlet=0.1 * side, dVerlet=0.05 * side, density={0: 2700.0, 1: 2700.0},
                 gravity=(0.0, 0.0, -9.81), update_nl=nl, nDriven=1, force_law="StickedLinks",
