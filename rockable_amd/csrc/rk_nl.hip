@@ -694,7 +694,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
   int* vg_l = lst;                  // gated vertices: i's as v, j's as maxv + v
   int* vjn_l = vg_l + 2 * cfg.maxv; // vertices of j near i's box (vertex-vertex)
   int* ef_l = vjn_l + cfg.maxv;     // gated edges and faces, tagged (see below)
-  int* cnt = ef_l + 2 * cfg.maxe + 2 * cfg.maxf;  // list sizes
+  int* cnt = ef_l + 2 * cfg.maxe + 2 * cfg.maxf;  // 8 spare ints, then maxwords ints (word prefix)
 
   // test order of the reference (AddOrRemoveInteractions_bruteForce): the
   // bit of test (segment s, own, oth) is segstart[s] + own * n_oth[s] + oth
@@ -825,7 +825,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
-  (void)cnt;
+
   __syncthreads();
   // counts per direction
   int cf = 0, cb = 0;
@@ -860,41 +860,62 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
     cnb[c] = cb;
   }
   if (local + cf + cb > near_rcap) return;  // host re-runs with a larger buffer
-  // write keys in test order: rank of a set bit = its output slot
-  int running = 0;
-  for (int w0 = 0; w0 < nwords; w0 += 64) {
-    const int w = w0 + lane;
-    const unsigned long long m = (w < nwords) ? bits[w] : 0ull;
-    const int pc = __popcll(m);
-    int incl = pc;
-    for (int off = 1; off < 64; off <<= 1) {
-      int y = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += y;
-    }
-    int pos = running + incl - pc;
-    unsigned long long mm = m;
-    while (mm) {
-      const int bit = __ffsll((long long)mm) - 1;
-      mm &= mm - 1ull;
-      const int t = w * 64 + bit;
-      int type, own, oth;
-      if (t < s1) {
-        type = VV, own = t / nvj, oth = t % nvj;
-      } else if (t < s2) {
-        type = VE, own = (t - s1) / nej, oth = (t - s1) % nej;
-      } else if (t < s3) {
-        type = VF, own = (t - s2) / nfj, oth = (t - s2) % nfj;
-      } else if (t < s4) {
-        type = EE, own = (t - s3) / nej, oth = (t - s3) % nej;
-      } else if (t < s5) {
-        type = VE, own = (t - s4) / nei, oth = (t - s4) % nei;
-      } else {
-        type = VF, own = (t - s5) / nfi, oth = (t - s5) % nfi;
+  // write the keys in test order (output slot = rank of the set bit), one
+  // key per lane: the exclusive prefix of the words' bit counts in LDS, then
+  // key q's word by a binary search over it and its bit as the
+  // (q - prefix)-th set bit of the word (a binary search over popcounts)
+  int* wpre = cnt + 8;
+  {
+    int running = 0;
+    for (int w0 = 0; w0 < nwords; w0 += 64) {
+      const int w = w0 + lane;
+      const int pc = w < nwords ? __popcll(bits[w]) : 0;
+      int incl = pc;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
       }
-      nearbuf[base_out + pos] = pack_sub(type, own, oth);
-      pos++;
+      if (w < nwords) wpre[w] = running + incl - pc;
+      running += __shfl(incl, 63, 64);
     }
-    running += __shfl(incl, 63, 64);
+  }
+  __syncthreads();
+  const int ntot = cf + cb;
+  for (int q = lane; q < ntot; q += 64) {
+    int lo = 0, hi = nwords - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (wpre[mid] <= q)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    unsigned long long m = bits[lo];
+    int r = q - wpre[lo], bit = 0;
+    for (int width = 32; width > 0; width >>= 1) {
+      const int c = __popcll(m & ((1ull << width) - 1ull));
+      if (r >= c) {
+        r -= c;
+        m >>= width;
+        bit += width;
+      }
+    }
+    const int t = lo * 64 + bit;
+    int type, own, oth;
+    if (t < s1) {
+      type = VV, own = t / nvj, oth = t % nvj;
+    } else if (t < s2) {
+      type = VE, own = (t - s1) / nej, oth = (t - s1) % nej;
+    } else if (t < s3) {
+      type = VF, own = (t - s2) / nfj, oth = (t - s2) % nfj;
+    } else if (t < s4) {
+      type = EE, own = (t - s3) / nej, oth = (t - s3) % nej;
+    } else if (t < s5) {
+      type = VE, own = (t - s4) / nei, oth = (t - s4) % nei;
+    } else {
+      type = VF, own = (t - s5) / nfi, oth = (t - s5) % nfi;
+    }
+    nearbuf[base_out + q] = pack_sub(type, own, oth);
   }
 }
 

@@ -64,10 +64,11 @@ struct NLParams {
 };
 
 // LDS of one narrow-phase workgroup: global vertices of both bodies, the
-// predicate bits, the lists of gated vertices, edges and faces.
+// predicate bits, the lists of gated vertices, edges and faces, the prefix
+// of the bit counts per word.
 inline size_t narrow_lds_bytes(const NLParams& p) {
   return sizeof(double) * 6 * p.maxv + sizeof(unsigned long long) * p.maxwords +
-         sizeof(int) * (3 * p.maxv + 2 * p.maxe + 2 * p.maxf + 8);
+         sizeof(int) * (3 * p.maxv + 2 * p.maxe + 2 * p.maxf + 8 + p.maxwords);
 }
 
 struct NLWork {
