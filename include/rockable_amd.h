@@ -138,7 +138,9 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * workgroup: 1, 2, 4; default 4), "body_block" (per-body kernels: 64, 128,
  * 256; default 256), "persistent" (0, the default: one tile per workgroup;
  * n in [8, 1048576]: the step kernels' grids capped at n workgroups, each
- * looping over its tiles).  No option changes results. */
+ * looping over its tiles), "narrow_pairs" (candidate body pairs per
+ * narrow-phase wavefront: 1, 2; 0, the default: 2 for shapes of up to 8
+ * vertices, else 1).  No option changes results. */
 int rk_set_option(rk_engine* eng, const char* name, double value);
 int rk_get_option(rk_engine* eng, const char* name, double* value);
 /* Autotuner (no counterpart in the reference) of fuse_law, law_group,

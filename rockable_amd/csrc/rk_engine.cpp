@@ -1478,6 +1478,10 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     if (!(value == 8.0 || value == 16.0)) return fail(eng, "gather_lanes must be 8 or 16");
     eng->gather_lanes = (int)value;
     if (eng->loaded) assemble_dev(eng);
+  } else if (n == "narrow_pairs") {
+    if (!(value == 0.0 || value == 1.0 || value == 2.0))
+      return fail(eng, "narrow_pairs must be 0 (by shape size), 1 or 2");
+    eng->w.narrow_pairs = (int)value;
   } else if (n == "gather_groups") {
     if (!(value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0 || value == 8.0 || value == 16.0))
       return fail(eng, "gather_groups must be 0 (by size), 1, 2, 4, 8 or 16");
@@ -1568,6 +1572,7 @@ int* tuned_option(rk_engine* eng, const std::string& n) {
   if (n == "fuse_law") return &eng->fuse_law;
   if (n == "gather_lanes") return &eng->gather_lanes;
   if (n == "gather_groups") return &eng->gather_groups;
+  if (n == "narrow_pairs") return &eng->w.narrow_pairs;
   if (n == "fuse_glob") return &eng->fuse_glob;
   if (n == "persistent") return &eng->persist;
   return nullptr;

@@ -658,78 +658,107 @@ __device__ __forceinline__ bool face_gate_pre(const float* g, double sum, double
 
 
 
-// Wavefront compaction of the indices v < n with pred(v) into list (in
-// ascending order); returns the count.
-template <class F>
-__device__ __forceinline__ int wave_compact(int n, int lane, int* list, F pred) {
+// Compaction of the indices v < n with pred(v) into list (in ascending
+// order) by the SW lanes of one pair (sl: lane within them, g: the pair's
+// group of lanes in the wavefront); returns the count.
+template <int SW, class F>
+__device__ __forceinline__ int group_compact(int n, int sl, int g, int* list, F pred) {
   int base = 0;
-  for (int v0 = 0; v0 < n; v0 += 64) {
-    const int v = v0 + lane;
+  for (int v0 = 0; v0 < n; v0 += SW) {
+    const int v = v0 + sl;
     const bool f = v < n && pred(v);
-    const unsigned long long m = __ballot(f);
-    if (f) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+    unsigned long long m = __ballot(f);
+    if constexpr (SW < 64) m = (m >> (g * SW)) & ((1ull << SW) - 1ull);
+    if (f) list[base + __popcll(m & ((1ull << sl) - 1ull))] = v;
     base += __popcll(m);
   }
   return base;
 }
 
+template <int SW>
+__device__ __forceinline__ int group_sum(int x) {
+  for (int off = SW / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, SW);
+  return x;
+}
+
+// P candidate pairs per wavefront (64 / P lanes each, LDS stride `stride`
+// bytes per pair): the work of a pair is mostly a fixed sequence of
+// wavefront instructions (gates, compactions, one pass per predicate, the
+// key output), which the pairs of a wavefront then share.
+template <int P>
 __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const double* baabb, NarrowCfg cfg, int nc,
                                                const uint64_t* cand, uint32_t* nearbuf, int* near_cnt,
-                                               int near_rcap, int32_t* cbase, int32_t* cnf, int32_t* cnb) {
-  extern __shared__ double lds[];
-  const int c = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  if (c >= nc) return;
+                                               int near_rcap, int32_t* cbase, int32_t* cnf, int32_t* cnb,
+                                               int stride) {
+  constexpr int SW = 64 / P;
+  extern __shared__ double lds_all[];
   const int lane = threadIdx.x;
-  const uint64_t key = cand[c];
+  // (with one pair per wavefront the pair's values stay wavefront-uniform,
+  // in scalar registers)
+  const int g = P == 1 ? 0 : lane / SW, sl = P == 1 ? lane : lane % SW;
+  const int c = xcd_block((int)blockIdx.x, (int)gridDim.x) * P + g;
+  const bool valid = c < nc;
+  if (P == 1 && !valid) return;
+  double* lds = lds_all + (size_t)g * (stride / 8);
+  uint64_t key = 0;
+  if (valid) key = cand[c];
   const int i = (int)(key >> 32), j = (int)(key & 0xffffffffu);
   const int shi = d.kin[i].shape, shj = d.kin[j].shape;
   const double Ri = d.kin[i].R, Rj = d.kin[j].R;
-  const int nvi = d.sh.nv[shi], nvj = d.sh.nv[shj];
-  const int nei = d.sh.ne[shi], nej = d.sh.ne[shj];
-  const int nfi = d.sh.nf[shi], nfj = d.sh.nf[shj];
-  double* gvi = lds;
-  double* gvj = lds + 3 * cfg.maxv;
-  unsigned long long* bits = (unsigned long long*)(lds + 6 * cfg.maxv);
+  // an invalid group (past the last pair) takes body 0 twice with no
+  // sub-element, so that it runs every pass with nothing to do
+  const int nvi = valid ? d.sh.nv[shi] : 0, nvj = valid ? d.sh.nv[shj] : 0;
+  const int nei = valid ? d.sh.ne[shi] : 0, nej = valid ? d.sh.ne[shj] : 0;
+  const int nfi = valid ? d.sh.nf[shi] : 0, nfj = valid ? d.sh.nf[shj] : 0;
+  double* pbox = lds;                // 48 doubles: both OBBs and both body boxes (several pairs per wavefront)
+  double* gvi = lds + 48;
+  double* gvj = gvi + 3 * cfg.maxv;
+  unsigned long long* bits = (unsigned long long*)(gvj + 3 * cfg.maxv);
   int* lst = (int*)(bits + cfg.maxwords);
   int* vg_l = lst;                  // gated vertices: i's as v, j's as maxv + v
   int* vjn_l = vg_l + 2 * cfg.maxv; // vertices of j near i's box (vertex-vertex)
   int* ef_l = vjn_l + cfg.maxv;     // gated edges and faces, tagged (see below)
-  int* cnt = ef_l + 2 * cfg.maxe + 2 * cfg.maxf;  // 8 spare ints, then maxwords ints (word prefix)
+  int* wpre = ef_l + 2 * cfg.maxe + 2 * cfg.maxf + 8;  // prefix of the bit counts per word
 
   // test order of the reference (AddOrRemoveInteractions_bruteForce): the
   // bit of test (segment s, own, oth) is segstart[s] + own * n_oth[s] + oth
   const int s1 = nvi * nvj, s2 = s1 + nvi * nej, s3 = s2 + nvi * nfj, s4 = s3 + nei * nej, s5 = s4 + nvj * nei;
   const int T = s5 + nvj * nfi, Tf = s4;
-  const int nwords = (T + 63) / 64;
-  if (nwords > cfg.maxwords) return;  // guarded on the host: shapes are checked at load
-  for (int w = lane; w < nwords; w += 64) bits[w] = 0ull;
+  // (nwords <= maxwords: the shapes are checked at load)
+  const int nwords = min((T + 63) / 64, cfg.maxwords);
+  for (int w = sl; w < nwords; w += SW) bits[w] = 0ull;
   {
     const double* src_i = d.gv + 3 * (size_t)d.bvoff[i];
     const double* src_j = d.gv + 3 * (size_t)d.bvoff[j];
-    for (int q = lane; q < 3 * nvi; q += 64) gvi[q] = src_i[q];
-    for (int q = lane; q < 3 * nvj; q += 64) gvj[q] = src_j[q];
+    for (int q = sl; q < 3 * nvi; q += SW) gvi[q] = src_i[q];
+    for (int q = sl; q < 3 * nvj; q += SW) gvj[q] = src_j[q];
   }
-  // body boxes (vertices +- Minkowski radius, from k_obb) for the culls
-  double bx[12];
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    bx[k] = baabb[6 * (size_t)i + k];
-    bx[6 + k] = baabb[6 * (size_t)j + k];
+  // the OBBs and the body boxes (vertices +- Minkowski radius, from k_obb)
+  // of both bodies: read where they lie when the pair's values are uniform
+  // over the wavefront (scalar loads), else from copies in the pair's LDS
+  // (vector registers would hold them for every lane)
+  const double *obi = bobb + 15 * (size_t)i, *obj = bobb + 15 * (size_t)j;
+  const double *bxi = baabb + 6 * (size_t)i, *bxj = baabb + 6 * (size_t)j;
+  if (P > 1) {
+    for (int q = sl; q < 42; q += SW)
+      pbox[q] = q < 15 ? obi[q] : q < 30 ? obj[q - 15] : q < 36 ? bxi[q - 30] : bxj[q - 36];
+    obi = pbox;
+    obj = pbox + 15;
+    bxi = pbox + 30;
+    bxj = pbox + 36;
   }
+  __syncthreads();
   double xmax = 0.0;
-#pragma unroll
-  for (int k = 0; k < 12; k++) xmax = fmax(xmax, fabs(bx[k]));
+  for (int k = 0; k < 6; k++) xmax = fmax(fmax(xmax, fabs(bxi[k])), fabs(bxj[k]));
   const double dmax = cfg.dmax;
   const double sum = Ri + Rj + dmax;
   const double margin = 1e-9 * (1.0 + xmax + sum);
-  const obb_t oi = load_obb(bobb + 15 * (size_t)i);
-  const obb_t oj = load_obb(bobb + 15 * (size_t)j);
-  __syncthreads();
 
   // reference vertex gates of both bodies in one pass: i's vertices against
   // j's OBB, j's against i's
-  const int nvg = wave_compact(nvi + nvj, lane, vg_l, [&](int v) {
-    return v < nvi ? vertex_gate(lds_v(gvi, v), Ri, oj, dmax) : vertex_gate(lds_v(gvj, v - nvi), Rj, oi, dmax);
+  const int nvg = group_compact<SW>(nvi + nvj, sl, g, vg_l, [&](int v) {
+    return v < nvi ? vertex_gate(lds_v(gvi, v), Ri, load_obb(obj), dmax)
+                   : vertex_gate(lds_v(gvj, v - nvi), Rj, load_obb(obi), dmax);
   });
   // edges and faces of both bodies in one pass; tag = kind * 2^20 + index,
   // kinds 0 edge of i, 1 edge of j, 2 face of i, 3 face of j
@@ -738,35 +767,33 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
   const float* gej = cfg.gedge + kGate * (size_t)d.beoff[j];
   const float* gfi = cfg.gface + kGate * (size_t)d.bfoff[i];
   const float* gfj = cfg.gface + kGate * (size_t)d.bfoff[j];
-  const int nefg = wave_compact(nef, lane, ef_l, [&](int q) {
-    if (q < nei) return edge_gate_pre(gei + kGate * q, sum, dmax, margin, bx + 6, bx + 9);
-    if (q < ne2) return edge_gate_pre(gej + kGate * (q - nei), sum, dmax, margin, bx, bx + 3);
-    if (q < ne2 + nfi) return face_gate_pre(gfi + kGate * (q - ne2), sum, margin, bx + 6, bx + 9);
-    return face_gate_pre(gfj + kGate * (q - ne2 - nfi), sum, margin, bx, bx + 3);
+  const int nefg = group_compact<SW>(nef, sl, g, ef_l, [&](int q) {
+    if (q < nei) return edge_gate_pre(gei + kGate * q, sum, dmax, margin, bxj, bxj + 3);
+    if (q < ne2) return edge_gate_pre(gej + kGate * (q - nei), sum, dmax, margin, bxi, bxi + 3);
+    if (q < ne2 + nfi) return face_gate_pre(gfi + kGate * (q - ne2), sum, margin, bxj, bxj + 3);
+    return face_gate_pre(gfj + kGate * (q - ne2 - nfi), sum, margin, bxi, bxi + 3);
   });
   // vertex-vertex partners: vertices of j within sum of i's box
   const double gvv = sum + margin;
-  const int nvj_n = wave_compact(nvj, lane, vjn_l, [&](int v) {
+  const int nvj_n = group_compact<SW>(nvj, sl, g, vjn_l, [&](int v) {
     const v3 p = lds_v(gvj, v);
-    return p.x >= bx[0] - gvv && p.x <= bx[3] + gvv && p.y >= bx[1] - gvv && p.y <= bx[4] + gvv &&
-           p.z >= bx[2] - gvv && p.z <= bx[5] + gvv;
+    return p.x >= bxi[0] - gvv && p.x <= bxi[3] + gvv && p.y >= bxi[1] - gvv && p.y <= bxi[4] + gvv &&
+           p.z >= bxi[2] - gvv && p.z <= bxi[5] + gvv;
   });
   // split the compacted lists by kind (they are in ascending q, so each kind
   // is a contiguous run)
   int nvi_g = 0, nei_g = 0, nej_g = 0, nfi_g = 0;
-  for (int k = lane; k < nvg; k += 64) nvi_g += vg_l[k] < nvi ? 1 : 0;
-  for (int k = lane; k < nefg; k += 64) {
+  for (int k = sl; k < nvg; k += SW) nvi_g += vg_l[k] < nvi ? 1 : 0;
+  for (int k = sl; k < nefg; k += SW) {
     const int q = ef_l[k];
     nei_g += q < nei ? 1 : 0;
     nej_g += (q >= nei && q < ne2) ? 1 : 0;
     nfi_g += (q >= ne2 && q < ne2 + nfi) ? 1 : 0;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    nvi_g += __shfl_xor(nvi_g, off, 64);
-    nei_g += __shfl_xor(nei_g, off, 64);
-    nej_g += __shfl_xor(nej_g, off, 64);
-    nfi_g += __shfl_xor(nfi_g, off, 64);
-  }
+  nvi_g = group_sum<SW>(nvi_g);
+  nei_g = group_sum<SW>(nei_g);
+  nej_g = group_sum<SW>(nej_g);
+  nfi_g = group_sum<SW>(nfi_g);
   const int nvj_g = nvg - nvi_g, nfj_g = nefg - nei_g - nej_g - nfi_g;
   const int* vi_l = vg_l;            // values v
   const int* vj_l = vg_l + nvi_g;    // values nvi + v
@@ -778,7 +805,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
 
   // one loop per segment, so that a wavefront runs one predicate at a time
   int n = nvi_g * nvj_n;
-  for (int k = lane; k < n; k += 64) {
+  for (int k = sl; k < n; k += SW) {
     const int a = vi_l[k / nvj_n], b = vjn_l[k % nvj_n];
     if (near_vv(gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = a * nvj + b;
@@ -786,7 +813,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
     }
   }
   n = nvi_g * nej_g;
-  for (int k = lane; k < n; k += 64) {
+  for (int k = sl; k < n; k += SW) {
     const int a = vi_l[k / nej_g], b = ej_l[k % nej_g] - nei;
     if (near_ve(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = s1 + a * nej + b;
@@ -794,7 +821,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
     }
   }
   n = nvi_g * nfj_g;
-  for (int k = lane; k < n; k += 64) {
+  for (int k = sl; k < n; k += SW) {
     const int a = vi_l[k / nfj_g], b = fj_l[k % nfj_g] - ne2 - nfi;
     if (near_vf(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = s2 + a * nfj + b;
@@ -802,7 +829,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
     }
   }
   n = nei_g * nej_g;
-  for (int k = lane; k < n; k += 64) {
+  for (int k = sl; k < n; k += SW) {
     const int a = ei_l[k / nej_g], b = ej_l[k % nej_g] - nei;
     if (near_ee(d.sh, shi, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = s3 + a * nej + b;
@@ -810,7 +837,7 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
     }
   }
   n = nvj_g * nei_g;
-  for (int k = lane; k < n; k += 64) {
+  for (int k = sl; k < n; k += SW) {
     const int a = vj_l[k / nei_g] - nvi, b = ei_l[k % nei_g];
     if (near_ve(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
       const int bit = s4 + a * nei + b;
@@ -818,18 +845,17 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
     }
   }
   n = nvj_g * nfi_g;
-  for (int k = lane; k < n; k += 64) {
+  for (int k = sl; k < n; k += SW) {
     const int a = vj_l[k / nfi_g] - nvi, b = fi_l[k % nfi_g] - ne2;
     if (near_vf(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
       const int bit = s5 + a * nfi + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
-
   __syncthreads();
   // counts per direction
   int cf = 0, cb = 0;
-  for (int w = lane; w < nwords; w += 64) {
+  for (int w = sl; w < nwords; w += SW) {
     const unsigned long long m = bits[w];
     const int lo = w * 64;
     if (lo + 64 <= Tf)
@@ -843,45 +869,41 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
       cb += __popcll(m & ~fm);
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    cf += __shfl_down(cf, off, 64);
-    cb += __shfl_down(cb, off, 64);
-  }
-  cf = __shfl(cf, 0, 64);
-  cb = __shfl(cb, 0, 64);
+  cf = group_sum<SW>(cf);
+  cb = group_sum<SW>(cb);
   const int stripe = c & (kStripes - 1);
   int local = 0;
-  if (lane == 0) local = atomicAdd(near_cnt + stripe * kStripeStride, cf + cb);
-  local = __shfl(local, 0, 64);
+  if (valid && sl == 0) local = atomicAdd(near_cnt + stripe * kStripeStride, cf + cb);
+  local = __shfl(local, 0, SW);
   const int base_out = stripe * near_rcap + local;
-  if (lane == 0) {
+  if (valid && sl == 0) {
     cbase[c] = base_out;
     cnf[c] = cf;
     cnb[c] = cb;
   }
-  if (local + cf + cb > near_rcap) return;  // host re-runs with a larger buffer
+  // over capacity: the host re-runs with a larger buffer
+  const bool room = valid && local + cf + cb <= near_rcap;
   // write the keys in test order (output slot = rank of the set bit), one
   // key per lane: the exclusive prefix of the words' bit counts in LDS, then
   // key q's word by a binary search over it and its bit as the
   // (q - prefix)-th set bit of the word (a binary search over popcounts)
-  int* wpre = cnt + 8;
   {
     int running = 0;
-    for (int w0 = 0; w0 < nwords; w0 += 64) {
-      const int w = w0 + lane;
+    for (int w0 = 0; w0 < nwords; w0 += SW) {
+      const int w = w0 + sl;
       const int pc = w < nwords ? __popcll(bits[w]) : 0;
       int incl = pc;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
+      for (int off = 1; off < SW; off <<= 1) {
+        const int y = __shfl_up(incl, off, SW);
+        if (sl >= off) incl += y;
       }
       if (w < nwords) wpre[w] = running + incl - pc;
-      running += __shfl(incl, 63, 64);
+      running += __shfl(incl, SW - 1, SW);
     }
   }
   __syncthreads();
-  const int ntot = cf + cb;
-  for (int q = lane; q < ntot; q += 64) {
+  const int ntot = room ? cf + cb : 0;
+  for (int q = sl; q < ntot; q += SW) {
     int lo = 0, hi = nwords - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -893,9 +915,9 @@ __global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const 
     unsigned long long m = bits[lo];
     int r = q - wpre[lo], bit = 0;
     for (int width = 32; width > 0; width >>= 1) {
-      const int c = __popcll(m & ((1ull << width) - 1ull));
-      if (r >= c) {
-        r -= c;
+      const int cnt = __popcll(m & ((1ull << width) - 1ull));
+      if (r >= cnt) {
+        r -= cnt;
         m >>= width;
         bit += width;
       }
@@ -1429,13 +1451,26 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     hipLaunchKernelGGL(k_gate_boxes, dim3((d.nb + 3) / 4), dim3(kBlock), 0, s, d, w.gedge, w.gface);
     cfg.gedge = w.gedge;
     cfg.gface = w.gface;
-    const size_t lds = narrow_lds_bytes(p);
+    // candidate pairs per wavefront: by default 2 for shapes of up to 8
+    // vertices, else 1 (measured: 2 pairs are faster for sphero-cubes, 1 for
+    // hulls of 8-14 vertices, whose passes fill a wavefront; more pairs cost
+    // occupancy, their values being per lane), then as many as the LDS holds
+    const size_t stride = (narrow_lds_bytes(p) + 7) & ~(size_t)7;
+    int P = w.narrow_pairs;
+    if (P <= 0) P = 2 * p.maxv <= 16 ? 2 : 1;
+    while (P > 1 && (size_t)P * stride > 65536) P >>= 1;
+    const size_t lds = (size_t)P * stride;
+    const dim3 grid((nc + P - 1) / P);
     for (int attempt = 0; attempt < 3; attempt++) {
       const int rcap = std::max(64, (w.near_hint + kStripes - 1) / kStripes);
       if (ensure(w, w.nearbuf, w.nearbuf_cap, (size_t)kStripes * rcap)) return -1;
       RK_CHECK(hipMemsetAsync(w.scnt, 0, sizeof(int) * kStripes * kStripeStride, s));
-      hipLaunchKernelGGL(k_narrow, dim3(nc), dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nc, w.cand, w.nearbuf,
-                         w.scnt, rcap, w.cbase, w.cnf, w.cnb);
+      if (P == 2)
+        hipLaunchKernelGGL(k_narrow<2>, grid, dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nc, w.cand, w.nearbuf,
+                           w.scnt, rcap, w.cbase, w.cnf, w.cnb, (int)stride);
+      else
+        hipLaunchKernelGGL(k_narrow<1>, grid, dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nc, w.cand, w.nearbuf,
+                           w.scnt, rcap, w.cbase, w.cnf, w.cnb, (int)stride);
       RK_CHECK(hipGetLastError());
       hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
       int tm[2] = {0, 0};

@@ -63,11 +63,11 @@ struct NLParams {
   int maxwords = 1;  // max 64-bit words of predicate bits per body pair
 };
 
-// LDS of one narrow-phase workgroup: global vertices of both bodies, the
-// predicate bits, the lists of gated vertices, edges and faces, the prefix
+// LDS of one narrow-phase pair: both bodies' OBBs and boxes, the global
+// vertices of both bodies, the predicate bits, the lists of gated vertices, edges and faces, the prefix
 // of the bit counts per word.
 inline size_t narrow_lds_bytes(const NLParams& p) {
-  return sizeof(double) * 6 * p.maxv + sizeof(unsigned long long) * p.maxwords +
+  return sizeof(double) * (48 + 6 * p.maxv) + sizeof(unsigned long long) * p.maxwords +
          sizeof(int) * (3 * p.maxv + 2 * p.maxe + 2 * p.maxf + 8 + p.maxwords);
 }
 
@@ -75,6 +75,7 @@ struct NLWork {
   char err[256] = {0};
   int cand_hint = 0, near_hint = 0, last_candidates = 0;
   int heavy_min = 256;  // slot count above which a body's sum runs block-wide
+  int narrow_pairs = 0;  // candidate pairs per narrow-phase wavefront (1, 2; 0: by shape size)
 #define RK_BUF(T, name) \
   T* name = nullptr;    \
   size_t name##_cap = 0;

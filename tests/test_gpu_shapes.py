@@ -1,6 +1,8 @@
 """Launch shapes and the autotuner: every choice of fuse_law, law_group,
-geom_block, gather_waves and body_block computes the same state bit for
-bit (the default engine, against which they are compared, fuses the
+geom_block, gather_lanes, gather_groups, fuse_glob, gather_waves,
+body_block, persistent, graph_steps and narrow_pairs (candidate pairs per
+narrow-phase wavefront; the default for these hulls is 1) computes the
+same state bit for bit (the default engine, against which they are compared, fuses the
 geometry and law kernels; the first three shapes use them apart), and the
 steps the autotuner takes while timing candidates are the steps rk_step
 takes."""
@@ -21,7 +23,9 @@ SHAPES = [dict(fuse_law=0, law_group=1, geom_block=64, gather_waves=1, body_bloc
           dict(fuse_law=0, persistent=16, law_group=2, geom_block=128, graph_steps=4),
           dict(gather_lanes=16, gather_groups=16, fuse_glob=1, gather_waves=1),
           dict(gather_lanes=8, gather_groups=4, fuse_glob=1, persistent=8, graph_steps=4),
-          dict(gather_lanes=8, gather_groups=16, fuse_glob=0, gather_waves=2)]
+          dict(gather_lanes=8, gather_groups=16, fuse_glob=0, gather_waves=2),
+          dict(narrow_pairs=2),
+          dict(narrow_pairs=2, gather_lanes=8)]
 
 
 def _hull_deck(tmp_path):
@@ -53,7 +57,7 @@ def test_bad_launch_shape_rejected(tmp_path):
     e = Engine(_hull_deck(tmp_path))
     for name, v in (("geom_block", 96), ("law_group", 3), ("gather_waves", 8), ("body_block", 512),
                     ("gather_lanes", 4), ("fuse_law", 2), ("model_scale", 0.0), ("persistent", 7),
-                    ("gather_groups", 3), ("fuse_glob", 2)):
+                    ("gather_groups", 3), ("fuse_glob", 2), ("narrow_pairs", 4)):
         with pytest.raises(RuntimeError):
             e.set_option(name, v)
 
