@@ -143,6 +143,8 @@ struct Dev {
   const int32_t* bvbody;
   int nvtot;
   int nvdriven;           // vertices of the driven bodies (the first ones)
+  const int32_t* wvstart; // free bodies' vertices packed by whole bodies into wavefronts of 64: [wvstart[w], wvstart[w+1])
+  int nwv;                // such wavefronts (0: a free body has more than 64 vertices)
   double* acc;
   double* arot;
   double* force;
@@ -192,7 +194,7 @@ struct Dev {
   int gather_waves;       // k_gather wavefronts per workgroup (1, 2, 4)
   int gather_lanes;       // k_gather lanes per body (8, 16)
   int gather_groups;      // k_gather slot groups per body wavefront (1 .. gather_lanes)
-  int fuse_glob;          // 1: k_predict writes the free bodies' global vertices
+  int fuse_glob;          // the free bodies' global vertices: 0 k_glob, 1 k_predict per body, 2 k_predict_vtx
   int body_block;         // workgroup size of the per-body kernels (64, 128, 256)
   int fuse_law;           // 1: geometry and law in one kernel (k_interact)
   int persist;            // step kernels' grid cap (0: one tile per workgroup)

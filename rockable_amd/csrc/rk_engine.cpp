@@ -82,6 +82,8 @@ struct rk_engine {
   int32_t *bvoff = nullptr, *bvbody = nullptr, *beoff = nullptr, *bfoff = nullptr;
   int netot = 0, nftot = 0;
   int nvtot = 0, nvdriven = 0;
+  int32_t* wvstart = nullptr;  // free bodies' vertices by wavefronts of whole bodies
+  int nwv = 0;
   // global vertices: of no body, of the listed bodies (those the force phase
   // reads), or of every body (the neighbour list reads them) current
   enum { GV_STALE, GV_LISTED, GV_ALL } gv_state = GV_STALE;
@@ -213,6 +215,9 @@ void free_device(rk_engine* eng) {
   eng->stat_part = eng->stat_out = nullptr;
   eng->stat_cap = 0;
   eng->nvtot = eng->nvdriven = 0;
+  if (eng->wvstart) (void)hipFree(eng->wvstart);
+  eng->wvstart = nullptr;
+  eng->nwv = 0;
   for (auto& E : eng->E) {
     entries_free(E);
     if (E.istart) (void)hipFree(E.istart);
@@ -275,6 +280,8 @@ void assemble_dev(rk_engine* eng) {
   d.bvbody = eng->bvbody;
   d.nvtot = eng->nvtot;
   d.nvdriven = eng->nvdriven;
+  d.wvstart = eng->wvstart;
+  d.nwv = eng->nwv;
   d.acc = eng->acc;
   d.arot = eng->arot;
   d.force = eng->force;
@@ -523,6 +530,22 @@ int upload_model(rk_engine* eng) {
     }
     eng->nvtot = bvoff[nb];
     eng->nvdriven = bvoff[m.nDriven];
+    // the free bodies' vertices packed into wavefronts of at most 64, whole
+    // bodies each (none if a free body has more than 64 vertices)
+    std::vector<int32_t> wv(1, eng->nvdriven);
+    bool fits = true;
+    for (int b = m.nDriven, used = 0; b < nb; b++) {
+      const int nvb = bvoff[b + 1] - bvoff[b];
+      if (nvb > 64) fits = false;
+      if (used + nvb > 64) {
+        wv.push_back(bvoff[b]);
+        used = 0;
+      }
+      used += nvb;
+    }
+    if (wv.back() != eng->nvtot) wv.push_back(eng->nvtot);
+    eng->nwv = fits && eng->nvtot > eng->nvdriven ? (int)wv.size() - 1 : 0;
+    if (upload(eng, eng->wvstart, wv)) return -1;
     if (upload(eng, eng->bvoff, bvoff) || upload(eng, eng->bvbody, bvbody) ||
         upload(eng, eng->gv, std::vector<double>(3 * (size_t)std::max(1, eng->nvtot), 0.0)))
       return -1;
@@ -1488,7 +1511,8 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     eng->gather_groups = (int)value;
     if (eng->loaded) assemble_dev(eng);
   } else if (n == "fuse_glob") {
-    if (!(value == -1.0 || value == 0.0 || value == 1.0)) return fail(eng, "fuse_glob must be -1 (by size), 0 or 1");
+    if (!(value == -1.0 || value == 0.0 || value == 1.0 || value == 2.0))
+      return fail(eng, "fuse_glob must be -1 (by size), 0, 1 or 2");
     eng->fuse_glob = (int)value;
     if (eng->loaded) assemble_dev(eng);
   } else if (n == "fuse_law") {
@@ -1593,7 +1617,7 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
                         {"geom_block", {64, 128, 256, 512}},
                         {"gather_lanes", {8, 16}},
                         {"gather_groups", {1, 4, 16}},
-                        {"fuse_glob", {0, 1}},
+                        {"fuse_glob", {0, 1, 2}},
                         {"gather_waves", {1, 2, 4}},
                         {"body_block", {64, 128, 256}},
                         {"persistent", {0, 2048, 8192}},

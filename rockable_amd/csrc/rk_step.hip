@@ -1284,162 +1284,206 @@ __global__ void __launch_bounds__(kBlock) k_glob(Dev d, int all, int nv) {
   }
 }
 
+// ServoFunction (DrivingSystem.cpp:242-268), then the controlled bodies in
+// list order (Rockable.cpp:1856-1900, 1988-2033), all by one thread.
+__device__ void drive_controls(const Dev& d, const StepParams& sp) {
+  if (d.servo != 0) {
+    const int* id = d.servo_ids;
+    const double Xmin = d.kin[id[0]].px + d.kin[id[0]].R;
+    const double Xmax = d.kin[id[1]].px - d.kin[id[1]].R;
+    const double Ymin = d.kin[id[2]].py + d.kin[id[2]].R;
+    const double Ymax = d.kin[id[3]].py - d.kin[id[3]].R;
+    const double Zmin = d.kin[id[4]].pz + d.kin[id[4]].R;
+    const double Zmax = d.kin[id[5]].pz - d.kin[id[5]].R;
+    const double dX = fabs(Xmax - Xmin), dY = fabs(Ymax - Ymin), dZ = fabs(Zmax - Zmin);
+    const double surfX = dY * dZ, surfY = dX * dZ, surfZ = dY * dX;
+    d.ctrl[d.servo_icontr + 1].value = -d.servo_pressure * surfX;
+    if (d.servo == 1) d.ctrl[d.servo_icontr + 3].value = -d.servo_pressure * surfY;
+    d.ctrl[d.servo_icontr + 5].value = -d.servo_pressure * surfZ;
+  }
+  for (int c = 0; c < d.nctrl; c++) {
+    const DevControl C = d.ctrl[c];
+    BodyKin* p = d.kin + C.i;
+    switch (C.type) {
+      case C_X_VEL:
+        p->vx = C.value;
+        p->px += sp.dt * C.value;
+        break;
+      case C_Y_VEL:
+        p->vy = C.value;
+        p->py += sp.dt * C.value;
+        break;
+      case C_Z_VEL:
+        p->vz = C.value;
+        p->pz += sp.dt * C.value;
+        break;
+      case C_XROT_VEL:
+      case C_YROT_VEL:
+      case C_ZROT_VEL:
+      case C_XYZROT_VEL: {
+        v3 w;
+        if (C.type == C_XROT_VEL)
+          w = V(C.value, 0.0, 0.0);
+        else if (C.type == C_YROT_VEL)
+          w = V(0.0, C.value, 0.0);
+        else if (C.type == C_ZROT_VEL)
+          w = V(0.0, 0.0, C.value);
+        else
+          w = V(C.vx, C.vy, C.vz);
+        qt Q = kq(*p);
+        Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+        qnormalize(&Q);
+        p->qw = Q.w;
+        p->qx = Q.x;
+        p->qy = Q.y;
+        p->qz = Q.z;
+      } break;
+      default: break;
+    }
+  }
+  for (int c = 0; c < d.nctrl; c++) {
+    const DevControl C = d.ctrl[c];
+    BodyKin* p = d.kin + C.i;
+    const double* a = d.acc + 3 * (size_t)C.i;
+    const double* ar = d.arot + 3 * (size_t)C.i;
+    switch (C.type) {
+      case C_X_FOR:
+        p->px += sp.dt * p->vx + sp.dt2_2 * a[0];
+        p->vx += sp.dt_2 * a[0];
+        break;
+      case C_Y_FOR:
+        p->py += sp.dt * p->vy + sp.dt2_2 * a[1];
+        p->vy += sp.dt_2 * a[1];
+        break;
+      case C_Z_FOR:
+        p->pz += sp.dt * p->vz + sp.dt2_2 * a[2];
+        p->vz += sp.dt_2 * a[2];
+        break;
+      case C_XROT_MOM:
+      case C_YROT_MOM:
+      case C_ZROT_MOM:
+      case C_XYZROT_MOM: {
+        v3 w;
+        if (C.type == C_XROT_MOM)
+          w = V(p->wx * 1.0, p->wx * 0.0, p->wx * 0.0);
+        else if (C.type == C_YROT_MOM)
+          w = V(p->wy * 0.0, p->wy * 1.0, p->wy * 0.0);
+        else if (C.type == C_ZROT_MOM)
+          w = V(p->wz * 0.0, p->wz * 0.0, p->wz * 1.0);
+        else
+          w = V(p->wx, p->wy, p->wz);
+        qt Q = kq(*p);
+        Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+        qnormalize(&Q);
+        p->qw = Q.w;
+        p->qx = Q.x;
+        p->qy = Q.y;
+        p->qz = Q.z;
+        if (C.type == C_XROT_MOM)
+          p->wx += sp.dt_2 * ar[0];
+        else if (C.type == C_YROT_MOM)
+          p->wy += sp.dt_2 * ar[1];
+        else if (C.type == C_ZROT_MOM)
+          p->wz += sp.dt_2 * ar[2];
+        else {
+          v3 nw = vadd(V(p->wx, p->wy, p->wz), vsc(sp.dt_2, V(ar[0], ar[1], ar[2])));
+          p->wx = nw.x;
+          p->wy = nw.y;
+          p->wz = nw.z;
+        }
+      } break;
+      default: break;
+    }
+  }
+}
+
+// predict_free: the new pose and velocities of free body b (Rockable.cpp:
+// 2036-2055) in P; store_predicted writes them back
+__device__ __forceinline__ void predict_free(const Dev& d, const StepParams& sp, int b, BodyKin& P) {
+  const v3 a = ld3(d.acc + 3 * (size_t)b);
+  const v3 ar = ld3(d.arot + 3 * (size_t)b);
+  v3 pos = vadd(kpos(P), vadd(vsc(sp.dt, kvel(P)), vsc(sp.dt2_2, a)));
+  v3 vel = vadd(kvel(P), vsc(sp.dt_2, a));
+  qt Q = kq(P);
+  const v3 w = kvrot(P);
+  Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+  Q = qadd_scaled(Q, qddot(Q, w, ar), sp.dt2_2);
+  qnormalize(&Q);
+  v3 w2 = vadd(w, vsc(sp.dt_2, ar));
+  P.px = pos.x;
+  P.py = pos.y;
+  P.pz = pos.z;
+  P.qw = Q.w;
+  P.qx = Q.x;
+  P.qy = Q.y;
+  P.qz = Q.z;
+  P.vx = vel.x;
+  P.vy = vel.y;
+  P.vz = vel.z;
+  P.wx = w2.x;
+  P.wy = w2.y;
+  P.wz = w2.z;
+}
+__device__ __forceinline__ void store_predicted(const Dev& d, int b, const BodyKin& P) {
+  BodyKin* out = d.kin + b;
+  out->px = P.px;
+  out->py = P.py;
+  out->pz = P.pz;
+  out->qw = P.qw;
+  out->qx = P.qx;
+  out->qy = P.qy;
+  out->qz = P.qz;
+  out->vx = P.vx;
+  out->vy = P.vy;
+  out->vz = P.vz;
+  out->wx = P.wx;
+  out->wy = P.wy;
+  out->wz = P.wz;
+}
+
 // glob: also the global vertices of the free bodies with listed slots, from
 // the new pose (the driven bodies' follow in k_glob)
 __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp, int glob) {
   if (frozen(d)) return;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid == 0 && d.nctrl > 0) {
-    // ServoFunction (DrivingSystem.cpp:242-268), then the controlled bodies in
-    // list order (Rockable.cpp:1856-1900, 1988-2033), all by one thread.
-    if (d.servo != 0) {
-      const int* id = d.servo_ids;
-      const double Xmin = d.kin[id[0]].px + d.kin[id[0]].R;
-      const double Xmax = d.kin[id[1]].px - d.kin[id[1]].R;
-      const double Ymin = d.kin[id[2]].py + d.kin[id[2]].R;
-      const double Ymax = d.kin[id[3]].py - d.kin[id[3]].R;
-      const double Zmin = d.kin[id[4]].pz + d.kin[id[4]].R;
-      const double Zmax = d.kin[id[5]].pz - d.kin[id[5]].R;
-      const double dX = fabs(Xmax - Xmin), dY = fabs(Ymax - Ymin), dZ = fabs(Zmax - Zmin);
-      const double surfX = dY * dZ, surfY = dX * dZ, surfZ = dY * dX;
-      d.ctrl[d.servo_icontr + 1].value = -d.servo_pressure * surfX;
-      if (d.servo == 1) d.ctrl[d.servo_icontr + 3].value = -d.servo_pressure * surfY;
-      d.ctrl[d.servo_icontr + 5].value = -d.servo_pressure * surfZ;
-    }
-    for (int c = 0; c < d.nctrl; c++) {
-      const DevControl C = d.ctrl[c];
-      BodyKin* p = d.kin + C.i;
-      switch (C.type) {
-        case C_X_VEL:
-          p->vx = C.value;
-          p->px += sp.dt * C.value;
-          break;
-        case C_Y_VEL:
-          p->vy = C.value;
-          p->py += sp.dt * C.value;
-          break;
-        case C_Z_VEL:
-          p->vz = C.value;
-          p->pz += sp.dt * C.value;
-          break;
-        case C_XROT_VEL:
-        case C_YROT_VEL:
-        case C_ZROT_VEL:
-        case C_XYZROT_VEL: {
-          v3 w;
-          if (C.type == C_XROT_VEL)
-            w = V(C.value, 0.0, 0.0);
-          else if (C.type == C_YROT_VEL)
-            w = V(0.0, C.value, 0.0);
-          else if (C.type == C_ZROT_VEL)
-            w = V(0.0, 0.0, C.value);
-          else
-            w = V(C.vx, C.vy, C.vz);
-          qt Q = kq(*p);
-          Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
-          qnormalize(&Q);
-          p->qw = Q.w;
-          p->qx = Q.x;
-          p->qy = Q.y;
-          p->qz = Q.z;
-        } break;
-        default: break;
-      }
-    }
-    for (int c = 0; c < d.nctrl; c++) {
-      const DevControl C = d.ctrl[c];
-      BodyKin* p = d.kin + C.i;
-      const double* a = d.acc + 3 * (size_t)C.i;
-      const double* ar = d.arot + 3 * (size_t)C.i;
-      switch (C.type) {
-        case C_X_FOR:
-          p->px += sp.dt * p->vx + sp.dt2_2 * a[0];
-          p->vx += sp.dt_2 * a[0];
-          break;
-        case C_Y_FOR:
-          p->py += sp.dt * p->vy + sp.dt2_2 * a[1];
-          p->vy += sp.dt_2 * a[1];
-          break;
-        case C_Z_FOR:
-          p->pz += sp.dt * p->vz + sp.dt2_2 * a[2];
-          p->vz += sp.dt_2 * a[2];
-          break;
-        case C_XROT_MOM:
-        case C_YROT_MOM:
-        case C_ZROT_MOM:
-        case C_XYZROT_MOM: {
-          v3 w;
-          if (C.type == C_XROT_MOM)
-            w = V(p->wx * 1.0, p->wx * 0.0, p->wx * 0.0);
-          else if (C.type == C_YROT_MOM)
-            w = V(p->wy * 0.0, p->wy * 1.0, p->wy * 0.0);
-          else if (C.type == C_ZROT_MOM)
-            w = V(p->wz * 0.0, p->wz * 0.0, p->wz * 1.0);
-          else
-            w = V(p->wx, p->wy, p->wz);
-          qt Q = kq(*p);
-          Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
-          qnormalize(&Q);
-          p->qw = Q.w;
-          p->qx = Q.x;
-          p->qy = Q.y;
-          p->qz = Q.z;
-          if (C.type == C_XROT_MOM)
-            p->wx += sp.dt_2 * ar[0];
-          else if (C.type == C_YROT_MOM)
-            p->wy += sp.dt_2 * ar[1];
-          else if (C.type == C_ZROT_MOM)
-            p->wz += sp.dt_2 * ar[2];
-          else {
-            v3 nw = vadd(V(p->wx, p->wy, p->wz), vsc(sp.dt_2, V(ar[0], ar[1], ar[2])));
-            p->wx = nw.x;
-            p->wy = nw.y;
-            p->wz = nw.z;
-          }
-        } break;
-        default: break;
-      }
-    }
-  }
+  if (gid == 0 && d.nctrl > 0) drive_controls(d, sp);
   // free bodies (Rockable.cpp:2036-2055), a grid-stride loop
   for (int b = gid; b < d.nb; b += gridDim.x * blockDim.x) {
     if (b < d.nDriven) continue;
     BodyKin P = d.kin[b];
-    const v3 a = ld3(d.acc + 3 * (size_t)b);
-    const v3 ar = ld3(d.arot + 3 * (size_t)b);
-    v3 pos = vadd(kpos(P), vadd(vsc(sp.dt, kvel(P)), vsc(sp.dt2_2, a)));
-    v3 vel = vadd(kvel(P), vsc(sp.dt_2, a));
-    qt Q = kq(P);
-    const v3 w = kvrot(P);
-    Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
-    Q = qadd_scaled(Q, qddot(Q, w, ar), sp.dt2_2);
-    qnormalize(&Q);
-    v3 w2 = vadd(w, vsc(sp.dt_2, ar));
-    BodyKin* out = d.kin + b;
-    out->px = pos.x;
-    out->py = pos.y;
-    out->pz = pos.z;
-    out->qw = Q.w;
-    out->qx = Q.x;
-    out->qy = Q.y;
-    out->qz = Q.z;
-    out->vx = vel.x;
-    out->vy = vel.y;
-    out->vz = vel.z;
-    out->wx = w2.x;
-    out->wy = w2.y;
-    out->wz = w2.z;
+    predict_free(d, sp, b, P);
+    store_predicted(d, b, P);
     if (glob && d.soff[b + 1] != d.soff[b]) {
-      P.px = pos.x;
-      P.py = pos.y;
-      P.pz = pos.z;
-      P.qw = Q.w;
-      P.qx = Q.x;
-      P.qy = Q.y;
-      P.qz = Q.z;
       const int v0 = d.bvoff[b], nv = d.bvoff[b + 1] - v0;
       for (int v = 0; v < nv; v++) st3(d.gv + 3 * (size_t)(v0 + v), glob_vertex(d.sh, P, v));
     }
+  }
+}
+
+// The predictor with one thread per vertex of the free bodies, each
+// wavefront holding whole bodies (wavefront w: vertices [wvstart[w],
+// wvstart[w + 1])): every lane computes its body's new state, the lane of
+// vertex 0 stores it and every lane of a listed body its global vertex.  The
+// lanes of a body read its old state in the same load instruction, before
+// any of them stores, so the body is read whole before it is overwritten.
+// The driven bodies (controls by thread 0) and their vertices are left to
+// k_glob after it.
+__global__ void __launch_bounds__(kBlock) k_predict_vtx(Dev d, StepParams sp) {
+  if (frozen(d)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && d.nctrl > 0) drive_controls(d, sp);
+  const int lane = threadIdx.x & 63;
+  for (int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wv < d.nwv;
+       wv += (int)((gridDim.x * blockDim.x) >> 6)) {
+    const int k = d.wvstart[wv] + lane;
+    if (k >= d.wvstart[wv + 1]) continue;
+    const int b = d.bvbody[k];
+    const int v = k - d.bvoff[b];
+    const bool listed = d.soff[b + 1] != d.soff[b];
+    if (v != 0 && !listed) continue;
+    BodyKin P = d.kin[b];
+    predict_free(d, sp, b, P);
+    if (v == 0) store_predicted(d, b, P);
+    if (listed) st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, v));
   }
 }
 
@@ -1513,8 +1557,15 @@ void launch_entry_desc(const Dev& d, hipStream_t s) {
 
 void launch_predict(const Dev& d, const StepParams& sp, int glob, hipStream_t s) {
   const int n = d.nb > 0 ? d.nb : 1, bs = body_block(d);
-  const bool listed = glob && d.soff;     // else every body's vertices
-  const int fused = listed && d.fuse_glob;  // the free bodies' in k_predict
+  const bool listed = glob && d.soff;  // else every body's vertices
+  if (listed && d.fuse_glob == 2 && d.nwv > 0) {
+    // one thread per vertex of the free bodies, wavefronts of whole bodies
+    const int wpb = kBlock / 64;
+    hipLaunchKernelGGL(k_predict_vtx, grid_for(d, (d.nwv + wpb - 1) / wpb), dim3(kBlock), 0, s, d, sp);
+    launch_glob(d, 0, d.nvdriven, s);
+    return;
+  }
+  const int fused = listed && d.fuse_glob != 0;  // the free bodies' in k_predict
   hipLaunchKernelGGL(k_predict, grid_for(d, (n + bs - 1) / bs), dim3(bs), 0, s, d, sp, fused);
   launch_glob(d, listed ? 0 : 1, fused ? d.nvdriven : d.nvtot, s);
 }
