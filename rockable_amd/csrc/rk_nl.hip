@@ -684,9 +684,11 @@ __device__ __forceinline__ int group_sum(int x) {
 // P candidate pairs per wavefront (64 / P lanes each, LDS stride `stride`
 // bytes per pair): the work of a pair is mostly a fixed sequence of
 // wavefront instructions (gates, compactions, one pass per predicate, the
-// key output), which the pairs of a wavefront then share.
+// key output), which the pairs of a wavefront then share.  Occupancy asked
+// of the compiler: 6 wavefronts per SIMD with one pair (80 VGPRs, no
+// spill), 3 with two.
 template <int P>
-__global__ void __launch_bounds__(64) k_narrow(Dev d, const double* bobb, const double* baabb, NarrowCfg cfg, int nc,
+__global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const double* bobb, const double* baabb, NarrowCfg cfg, int nc,
                                                const uint64_t* cand, uint32_t* nearbuf, int* near_cnt,
                                                int near_rcap, int32_t* cbase, int32_t* cnf, int32_t* cnb,
                                                int stride) {

@@ -934,8 +934,10 @@ __device__ __forceinline__ double sum_slots(const Dev& d, bool run, int k0, int 
 // and store each chunk's partial sums (from -0.0, the identity of addition)
 // for k_heavy_finish.  With guard, the displacement check accumulates each
 // finished body and every workgroup stores its maxima in gpart[2 blockIdx.x].
+// Occupancy asked of the compiler: 5 wavefronts per SIMD without a body force
+// (96 VGPRs, no spill), 3 with one (its code needs more registers).
 template <int L, bool BF>  // BF: the deck has a body force
-__global__ void __launch_bounds__(kBlock) k_gather(Dev d, StepParams sp, int kick, int guard) {
+__global__ void __launch_bounds__(kBlock, BF ? 3 : 5) k_gather(Dev d, StepParams sp, int kick, int guard) {
   constexpr int kPer = 64 / L;  // bodies per slot group (chunks per chunk wavefront)
   __shared__ unsigned long long red[2][kBlock / 64];
   if (frozen(d)) return;
