@@ -218,11 +218,21 @@ def _kat_case(name, d):
         return path, drv, 4000, True
     if name == "dynamic_nl":
         return K.single_sphere(d, vel=(0.3, 0, 0), dynamic=(0.01, 45.0)), None, 2000, False
+    if name == "moment_controls":
+        # a plate spun by imposed moments about one axis, then about all three
+        # (the rotational branch of the driven bodies' accelerations and kick)
+        path, drv = K.driven_plate(d, ["Control _zrot_Mom_ 0 2.0", "Control _x_For_ 0 50.0",
+                                       "Control _xrot_Vel_ 0 0.3"])
+        return path, drv, 3000, True
+    if name == "moment_xyz":
+        path, drv = K.driven_plate(d, ["Control _xyzrot_Mom_ 0 1.0 0.5 -0.25", "Control _y_Vel_ 0 0.2"])
+        return path, drv, 3000, True
     raise ValueError(name)
 
 
 @pytest.mark.parametrize("name", ["avalanche_restitution", "avalanche_friction", "bond_breaks", "bond_holds",
-                                  "attracting_point", "viscous_fluid", "controls", "dynamic_nl"])
+                                  "attracting_point", "viscous_fluid", "controls", "dynamic_nl", "moment_controls",
+                                  "moment_xyz"])
 def test_known_answer_decks_bit_exact(tmp_path, name):
     """The decks that pin the Avalanche and StickedLinks laws, breakage, the
     body forces, the driving controls and dynamicUpdateNL against physics:
