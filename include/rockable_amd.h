@@ -126,7 +126,8 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * "model_scale" (the rebuild-interval network's proposals are multiplied by
  * it; default 0.75),
  * launch shapes: "fuse_law" (1, the default: contact kinematics and force law
- * in one kernel; 0: two), "law_group" (geometry blocks per force-law
+ * in one kernel; 0: two), "type_sort" (1, the default: the fused kernel takes
+ * a block's entries by contact type; 0: in list order), "law_group" (geometry blocks per force-law
  * wavefront when not fused: 1, 2, 4, 8; default 1), "geom_block"
  * (contact-kinematics workgroup when not fused: 64, 128, 256, 512; default
  * 256), "gather_lanes" (lanes per body in the resultant sums: 8, 16; default
@@ -146,7 +147,7 @@ int rk_set_option(rk_engine* eng, const char* name, double value);
 int rk_get_option(rk_engine* eng, const char* name, double* value);
 /* Autotuner (no counterpart in the reference) of fuse_law, law_group,
  * geom_block, gather_lanes, gather_groups, fuse_glob, gather_waves,
- * body_block, persistent and graph_steps, one after the other: the
+ * body_block, persistent, graph_steps and type_sort, one after the other: the
  * simulation runs on, trial after trial of steps_per_trial steps, each
  * candidate value twice, and the fastest (wall time per step, rebuilds
  * excluded) is kept.  Every candidate computes the same state, so the steps

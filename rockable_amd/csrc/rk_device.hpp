@@ -197,6 +197,7 @@ struct Dev {
   int fuse_glob;          // the free bodies' global vertices: 0 k_glob, 1 k_predict per body, 2 k_predict_vtx
   int body_block;         // workgroup size of the per-body kernels (64, 128, 256)
   int fuse_law;           // 1: geometry and law in one kernel (k_interact)
+  int type_sort;          // 1: k_interact takes a block's entries by contact type
   int persist;            // step kernels' grid cap (0: one tile per workgroup)
   // device displacement guard (dynamicUpdateNL)
   int guard_on;

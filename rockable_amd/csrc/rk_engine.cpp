@@ -44,6 +44,7 @@ struct rk_engine {
   // 0 / -1: chosen from the packing's size (large packings: 64 bodies per
   // resultant wavefront, global vertices written by the predictor)
   int gather_groups = 0, fuse_glob = -1;
+  int type_sort = 1;
   int persist = 0;
   // host wall time of the rebuilds while the autotuner runs (it synchronises
   // after each rebuild then)
@@ -345,6 +346,7 @@ void assemble_dev(rk_engine* eng) {
   d.body_block = eng->body_block;
   d.fuse_law = eng->fuse_law;
   d.gather_lanes = eng->gather_lanes;
+  d.type_sort = eng->type_sort;
   {
     const bool large = d.nb >= kLargePacking;
     const int g = eng->gather_groups > 0 ? eng->gather_groups : large ? eng->gather_lanes : 1;
@@ -1501,6 +1503,10 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     if (!(value == 8.0 || value == 16.0)) return fail(eng, "gather_lanes must be 8 or 16");
     eng->gather_lanes = (int)value;
     if (eng->loaded) assemble_dev(eng);
+  } else if (n == "type_sort") {
+    if (!(value == 0.0 || value == 1.0)) return fail(eng, "type_sort must be 0 or 1");
+    eng->type_sort = (int)value;
+    if (eng->loaded) assemble_dev(eng);
   } else if (n == "narrow_pairs") {
     if (!(value == 0.0 || value == 1.0 || value == 2.0))
       return fail(eng, "narrow_pairs must be 0 (by shape size), 1 or 2");
@@ -1597,6 +1603,7 @@ int* tuned_option(rk_engine* eng, const std::string& n) {
   if (n == "gather_lanes") return &eng->gather_lanes;
   if (n == "gather_groups") return &eng->gather_groups;
   if (n == "narrow_pairs") return &eng->w.narrow_pairs;
+  if (n == "type_sort") return &eng->type_sort;
   if (n == "fuse_glob") return &eng->fuse_glob;
   if (n == "persistent") return &eng->persist;
   return nullptr;
@@ -1613,6 +1620,7 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
     std::vector<int> values;
   };
   const Knob knobs[] = {{"fuse_law", {0, 1}},
+                        {"type_sort", {1, 0}},
                         {"law_group", {1, 2, 4, 8}},
                         {"geom_block", {64, 128, 256, 512}},
                         {"gather_lanes", {8, 16}},

@@ -384,13 +384,15 @@ __device__ void law_sticked(const Dev& d, const StepParams& sp, const Kin& k, do
 // The geometry pass of one block of BS entries (block index blk): returns
 // whether this thread's entry is active, with its kinematics in a.  Every
 // thread of the block calls it (it synchronises the block).
-template <bool GEOM_ALL, int BS>
+template <bool GEOM_ALL, int BS, bool SORT = true>
 __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5], short* perm, ActiveKin& a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // The block's entries are taken in the order of their contact type (a
-  // counting sort in LDS; entries past the end last), so that a wavefront
-  // runs one branch of update_kinematics instead of all four.
-  {
+  // SORT: the block's entries are taken in the order of their contact type
+  // (a counting sort in LDS; entries past the end last), so that a wavefront
+  // runs one branch of update_kinematics instead of all four; without it
+  // each thread takes its own entry, one dependent load and two barriers
+  // sooner, its wavefront running the branches of the types it holds.
+  if (SORT) {
     const int e0 = blk * BS + (int)threadIdx.x;
     const int ty = e0 < d.ne ? (int)(d.esub[e0] >> 30) : 4;
     unsigned long long mine = 0ull;
@@ -408,7 +410,7 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
     perm[rank] = (short)threadIdx.x;
     __syncthreads();
   }
-  const int e = blk * BS + perm[threadIdx.x];
+  const int e = blk * BS + (SORT ? (int)perm[threadIdx.x] : (int)threadIdx.x);
   bool act = false;
   if (e < d.ne) {
     const int i = d.ei[e], j = d.ej[e];
@@ -651,7 +653,7 @@ __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_
 // of the global active list, and the block's first threads apply the law to
 // them after the geometry pass (no second launch, no round trip of the
 // active kinematics through memory).
-template <int LAW, bool GEOM_ALL>
+template <int LAW, bool GEOM_ALL, bool SORT>
 __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int write_geom_all, int pii) {
   __shared__ int wcount[kBlock / 64];
   __shared__ int wtype[kBlock / 64][5];
@@ -660,7 +662,7 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
   if (frozen(d)) return;
   for (TileLoop L = xcd_tiles((d.ne + kBlock - 1) / kBlock); L.t < L.end; L.t += L.step) {
     ActiveKin a;
-    const bool act = geom_pass<GEOM_ALL, kBlock>(d, L.t, wtype, perm, a);
+    const bool act = geom_pass<GEOM_ALL, kBlock, SORT>(d, L.t, wtype, perm, a);
     int total = 0;
     const int r = active_rank<kBlock>(act, wcount, &total);
     if (act) sa[r] = a;
@@ -1581,15 +1583,24 @@ void launch_geom(const Dev& d, int write_geom_all, hipStream_t s) {
     hipLaunchKernelGGL((k_geom<false, BS>), g, bl, 0, s, d);
 }
 
-template <bool GEOM_ALL>
-void launch_fused(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
+template <bool GEOM_ALL, bool SORT>
+void launch_fused_sorted(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
   const dim3 g = grid_for(d, (d.ne + kBlock - 1) / kBlock), bl(kBlock);
   if (sp.law == LAW_AVALANCHE)
-    hipLaunchKernelGGL((k_interact<LAW_AVALANCHE, GEOM_ALL>), g, bl, 0, s, d, sp, write_geom_all, 0);
+    hipLaunchKernelGGL((k_interact<LAW_AVALANCHE, GEOM_ALL, SORT>), g, bl, 0, s, d, sp, write_geom_all, 0);
   else if (sp.law == LAW_STICKED)
-    hipLaunchKernelGGL((k_interact<LAW_STICKED, GEOM_ALL>), g, bl, 0, s, d, sp, write_geom_all, d.paramsInInterfaces);
+    hipLaunchKernelGGL((k_interact<LAW_STICKED, GEOM_ALL, SORT>), g, bl, 0, s, d, sp, write_geom_all,
+                       d.paramsInInterfaces);
   else
-    hipLaunchKernelGGL((k_interact<LAW_DEFAULT, GEOM_ALL>), g, bl, 0, s, d, sp, write_geom_all, 0);
+    hipLaunchKernelGGL((k_interact<LAW_DEFAULT, GEOM_ALL, SORT>), g, bl, 0, s, d, sp, write_geom_all, 0);
+}
+
+template <bool GEOM_ALL>
+void launch_fused(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
+  if (d.type_sort)
+    launch_fused_sorted<GEOM_ALL, true>(d, sp, write_geom_all, s);
+  else
+    launch_fused_sorted<GEOM_ALL, false>(d, sp, write_geom_all, s);
 }
 
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {

@@ -186,7 +186,7 @@ class Engine:
         self._check(self.L.rk_get_option(self.h, name.encode(), ctypes.byref(v)))
         return v.value
 
-    TUNED = ("fuse_law", "law_group", "geom_block", "gather_lanes", "gather_groups", "fuse_glob", "gather_waves",
+    TUNED = ("fuse_law", "type_sort", "law_group", "geom_block", "gather_lanes", "gather_groups", "fuse_glob", "gather_waves",
              "body_block", "persistent",
              "graph_steps")
 

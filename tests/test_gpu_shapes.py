@@ -1,6 +1,6 @@
 """Launch shapes and the autotuner: every choice of fuse_law, law_group,
 geom_block, gather_lanes, gather_groups, fuse_glob, gather_waves,
-body_block, persistent, graph_steps and narrow_pairs (candidate pairs per
+body_block, persistent, graph_steps, type_sort and narrow_pairs (candidate pairs per
 narrow-phase wavefront; the default for these hulls is 1) computes the
 same state bit for bit (the default engine, against which they are compared, fuses the
 geometry and law kernels; the first three shapes use them apart), and the
@@ -26,7 +26,7 @@ SHAPES = [dict(fuse_law=0, law_group=1, geom_block=64, gather_waves=1, body_bloc
           dict(gather_lanes=8, gather_groups=16, fuse_glob=0, gather_waves=2),
           dict(narrow_pairs=2),
           dict(narrow_pairs=2, gather_lanes=8),
-          dict(fuse_glob=2),
+          dict(fuse_glob=2, type_sort=0),
           dict(fuse_glob=2, persistent=8, gather_groups=4, graph_steps=4)]
 
 
@@ -59,7 +59,7 @@ def test_bad_launch_shape_rejected(tmp_path):
     e = Engine(_hull_deck(tmp_path))
     for name, v in (("geom_block", 96), ("law_group", 3), ("gather_waves", 8), ("body_block", 512),
                     ("gather_lanes", 4), ("fuse_law", 2), ("model_scale", 0.0), ("persistent", 7),
-                    ("gather_groups", 3), ("fuse_glob", 3), ("narrow_pairs", 4)):
+                    ("gather_groups", 3), ("fuse_glob", 3), ("narrow_pairs", 4), ("type_sort", 2)):
         with pytest.raises(RuntimeError):
             e.set_option(name, v)
 
