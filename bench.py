@@ -419,8 +419,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
         return
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (how the step would shard over "
-              "GPUs is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+        print("bench.py: the engine runs on one GPU; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
     def log(msg):
