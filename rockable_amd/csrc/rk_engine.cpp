@@ -473,6 +473,9 @@ int upload_model(rk_engine* eng) {
   eng->nlp.maxe = maxne;
   eng->nlp.maxf = maxnf;
   eng->nlp.maxwords = (int)((maxT + 63) / 64);
+  eng->nlp.has_points = 0;
+  for (const Shape& sh : m.shapes)
+    if (sh.nv() == 1 && sh.edge.empty() && sh.nf() == 0) eng->nlp.has_points = 1;
   const size_t lds = narrow_lds_bytes(eng->nlp);
   if (lds > 65536) return fail(eng, "shapes have too many sub-elements for the narrow-phase LDS budget");
   if (upload(eng, eng->sh_nv, nv) || upload(eng, eng->sh_ne, ne) || upload(eng, eng->sh_nf, nf) ||

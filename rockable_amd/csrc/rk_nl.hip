@@ -681,6 +681,59 @@ __device__ __forceinline__ int group_sum(int x) {
   return x;
 }
 
+// Candidate pairs of two point shapes (one vertex, no edge or face: the
+// spheres), one pair per lane.  Their only test is vertex-vertex, so their
+// result is k_narrow's computed directly: the key (VV, 0, 0) when i's vertex
+// passes the reference's gate against j's OBB, j's vertex lies within sum of
+// i's box and VertexIsNearVertex holds.  A wavefront's near keys go to one
+// stripe (one atomic); rest[c] flags the other candidates for k_narrow.
+__global__ void __launch_bounds__(kBlock) k_narrow_points(Dev d, const double* bobb, const double* baabb,
+                                                          NarrowCfg cfg, int nc, const uint64_t* cand,
+                                                          uint32_t* nearbuf, int* near_cnt, int near_rcap,
+                                                          int32_t* cbase, int32_t* cnf, int32_t* cnb, int32_t* rest) {
+  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int lane = threadIdx.x & 63;
+  bool point = false, near = false;
+  if (c < nc) {
+    const uint64_t key = cand[c];
+    const int i = (int)(key >> 32), j = (int)(key & 0xffffffffu);
+    const int shi = d.kin[i].shape, shj = d.kin[j].shape;
+    point = d.sh.nv[shi] == 1 && d.sh.ne[shi] == 0 && d.sh.nf[shi] == 0 && d.sh.nv[shj] == 1 && d.sh.ne[shj] == 0 &&
+            d.sh.nf[shj] == 0;
+    rest[c] = point ? 0 : 1;
+    if (point) {
+      const double Ri = d.kin[i].R, Rj = d.kin[j].R;
+      const double *bxi = baabb + 6 * (size_t)i, *bxj = baabb + 6 * (size_t)j;
+      double xmax = 0.0;
+      for (int k = 0; k < 6; k++) xmax = fmax(fmax(xmax, fabs(bxi[k])), fabs(bxj[k]));
+      const double dmax = cfg.dmax;
+      const double sum = Ri + Rj + dmax;
+      const double margin = 1e-9 * (1.0 + xmax + sum);
+      const double* gvi = d.gv + 3 * (size_t)d.bvoff[i];
+      const double* gvj = d.gv + 3 * (size_t)d.bvoff[j];
+      const double gvv = sum + margin;
+      const v3 pj = lds_v(gvj, 0);
+      const bool jnear = pj.x >= bxi[0] - gvv && pj.x <= bxi[3] + gvv && pj.y >= bxi[1] - gvv &&
+                         pj.y <= bxi[4] + gvv && pj.z >= bxi[2] - gvv && pj.z <= bxi[5] + gvv;
+      near = jnear && vertex_gate(lds_v(gvi, 0), Ri, load_obb(bobb + 15 * (size_t)j), dmax) &&
+             near_vv(gvi, gvj, Ri, Rj, 0, 0, dmax);
+    }
+  }
+  const unsigned long long m = __ballot(near);
+  const int stripe = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & (kStripes - 1);
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(near_cnt + stripe * kStripeStride, __popcll(m));
+  base = __shfl(base, 0, 64);
+  if (point) {
+    const int local = base + __popcll(m & ((1ull << lane) - 1ull));
+    cbase[c] = stripe * near_rcap + local;
+    cnf[c] = near ? 1 : 0;
+    cnb[c] = 0;
+    // over capacity: the host re-runs with a larger buffer
+    if (near && local < near_rcap) nearbuf[stripe * near_rcap + local] = pack_sub(VV, 0, 0);
+  }
+}
+
 // P candidate pairs per wavefront (64 / P lanes each, LDS stride `stride`
 // bytes per pair): the work of a pair is mostly a fixed sequence of
 // wavefront instructions (gates, compactions, one pass per predicate, the
@@ -691,16 +744,19 @@ template <int P>
 __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const double* bobb, const double* baabb, NarrowCfg cfg, int nc,
                                                const uint64_t* cand, uint32_t* nearbuf, int* near_cnt,
                                                int near_rcap, int32_t* cbase, int32_t* cnf, int32_t* cnb,
-                                               int stride) {
+                                               int stride, const int32_t* clist) {
   constexpr int SW = 64 / P;
   extern __shared__ double lds_all[];
   const int lane = threadIdx.x;
   // (with one pair per wavefront the pair's values stay wavefront-uniform,
   // in scalar registers)
   const int g = P == 1 ? 0 : lane / SW, sl = P == 1 ? lane : lane % SW;
-  const int c = xcd_block((int)blockIdx.x, (int)gridDim.x) * P + g;
-  const bool valid = c < nc;
+  // the idx-th pair to test: candidate clist[idx] (all candidates in order
+  // without a list)
+  const int idx = xcd_block((int)blockIdx.x, (int)gridDim.x) * P + g;
+  const bool valid = idx < nc;
   if (P == 1 && !valid) return;
+  const int c = valid ? (clist ? clist[idx] : idx) : 0;
   double* lds = lds_all + (size_t)g * (stride / 8);
   uint64_t key = 0;
   if (valid) key = cand[c];
@@ -1462,17 +1518,36 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     if (P <= 0) P = 2 * p.maxv <= 16 ? 2 : 1;
     while (P > 1 && (size_t)P * stride > 65536) P >>= 1;
     const size_t lds = (size_t)P * stride;
-    const dim3 grid((nc + P - 1) / P);
     for (int attempt = 0; attempt < 3; attempt++) {
       const int rcap = std::max(64, (w.near_hint + kStripes - 1) / kStripes);
       if (ensure(w, w.nearbuf, w.nearbuf_cap, (size_t)kStripes * rcap)) return -1;
       RK_CHECK(hipMemsetAsync(w.scnt, 0, sizeof(int) * kStripes * kStripeStride, s));
-      if (P == 2)
-        hipLaunchKernelGGL(k_narrow<2>, grid, dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nc, w.cand, w.nearbuf,
-                           w.scnt, rcap, w.cbase, w.cnf, w.cnb, (int)stride);
-      else
-        hipLaunchKernelGGL(k_narrow<1>, grid, dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nc, w.cand, w.nearbuf,
-                           w.scnt, rcap, w.cbase, w.cnf, w.cnb, (int)stride);
+      // pairs of point shapes one per lane; the others (all of them without
+      // point shapes) one or two per wavefront
+      int nrest = nc;
+      const int32_t* clist = nullptr;
+      if (p.has_points) {
+        if (ensure(w, w.pflag, w.pflag_cap, (size_t)nc) || ensure(w, w.ppos, w.ppos_cap, (size_t)nc) ||
+            ensure(w, w.plist, w.plist_cap, (size_t)nc))
+          return -1;
+        hipLaunchKernelGGL(k_narrow_points, dim3(nblk(nc)), dim3(kBlock), 0, s, d, w.bobb, w.baabb, cfg, nc, w.cand,
+                           w.nearbuf, w.scnt, rcap, w.cbase, w.cnf, w.cnb, w.pflag);
+        if (scan_exclusive(w, w.pflag, w.ppos, nc, s)) return -1;
+        hipLaunchKernelGGL(k_heavy_list, dim3(nblk(nc)), dim3(kBlock), 0, s, nc, w.pflag, w.ppos, w.plist);
+        int last[2] = {0, 0};
+        RK_CHECK(hipMemcpyAsync(&last[0], w.ppos + nc - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        RK_CHECK(hipMemcpyAsync(&last[1], w.pflag + nc - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        RK_CHECK(hipStreamSynchronize(s));
+        nrest = last[0] + last[1];
+        clist = w.plist;
+      }
+      const dim3 grid((nrest + P - 1) / P);
+      if (nrest > 0 && P == 2)
+        hipLaunchKernelGGL(k_narrow<2>, grid, dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nrest, w.cand, w.nearbuf,
+                           w.scnt, rcap, w.cbase, w.cnf, w.cnb, (int)stride, clist);
+      else if (nrest > 0)
+        hipLaunchKernelGGL(k_narrow<1>, grid, dim3(64), lds, s, d, w.bobb, w.baabb, cfg, nrest, w.cand, w.nearbuf,
+                           w.scnt, rcap, w.cbase, w.cnf, w.cnb, (int)stride, clist);
       RK_CHECK(hipGetLastError());
       hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
       int tm[2] = {0, 0};
@@ -1635,7 +1710,8 @@ void entries_free(EntryArrays& E) {
 }
 
 void nl_work_free(NLWork& w) {
-  void* ps[] = {w.cstripe, w.scnt, w.soff, w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
+  void* ps[] = {w.pflag, w.ppos, w.plist,
+                w.cstripe, w.scnt, w.soff, w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
                 w.gk,    w.gk2,   w.gv,      w.gv2,   w.gflag, w.gpos,  w.ukey,  w.ucand,   w.uold,  w.gcount,
                 w.goff,  w.tmp,   w.i32a,    w.i32b,  w.i32c,  w.f64a,  w.cell_of, w.ccount, w.cstart, w.cfill,
                 w.cbody, w.over, w.gedge, w.gface, w.kout};

@@ -61,6 +61,7 @@ struct NLParams {
   int maxe = 0;      // max edges per shape
   int maxf = 0;      // max faces per shape
   int maxwords = 1;  // max 64-bit words of predicate bits per body pair
+  int has_points = 0;  // some shape is a point (one vertex, no edge or face)
 };
 
 // LDS of one narrow-phase pair: both bodies' OBBs and boxes, the global
@@ -81,6 +82,9 @@ struct NLWork {
   size_t name##_cap = 0;
   RK_BUF(double, bobb)
   RK_BUF(double, baabb)
+  RK_BUF(int32_t, pflag)  // narrow phase: candidates that are not point pairs, their prefix, their list
+  RK_BUF(int32_t, ppos)
+  RK_BUF(int32_t, plist)
   RK_BUF(float, gedge)
   RK_BUF(float, gface)
   RK_BUF(int32_t, kout)
