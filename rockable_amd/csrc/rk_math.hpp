@@ -144,22 +144,23 @@ RK_HD void obb_enlarge(obb_t* o, double d) {
 }
 // Separating-axis test over the 15 candidate axes (Gottschalk, Lin, Manocha
 // 1996); touching boxes intersect.
+// Row i of the rotation (and t[i]) is computed just before the test on a's
+// axis i needs it, so that a pair separated on an early axis skips the rest;
+// every value is the same expression as computed all at once.
 RK_HD bool obb_intersect(const obb_t& a, const obb_t& b) {
-  double R[3][3], AR[3][3];
+  double R[3][3], AR[3][3], t[3];
   v3 d = vsub(b.c, a.c);
-  double t[3] = {vdot(d, a.e[0]), vdot(d, a.e[1]), vdot(d, a.e[2])};
   double ea[3] = {a.ext.x, a.ext.y, a.ext.z};
   double eb[3] = {b.ext.x, b.ext.y, b.ext.z};
+  double ra, rb;
 #pragma unroll
-  for (int i = 0; i < 3; i++)
+  for (int i = 0; i < 3; i++) {
+    t[i] = vdot(d, a.e[i]);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       R[i][k] = vdot(a.e[i], b.e[k]);
       AR[i][k] = fabs(R[i][k]);
     }
-  double ra, rb;
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
     ra = ea[i];
     rb = eb[0] * AR[i][0] + eb[1] * AR[i][1] + eb[2] * AR[i][2];
     if (fabs(t[i]) > ra + rb) return false;
