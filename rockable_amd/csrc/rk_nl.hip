@@ -658,6 +658,14 @@ __device__ __forceinline__ bool face_gate_pre(const float* g, double sum, double
 
 
 
+// floor(k / n) for 0 <= k < 2^20 and n >= 1, with inv = 1 / n in FP32:
+// (k + 0.5) / n lies at least 0.5 / n from an integer, and the FP32 product
+// is within 3e-7 of it relative, less than that for k < 2^20.  Larger k
+// (shapes of thousands of sub-elements) take the integer division.
+__device__ __forceinline__ int quot_small(int k, int n, float inv) {
+  return k < (1 << 20) ? (int)(((float)k + 0.5f) * inv) : k / n;
+}
+
 // Compaction of the indices v < n with pred(v) into list (in ascending
 // order) by the SW lanes of one pair (sl: lane within them, g: the pair's
 // group of lanes in the wavefront); returns the count.
@@ -863,48 +871,60 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
 
   // one loop per segment, so that a wavefront runs one predicate at a time
   int n = nvi_g * nvj_n;
+  float inv = 1.0f / (float)max(nvj_n, 1);
   for (int k = sl; k < n; k += SW) {
-    const int a = vi_l[k / nvj_n], b = vjn_l[k % nvj_n];
+    const int q = quot_small(k, nvj_n, inv);
+    const int a = vi_l[q], b = vjn_l[k - q * nvj_n];
     if (near_vv(gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = a * nvj + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
   n = nvi_g * nej_g;
+  inv = 1.0f / (float)max(nej_g, 1);
   for (int k = sl; k < n; k += SW) {
-    const int a = vi_l[k / nej_g], b = ej_l[k % nej_g] - nei;
+    const int q = quot_small(k, nej_g, inv);
+    const int a = vi_l[q], b = ej_l[k - q * nej_g] - nei;
     if (near_ve(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = s1 + a * nej + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
   n = nvi_g * nfj_g;
+  inv = 1.0f / (float)max(nfj_g, 1);
   for (int k = sl; k < n; k += SW) {
-    const int a = vi_l[k / nfj_g], b = fj_l[k % nfj_g] - ne2 - nfi;
+    const int q = quot_small(k, nfj_g, inv);
+    const int a = vi_l[q], b = fj_l[k - q * nfj_g] - ne2 - nfi;
     if (near_vf(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = s2 + a * nfj + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
   n = nei_g * nej_g;
+  inv = 1.0f / (float)max(nej_g, 1);
   for (int k = sl; k < n; k += SW) {
-    const int a = ei_l[k / nej_g], b = ej_l[k % nej_g] - nei;
+    const int q = quot_small(k, nej_g, inv);
+    const int a = ei_l[q], b = ej_l[k - q * nej_g] - nei;
     if (near_ee(d.sh, shi, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = s3 + a * nej + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
   n = nvj_g * nei_g;
+  inv = 1.0f / (float)max(nei_g, 1);
   for (int k = sl; k < n; k += SW) {
-    const int a = vj_l[k / nei_g] - nvi, b = ei_l[k % nei_g];
+    const int q = quot_small(k, nei_g, inv);
+    const int a = vj_l[q] - nvi, b = ei_l[k - q * nei_g];
     if (near_ve(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
       const int bit = s4 + a * nei + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
   n = nvj_g * nfi_g;
+  inv = 1.0f / (float)max(nfi_g, 1);
   for (int k = sl; k < n; k += SW) {
-    const int a = vj_l[k / nfi_g] - nvi, b = fi_l[k % nfi_g] - ne2;
+    const int q = quot_small(k, nfi_g, inv);
+    const int a = vj_l[q] - nvi, b = fi_l[k - q * nfi_g] - ne2;
     if (near_vf(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
       const int bit = s5 + a * nfi + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
