@@ -1001,21 +1001,24 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
       }
     }
     const int t = lo * 64 + bit;
-    int type, own, oth;
+    // the segment of test t: its type, first bit and count of "other" elements
+    int type, first, noth;
     if (t < s1) {
-      type = VV, own = t / nvj, oth = t % nvj;
+      type = VV, first = 0, noth = nvj;
     } else if (t < s2) {
-      type = VE, own = (t - s1) / nej, oth = (t - s1) % nej;
+      type = VE, first = s1, noth = nej;
     } else if (t < s3) {
-      type = VF, own = (t - s2) / nfj, oth = (t - s2) % nfj;
+      type = VF, first = s2, noth = nfj;
     } else if (t < s4) {
-      type = EE, own = (t - s3) / nej, oth = (t - s3) % nej;
+      type = EE, first = s3, noth = nej;
     } else if (t < s5) {
-      type = VE, own = (t - s4) / nei, oth = (t - s4) % nei;
+      type = VE, first = s4, noth = nei;
     } else {
-      type = VF, own = (t - s5) / nfi, oth = (t - s5) % nfi;
+      type = VF, first = s5, noth = nfi;
     }
-    nearbuf[base_out + q] = pack_sub(type, own, oth);
+    const int rt = t - first;
+    const int own = quot_small(rt, noth, 1.0f / (float)noth);
+    nearbuf[base_out + q] = pack_sub(type, own, rt - own * noth);
   }
 }
 
