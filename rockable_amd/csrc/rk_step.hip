@@ -879,27 +879,28 @@ __device__ __forceinline__ void guard_body(const Dev& d, const StepParams& sp, i
 __device__ __forceinline__ bool guard_reset(const Dev& d) { return *(volatile const int*)&d.guard->reset != 0; }
 
 // compute_resultants (Rockable.cpp:3267-3272).  L lanes per body (8 or 16;
-// 64 / L bodies per wavefront).  The lanes read the slot activity bytes 4 L
-// at a time (four each) and join them into one mask; lanes 0-5, one per
+// 64 / L bodies per wavefront).  The lanes read the slot activity bytes 64
+// at a time (64 / L each) and join them into one mask; lanes 0-5, one per
 // component of (F, M), then fetch the terms of the active slots eight at a
 // time (independent loads) and add them in slot order.  Called by every
 // lane of the wavefront; returns the updated sum in lanes gl < 6.
-template <int L>  // lanes per body: 8 or 16 (windows of 4 L slots)
+template <int L>  // lanes per body: 8 or 16 (windows of 64 slots, 64 / L per lane)
 __device__ __forceinline__ double sum_slots(const Dev& d, bool run, int k0, int k1, int gl, double s) {
+  constexpr int B = 64 / L;  // activity bytes read per lane and window
   const double* val = d.sval + (gl < 6 ? gl : 0);
   for (int it = 0;; it++) {
-    const int w = k0 + 4 * L * it;
+    const int w = k0 + 64 * it;
     const bool live = run && w < k1;
     if (!__any(live)) break;
-    unsigned m4 = 0u;
+    unsigned mb = 0u;
     if (live) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int k = w + 4 * gl + q;
-        if (k < k1 && d.sact[k]) m4 |= 1u << q;
+      for (int q = 0; q < B; q++) {
+        const int k = w + B * gl + q;
+        if (k < k1 && d.sact[k]) mb |= 1u << q;
       }
     }
-    unsigned long long rest = (unsigned long long)m4 << (4 * gl);
+    unsigned long long rest = (unsigned long long)mb << (B * gl);
     for (int o = 1; o < L; o <<= 1) rest |= __shfl_xor(rest, o, L);
     if (gl < 6) {
       while (rest) {
