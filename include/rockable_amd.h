@@ -135,8 +135,9 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * wavefront: 1, 2, 4, 8, 16, at most gather_lanes; 0, the default: 1 below
  * 262144 bodies, gather_lanes from there), "fuse_glob" (1: the predictor
  * writes the free bodies' global vertices, one thread per body; 2: one
- * thread per vertex; 0: a kernel of its own; -1, the default: 1 from 262144
- * bodies), "gather_waves" (wavefronts per resultant
+ * thread per vertex; 3: workgroups of 64 bodies, their vertices shared by the
+ * workgroup; 0: a kernel of its own; -1, the default: 1 from 262144
+ * bodies, else 3), "gather_waves" (wavefronts per resultant
  * workgroup: 1, 2, 4; default 4), "body_block" (per-body kernels: 64, 128,
  * 256; default 256), "persistent" (0, the default: one tile per workgroup;
  * n in [8, 1048576]: the step kernels' grids capped at n workgroups, each

@@ -194,7 +194,7 @@ struct Dev {
   int gather_waves;       // k_gather wavefronts per workgroup (1, 2, 4)
   int gather_lanes;       // k_gather lanes per body (8, 16)
   int gather_groups;      // k_gather slot groups per body wavefront (1 .. gather_lanes)
-  int fuse_glob;          // the free bodies' global vertices: 0 k_glob, 1 k_predict per body, 2 k_predict_vtx
+  int fuse_glob;          // the free bodies' global vertices: 0 k_glob, 1 k_predict per body, 2 k_predict_vtx, 3 k_predict_blk
   int body_block;         // workgroup size of the per-body kernels (64, 128, 256)
   int fuse_law;           // 1: geometry and law in one kernel (k_interact)
   int type_sort;          // 1: k_interact takes a block's entries by contact type

@@ -42,7 +42,8 @@ struct rk_engine {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int law_group = 1, geom_block = 256, gather_waves = 4, body_block = 256, fuse_law = 1, gather_lanes = 16;
   // 0 / -1: chosen from the packing's size (large packings: 64 bodies per
-  // resultant wavefront, global vertices written by the predictor)
+  // resultant wavefront, global vertices written by the predictor per body;
+  // smaller ones: by the predictor's workgroups of 64 bodies)
   int gather_groups = 0, fuse_glob = -1;
   int type_sort = 1;
   int persist = 0;
@@ -351,7 +352,7 @@ void assemble_dev(rk_engine* eng) {
     const bool large = d.nb >= kLargePacking;
     const int g = eng->gather_groups > 0 ? eng->gather_groups : large ? eng->gather_lanes : 1;
     d.gather_groups = std::min(g, eng->gather_lanes);
-    d.fuse_glob = eng->fuse_glob >= 0 ? eng->fuse_glob : large ? 1 : 0;
+    d.fuse_glob = eng->fuse_glob >= 0 ? eng->fuse_glob : large ? 1 : 3;
   }
   d.persist = eng->persist;
   d.istart = E.istart;
@@ -1520,8 +1521,8 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     eng->gather_groups = (int)value;
     if (eng->loaded) assemble_dev(eng);
   } else if (n == "fuse_glob") {
-    if (!(value == -1.0 || value == 0.0 || value == 1.0 || value == 2.0))
-      return fail(eng, "fuse_glob must be -1 (by size), 0, 1 or 2");
+    if (!(value == -1.0 || value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0))
+      return fail(eng, "fuse_glob must be -1 (by size), 0, 1, 2 or 3");
     eng->fuse_glob = (int)value;
     if (eng->loaded) assemble_dev(eng);
   } else if (n == "fuse_law") {
@@ -1628,7 +1629,7 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
                         {"geom_block", {64, 128, 256, 512}},
                         {"gather_lanes", {8, 16}},
                         {"gather_groups", {1, 4, 16}},
-                        {"fuse_glob", {0, 1, 2}},
+                        {"fuse_glob", {0, 1, 2, 3}},
                         {"gather_waves", {1, 2, 4}},
                         {"body_block", {64, 128, 256}},
                         {"persistent", {0, 2048, 8192}},

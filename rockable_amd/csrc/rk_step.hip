@@ -1465,6 +1465,73 @@ __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp, int gl
   }
 }
 
+// The predictor by workgroups of 64 free bodies: one thread per body
+// computes and stores its new state and keeps the pose in LDS, then the 256
+// threads share those bodies' vertices (global vertices of the listed ones).
+// The driven bodies are moved by thread 0 of workgroup 0 (controls); with
+// driven_glob, workgroup 0 then writes their listed global vertices (else
+// k_glob does after this kernel).
+__global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, int driven_glob) {
+  __shared__ double pose[64][8];  // px py pz qw qx qy qz h
+  __shared__ int shp[64];         // shape, or -1 if the body is not listed
+  if (frozen(d)) return;
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0) {
+    if (t == 0 && d.nctrl > 0) drive_controls(d, sp);
+    if (driven_glob) {
+      __syncthreads();  // the driven poses, written by thread 0, are visible to the workgroup
+      for (int k = t; k < d.nvdriven; k += blockDim.x) {
+        const int b = d.bvbody[k];
+        if (d.soff[b + 1] == d.soff[b]) continue;
+        const BodyKin& P = d.kin[b];
+        st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, k - d.bvoff[b]));
+      }
+    }
+  }
+  const int nfree = d.nb - d.nDriven;
+  for (int tile = blockIdx.x; tile * 64 < nfree; tile += gridDim.x) {
+    const int b0 = d.nDriven + tile * 64;
+    const int bend = min(b0 + 64, d.nb);
+    if (t < 64) {
+      const int b = b0 + t;
+      shp[t] = -1;
+      if (b < bend) {
+        BodyKin P = d.kin[b];
+        predict_free(d, sp, b, P);
+        store_predicted(d, b, P);
+        if (d.soff[b + 1] != d.soff[b]) {
+          pose[t][0] = P.px;
+          pose[t][1] = P.py;
+          pose[t][2] = P.pz;
+          pose[t][3] = P.qw;
+          pose[t][4] = P.qx;
+          pose[t][5] = P.qy;
+          pose[t][6] = P.qz;
+          pose[t][7] = P.h;
+          shp[t] = P.shape;
+        }
+      }
+    }
+    __syncthreads();
+    for (int k = d.bvoff[b0] + t; k < d.bvoff[bend]; k += blockDim.x) {
+      const int b = d.bvbody[k], lb = b - b0;
+      if (shp[lb] < 0) continue;
+      BodyKin P;
+      P.px = pose[lb][0];
+      P.py = pose[lb][1];
+      P.pz = pose[lb][2];
+      P.qw = pose[lb][3];
+      P.qx = pose[lb][4];
+      P.qy = pose[lb][5];
+      P.qz = pose[lb][6];
+      P.h = pose[lb][7];
+      P.shape = shp[lb];
+      st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, k - d.bvoff[b]));
+    }
+    __syncthreads();
+  }
+}
+
 // The predictor with one thread per vertex of the free bodies, each
 // wavefront holding whole bodies (wavefront w: vertices [wvstart[w],
 // wvstart[w + 1])): every lane computes its body's new state, the lane of
@@ -1563,6 +1630,15 @@ void launch_entry_desc(const Dev& d, hipStream_t s) {
 void launch_predict(const Dev& d, const StepParams& sp, int glob, hipStream_t s) {
   const int n = d.nb > 0 ? d.nb : 1, bs = body_block(d);
   const bool listed = glob && d.soff;  // else every body's vertices
+  if (listed && d.fuse_glob == 3 && d.nb > d.nDriven) {
+    // workgroups of 64 free bodies, their vertices from the poses in LDS; a
+    // few driven bodies' vertices by workgroup 0 after the controls
+    const int tiles = (d.nb - d.nDriven + 63) / 64;
+    const int driven_glob = d.nvdriven <= 1024 ? 1 : 0;
+    hipLaunchKernelGGL(k_predict_blk, grid_for(d, tiles), dim3(kBlock), 0, s, d, sp, driven_glob);
+    if (!driven_glob) launch_glob(d, 0, d.nvdriven, s);
+    return;
+  }
   if (listed && d.fuse_glob == 2 && d.nwv > 0) {
     // one thread per vertex of the free bodies, wavefronts of whole bodies
     const int wpb = kBlock / 64;

@@ -27,7 +27,9 @@ SHAPES = [dict(fuse_law=0, law_group=1, geom_block=64, gather_waves=1, body_bloc
           dict(narrow_pairs=2),
           dict(narrow_pairs=2, gather_lanes=8),
           dict(fuse_glob=2, type_sort=0),
-          dict(fuse_glob=2, persistent=8, gather_groups=4, graph_steps=4)]
+          dict(fuse_glob=2, persistent=8, gather_groups=4, graph_steps=4),
+          dict(fuse_glob=3),
+          dict(fuse_glob=3, persistent=8, gather_lanes=8, graph_steps=4)]
 
 
 def _hull_deck(tmp_path):
@@ -59,7 +61,7 @@ def test_bad_launch_shape_rejected(tmp_path):
     e = Engine(_hull_deck(tmp_path))
     for name, v in (("geom_block", 96), ("law_group", 3), ("gather_waves", 8), ("body_block", 512),
                     ("gather_lanes", 4), ("fuse_law", 2), ("model_scale", 0.0), ("persistent", 7),
-                    ("gather_groups", 3), ("fuse_glob", 3), ("narrow_pairs", 4), ("type_sort", 2)):
+                    ("gather_groups", 3), ("fuse_glob", 4), ("narrow_pairs", 4), ("type_sort", 2)):
         with pytest.raises(RuntimeError):
             e.set_option(name, v)
 
