@@ -1036,10 +1036,10 @@ __global__ void __launch_bounds__(kBlock, BF ? 3 : 5) k_gather(Dev d, StepParams
 
 // A passive heavy body (its force and moment feed nothing back into the
 // motion; exact_sums off): the initial force and moment plus its chunks'
-// partial sums, reduced by one wavefront in a fixed order (lane l adds chunks
-// l, l + 64, ... in turn, then a butterfly over the lanes; IEEE addition is
-// commutative, so every lane holds the same sum).  Deterministic; it differs
-// from the serial sum only by rounding.
+// partial sums, reduced by one workgroup of four wavefronts in a fixed order
+// (thread t adds chunks t, t + 256, ... in turn, a butterfly over each
+// wavefront's lanes, then (w0 + w1) + (w2 + w3)).  Deterministic; it
+// differs from the serial sum only by rounding.
 __device__ __forceinline__ void guard_heavy(const Dev& d, const StepParams& sp, int b, const BodyKin& P) {
   GuardAcc a;
   a.kp = 0ull;
@@ -1049,32 +1049,42 @@ __device__ __forceinline__ void guard_heavy(const Dev& d, const StepParams& sp, 
   if (a.kw != ~0ull) atomicMin(&d.guard->minw, a.kw);
 }
 
-__global__ void __launch_bounds__(64) k_heavy_finish(Dev d, StepParams sp, int kick, int guard) {
+static_assert(kBlock == 256, "k_heavy_finish combines four wavefronts");
+__global__ void __launch_bounds__(kBlock) k_heavy_finish(Dev d, StepParams sp, int kick, int guard) {
+  __shared__ double red[kBlock / 64][6];
   if (frozen(d)) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (TileLoop T = xcd_tiles(d.nheavy); T.t < T.end; T.t += T.step) {
-  const int h = T.t;
-  const int b = d.heavy[h];
-  if (!d.passive[b] || d.exact_sums) continue;
-  const int lane = threadIdx.x;
-  double a[6] = {-0.0, -0.0, -0.0, -0.0, -0.0, -0.0};
-  for (int c = d.hch_off[h] + lane; c < d.hch_off[h + 1]; c += 64) {
-    const double* p = d.hpart + 6 * (size_t)c;
+    const int h = T.t;
+    const int b = d.heavy[h];
+    if (!d.passive[b] || d.exact_sums) continue;  // the same for the whole workgroup
+    double a[6] = {-0.0, -0.0, -0.0, -0.0, -0.0, -0.0};
+    for (int c = d.hch_off[h] + t; c < d.hch_off[h + 1]; c += kBlock) {
+      const double* p = d.hpart + 6 * (size_t)c;
 #pragma unroll
-    for (int q = 0; q < 6; q++) a[q] += p[q];
-  }
+      for (int q = 0; q < 6; q++) a[q] += p[q];
+    }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-    for (int q = 0; q < 6; q++) a[q] += __shfl_xor(a[q], off, 64);
-  if (lane != 0) continue;
-  BodyKin P = d.kin[b];
-  const bool driven = b < d.nDriven;
-  v3 F, M;
-  int c0, c1;
-  init_resultants(d, b, P, driven, F, M, c0, c1);
-  finish_body(d, sp, kick, b, P, driven, vadd(F, V(a[0], a[1], a[2])), vadd(M, V(a[3], a[4], a[5])), c0, c1,
-              d.mass[b], ld3(d.inertia + 3 * (size_t)b));
-  if (guard) guard_heavy(d, sp, b, P);
+      for (int q = 0; q < 6; q++) a[q] += __shfl_xor(a[q], off, 64);
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 6; q++) red[wv][q] = a[q];
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) a[q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+      BodyKin P = d.kin[b];
+      const bool driven = b < d.nDriven;
+      v3 F, M;
+      int c0, c1;
+      init_resultants(d, b, P, driven, F, M, c0, c1);
+      finish_body(d, sp, kick, b, P, driven, vadd(F, V(a[0], a[1], a[2])), vadd(M, V(a[3], a[4], a[5])), c0, c1,
+                  d.mass[b], ld3(d.inertia + 3 * (size_t)b));
+      if (guard) guard_heavy(d, sp, b, P);
+    }
+    __syncthreads();
   }
 }
 
@@ -1738,7 +1748,7 @@ void launch_gather(const Dev& d, const StepParams& sp, int kick, int guard, hipS
   else
     hipLaunchKernelGGL((k_gather<16, false>), blocks, dim3(64 * wpb), 0, s, d, sp, kick, guard);
   if (d.nheavy > 0) {
-    hipLaunchKernelGGL(k_heavy_finish, grid_for(d, d.nheavy), dim3(64), 0, s, d, sp, kick, guard);
+    hipLaunchKernelGGL(k_heavy_finish, grid_for(d, d.nheavy), dim3(kBlock), 0, s, d, sp, kick, guard);
     if (d.nheavy_exact > 0)
       hipLaunchKernelGGL(k_gather_heavy, dim3(d.nheavy), dim3(kBlock), 0, s, d, sp, kick, guard);
   }
