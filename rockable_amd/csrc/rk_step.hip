@@ -420,17 +420,14 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
     const int stick = d.ifc.n > 0 ? d.estick[e] : -1;  // bonds exist only with interfaces
     Kin k;
     k.deact = false;
-    k.prev_dn = d.edn[e];
+    k.prev_dn = 0.0;  // the previous dn is read by the law, for active entries only
     if (stick >= 0) {  // a bond may run on stale geometry after a deactivation
       const EGeom& g = d.egeo[e];
       k.n = V(g.nx, g.ny, g.nz);
       k.pos = V(g.px, g.py, g.pz);
     }
     update_kinematics<false>(d, type, d.edesc[e], Pi, Pj, k);
-    if (k.deact) {  // Interaction::deactivate (Interaction.cpp:123-128)
-      k.dn = 0.0;
-      k.prev_dn = 0.0;
-    }
+    if (k.deact) k.dn = 0.0;  // Interaction::deactivate (Interaction.cpp:123-128)
     act = (k.dn < 0.0) || (stick >= 0);
     if (act) {
       a.nx = k.n.x;
@@ -440,7 +437,7 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
       a.py = k.pos.y;
       a.pz = k.pos.z;
       a.dn = k.dn;
-      a.prev_dn = k.prev_dn;
+      a.prev_dn = 0.0;  // (unused: the law reads the previous dn itself)
       a.e = e;
       a.deact = k.deact ? 1 : 0;
       a.i = i;
@@ -539,7 +536,8 @@ __device__ __forceinline__ void law_item(const Dev& d, const StepParams& sp, int
   k.n = V(a.nx, a.ny, a.nz);
   k.pos = V(a.px, a.py, a.pz);
   k.dn = a.dn;
-  k.prev_dn = a.prev_dn;
+  // the previous step's dn (the Avalanche law's only; a deactivation zeroes it)
+  k.prev_dn = (LAW == LAW_AVALANCHE && !a.deact) ? d.edn[e] : 0.0;
   k.deact = a.deact != 0;
   Hist h;
   if (k.deact) {  // a bond keeps its stale velocity; the history restarts
