@@ -52,8 +52,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per force-phase launch (k_interact + k_gather + k_heavy_finish on
 # the 50 005-body settled pour): rocprofv3 FETCH_SIZE and WRITE_SIZE, each
 # scaled by what it reads for a plain 256 MiB device copy (factors 2.0 and
-# 1.0), 177.2 MB read + 36.6 MB written; profiles/c2_force_phase_traffic.md
-TRAFFIC_BYTES_PER_LAUNCH = 213.8e6
+# 1.0), 171.3 MB read + 36.3 MB written; profiles/c2_force_phase_traffic.md
+TRAFFIC_BYTES_PER_LAUNCH = 207.6e6
 
 
 def make_settled_deck(n_bodies, seed, dirpath, settle_steps, relax_steps, log):
