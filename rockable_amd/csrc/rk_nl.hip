@@ -20,8 +20,11 @@
 //                    then written as keys in (type, isub, jsub) order with a
 //                    wavefront prefix scan (no atomics per key)
 //   k_merge          per directed group, merge of the sorted old entries and
-//                    the sorted near keys (count pass, scan, write pass); the
-//                    output is again sorted by (i, j, type, isub, jsub)
+//                    the sorted near keys (count pass, scan, write pass of the
+//                    keys and of each entry's source); the output is again
+//                    sorted by (i, j, type, isub, jsub)
+//   k_merge_fill     one thread per new entry: its state record, copied from
+//                    the old entry it keeps or started from zero
 //   CSR              istart by owner; jlist = entries sorted (stably) by j
 #include <hip/hip_runtime.h>
 
@@ -1048,6 +1051,7 @@ struct MergeIn {
 struct MergeOut {
   int32_t* count;   // per group (count pass)
   const int32_t* offset;  // per group (write pass)
+  int32_t* src;     // per new entry (write pass): the old entry kept, -1 for a fresh one
   int32_t* ei;
   int32_t* ej;
   uint32_t* esub;
@@ -1130,35 +1134,13 @@ __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
     }
     if (take != -1) {
       if (WRITE) {
+        // Records are filled by k_merge_fill, one thread per entry: written
+        // here, a group's records would be strided stores across the wavefront.
         const int e = cnt;
         out.ei[e] = owner;
         out.ej[e] = other;
-        if (take == 1) {
-          out.esub[e] = sb;
-          EState s;
-          s.fn = 0.0;
-          s.ftx = s.fty = s.ftz = 0.0;
-          s.momx = s.momy = s.momz = 0.0;
-          out.est[e] = s;
-          out.edn[e] = 0.0;
-          EGeom q;
-          q.nx = q.ny = q.nz = q.px = q.py = q.pz = q.vx = q.vy = q.vz = 0.0;
-          out.egeo[e] = q;
-          out.edamp[e] = pair_damp(d, pi, pj);
-          out.estick[e] = -1;
-          out.eact[e] = 0;
-        } else {
-          // a kept entry stays active as it was at the last accelerations()
-          const int o = os + a;
-          const double dn = in.oedn[o];
-          out.esub[e] = in.osub[o];
-          out.est[e] = in.oest[o];
-          out.edn[e] = dn;
-          out.egeo[e] = in.ogeo[o];
-          out.edamp[e] = in.odamp[o];
-          out.estick[e] = in.ostick[o];
-          out.eact[e] = ((dn < 0.0 || in.ostick[o] >= 0) ? 1 : 0) | 2;
-        }
+        out.esub[e] = take == 1 ? sb : in.osub[os + a];
+        out.src[e] = take == 1 ? -1 : os + a;
       }
       cnt++;
     }
@@ -1171,6 +1153,39 @@ __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
       a++;
   }
   if (!WRITE) out.count[g] = cnt;
+}
+
+// Records of the merged entries: a fresh entry starts from zero state with the
+// damp of its body pair; a kept one carries its old state, active as it was at
+// the last accelerations() (Rockable.cpp:1356-1376 for the damp).
+__global__ void k_merge_fill(Dev d, int n, MergeIn in, MergeOut out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int o = out.src[e];
+  if (o < 0) {
+    const int i = out.ei[e], j = out.ej[e];
+    EState s;
+    s.fn = 0.0;
+    s.ftx = s.fty = s.ftz = 0.0;
+    s.momx = s.momy = s.momz = 0.0;
+    out.est[e] = s;
+    out.edn[e] = 0.0;
+    EGeom q;
+    q.nx = q.ny = q.nz = q.px = q.py = q.pz = q.vx = q.vy = q.vz = 0.0;
+    out.egeo[e] = q;
+    out.edamp[e] = pair_damp(d, i < j ? i : j, i < j ? j : i);
+    out.estick[e] = -1;
+    out.eact[e] = 0;
+  } else {
+    const double dn = in.oedn[o];
+    const int stick = in.ostick[o];
+    out.est[e] = in.oest[o];
+    out.edn[e] = dn;
+    out.egeo[e] = in.ogeo[o];
+    out.edamp[e] = in.odamp[o];
+    out.estick[e] = stick;
+    out.eact[e] = ((dn < 0.0 || stick >= 0) ? 1 : 0) | 2;
+  }
 }
 
 // --- CSR -----------------------------------------------------------------
@@ -1191,11 +1206,6 @@ __global__ void k_count_sorted(int n, const int32_t* key, int32_t* cnt) {
       hi = mid;
   }
   cnt[k] = e - lo + 1;
-}
-__global__ void k_count_by(int n, const int32_t* key, int32_t* cnt) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  atomicAdd(cnt + key[e], 1);
 }
 __global__ void k_iota(int n, int32_t* v) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1341,10 +1351,6 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipS
   RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));
   if (n > 0) hipLaunchKernelGGL(k_count_sorted, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ei, w.i32a);
   if (scan_exclusive(w, w.i32a, E.istart, nb + 1, s)) return -1;
-  // jstart
-  RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));
-  if (n > 0) hipLaunchKernelGGL(k_count_by, dim3(nblk(n)), dim3(kBlock), 0, s, n, E.ej, w.i32a);
-  if (scan_exclusive(w, w.i32a, E.jstart, nb + 1, s)) return -1;
   // jlist: stable radix sort of entry indices by j
   if (n > 0) {
     hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(kBlock), 0, s, n, w.i32b);
@@ -1354,6 +1360,11 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipS
                    bits_for((uint64_t)(nb > 1 ? nb - 1 : 1)), s))
       return -1;
   }
+  // jstart: counts of the sorted j keys (a wall is the j of thousands of
+  // entries: counted by atomics, they would serialise on its counter)
+  RK_CHECK(hipMemsetAsync(w.i32a, 0, sizeof(int32_t) * (nb + 1), s));
+  if (n > 0) hipLaunchKernelGGL(k_count_sorted, dim3(nblk(n)), dim3(kBlock), 0, s, n, (const int32_t*)w.kout, w.i32a);
+  if (scan_exclusive(w, w.i32a, E.jstart, nb + 1, s)) return -1;
   // resultant slots
   if (ensure(w, E.soff, E.soff_cap, (size_t)nb + 1)) return -1;
   hipLaunchKernelGGL(k_slot_body, dim3(nblk(nb + 1)), dim3(kBlock), 0, s, nb, E.istart, E.jstart, E.jlist, E.soff,
@@ -1680,7 +1691,10 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     out.edamp = nxt.edamp;
     out.estick = nxt.estick;
     out.eact = nxt.eact;
+    if (ensure(w, w.msrc, w.msrc_cap, (size_t)nnew)) return -1;
+    out.src = w.msrc;
     hipLaunchKernelGGL(k_merge<true>, dim3(nblk(ng)), dim3(kBlock), 0, s, d, in, out);
+    hipLaunchKernelGGL(k_merge_fill, dim3(nblk(nnew)), dim3(kBlock), 0, s, d, nnew, in, out);
   }
   RK_CHECK(hipGetLastError());
   if (nl_build_csr(w, nb, nxt, d.passive, s) != 0) return -1;
@@ -1737,7 +1751,7 @@ void nl_work_free(NLWork& w) {
                 w.cstripe, w.scnt, w.soff, w.bobb,  w.baabb, w.counter, w.cand,  w.cbase, w.cnf,   w.cnb,   w.nearbuf, w.okey,  w.ostart,
                 w.gk,    w.gk2,   w.gv,      w.gv2,   w.gflag, w.gpos,  w.ukey,  w.ucand,   w.uold,  w.gcount,
                 w.goff,  w.tmp,   w.i32a,    w.i32b,  w.i32c,  w.f64a,  w.cell_of, w.ccount, w.cstart, w.cfill,
-                w.cbody, w.over, w.gedge, w.gface, w.kout};
+                w.cbody, w.over, w.gedge, w.gface, w.kout, w.msrc};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   NLWork z;

@@ -110,6 +110,7 @@ struct NLWork {
   RK_BUF(int32_t, uold)
   RK_BUF(int32_t, gcount)
   RK_BUF(int32_t, goff)
+  RK_BUF(int32_t, msrc)  // merge: per new entry, the old entry it keeps (-1 for a fresh one)
   RK_BUF(unsigned char, tmp)
   RK_BUF(int32_t, i32a)
   RK_BUF(int32_t, i32b)
