@@ -1735,6 +1735,7 @@ int rk_integrate(rk_engine* eng, const char* outdir, int64_t max_steps, int64_t*
 
 int rk_save_conf(rk_engine* eng, const char* path) {
   if (check_loaded(eng)) return -1;
+  if (!path) return fail(eng, "no path given");
   return save_conf_now(eng, path);
 }
 

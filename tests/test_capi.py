@@ -51,3 +51,26 @@ def test_product_never_links_the_oracle():
     out = subprocess.run(["ldd", os.path.join(ROOT, "rockable_amd", "librockable_amd.so")], capture_output=True,
                          text=True).stdout
     assert "rk_oracle" not in out
+
+
+def test_null_engine_is_rejected_without_a_device_call():
+    """Every entry point that takes an engine returns -1 for a null handle,
+    before any HIP call (so this runs without a GPU)."""
+    import ctypes
+    from rockable_amd.engine import load_library
+    L = load_library()
+    assert L.rk_last_error(None) == b"null engine"
+    assert L.rk_load_conf(None, b"x") == -1
+    assert L.rk_load_driving_system(None, b"x") == -1
+    assert L.rk_set_force_law(None, b"Avalanche") == -1
+    assert L.rk_set_option(None, b"fuse_law", 1.0) == -1
+    assert L.rk_update_nl(None) == -1
+    assert L.rk_accelerations(None) == -1
+    assert L.rk_step(None, 1) == -1
+    assert L.rk_integrate(None, None, 1, None) == -1
+    assert L.rk_save_conf(None, None) == -1
+    assert L.rk_synchronize(None) == -1
+    assert L.rk_get_counts(None, None) == -1
+    assert L.rk_reset_kernel_time(None) == -1
+    v = ctypes.c_double()
+    assert L.rk_get_option(None, b"fuse_law", ctypes.byref(v)) == -1
