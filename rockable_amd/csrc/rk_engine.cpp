@@ -1452,6 +1452,7 @@ int rk_set_force_law(rk_engine* eng, const char* name) {
 
 int rk_set_body_force(rk_engine* eng, const char* name, const double* a, int nargs) {
   if (check_loaded(eng)) return -1;
+  if (nargs < 0 || (nargs > 0 && !a)) return fail(eng, "body force arguments missing");
   std::string n = name ? name : "";
   Model& m = eng->m;
   if (n.empty()) {
