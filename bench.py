@@ -103,6 +103,38 @@ def make_settled_deck(n_bodies, seed, dirpath, settle_steps, relax_steps, log):
     return path, int((~inside).sum())
 
 
+def interverlet_steps(e):
+    """Steps between two scheduled rebuilds of the engine's deck
+    (interVerlet / dt, the loop's own test at Rockable.cpp:2693)."""
+    import math
+    c = e.counts()
+    iv = e.conf_scalar("interVerlet")
+    return max(1, int(math.ceil(iv / c["dt"] - 0.5)))
+
+
+def form_roofline(e, steps, law):
+    """Force-phase roofline of a form, on the engine in its timed state:
+    steps more steps with HIP events bracketing the force phase on the
+    engine stream, the algorithmic bytes of each launch over its average
+    duration."""
+    c0 = e.counts()
+    e.set_option("profile", 1)
+    e.reset_kernel_time()
+    e.step(steps)
+    e.synchronize()
+    n, ms = e.kernel_time("force")
+    e.set_option("profile", 0)
+    c1 = e.counts()
+    nl_listed = 0.5 * (c0["nlisted"] + c1["nlisted"])
+    b = force_phase_bytes(c1["nbodies"], nl_listed, law)
+    t = ms / max(n, 1) * 1e-3
+    ach = b / t / 1e9 if t > 0 else 0.0
+    return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+            "traffic": None, "kernel": "force phase (%s law)" % law, "algorithmic_bytes_per_launch": b,
+            "force_phase_ms": 1e3 * t, "launches": n, "listed_per_body": nl_listed / c1["nbodies"],
+            "note": "HIP events over %d steps right after the timed region, same state" % steps}
+
+
 def force_phase_bytes(nb, nl, law):
     """Algorithmic HBM bytes of one force phase: each body's kinematics read
     once (pos 24 + Q 32 + vel 24 + vrot 24 + group 4 = 108 B) and its force
@@ -127,24 +159,66 @@ def _oracle_rate(deck_path, target_seconds, threads):
     return o.nbodies * steps / dt, steps, dt
 
 
-def cpu_model():
+def lscpu_summary():
+    """CPU model and core counts as lscpu reports them."""
+    import subprocess
+    info = {}
     try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    model = info.get("Model name")
+    if not model:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return {"cpu_model": model or "unknown", "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "logical_cpus": info.get("CPU(s)")}
 
 
-def cpu_baseline(deck_path, target_seconds, threads):
-    v, steps, dt = _oracle_rate(deck_path, target_seconds, threads)
-    v1, steps1, dt1 = _oracle_rate(deck_path, max(2.0, target_seconds / 3.0), 1)
-    return {"value": v, "unit": "body-steps/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
-            "value_1_thread": v1,
-            "sample": "CPU oracle (OpenMP restatement of Rockable's DefaultParallelMode step, %d threads), the same "
-                      "settled packing, %d steps after 20 warm-up steps, %.1f s; 1 thread: %d steps, %.1f s"
-                      % (threads, steps, dt, steps1, dt1)}
+def _oracle_rate_child(deck_path, seconds, threads):
+    """_oracle_rate in a child process whose OpenMP runtime starts with
+    threads threads bound one per core (OMP_PROC_BIND=close,
+    OMP_PLACES=cores): the settings must be in the environment before the
+    runtime starts, and the parent's runtime has started already."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", deck_path, str(seconds),
+                        str(threads)], capture_output=True, text=True, timeout=600, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr.strip()[-300:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(deck_path, target_seconds, threads=None, repeats=3):
+    """The CPU oracle on the cores this process may run on (one OpenMP thread
+    per core, bound), repeats runs of target_seconds / repeats each, the
+    median reported; and one 1-thread run for scale."""
+    import statistics
+    if threads is None:
+        threads = len(os.sched_getaffinity(0))
+    per = max(2.0, target_seconds / repeats)
+    runs = [_oracle_rate_child(deck_path, per, threads) for _ in range(repeats)]
+    one = _oracle_rate_child(deck_path, max(2.0, per), 1)
+    med = statistics.median(r["value"] for r in runs)
+    cpu = lscpu_summary()
+    out = {"value": med, "unit": "body-steps/s", "cores": threads, "kind": "port", "cpu_model": cpu["cpu_model"],
+           "lscpu": cpu, "affinity_cpus": len(os.sched_getaffinity(0)),
+           "runs": [r["value"] for r in runs], "value_1_thread": one["value"],
+           "sample": "CPU oracle (OpenMP restatement of Rockable's DefaultParallelMode step; %d threads, "
+                     "OMP_PROC_BIND=close, OMP_PLACES=cores), the same settled packing, median of %d runs of "
+                     "%d-%d steps after 20 warm-up steps (%.1f s each); 1 thread: %d steps, %.1f s"
+                     % (threads, repeats, min(r["steps"] for r in runs), max(r["steps"] for r in runs), per,
+                        one["steps"], one["seconds"])}
+    return out
 
 
 def copy_bandwidth_child(mib=1024):
@@ -210,6 +284,7 @@ def batched_form(a, log):
     wall = time.perf_counter() - t0
     c1 = e.counts()
     nb = c1["nbodies"]
+    roof = form_roofline(e, min(500, a.batched_steps), "Default")
     e.close()
     # unbatched sample: every 32nd packing as an engine of its own, default shapes
     rng = np.random.default_rng(0)
@@ -234,7 +309,7 @@ def batched_form(a, log):
             "workload": "%d packings of 500-20000 spheres (R 0.5 mm, 2.2 R lattice, floor + 4 walls each), "
                         "per-packing Default-law mu in [0.2, 0.8] and en in [0.1, 0.9], linkCells, "
                         "dt 1e-6 s, NL every 100 steps" % len(info),
-            "autotuned": choice, "autotune_steps": ntune,
+            "autotuned": choice, "autotune_steps": ntune, "roofline": roof,
             "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
             "graph_launches": c1["graph_launches"] - c0["graph_launches"],
             "unbatched_sample": {"packings": sample, "value": work / swall, "unit": "body-steps/s",
@@ -273,9 +348,10 @@ def bonds_form(a, log):
     c1 = e.counts()
     st = e.stats()
     nb = c1["nbodies"]
+    roof = form_roofline(e, 200, "StickedLinks")
     e.close()
     return {"value": nb * a.bond_steps / wall, "unit": "body-steps/s", "ms_per_step": 1e3 * wall / a.bond_steps,
-            "steps": a.bond_steps, "bodies": nb, "autotuned": choice, "autotune_steps": ntune,
+            "steps": a.bond_steps, "bodies": nb, "autotuned": choice, "autotune_steps": ntune, "roofline": roof,
             "interfaces_before": c0["ninterfaces"], "interfaces_after": c1["ninterfaces"],
             "broken_area": c1["total_broken_area"], "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
             "active_per_body": c1["nactive"] / nb, "kinetic_energy": st["kinetic_translational"] +
@@ -349,6 +425,7 @@ def avalanche_forms(a, log):
         finals[name] = {k: b[k] for k in ("pos", "vel", "Q", "vrot")}
         if name == "plain":
             stats, c = e.stats(), c1
+            roof = form_roofline(e, 200, "Avalanche")
         e.close()
         log("avalanche %s: %.3f ms/step" % (name, forms[name]["ms_per_step"]))
     mlp.close()
@@ -363,6 +440,7 @@ def avalanche_forms(a, log):
     out["kinetic_energy"] = stats["kinetic_translational"] + stats["kinetic_rotational"]
     out["coordination"] = stats["coordination"]
     out["autotuned"] = choice
+    out["roofline"] = roof
     out["workload"] = ("%d hull polyhedra (config of the plain GPU form) down a 30 deg slope, dt 2e-7 s, "
                        "dynamicUpdateNL (disp DVerlet/4, 1 rad); %d steps (autotuning and warm-up) untimed, then "
                        "the same %d steps timed in each form" % (nb, start, a.avalanche_steps))
@@ -379,7 +457,10 @@ def main():
     ap.add_argument("--settle", type=int, default=45000)
     ap.add_argument("--relax", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="CPU baseline threads (default: the cores this process may run on)")
+    ap.add_argument("--cpu-child", nargs=3, default=None, metavar=("DECK", "SECONDS", "THREADS"),
+                    help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--deck-dir", default=None)
     ap.add_argument("--packings", type=int, default=256)
@@ -403,6 +484,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (calibrates the HBM byte counters)")
     a = ap.parse_args()
+    if a.cpu_child:
+        v, steps, sec = _oracle_rate(a.cpu_child[0], float(a.cpu_child[1]), int(a.cpu_child[2]))
+        print(json.dumps({"value": v, "steps": steps, "seconds": sec}), flush=True)
+        return
     if a.copy_rate > 0:
         print(json.dumps({"copy_GBs": copy_bandwidth(a.copy_rate)}), flush=True)
         return
@@ -452,35 +537,42 @@ def main():
     log("warm-up: %d steps (%d of them autotuning: %s) in %.2f s" % (max(a.warmup, tuned), tuned, choice,
                                                                    time.perf_counter() - t0))
 
-    # timed region: K steps, device state resident; HIP events on the engine
-    # stream bracket the force phase and the neighbour-list rebuilds
+    # timed region: whole periods of the neighbour-list rebuild (the rebuild
+    # is part of the step loop, so the timed steps are --steps rounded up to
+    # a multiple of interVerlet), device state resident; HIP events on the
+    # engine stream bracket the force phase and the neighbour-list rebuilds
+    period = interverlet_steps(e)
+    K = max(period, -(-a.steps // period) * period)
     e.set_option("profile", 1)
     e.reset_kernel_time()
     c0 = e.counts()
     e.synchronize()
     t0 = time.perf_counter()
-    e.step(a.steps)
+    e.step(K)
     e.synchronize()
     wall = time.perf_counter() - t0
     c1 = e.counts()
     nb = c1["nbodies"]
     kt = {k: e.kernel_time(k) for k in ("force", "nl")}
-    value = nb * a.steps / wall
+    value = nb * K / wall
+    rebuilds = c1["nl_updates"] - c0["nl_updates"]
+    if rebuilds < K // period:
+        raise RuntimeError("timed region of %d steps holds %d rebuilds, expected %d" % (K, rebuilds, K // period))
     # per-kernel breakdown over the next K steps (one event pair per kernel)
     e.set_option("profile", 2)
     e.reset_kernel_time()
-    e.step(a.steps)
+    e.step(K)
     e.synchronize()
     kt2 = {k: e.kernel_time(k) for k in ("predict", "interact", "gather", "breakage", "nl")}
     e.set_option("profile", 0)
     # and without events, to show what the brackets cost
     e.synchronize()
     t1 = time.perf_counter()
-    e.step(a.steps)
+    e.step(K)
     e.synchronize()
     wall_noev = time.perf_counter() - t1
-    log("timed: %.4f ms/step with force-phase events, %.4f ms/step without" % (1e3 * wall / a.steps,
-                                                                             1e3 * wall_noev / a.steps))
+    log("timed: %d steps (%d rebuilds), %.4f ms/step with force-phase events, %.4f ms/step without" % (
+        K, rebuilds, 1e3 * wall / K, 1e3 * wall_noev / K))
 
     # roofline of the force phase (interact + gather), per launch
     n_force, ms_force = kt["force"]
@@ -490,20 +582,21 @@ def main():
     achieved = bytes_per / t_force / 1e9
 
     out = {
-        "metric": "body-steps/s", "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": max(a.warmup, tuned), "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "metric": "body-steps/s", "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": K,
+        "steps_requested": a.steps, "warmup": max(a.warmup, tuned), "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "plain GPU form: %d convex sphero-polyhedra (hulls of 8-14 points on a 1:0.8:0.6 "
                                "ellipsoid, 4-6 mm), settled pour in a 40x40x200-size box, linkCells, ForceLaw "
                                "Avalanche, dt 2e-7 s, interVerlet 100 steps" % nb,
                    "bodies": nb, "bodies_dropped_after_settling": dropped,
                    "listed_per_body": c1["nlisted"] / nb, "active_per_body": c1["nactive"] / nb,
-                   "nl_rebuilds_in_timed_region": c1["nl_updates"] - c0["nl_updates"],
+                   "nl_rebuilds_in_timed_region": rebuilds, "interverlet_steps": period,
                    "nl_candidate_pairs": c1["last_candidates"],
                    "force_phase_ms": 1e3 * t_force,
                    "nl_rebuild_ms": kt["nl"][1] / max(kt["nl"][0], 1),
                    "kernel_ms_per_launch": {k: (v[1] / max(v[0], 1)) for k, v in kt2.items() if v[0]},
-                   "ms_per_step_without_events": 1e3 * wall_noev / a.steps,
+                   "ms_per_step_without_events": 1e3 * wall_noev / K,
                    "autotuned": choice,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -81,6 +81,20 @@ typedef struct {
   double total_broken_area;      /* TrackDamage's broken area */
 } rk_stats;
 
+/* What rk_inspect_conf reports of a deck. */
+typedef struct {
+  int64_t nbodies, ndriven, nshapes, ninteractions, ninterfaces, ngroups;
+  int32_t law;       /* 0 Default, 1 Avalanche, 2 StickedLinks */
+  int32_t linkcells; /* UpdateNL linkCells (1) or bruteForce (0) */
+} rk_conf_info;
+
+/* The deck reader of rk_load_conf alone, on the host, with no device and no
+ * engine (initParser and the interface linking of loadConf,
+ * Rockable.cpp:643-1069, 956-1002): parses path (its redirection and shape
+ * library included) and fills *out.  On error returns a negative code and
+ * writes the message into err (at most errlen bytes, NUL-terminated). */
+int rk_inspect_conf(const char* path, rk_conf_info* out, char* err, size_t errlen);
+
 int rk_engine_create(rk_engine** out, int device);
 void rk_engine_destroy(rk_engine* eng);
 const char* rk_last_error(const rk_engine* eng);
@@ -145,6 +159,8 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * narrow-phase wavefront: 1, 2; 0, the default: 2 for shapes of up to 8
  * vertices, else 1).  No option changes results. */
 int rk_set_option(rk_engine* eng, const char* name, double value);
+/* Also reads the deck's scalars "dt", "interVerlet", "interConf", "DVerlet",
+ * "dVerlet" and "tmax". */
 int rk_get_option(rk_engine* eng, const char* name, double* value);
 /* Autotuner (no counterpart in the reference) of fuse_law, law_group,
  * geom_block, gather_lanes, gather_groups, fuse_glob, gather_waves,

@@ -147,12 +147,17 @@ int merge_models(std::vector<Model>& parts, Model& out, std::vector<PackingRange
   for (size_t p = 0; p < parts.size(); p++) {
     const Model& m = parts[p];
     for (const auto& a : group_map[p]) {
-      if (a.first >= 0 && a.first < kMaxGroups) out.density[a.second] = m.density[a.first];
+      // the loader accepts only groups in [0, kMaxGroups)
+      if (a.first < 0 || a.first >= kMaxGroups) {
+        err = "batch: group " + std::to_string(a.first) + " out of range";
+        return -1;
+      }
+      out.density[a.second] = m.density[a.first];
       out.ndensity = std::max(out.ndensity, a.second + 1);
       for (const auto& b : group_map[p])
         for (int id = 0; id < P_COUNT; id++) {
           const size_t k = ((size_t)id * kMaxGroups + a.first) * kMaxGroups + b.first;
-          if (a.first < kMaxGroups && b.first < kMaxGroups && m.defined[k]) out.set_param(id, a.second, b.second, m.table[k]);
+          if (m.defined[k]) out.set_param(id, a.second, b.second, m.table[k]);
         }
     }
   }

@@ -1285,6 +1285,28 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
 
 extern "C" {
 
+int rk_inspect_conf(const char* path, rk_conf_info* out, char* err, size_t errlen) {
+  auto report = [&](const std::string& msg) {
+    if (err && errlen > 0) snprintf(err, errlen, "%s", msg.c_str());
+    return -1;
+  };
+  if (!path) return report("no path given");
+  if (!out) return report("no output record given");
+  Model m;
+  std::string e;
+  if (load_conf(m, path, e) != 0) return report(e);
+  out->nbodies = (int64_t)m.bodies.size();
+  out->ndriven = m.nDriven;
+  out->nshapes = (int64_t)m.shapes.size();
+  out->ninteractions = (int64_t)m.inter.size();
+  out->ninterfaces = (int64_t)m.ifc.size();
+  out->ngroups = m.ngroup;
+  out->law = m.law;
+  out->linkcells = m.nl_linkcells;
+  if (err && errlen > 0) err[0] = 0;
+  return 0;
+}
+
 int rk_engine_create(rk_engine** out, int device) {
   if (!out) return -1;
   *out = nullptr;
@@ -1398,6 +1420,7 @@ int rk_load_conf(rk_engine* eng, const char* path) {
 
 int rk_load_driving_system(rk_engine* eng, const char* path) {
   if (check_loaded(eng)) return -1;
+  if (!path) return fail(eng, "no path given");
   std::string err;
   if (load_driving(eng->m, path, err) != 0) return fail(eng, err);
   std::vector<DevControl> c;
@@ -1705,6 +1728,18 @@ int rk_get_option(rk_engine* eng, const char* name, double* value) {
     *value = eng->m.gravity.y;
   else if (n == "gravity_z")
     *value = eng->m.gravity.z;
+  else if (n == "dt")  // the deck's scalars, read-only
+    *value = eng->m.dt;
+  else if (n == "interVerlet")
+    *value = eng->m.interVerlet;
+  else if (n == "interConf")
+    *value = eng->m.interConf;
+  else if (n == "DVerlet")
+    *value = eng->m.DVerlet;
+  else if (n == "dVerlet")
+    *value = eng->m.dVerlet;
+  else if (n == "tmax")
+    *value = eng->m.tmax;
   else
     return fail(eng, "unknown option " + n);
   return 0;

@@ -17,6 +17,8 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <string>
+#include <unordered_set>
 
 #include "rk_model.hpp"
 
@@ -330,7 +332,13 @@ static int parse_conf(Model& m, const std::string& fullpath, std::string& err) {
         }
         Body P;
         P.shape_name = nm;
-        P.group = (int)r.l();
+        const long group = r.l();
+        if (group < 0 || group >= kMaxGroups) {
+          err = "body " + std::to_string(i) + ": group " + std::to_string(group) + " outside [0, " +
+                std::to_string(kMaxGroups) + ")";
+          return -1;
+        }
+        P.group = (int)group;
         P.cluster = (int)r.l();
         P.homothety = r.d();
         P.pos = r.vec();
@@ -345,7 +353,7 @@ static int parse_conf(Model& m, const std::string& fullpath, std::string& err) {
           return -1;
         }
         double h = P.homothety;
-        double dens = (P.group >= 0 && P.group < kMaxGroups) ? m.density[P.group] : 0.0;
+        const double dens = m.density[P.group];
         P.mass = (h * h * h * m.shapes[P.shape].volume) * dens;
         P.inertia = vsc(P.mass, vsc(h * h, m.shapes[P.shape].inertia_mass));
         m.bodies.push_back(P);
@@ -462,11 +470,19 @@ int load_conf(Model& m, const std::string& file, std::string& err) {
   }
   // Link interfaces to their bonds (Rockable.cpp:956-1002); interfaces whose
   // bonds are not all listed, or that repeat an (i, j), are dropped.
+  // (the pairs already kept are looked up in a hash set: decks hold
+  // hundreds of thousands of interfaces)
   std::vector<Interface> kept;
+  std::unordered_set<uint64_t> kept_pairs;
+  kept_pairs.reserve(2 * m.ifc.size());
   for (Interface& BI : m.ifc) {
-    bool missed = false, dup = false;
-    for (const Interface& K : kept)
-      if (K.i == BI.i && K.j == BI.j) dup = true;
+    if (BI.i < 0 || BI.i >= nb || BI.j < 0 || BI.j >= nb) {
+      err = "interface body index out of range";
+      return -1;
+    }
+    bool missed = false;
+    const uint64_t pair = ((uint64_t)(uint32_t)BI.i << 32) | (uint32_t)BI.j;
+    const bool dup = kept_pairs.count(pair) != 0;
     for (size_t u = 0; u < BI.bkey.size(); u += 3) {
       Inter key;
       key.i = BI.i;
@@ -477,7 +493,10 @@ int load_conf(Model& m, const std::string& file, std::string& err) {
       auto it = std::lower_bound(m.inter.begin(), m.inter.end(), key, key_less);
       if (it == m.inter.end() || key_less(key, *it)) missed = true;
     }
-    if (!missed && !dup) kept.push_back(BI);
+    if (!missed && !dup) {
+      kept.push_back(BI);
+      kept_pairs.insert(pair);
+    }
   }
   std::sort(kept.begin(), kept.end(),
             [](const Interface& a, const Interface& b) { return a.i != b.i ? a.i < b.i : a.j < b.j; });

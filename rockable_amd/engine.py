@@ -39,7 +39,13 @@ class RkStats(ctypes.Structure):
                 ("broken_interfaces", ctypes.c_int64), ("total_broken_area", ctypes.c_double)]
 
 
-EXPORTED = ["rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_conf", "rk_load_driving_system",
+class RkConfInfo(ctypes.Structure):
+    _fields_ = [("nbodies", ctypes.c_int64), ("ndriven", ctypes.c_int64), ("nshapes", ctypes.c_int64),
+                ("ninteractions", ctypes.c_int64), ("ninterfaces", ctypes.c_int64), ("ngroups", ctypes.c_int64),
+                ("law", ctypes.c_int32), ("linkcells", ctypes.c_int32)]
+
+
+EXPORTED = ["rk_inspect_conf", "rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_conf", "rk_load_driving_system",
             "rk_set_force_law", "rk_set_body_force", "rk_set_option", "rk_update_nl", "rk_accelerations",
             "rk_step", "rk_integrate", "rk_save_conf", "rk_synchronize", "rk_get_counts", "rk_get_bodies",
             "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time",
@@ -60,6 +66,7 @@ def load_library(path=LIB_PATH):
         raise RuntimeError("librockable_amd.so is not built (run `python -m rockable_amd.build`)")
     L = ctypes.CDLL(path)
     P = ctypes.c_void_p
+    L.rk_inspect_conf.argtypes = [ctypes.c_char_p, ctypes.POINTER(RkConfInfo), ctypes.c_char_p, ctypes.c_size_t]
     L.rk_engine_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
     L.rk_engine_destroy.argtypes = [P]
     L.rk_last_error.argtypes = [P]
@@ -107,6 +114,18 @@ def load_library(path=LIB_PATH):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def inspect_conf(path):
+    """The deck reader alone, on the host (no device): a dict of the deck's
+    body, shape, interaction, interface and group counts, its force law and
+    neighbour-list method.  Raises RuntimeError with the reader's message."""
+    L = load_library()
+    info = RkConfInfo()
+    err = ctypes.create_string_buffer(512)
+    if L.rk_inspect_conf(os.fspath(path).encode(), ctypes.byref(info), err, len(err)) != 0:
+        raise RuntimeError(err.value.decode())
+    return {k: getattr(info, k) for k, _ in RkConfInfo._fields_}
 
 
 class Engine:
@@ -185,6 +204,11 @@ class Engine:
         v = ctypes.c_double(0.0)
         self._check(self.L.rk_get_option(self.h, name.encode(), ctypes.byref(v)))
         return v.value
+
+    def conf_scalar(self, name):
+        """A scalar of the loaded deck: dt, interVerlet, interConf, DVerlet,
+        dVerlet or tmax."""
+        return self.get_option(name)
 
     TUNED = ("fuse_law", "type_sort", "law_group", "geom_block", "gather_lanes", "gather_groups", "fuse_glob", "gather_waves",
              "body_block", "persistent",
