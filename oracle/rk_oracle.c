@@ -1031,7 +1031,7 @@ static void law_default(rko_sim* s, inter_t* I) {
   if (I->fn < 0.0) I->fn = 0.0;
   v3 vt = vsub(I->vel, vsc(vn, I->n));
   I->ft = vadd(I->ft, vsc(kt, vsc(s->dt, vt)));
-  double threshold = fabs(mu * fne);
+  double threshold = fabs(mu * ((g_var[VAR_MUTANT] == 8) ? I->fn : fne));
   double ft_square = vdot(I->ft, I->ft);
   if (ft_square > 0.0 && ft_square >= threshold * threshold) I->ft = vsc(threshold / sqrt(ft_square), I->ft);
   if (kr > 0.0) {
@@ -1072,7 +1072,7 @@ static void law_avalanche(rko_sim* s, inter_t* I) {
   const body_t* P = (I->i < s->nDriven) ? &s->b[I->j] : &s->b[I->i];
   branch = vsub(I->pos, P->pos);
   double r = vdot(branch, P->vrot) / vdot(P->vrot, P->vrot);
-  branch = vsub(branch, vsc(r, P->vrot));
+  if (g_var[VAR_MUTANT] != 10) branch = vsub(branch, vsc(r, P->vrot));
   double threshold_mom = fabs(mur * sqrt(vnorm2(branch)) * I->fn);
   double mom_square = vdot(I->mom, I->mom);
   if (mom_square > 0.0 && mom_square > threshold_mom * threshold_mom)
@@ -1124,9 +1124,10 @@ static void law_sticked(rko_sim* s, inter_t* I) {
     v3 vt = vsub(I->vel, vsc(vn, I->n));
     I->ft = vadd(I->ft, vsc(kt, vsc(s->dt, vt)));
     v3 ftv = vsc(I->damp, vt);
-    I->ft = vadd(I->ft, ftv);
+    if (g_var[VAR_MUTANT] != 11) I->ft = vadd(I->ft, ftv);
     I->mom = vadd(I->mom, vsc(s->dt, vsc(kr, vsub(s->b[I->j].vrot, s->b[I->i].vrot))));
-    double yield = pow(sqrt(vnorm2(I->ft)) / ft0, power) + pow(sqrt(vnorm2(I->mom)) / mom0, power) - I->fn / fn0 - 1.0;
+    double yield = pow(sqrt(vnorm2(I->ft)) / ft0, power) +
+                   ((g_var[VAR_MUTANT] == 12) ? 0.0 : pow(sqrt(vnorm2(I->mom)) / mom0, power)) - I->fn / fn0 - 1.0;
     if (yield > 0.0) s->ifc_break[I->stick] = 1; /* idempotent flag, read after the resultants */
   } else {
     if (I->dn > 0.0) {
@@ -1178,7 +1179,7 @@ static void body_force(const rko_sim* s, int i, v3* F, v3* M) {
     double cosa = vdot(axisB, s->bf_axis);
     double angle = acos(cosa);
     double sgn = -1.0;
-    if (cosa < 0.0) {
+    if (cosa < 0.0 && g_var[VAR_MUTANT] != 13) {
       sgn = 1.0;
       angle = M_PI - angle;
     }
@@ -1357,7 +1358,7 @@ static void accelerations_from_resultants(rko_sim* s) { /* Rockable.cpp:3419-355
         else
           w = p->vrot;
         v3 omega = qrot(Qinv, w);
-        v3 M = qrot(Qinv, p->moment);
+        v3 M = (g_var[VAR_MUTANT] == 15) ? p->moment : qrot(Qinv, p->moment);
         v3 dw = qrot(p->Q, euler_domega(p, omega, M));
         if (C->type == CTRL_XROT_MOM)
           p->arot.x = dw.x;
@@ -1419,7 +1420,7 @@ static void servo(rko_sim* s) {
   const double dX = fabs(Xmax - Xmin), dY = fabs(Ymax - Ymin), dZ = fabs(Zmax - Zmin);
   const double surfX = dY * dZ, surfY = dX * dZ, surfZ = dY * dX;
   int ic = s->servo_icontr;
-  s->ctrl[ic + 1].value = -s->servo_pressure * surfX;
+  s->ctrl[ic + 1].value = -s->servo_pressure * ((g_var[VAR_MUTANT] == 14 && s->servo == 2) ? surfY : surfX);
   if (s->servo == 1) s->ctrl[ic + 3].value = -s->servo_pressure * surfY;
   s->ctrl[ic + 5].value = -s->servo_pressure * surfZ;
 }
@@ -1579,7 +1580,7 @@ static double pair_damp(const rko_sim* s, int i, int j) {
     double dampRate = -logen / sqrt(logen * logen + M_PI * M_PI);
     return dampRate * 2.0 * sqrt(kn * meff);
   } else if (en2 <= 0.0) {
-    return 2.0 * sqrt(kn * meff);
+    return (g_var[VAR_MUTANT] == 9) ? 0.0 : 2.0 * sqrt(kn * meff);
   }
   return 0.0;
 }

@@ -15,14 +15,15 @@ def sphere_mass(R):
 
 
 def two_spheres(dirpath, R, x1, x2, v1=(0, 0, 0), v2=(0, 0, 0), kn=1e5, en2=0.25, mu=0.0, law="Default",
-                dt=1e-6, density=RHO, kr=0.0, mur=0.0):
-    """Two spheres of radius R, no gravity, one group."""
+                dt=1e-6, density=RHO, kr=0.0, mur=0.0, w1=(0, 0, 0), w2=(0, 0, 0)):
+    """Two spheres of radius R (angular velocities w1, w2), no gravity, one group."""
     shapes = [D.sphere_shape("S", R)]
     D.write_shp(os.path.join(dirpath, "shapes.shp"), shapes)
     deck = D.Deck(dt=dt, tmax=1e3, interVerlet=1e-4, interConf=1e3, DVerlet=0.1 * R, dVerlet=0.05 * R,
                   density={0: density}, gravity=(0.0, 0.0, 0.0), force_law=law,
                   params=D.contact_params(kn, 0.8 * kn, en2, mu, kr=kr, mur=mur))
-    deck.bodies = [D.Body("S", pos=tuple(x1), vel=tuple(v1)), D.Body("S", pos=tuple(x2), vel=tuple(v2))]
+    deck.bodies = [D.Body("S", pos=tuple(x1), vel=tuple(v1), vrot=tuple(w1)),
+                   D.Body("S", pos=tuple(x2), vel=tuple(v2), vrot=tuple(w2))]
     path = os.path.join(dirpath, "deck.conf")
     D.write_conf(path, deck)
     return path
@@ -96,3 +97,27 @@ def hull_pair_decks(dirpath, n_bodies=300, law="Default"):
     cells = D.write_deck(dirpath, shapes, deck, name="cells.conf")
     brute = D.conf_with(cells, os.path.join(dirpath, "brute.conf"), UpdateNL="bruteForce")
     return cells, brute
+
+
+def spun_bond(dirpath, R=0.5, w=2.0, kn=1e5, kt=8e4, kr=0.0, fn0=10.0, ft0=1.0, mom0=1.0, power=2.0, damp=0.0,
+              dt=1e-6, density=1e15):
+    """Two touching spheres of huge mass (dn = dn0 = 0) joined by one
+    StickedLinks bond with its parameters in the interface
+    (ParamsInInterfaces 1); the bonded vertex-vertex entry carries the
+    damping coefficient damp.  Body 1 spins at w about z, body 0 is at rest."""
+    shapes = [D.sphere_shape("S", R)]
+    D.write_shp(os.path.join(dirpath, "shapes.shp"), shapes)
+    deck = D.Deck(dt=dt, tmax=1e3, interVerlet=1e-4, interConf=1e3, DVerlet=0.1 * R, dVerlet=0.05 * R,
+                  density={0: density}, gravity=(0.0, 0.0, 0.0), force_law="StickedLinks",
+                  params=D.contact_params(kn, 0.8 * kn, 0.25, 0.3))
+    x0, x1 = np.zeros(3), np.array([2 * R, 0.0, 0.0])
+    deck.bodies = [D.Body("S", pos=tuple(x0)), D.Body("S", pos=tuple(x1), vrot=(0.0, 0.0, w))]
+    n = (x0 - x1) / np.linalg.norm(x0 - x1)
+    pos = x0 - n * R
+    rec = (0, 1, 0, 0, 0) + tuple(n) + (0.0,) + tuple(pos) + (0.0, 0.0, 0.0) + (0.0,) + (0.0,) * 6 + (damp,)
+    deck.interactions = [rec]
+    deck.params_in_interfaces = 1
+    deck.interfaces = [(0, 1, 0.0, (kn, kt, kr, fn0, ft0, mom0, power, 0.0), [(0, 0, 0)])]
+    path = os.path.join(dirpath, "deck.conf")
+    D.write_conf(path, deck)
+    return path
