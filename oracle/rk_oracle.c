@@ -482,9 +482,19 @@ struct rko_sim {
   /* driving system */
   control_t* ctrl;
   int nctrl;
-  int servo; /* 0 none, 1 isostatic, 2 biaxial */
+  /* ServoFunction (DrivingSystem.cpp:188-496): 0 none, 1 tritriIsostaticCompression,
+   * 2 tritriBiaxialCompression, 3 tritriCustom, 4 tritriLodeAngle, 5 shaker,
+   * 6 triangle_shaker, 7 sawtooth_shaker, 8 ramp; the last Servo line read wins */
+  int servo;
   int servo_icontr, servo_ids[6];
   double servo_pressure, servo_velocity;
+  double servo_val[6];                 /* tritriCustom: xmin xmax ymin ymax zmin zmax values */
+  double lode_a, lode_b, lode_rate;    /* tritriLodeAngle */
+  v3 shake_dir;                        /* shakers: normalised direction, amplitude, */
+  double shake_A, shake_omega, shake_T, shake_tini; /* angular frequency, period, start */
+  double ramp_vb, ramp_ve, ramp_tb, ramp_te, ramp_slope;
+  int ramp_slope_set;                  /* the slope is a function-static of the first call */
+  char driving_path[1024];             /* re-read at each conf save */
   /* body force */
   int bf;
   v3 bf_point;
@@ -1408,21 +1418,75 @@ void rko_accelerations(rko_sim* s) { /* Rockable.cpp:3562-3719, DefaultParallelM
 /* ------------------------------------------------------------------------- */
 /* Driving system (DrivingSystem.cpp:242-268) and velocity-Verlet step        */
 /* ------------------------------------------------------------------------- */
+/* The ServoFunction lambdas of DrivingSystem::read (DrivingSystem.cpp:251-494),
+ * called with the box time t at the start of each step and after each conf
+ * save.  The tritri servos read the six walls' inner faces; the shakers and
+ * the ramp depend on t only. */
 static void servo(rko_sim* s) {
-  if (s->servo == 0) return;
-  const int* id = s->servo_ids;
-  const double Xmin = s->b[id[0]].pos.x + mink(s, &s->b[id[0]]);
-  const double Xmax = s->b[id[1]].pos.x - mink(s, &s->b[id[1]]);
-  const double Ymin = s->b[id[2]].pos.y + mink(s, &s->b[id[2]]);
-  const double Ymax = s->b[id[3]].pos.y - mink(s, &s->b[id[3]]);
-  const double Zmin = s->b[id[4]].pos.z + mink(s, &s->b[id[4]]);
-  const double Zmax = s->b[id[5]].pos.z - mink(s, &s->b[id[5]]);
-  const double dX = fabs(Xmax - Xmin), dY = fabs(Ymax - Ymin), dZ = fabs(Zmax - Zmin);
-  const double surfX = dY * dZ, surfY = dX * dZ, surfZ = dY * dX;
-  int ic = s->servo_icontr;
-  s->ctrl[ic + 1].value = -s->servo_pressure * ((g_var[VAR_MUTANT] == 14 && s->servo == 2) ? surfY : surfX);
-  if (s->servo == 1) s->ctrl[ic + 3].value = -s->servo_pressure * surfY;
-  s->ctrl[ic + 5].value = -s->servo_pressure * surfZ;
+  const int ic = s->servo_icontr;
+  if (s->servo >= 1 && s->servo <= 4) {
+    if (ic + 5 >= s->nctrl) return;
+    const int* id = s->servo_ids;
+    const double Xmin = s->b[id[0]].pos.x + mink(s, &s->b[id[0]]);
+    const double Xmax = s->b[id[1]].pos.x - mink(s, &s->b[id[1]]);
+    const double Ymin = s->b[id[2]].pos.y + mink(s, &s->b[id[2]]);
+    const double Ymax = s->b[id[3]].pos.y - mink(s, &s->b[id[3]]);
+    const double Zmin = s->b[id[4]].pos.z + mink(s, &s->b[id[4]]);
+    const double Zmax = s->b[id[5]].pos.z - mink(s, &s->b[id[5]]);
+    const double dX = fabs(Xmax - Xmin), dY = fabs(Ymax - Ymin), dZ = fabs(Zmax - Zmin);
+    const double surfX = dY * dZ, surfY = dX * dZ, surfZ = dY * dX;
+    const double p = s->servo_pressure;
+    control_t* C = s->ctrl + ic;
+    if (s->servo == 1) { /* 251-268 */
+      C[1].value = -p * surfX;
+      C[3].value = -p * surfY;
+      C[5].value = -p * surfZ;
+    } else if (s->servo == 2) { /* 280-298 */
+      C[1].value = -p * ((g_var[VAR_MUTANT] == 14) ? surfY : surfX);
+      C[5].value = -p * surfZ;
+    } else if (s->servo == 3) { /* 328-353 */
+      const double* v = s->servo_val;
+      if (C[0].type == CTRL_X_FOR) C[0].value = v[0] * surfX;
+      if (C[1].type == CTRL_X_FOR) C[1].value = -v[1] * surfX;
+      if (C[2].type == CTRL_Y_FOR) C[2].value = v[2] * surfY;
+      if (C[3].type == CTRL_Y_FOR) C[3].value = -v[3] * surfY;
+      if (C[4].type == CTRL_Z_FOR) C[4].value = v[4] * surfZ;
+      if (C[5].type == CTRL_Z_FOR) C[5].value = -v[5] * surfZ;
+    } else { /* 369-392 */
+      const double dSig = s->lode_rate * s->t;
+      C[1].value = -(p + dSig) * surfX;
+      C[3].value = -(p + s->lode_a * dSig) * surfY;
+      C[5].value = -(p + s->lode_b * dSig) * surfZ;
+    }
+  } else if (s->servo >= 5 && s->servo <= 7) {
+    if (ic + 2 >= s->nctrl) return;
+    v3 v;
+    if (s->servo == 5) { /* 413-418 */
+      const double a = s->shake_A * s->shake_omega * cos(s->shake_omega * s->t);
+      v = vsc(a, s->shake_dir);
+    } else { /* 436-444, 467-475 */
+      const double T = s->shake_T;
+      v = vsc(4.0 * s->shake_A / T, s->shake_dir);
+      double phase = (s->servo == 6) ? s->t - floor(s->t / T) * T : s->t - floor((s->t - s->shake_tini) / T) * T;
+      phase /= T;
+      if (phase >= 0.25 && phase < 0.75) v = vsc(-1.0, v);
+    }
+    s->ctrl[ic].value = v.x;
+    s->ctrl[ic + 1].value = v.y;
+    s->ctrl[ic + 2].value = v.z;
+  } else if (s->servo == 8) { /* 485-493 */
+    if (ic >= s->nctrl) return;
+    if (!s->ramp_slope_set) {
+      s->ramp_slope = (s->ramp_ve - s->ramp_vb) / (s->ramp_te - s->ramp_tb);
+      s->ramp_slope_set = 1;
+    }
+    if (s->t <= s->ramp_tb)
+      s->ctrl[ic].value = s->ramp_vb;
+    else if (s->t >= s->ramp_te)
+      s->ctrl[ic].value = s->ramp_ve;
+    else
+      s->ctrl[ic].value = s->ramp_vb + s->ramp_slope * (s->t - s->ramp_tb);
+  }
 }
 
 static void velocity_controlled_drive(rko_sim* s) { /* Rockable.cpp:1856-1900 */
@@ -2178,9 +2242,11 @@ int rko_load_conf(rko_sim* s, const char* path, int run_fake_step) {
   return 0;
 }
 
-int rko_read_driving(rko_sim* s, const char* path) { /* DrivingSystem.cpp:44-268 */
+int rko_read_driving(rko_sim* s, const char* path) { /* DrivingSystem.cpp:44-496 */
   reader_t r;
-  if (rd_open(&r, path) != 0) return 0; /* no file: the first nDriven bodies do not move */
+  /* read again at each conf save (Rockable.cpp:2682-2688), even if absent now */
+  if (path != s->driving_path) snprintf(s->driving_path, sizeof s->driving_path, "%s", path);
+  if (rd_open(&r, s->driving_path) != 0) return 0; /* no file: the first nDriven bodies do not move */
   free(s->ctrl);
   s->ctrl = NULL;
   s->nctrl = 0;
@@ -2225,33 +2291,82 @@ int rko_read_driving(rko_sim* s, const char* path) { /* DrivingSystem.cpp:44-268
     } else if (!strcmp(tok, "Servo")) {
       char nm[128];
       rd_tok(&r, nm, sizeof nm);
-      if (strncmp(nm, "tritri", 6) != 0) {
-        rd_close(&r);
-        return seterr(s, "Servo %s is not supported", nm);
-      }
-      int id[6];
-      for (int k = 0; k < 6; k++) id[k] = (int)rd_l(&r);
-      int icontr = s->nctrl;
-      PUSHC(CTRL_X_VEL, id[0], 0.0);
-      PUSHC(CTRL_X_FOR, id[1], 0.0);
-      PUSHC(CTRL_Y_VEL, id[2], 0.0);
-      PUSHC(CTRL_Y_FOR, id[3], 0.0);
-      PUSHC(CTRL_Z_VEL, id[4], 0.0);
-      PUSHC(CTRL_Z_FOR, id[5], 0.0);
-      s->servo_icontr = icontr;
-      memcpy(s->servo_ids, id, sizeof id);
-      if (!strcmp(nm, "tritriIsostaticCompression")) {
-        s->servo = 1;
-        s->servo_pressure = rd_d(&r);
-      } else if (!strcmp(nm, "tritriBiaxialCompression")) {
-        s->servo = 2;
-        s->servo_pressure = rd_d(&r);
-        s->servo_velocity = rd_d(&r);
-        s->ctrl[icontr + 3].type = CTRL_Y_VEL;
-        s->ctrl[icontr + 3].value = -s->servo_velocity;
-      } else {
-        rd_close(&r);
-        return seterr(s, "Servo %s is not supported", nm);
+      if (strncmp(nm, "tritri", 6) == 0) { /* DrivingSystem.cpp:191-392 */
+        int id[6];
+        for (int k = 0; k < 6; k++) id[k] = (int)rd_l(&r);
+        int icontr = s->nctrl;
+        PUSHC(CTRL_X_VEL, id[0], 0.0);
+        PUSHC(CTRL_X_FOR, id[1], 0.0);
+        PUSHC(CTRL_Y_VEL, id[2], 0.0);
+        PUSHC(CTRL_Y_FOR, id[3], 0.0);
+        PUSHC(CTRL_Z_VEL, id[4], 0.0);
+        PUSHC(CTRL_Z_FOR, id[5], 0.0);
+        int kind = 0;
+        if (!strcmp(nm, "tritriIsostaticCompression")) {
+          kind = 1;
+          s->servo_pressure = rd_d(&r);
+        } else if (!strcmp(nm, "tritriBiaxialCompression")) {
+          kind = 2;
+          s->servo_pressure = rd_d(&r);
+          s->servo_velocity = rd_d(&r);
+          s->ctrl[icontr + 3].type = CTRL_Y_VEL;
+          s->ctrl[icontr + 3].value = -s->servo_velocity;
+        } else if (!strcmp(nm, "tritriCustom")) {
+          kind = 3;
+          static const int vel[3] = {CTRL_X_VEL, CTRL_Y_VEL, CTRL_Z_VEL};
+          static const int frc[3] = {CTRL_X_FOR, CTRL_Y_FOR, CTRL_Z_FOR};
+          for (int k = 0; k < 6; k++) {
+            int ty = (int)rd_l(&r);
+            s->servo_val[k] = rd_d(&r);
+            s->ctrl[icontr + k].type = (ty == 0) ? vel[k / 2] : frc[k / 2];
+            s->ctrl[icontr + k].value = (k & 1) ? -s->servo_val[k] : s->servo_val[k];
+          }
+        } else if (!strcmp(nm, "tritriLodeAngle")) {
+          kind = 4;
+          s->servo_pressure = rd_d(&r);
+          double lode = rd_d(&r);
+          s->lode_rate = rd_d(&r);
+          double rad = lode * M_PI / 180.0;
+          s->lode_a = (3.0 * tan(rad) - sqrt(3.0)) / (2.0 * sqrt(3.0));
+          s->lode_b = -(1.0 + s->lode_a);
+        }
+        if (kind) { /* an unknown tritri name keeps its controls and no new function */
+          s->servo = kind;
+          s->servo_icontr = icontr;
+          memcpy(s->servo_ids, id, sizeof id);
+        }
+      } else if (!strcmp(nm, "shaker") || !strcmp(nm, "triangle_shaker") || !strcmp(nm, "sawtooth_shaker")) {
+        /* 397-477: velocity of one body along dir */
+        int i = (int)rd_l(&r);
+        v3 dir = rd_v3(&r);
+        double A = rd_d(&r), freq = rd_d(&r);
+        double tini = 0.0;
+        if (!strcmp(nm, "sawtooth_shaker")) tini = rd_d(&r);
+        vnormalize(&dir);
+        s->shake_dir = dir;
+        s->shake_A = A;
+        s->shake_omega = 2.0 * M_PI * freq;
+        s->shake_T = 1.0 / freq;
+        s->shake_tini = tini;
+        s->servo_icontr = s->nctrl;
+        s->servo = !strcmp(nm, "shaker") ? 5 : !strcmp(nm, "triangle_shaker") ? 6 : 7;
+        PUSHC(CTRL_X_VEL, i, 0.0);
+        PUSHC(CTRL_Y_VEL, i, 0.0);
+        PUSHC(CTRL_Z_VEL, i, 0.0);
+      } else if (!strcmp(nm, "ramp")) { /* 479-494 */
+        char ty[64];
+        rd_tok(&r, ty, sizeof ty);
+        int code = 0;
+        for (int k = 0; k < 14; k++)
+          if (!strcmp(ty, names[k])) code = codes[k];
+        int i = (int)rd_l(&r);
+        s->ramp_vb = rd_d(&r);
+        s->ramp_ve = rd_d(&r);
+        s->ramp_tb = rd_d(&r);
+        s->ramp_te = rd_d(&r);
+        s->servo_icontr = s->nctrl;
+        s->servo = 8;
+        PUSHC(code, i, s->ramp_vb);
       }
     }
   }
@@ -2436,6 +2551,9 @@ static int loop_body(rko_sim* s, const char* outdir) {
       snprintf(fn, sizeof fn, "%s/conf%d", outdir, s->iconf);
       if (rko_save_conf(s, fn) != 0) return -1;
     }
+    /* re-read the driving file, then the servo on the new controls
+       (Rockable.cpp:2682-2688) */
+    if (s->driving_path[0] && rko_read_driving(s, s->driving_path) != 0) return -1;
     servo(s);
     s->interConfC = 0.0;
   }

@@ -215,8 +215,14 @@ struct Dev {
   DevControl* ctrl;
   const int32_t* ctrl_off;   // per driven body, [ctrl_off[b], ctrl_off[b+1]) into ctrl_idx
   const int32_t* ctrl_idx;
+  // ServoFunction: 0 none, 1 tritriIsostaticCompression, 2 tritriBiaxialCompression,
+  // 3 tritriCustom, 4 tritriLodeAngle (the walls' gaps, on the device);
+  // 5-8 shakers and ramp (functions of t only: the host evaluates them and
+  // passes the values in StepParams.servo_v)
   int servo, servo_icontr, servo_ids[6];
   double servo_pressure;
+  double servo_val[6];        // tritriCustom values (xmin xmax ymin ymax zmin zmax)
+  double lode_a, lode_b, lode_rate;
   // body force
   int bf;
   double bf_point[3], bf_acc, bf_axisBody[3], bf_axis[3], bf_Kr, bf_fluidDensity;
@@ -230,6 +236,8 @@ struct StepParams {
   double dt, dt_2, dt2_2;
   double gx, gy, gz;
   int law;
+  double t;           // box time of the step (after its increment)
+  double servo_v[3];  // values of a time-only servo's controls at t
 };
 
 RK_HD uint32_t pack_sub(int type, int isub, int jsub) {

@@ -138,6 +138,7 @@ struct rk_engine {
   bool needUpdate = false;
   int64_t steps = 0, nl_updates = 0;
   int live_interfaces = 0;
+  std::vector<double> ctrl_sig;  // the driving file's content last uploaded
 };
 
 namespace {
@@ -366,6 +367,10 @@ void assemble_dev(rk_engine* eng) {
   d.servo_icontr = m.servo_icontr;
   for (int k = 0; k < 6; k++) d.servo_ids[k] = m.servo_ids[k];
   d.servo_pressure = m.servo_pressure;
+  for (int k = 0; k < 6; k++) d.servo_val[k] = m.servo_val[k];
+  d.lode_a = m.lode_a;
+  d.lode_b = m.lode_b;
+  d.lode_rate = m.lode_rate;
   d.bf = m.bf;
   d.bf_point[0] = m.bf_point.x;
   d.bf_point[1] = m.bf_point.y;
@@ -392,8 +397,9 @@ void assemble_dev(rk_engine* eng) {
   d.flags = eng->flags;
 }
 
-StepParams step_params(const rk_engine* eng) {
+StepParams step_params(rk_engine* eng) {
   StepParams sp;
+  memset(&sp, 0, sizeof sp);  // the captured-step cache compares it bytewise
   sp.dt = eng->m.dt;
   sp.dt_2 = eng->m.dt_2;
   sp.dt2_2 = eng->m.dt2_2;
@@ -401,6 +407,10 @@ StepParams step_params(const rk_engine* eng) {
   sp.gy = eng->m.gravity.y;
   sp.gz = eng->m.gravity.z;
   sp.law = eng->m.law;
+  if (eng->m.servo_timed()) {  // such steps are launched one by one, at their time
+    sp.t = eng->m.t;
+    eng->m.servo_time_values(eng->m.t, sp.servo_v);
+  }
   return sp;
 }
 
@@ -868,7 +878,7 @@ int launch_one_step(rk_engine* eng, bool with_guard) {
 
 // What a captured run of steps depends on: the device view (pointers and
 // sizes), the step parameters and the launch options.
-std::vector<unsigned char> graph_key(const rk_engine* eng, int n, bool with_guard) {
+std::vector<unsigned char> graph_key(rk_engine* eng, int n, bool with_guard) {
   const StepParams sp = step_params(eng);
   const double spv[6] = {sp.dt, sp.dt_2, sp.dt2_2, sp.gx, sp.gy, sp.gz};
   const int iv[5] = {n, with_guard ? 1 : 0, sp.law, eng->record_geometry, eng->live_interfaces};
@@ -974,7 +984,7 @@ int flush_steps(rk_engine* eng) {
 int enqueue_step(rk_engine* eng, bool write_geom, bool with_guard) {
   const bool bonds = eng->m.law == LAW_STICKED && eng->live_interfaces > 0;
   if (eng->pending > 0 && eng->pending_guard != with_guard && flush_steps(eng) != 0) return -1;
-  if (write_geom || bonds || eng->graph_steps <= 1) {
+  if (write_geom || bonds || eng->graph_steps <= 1 || eng->m.servo_timed()) {
     if (flush_steps(eng) != 0) return -1;
     return vv_step(eng, write_geom ? 1 : 0, with_guard);
   }
@@ -1092,6 +1102,38 @@ int integrate_prepare(rk_engine* eng, bool reset) {
   return 0;
 }
 
+int upload_controls(rk_engine* eng);
+
+// The controls and servo parameters as read from the driving file: equal
+// signatures mean that a re-read changed nothing.
+std::vector<double> control_signature(const Model& m) {
+  std::vector<double> v;
+  for (const Control& c : m.ctrl) {
+    const double x[6] = {(double)c.type, (double)c.i, c.value, c.vec.x, c.vec.y, c.vec.z};
+    v.insert(v.end(), x, x + 6);
+  }
+  const double sv[] = {(double)m.servo, (double)m.servo_icontr, m.servo_pressure, m.servo_velocity,
+                       m.servo_val[0], m.servo_val[1], m.servo_val[2], m.servo_val[3], m.servo_val[4],
+                       m.servo_val[5], m.lode_a, m.lode_b, m.lode_rate, m.shake_dir.x, m.shake_dir.y,
+                       m.shake_dir.z, m.shake_A, m.shake_omega, m.shake_T, m.shake_tini, m.ramp_vb, m.ramp_ve,
+                       m.ramp_tb, m.ramp_te};
+  v.insert(v.end(), sv, sv + sizeof sv / sizeof sv[0]);
+  for (int k = 0; k < 6; k++) v.push_back(m.servo_ids[k]);
+  return v;
+}
+
+// At each conf save the reference reads its driving file again and applies
+// the servo (Rockable.cpp:2682-2688): new controls take effect from the next
+// step.  The device copy is refreshed only when the file's content changed.
+int reread_driving(rk_engine* eng) {
+  if (eng->m.driving_path.empty()) return 0;
+  if (flush_steps(eng) != 0) return -1;
+  std::string err;
+  if (load_driving(eng->m, eng->m.driving_path, err) != 0) return fail(eng, err);
+  if (control_signature(eng->m) == eng->ctrl_sig) return 0;
+  return upload_controls(eng);
+}
+
 // one iteration of the integrate() loop (Rockable.cpp:2584-2745)
 int loop_body(rk_engine* eng, const char* outdir) {
   Model& m = eng->m;
@@ -1108,6 +1150,7 @@ int loop_body(rk_engine* eng, const char* outdir) {
       snprintf(fn, sizeof fn, "%s/conf%d", outdir, m.iconf);
       if (save_conf_now(eng, fn) != 0) return -1;
     }
+    if (reread_driving(eng) != 0) return -1;
     m.interConfC = 0.0;
   }
   if (eng->needUpdate || m.interVerletC >= m.interVerlet - m.dt_2) {
@@ -1222,8 +1265,8 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       hist.push_back(loop_scalars(eng));
       conf_at.push_back(conf_due ? 1 : 0);
       k++;
-      if ((conf_due && outdir) || eng->needUpdate || m.interVerletC >= m.interVerlet - m.dt_2 ||
-          (until_tmax && m.t > m.tmax))
+      if ((conf_due && (outdir || !m.driving_path.empty())) || eng->needUpdate ||
+          m.interVerletC >= m.interVerlet - m.dt_2 || (until_tmax && m.t > m.tmax))
         break;
       if (pipe && --chunk_left == 0) {
         if (flush_steps(eng) != 0) return -1;
@@ -1268,6 +1311,7 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       snprintf(fn, sizeof fn, "%s/conf%d", outdir, m.iconf);
       if (save_conf_now(eng, fn) != 0) return -1;
     }
+    if (conf_at[executed] && reread_driving(eng) != 0) return -1;
     if (need || m.interVerletC >= m.interVerlet - m.dt_2) {
       if (do_update_nl(eng) != 0) return -1;
       if (!need) m.interVerletC = 0.0;
@@ -1421,8 +1465,15 @@ int rk_load_conf(rk_engine* eng, const char* path) {
 int rk_load_driving_system(rk_engine* eng, const char* path) {
   if (check_loaded(eng)) return -1;
   if (!path) return fail(eng, "no path given");
+  (void)hipSetDevice(eng->device);
+  if (flush_steps(eng) != 0) return -1;
   std::string err;
   if (load_driving(eng->m, path, err) != 0) return fail(eng, err);
+  return upload_controls(eng);
+}
+
+namespace {
+int upload_controls(rk_engine* eng) {
   std::vector<DevControl> c;
   for (const Control& x : eng->m.ctrl) {
     DevControl dc;
@@ -1449,8 +1500,10 @@ int rk_load_driving_system(rk_engine* eng, const char* path) {
   if (ensure_gpart(eng) != 0) return -1;
   assemble_dev(eng);
   HIPCHK(hipStreamSynchronize(eng->stream));
+  eng->ctrl_sig = control_signature(eng->m);
   return 0;
 }
+}  // namespace
 
 int rk_set_force_law(rk_engine* eng, const char* name) {
   if (check_loaded(eng)) return -1;

@@ -520,13 +520,13 @@ int load_conf(Model& m, const std::string& file, std::string& err) {
 
 int load_driving(Model& m, const std::string& file, std::string& err) {
   Reader r;
+  m.driving_path = file;  // read again at each conf save, even if absent now
   if (!r.open(file)) return 0;
   static const char* names[] = {"_x_Vel_",  "_y_Vel_",   "_z_Vel_",    "_xrot_Vel_", "_yrot_Vel_",
                                 "_zrot_Vel_", "_x_For_", "_y_For_",    "_z_For_",    "_xrot_Mom_",
                                 "_yrot_Mom_", "_zrot_Mom_", "_xyzrot_Vel_", "_xyzrot_Mom_"};
   static const int codes[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 100, 101};
-  m.ctrl.clear();
-  m.servo = 0;
+  m.ctrl.clear();  // the servo function stays until a Servo line replaces it
   std::string t;
   while (r.tok(t)) {
     if (is_comment(t)) {
@@ -548,35 +548,87 @@ int load_driving(Model& m, const std::string& file, std::string& err) {
     } else if (t == "Servo") {
       std::string nm;
       r.tok(nm);
-      if (nm.compare(0, 6, "tritri") != 0) {
-        err = "Servo " + nm + " is not supported";
-        return -1;
-      }
-      int id[6];
-      for (int k = 0; k < 6; k++) id[k] = (int)r.l();
-      int icontr = (int)m.ctrl.size();
-      const int types[6] = {C_X_VEL, C_X_FOR, C_Y_VEL, C_Y_FOR, C_Z_VEL, C_Z_FOR};
-      for (int k = 0; k < 6; k++) {
+      if (nm.compare(0, 6, "tritri") == 0) {  // DrivingSystem.cpp:191-392
+        int id[6];
+        for (int k = 0; k < 6; k++) id[k] = (int)r.l();
+        const int icontr = (int)m.ctrl.size();
+        const int types[6] = {C_X_VEL, C_X_FOR, C_Y_VEL, C_Y_FOR, C_Z_VEL, C_Z_FOR};
+        for (int k = 0; k < 6; k++) {
+          Control c;
+          c.type = types[k];
+          c.i = id[k];
+          m.ctrl.push_back(c);
+        }
+        int kind = 0;
+        if (nm == "tritriIsostaticCompression") {
+          kind = 1;
+          m.servo_pressure = r.d();
+        } else if (nm == "tritriBiaxialCompression") {
+          kind = 2;
+          m.servo_pressure = r.d();
+          m.servo_velocity = r.d();
+          m.ctrl[icontr + 3].type = C_Y_VEL;
+          m.ctrl[icontr + 3].value = -m.servo_velocity;
+        } else if (nm == "tritriCustom") {
+          kind = 3;
+          const int vel[3] = {C_X_VEL, C_Y_VEL, C_Z_VEL}, frc[3] = {C_X_FOR, C_Y_FOR, C_Z_FOR};
+          for (int k = 0; k < 6; k++) {
+            const long ty = r.l();
+            m.servo_val[k] = r.d();
+            m.ctrl[icontr + k].type = ty == 0 ? vel[k / 2] : frc[k / 2];
+            m.ctrl[icontr + k].value = (k & 1) ? -m.servo_val[k] : m.servo_val[k];
+          }
+        } else if (nm == "tritriLodeAngle") {
+          kind = 4;
+          m.servo_pressure = r.d();
+          const double lode = r.d();
+          m.lode_rate = r.d();
+          const double rad = lode * M_PI / 180.0;
+          m.lode_a = (3.0 * tan(rad) - sqrt(3.0)) / (2.0 * sqrt(3.0));
+          m.lode_b = -(1.0 + m.lode_a);
+        }
+        if (kind) {  // an unknown tritri name keeps its controls and adds no function
+          m.servo = kind;
+          m.servo_icontr = icontr;
+          for (int k = 0; k < 6; k++) m.servo_ids[k] = id[k];
+        }
+      } else if (nm == "shaker" || nm == "triangle_shaker" || nm == "sawtooth_shaker") {  // 397-477
         Control c;
-        c.type = types[k];
-        c.i = id[k];
+        c.i = (int)r.l();
+        v3 dir = r.vec();
+        const double A = r.d(), freq = r.d();
+        const double tini = nm == "sawtooth_shaker" ? r.d() : 0.0;
+        vnormalize(&dir);
+        m.shake_dir = dir;
+        m.shake_A = A;
+        m.shake_omega = 2.0 * M_PI * freq;
+        m.shake_T = 1.0 / freq;
+        m.shake_tini = tini;
+        m.servo_icontr = (int)m.ctrl.size();
+        m.servo = nm == "shaker" ? 5 : nm == "triangle_shaker" ? 6 : 7;
+        const int types[3] = {C_X_VEL, C_Y_VEL, C_Z_VEL};
+        for (int k = 0; k < 3; k++) {
+          c.type = types[k];
+          m.ctrl.push_back(c);
+        }
+      } else if (nm == "ramp") {  // 479-494
+        std::string ty;
+        r.tok(ty);
+        Control c;
+        c.type = 0;  // an unknown control name maps to _x_Vel_ (std::map default)
+        for (int k = 0; k < 14; k++)
+          if (ty == names[k]) c.type = codes[k];
+        c.i = (int)r.l();
+        m.ramp_vb = r.d();
+        m.ramp_ve = r.d();
+        m.ramp_tb = r.d();
+        m.ramp_te = r.d();
+        c.value = m.ramp_vb;
+        m.servo_icontr = (int)m.ctrl.size();
+        m.servo = 8;
         m.ctrl.push_back(c);
       }
-      m.servo_icontr = icontr;
-      for (int k = 0; k < 6; k++) m.servo_ids[k] = id[k];
-      if (nm == "tritriIsostaticCompression") {
-        m.servo = 1;
-        m.servo_pressure = r.d();
-      } else if (nm == "tritriBiaxialCompression") {
-        m.servo = 2;
-        m.servo_pressure = r.d();
-        m.servo_velocity = r.d();
-        m.ctrl[icontr + 3].type = C_Y_VEL;
-        m.ctrl[icontr + 3].value = -m.servo_velocity;
-      } else {
-        err = "Servo " + nm + " is not supported";
-        return -1;
-      }
+      // other servo names: ignored, as the reference's chain of names ignores them
     }
   }
   for (const Control& c : m.ctrl)
@@ -584,7 +636,43 @@ int load_driving(Model& m, const std::string& file, std::string& err) {
       err = "controls must address driven bodies (index < nDriven)";
       return -1;
     }
+  if (m.servo >= 1 && m.servo <= 4)
+    for (int k = 0; k < 6; k++)
+      if (m.servo_ids[k] < 0 || m.servo_ids[k] >= (int)m.bodies.size()) {
+        err = "servo wall index out of range";
+        return -1;
+      }
   return 0;
+}
+
+void Model::servo_time_values(double t, double out[3]) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (servo >= 5 && servo <= 7) {
+    v3 v;
+    if (servo == 5) {  // DrivingSystem.cpp:413-418
+      v = vsc(shake_A * shake_omega * cos(shake_omega * t), shake_dir);
+    } else {  // 436-444, 467-475
+      const double T = shake_T;
+      v = vsc(4.0 * shake_A / T, shake_dir);
+      double phase = servo == 6 ? t - std::floor(t / T) * T : t - std::floor((t - shake_tini) / T) * T;
+      phase /= T;
+      if (phase >= 0.25 && phase < 0.75) v = vsc(-1.0, v);
+    }
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
+  } else if (servo == 8) {  // 485-493
+    if (!ramp_slope_set) {
+      ramp_slope = (ramp_ve - ramp_vb) / (ramp_te - ramp_tb);
+      ramp_slope_set = true;
+    }
+    if (t <= ramp_tb)
+      out[0] = ramp_vb;
+    else if (t >= ramp_te)
+      out[0] = ramp_ve;
+    else
+      out[0] = ramp_vb + ramp_slope * (t - ramp_tb);
+  }
 }
 
 double pair_damp(const Model& m, int i, int j) {

@@ -131,9 +131,25 @@ struct Model {
   std::vector<Interface> ifc;  // sorted by (i, j)
   // driving system
   std::vector<Control> ctrl;
-  int servo = 0;  // 0 none, 1 tritriIsostaticCompression, 2 tritriBiaxialCompression
+  // ServoFunction (DrivingSystem.cpp:188-496), the last Servo line read:
+  // 0 none, 1 tritriIsostaticCompression, 2 tritriBiaxialCompression,
+  // 3 tritriCustom, 4 tritriLodeAngle, 5 shaker, 6 triangle_shaker,
+  // 7 sawtooth_shaker, 8 ramp
+  int servo = 0;
   int servo_icontr = 0, servo_ids[6] = {0, 0, 0, 0, 0, 0};
   double servo_pressure = 0.0, servo_velocity = 0.0;
+  double servo_val[6] = {0, 0, 0, 0, 0, 0};
+  double lode_a = 0.0, lode_b = 0.0, lode_rate = 0.0;
+  v3 shake_dir{0, 0, 0};
+  double shake_A = 0.0, shake_omega = 0.0, shake_T = 1.0, shake_tini = 0.0;
+  double ramp_vb = 0.0, ramp_ve = 0.0, ramp_tb = 0.0, ramp_te = 0.0, ramp_slope = 0.0;
+  bool ramp_slope_set = false;  // the slope is a function-static of the ramp's first call
+  std::string driving_path;     // re-read at each conf save (Rockable.cpp:2682-2688)
+  // servos 4-8 depend on the time; their steps are launched one by one
+  bool servo_timed() const { return servo >= 4; }
+  // the controls' values of a time-only servo (5-8) at time t (the lambdas
+  // of DrivingSystem.cpp:413-493, evaluated on the host with the C library)
+  void servo_time_values(double t, double out[3]);
   // body force
   int bf = BF_NONE;
   v3 bf_point{0, 0, 0};

@@ -1305,10 +1305,13 @@ __global__ void __launch_bounds__(kBlock) k_glob(Dev d, int all, int nv) {
   }
 }
 
-// ServoFunction (DrivingSystem.cpp:242-268), then the controlled bodies in
-// list order (Rockable.cpp:1856-1900, 1988-2033), all by one thread.
+// ServoFunction (DrivingSystem.cpp:251-494), then the controlled bodies in
+// list order (Rockable.cpp:1856-1900, 1988-2033), all by one thread.  The
+// tritri servos read the walls' inner faces here; the shakers' and the
+// ramp's values (functions of t only) come from the host in sp.servo_v.
 __device__ void drive_controls(const Dev& d, const StepParams& sp) {
-  if (d.servo != 0) {
+  const int ic = d.servo_icontr;
+  if (d.servo >= 1 && d.servo <= 4 && ic + 5 < d.nctrl) {
     const int* id = d.servo_ids;
     const double Xmin = d.kin[id[0]].px + d.kin[id[0]].R;
     const double Xmax = d.kin[id[1]].px - d.kin[id[1]].R;
@@ -1318,9 +1321,35 @@ __device__ void drive_controls(const Dev& d, const StepParams& sp) {
     const double Zmax = d.kin[id[5]].pz - d.kin[id[5]].R;
     const double dX = fabs(Xmax - Xmin), dY = fabs(Ymax - Ymin), dZ = fabs(Zmax - Zmin);
     const double surfX = dY * dZ, surfY = dX * dZ, surfZ = dY * dX;
-    d.ctrl[d.servo_icontr + 1].value = -d.servo_pressure * surfX;
-    if (d.servo == 1) d.ctrl[d.servo_icontr + 3].value = -d.servo_pressure * surfY;
-    d.ctrl[d.servo_icontr + 5].value = -d.servo_pressure * surfZ;
+    const double p = d.servo_pressure;
+    DevControl* C = d.ctrl + ic;
+    if (d.servo == 1) {  // isostatic (251-268)
+      C[1].value = -p * surfX;
+      C[3].value = -p * surfY;
+      C[5].value = -p * surfZ;
+    } else if (d.servo == 2) {  // biaxial (280-298)
+      C[1].value = -p * surfX;
+      C[5].value = -p * surfZ;
+    } else if (d.servo == 3) {  // custom (328-353): force-driven walls only
+      const double* v = d.servo_val;
+      if (C[0].type == C_X_FOR) C[0].value = v[0] * surfX;
+      if (C[1].type == C_X_FOR) C[1].value = -v[1] * surfX;
+      if (C[2].type == C_Y_FOR) C[2].value = v[2] * surfY;
+      if (C[3].type == C_Y_FOR) C[3].value = -v[3] * surfY;
+      if (C[4].type == C_Z_FOR) C[4].value = v[4] * surfZ;
+      if (C[5].type == C_Z_FOR) C[5].value = -v[5] * surfZ;
+    } else {  // Lode angle (369-392)
+      const double dSig = d.lode_rate * sp.t;
+      C[1].value = -(p + dSig) * surfX;
+      C[3].value = -(p + d.lode_a * dSig) * surfY;
+      C[5].value = -(p + d.lode_b * dSig) * surfZ;
+    }
+  } else if (d.servo >= 5 && d.servo <= 7 && ic + 2 < d.nctrl) {  // shakers
+    d.ctrl[ic].value = sp.servo_v[0];
+    d.ctrl[ic + 1].value = sp.servo_v[1];
+    d.ctrl[ic + 2].value = sp.servo_v[2];
+  } else if (d.servo == 8 && ic < d.nctrl) {  // ramp
+    d.ctrl[ic].value = sp.servo_v[0];
   }
   for (int c = 0; c < d.nctrl; c++) {
     const DevControl C = d.ctrl[c];
