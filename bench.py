@@ -33,8 +33,9 @@ Beside the plain form, the JSON line carries "forms": "batched" (256
 independent sphere packings in one engine, autotuned launch shapes and
 graph-captured steps, with a sample of the same packings run one engine
 after another) and "avalanche" (a 1,000,000-body polyhedral avalanche with
-the displacement-check rebuild: the plain GPU form against the batched +
-learned-rebuild form, timed on the same steps of the same trajectory) and
+the displacement-check rebuild: the settled packing tiled and released on a
+slope; the plain GPU form, fixed batches and the batched + learned-rebuild
+form, timed on the same steps of the same trajectory) and
 "bonds" (200,000 sphero-cubes in 2,000 bonded clusters hitting a plane, bonds
 breaking on impact).  --forms "" leaves them out.
 """
@@ -361,29 +362,36 @@ def bonds_form(a, log):
                         "through the impact" % (a.bond_clusters, nb, ntune, a.bond_steps)}
 
 
-def avalanche_forms(a, log):
-    """Learned-rebuild workload: a.avalanche_bodies of the plain GPU form's
-    hull polyhedra packed on the floor of a 400 x 100-size box and sent down
-    a 30 deg slope (gravity tilted; 1 m/s downslope plus velocities in
-    [-0.2, 0.2] m/s), dt 2e-7 s, the neighbour list rebuilt by the
-    displacement check alone (dynamicUpdateNL, dispUpdateNL = DVerlet / 4,
-    angleUpdateNL = 1 rad, the reference default).  The launch shapes are
-    autotuned on the running avalanche, then after a warm-up the same
-    a.avalanche_steps steps are timed twice, from the same state: in the
-    plain GPU form (the check read by the host after every step, every kernel
-    launched on its own) and in the batched + learned form on a second engine
-    brought to that state (batches of steps sized by the rebuild-interval
-    network and enqueued without host synchronisation, as captured graphs
-    when the autotuner finds them faster, the device check freezing any step
-    after a rebuild falls due).  The two final states must be identical."""
+def avalanche_forms(a, log, settled_path):
+    """Learned-rebuild workload: the settled plain-form packing (50 005
+    bodies: the hull polyhedra poured and settled in their 40 x 40-size box)
+    tiled along x into a bed of about a.avalanche_bodies bodies and released
+    on a 30 deg slope (gravity tilted; the box's downslope wall moved 0.5 m
+    past the bed's end; 1 m/s added downslope to every body), dt 2e-7 s,
+    ForceLaw Avalanche, the neighbour list rebuilt by the displacement check
+    alone (dynamicUpdateNL, dispUpdateNL = DVerlet / 4, angleUpdateNL = 1 rad,
+    the reference default).  The launch shapes are autotuned on the running
+    avalanche, a.avalanche_warmup more steps are run, and that state is saved;
+    then three forms each load it and run the same a.avalanche_steps steps:
+    "plain" (the check read by the host after every step, as the reference
+    runs it, every kernel launched on its own), "fixed_batches" (the engine's
+    default: batches of 16 steps between host reads of the check, captured
+    graphs when the autotuner keeps them) and "batched_learned" (batches sized
+    by the rebuild-interval network, asynchronous polls of the check).  The
+    three final states must be identical."""
+    import math
     import numpy as np
     from rockable_amd import decks as D
     from rockable_amd.engine import Engine, Mlp
     d = tempfile.mkdtemp(prefix="rk_aval_")
     t0 = time.perf_counter()
-    shapes, deck = D.avalanche_deck(n_bodies=a.avalanche_bodies, dt=2e-7, gap_frac=0.005, vmax=0.2)
-    path = D.write_deck(d, shapes, deck)
-    log("avalanche deck: %d bodies, written in %.1f s" % (a.avalanche_bodies, time.perf_counter() - t0))
+    per_tile = D.read_conf_bodies(settled_path)[1]["names"].size
+    nd0 = 5
+    tiles = max(1, int(math.ceil(a.avalanche_bodies / float(per_tile - nd0))))
+    pool = D.hull_shape_pool(64, a.seed)
+    path = D.slope_from_settled(settled_path, pool, d, tiles=tiles, slope_deg=30.0, flow_speed=1.0, runout=0.5,
+                                tile_gap=2e-3)
+    log("avalanche deck: %d tiles of the settled packing, written in %.1f s" % (tiles, time.perf_counter() - t0))
     mlp = Mlp(device=0)
     mlp.set_params(np.load(os.path.join(ROOT, "rockable_amd", "data", "rebuild_mlp.npy")))
     e = Engine(path, device=0)
@@ -393,20 +401,25 @@ def avalanche_forms(a, log):
     e.step(a.avalanche_warmup)
     start = ntune + a.avalanche_warmup
     log("autotune: %d steps, %s" % (ntune, choice))
+    start_conf = os.path.join(d, "start.conf")
+    e.save_conf(start_conf)
+    e.close()
     forms, finals = {}, {}
-    for name in ("plain", "batched_learned"):
-        if name == "batched_learned":
-            e = Engine(path, device=0)
-            e.set_option("record_geometry", 0)
-            for k, v in choice.items():
-                e.set_option(k, v)
-            e.update_nl()
-            e.step(start)
-            e.set_rebuild_model(mlp)
-        else:
-            e.set_rebuild_model(None)
+    for name in ("plain", "fixed_batches", "batched_learned"):
+        e = Engine(start_conf, device=0)
+        e.set_option("record_geometry", 0)
+        for k, v in choice.items():
+            e.set_option(k, v)
+        if name == "plain":
             e.set_option("guard_batch", 1)
             e.set_option("graph_steps", 0)
+        elif name == "fixed_batches":
+            e.set_option("guard_batch", 16)
+        else:
+            e.set_rebuild_model(mlp)
+        e.update_nl()
+        if name == "plain":
+            st0 = e.stats()
         c0 = e.counts()
         e.synchronize()
         t1 = time.perf_counter()
@@ -432,18 +445,21 @@ def avalanche_forms(a, log):
     nb = c["nbodies"]
     out = dict(forms)
     out["batched_learned_over_plain"] = forms["batched_learned"]["value"] / forms["plain"]["value"]
-    out["identical_final_state"] = all(np.array_equal(finals["plain"][k], finals["batched_learned"][k])
-                                       for k in finals["plain"])
+    out["batched_learned_over_fixed_batches"] = forms["batched_learned"]["value"] / forms["fixed_batches"]["value"]
+    out["identical_final_state"] = all(np.array_equal(finals["plain"][k], finals[f][k])
+                                       for f in ("fixed_batches", "batched_learned") for k in finals["plain"])
     out["bodies"] = nb
     out["listed_per_body"] = c["nlisted"] / nb
     out["active_per_body"] = c["nactive"] / nb
+    out["coordination_at_start"] = st0["coordination"]
+    out["coordination_at_end"] = stats["coordination"]
     out["kinetic_energy"] = stats["kinetic_translational"] + stats["kinetic_rotational"]
-    out["coordination"] = stats["coordination"]
     out["autotuned"] = choice
     out["roofline"] = roof
-    out["workload"] = ("%d hull polyhedra (config of the plain GPU form) down a 30 deg slope, dt 2e-7 s, "
-                       "dynamicUpdateNL (disp DVerlet/4, 1 rad); %d steps (autotuning and warm-up) untimed, then "
-                       "the same %d steps timed in each form" % (nb, start, a.avalanche_steps))
+    out["workload"] = ("%d hull polyhedra: the settled plain-form packing tiled %d times and released down a "
+                       "30 deg slope at 1 m/s, dt 2e-7 s, dynamicUpdateNL (disp DVerlet/4, 1 rad); %d steps "
+                       "(autotuning and warm-up) untimed, then the same %d steps timed in each form from the "
+                       "saved state" % (nb, tiles, start, a.avalanche_steps))
     return out
 
 
@@ -618,7 +634,7 @@ def main():
     if "batched" in forms:
         out["forms"]["batched"] = batched_form(a, log)
     if "avalanche" in forms:
-        out["forms"]["avalanche"] = avalanche_forms(a, log)
+        out["forms"]["avalanche"] = avalanche_forms(a, log, path)
     if "bonds" in forms:
         out["forms"]["bonds"] = bonds_form(a, log)
     if not a.no_cpu:

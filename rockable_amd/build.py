@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librockable_amd.so")
+EXAMPLE = os.path.join(HERE, "rk_run")
 SOURCES = ["rk_io.cpp", "rk_batch.cpp", "rk_step.hip", "rk_nl.hip", "rk_mlp.hip", "rk_engine.cpp"]
 ARCH = os.environ.get("RK_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
@@ -49,7 +50,25 @@ def build(jobs=4, verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_example(verbose)
     return LIB
+
+
+def build_example(verbose=False):
+    """rk_run (examples/rk_run.cpp): the C++ caller of the C ABI, compiled by
+    the host compiler against include/rockable_amd.h alone and linked to the
+    library next to it."""
+    src = os.path.join(HERE, "examples", "rk_run.cpp")
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", inc, src, "-o", EXAMPLE + ".tmp", "-L", HERE,
+           "-lrockable_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("example build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(EXAMPLE + ".tmp", EXAMPLE)
+    if verbose:
+        print("built", EXAMPLE)
+    return EXAMPLE
 
 
 if __name__ == "__main__":

@@ -867,3 +867,135 @@ def single_packing_of(shapes, deck, info, p):
                     "pos": blk["pos"][f:f + n], "vel": blk["vel"][f:f + n], "Q": blk["Q"][f:f + n]}
     d.nDriven = 5
     return shapes, d
+
+
+def read_conf_bodies(path):
+    """The header scalars and the Particles block of a conf file: a dict of
+    header keyword -> list of tokens (the first occurrence), and arrays of
+    the bodies' names, group, cluster, homothety, pos, vel, acc, Q, vrot,
+    arot (Rockable.cpp:895-909 column order)."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    header = {}
+    k = 0
+    while k < len(lines):
+        tok = lines[k].split()
+        k += 1
+        if not tok or tok[0].startswith(("#", "!", "/")):
+            continue
+        if tok[0] == "Particles":
+            n = int(tok[1])
+            break
+        header.setdefault(tok[0], tok[1:])
+    rows = []
+    while len(rows) < n and k < len(lines):
+        ln = lines[k]
+        k += 1
+        if not ln.strip() or ln.lstrip().startswith("#"):
+            continue
+        rows.append(ln.split())
+    names = np.array([r[0] for r in rows])
+    num = np.array([[float(x) for x in r[1:23]] for r in rows])
+    return header, {"names": names, "group": num[:, 0].astype(int), "cluster": num[:, 1].astype(int),
+                    "h": num[:, 2], "pos": num[:, 3:6], "vel": num[:, 6:9], "acc": num[:, 9:12],
+                    "Q": num[:, 12:16], "vrot": num[:, 16:19], "arot": num[:, 19:22]}
+
+
+def slope_from_settled(settled_conf, pool, dirpath, tiles=1, slope_deg=30.0, flow_speed=1.0, runout=1.0,
+                       disp_frac=0.25, angle_deg=57.29577951308232, tile_gap=None, name="slope.conf"):
+    """An avalanche released from a settled bed: the free bodies of
+    settled_conf (a packing settled under vertical gravity between the walls
+    of a box, its walls the first nDriven bodies) tiled `tiles` times along
+    x (each copy one box length plus tile_gap further), set on a floor
+    tilted by slope_deg (gravity (g sin, 0, -g cos); downslope is +x) with
+    flow_speed added downslope to every body's velocity.  New walls: the
+    floor, the upslope wall and the two side walls of the tiled bed, and a
+    downslope wall `runout` metres past its end.  The settled contacts'
+    histories are dropped (the first neighbour-list update finds the
+    contacts again).  The neighbour list follows the displacement check
+    alone (dynamicUpdateNL, dispUpdateNL = disp_frac DVerlet, angleUpdateNL
+    angle_deg), with the settled deck's law, parameters, time step and cell
+    size.  pool: the free bodies' shapes (written with the walls' shape)."""
+    import os
+    hdr, B = read_conf_bodies(settled_conf)
+    nd = int(hdr["nDriven"][0])
+    wpos = B["pos"][:nd]
+    wr = float(B["h"][0])  # walls: homothety 1 -> the Minkowski radius is the shape's
+    free = slice(nd, len(B["names"]))
+    # the settled box's inner faces: floor (0), x0 (1), x1 (2), y0 (3), y1 (4)
+    walls_r = {}
+    with open(os.path.join(os.path.dirname(settled_conf), hdr["shapeFile"][0])) as fh:
+        shp_text = fh.read()
+    for blk in shp_text.split("<")[1:]:
+        t = blk.split()
+        if "name" in t and "radius" in t:
+            walls_r[t[t.index("name") + 1]] = float(t[t.index("radius") + 1])
+    r = walls_r.get(str(B["names"][0]), 0.0) * wr
+    x0 = wpos[1, 0] + r
+    x1 = wpos[2, 0] - r
+    y0 = wpos[3, 1] + r
+    y1 = wpos[4, 1] - r
+    z0 = wpos[0, 2] + r
+    L = x1 - x0
+    gap = float(tile_gap) if tile_gap is not None else 0.0
+    n1 = len(B["names"]) - nd
+    reps = lambda a: np.concatenate([a[free]] * tiles)
+    pos = reps(B["pos"]).copy()
+    for k in range(tiles):
+        pos[k * n1:(k + 1) * n1, 0] += k * (L + gap)
+    vel = reps(B["vel"]).copy()
+    vel[:, 0] += flow_speed
+    top = float(pos[:, 2].max()) + 0.1 * L
+    hi_x = x0 + tiles * L + (tiles - 1) * gap + runout
+    lo = np.array([x0, y0, z0])
+    hi = np.array([hi_x, y1, top])
+    c = 0.5 * (lo + hi)
+    half = 0.5 * float((hi - lo).max()) + 2 * r
+    wall = square_plane_shape("SLOPEWALL", half, r)
+    s2 = np.sqrt(0.5)
+    q_x = (s2, 0.0, s2, 0.0)
+    q_y = (s2, -s2, 0.0, 0.0)
+    walls = [Body("SLOPEWALL", group=1, pos=(c[0], c[1], lo[2] - r)),
+             Body("SLOPEWALL", group=1, pos=(lo[0] - r, c[1], c[2]), Q=q_x),
+             Body("SLOPEWALL", group=1, pos=(hi[0] + r, c[1], c[2]), Q=q_x),
+             Body("SLOPEWALL", group=1, pos=(c[0], lo[1] - r, c[2]), Q=q_y),
+             Body("SLOPEWALL", group=1, pos=(c[0], hi[1] + r, c[2]), Q=q_y)]
+    th = np.radians(slope_deg)
+    g = 9.81
+    dt = float(hdr["dt"][0])
+    DV = float(hdr["DVerlet"][0])
+    params = []
+    with open(settled_conf) as fh:
+        for ln in fh:
+            t = ln.split()
+            if t and t[0].endswith("Contact") and len(t) == 4:
+                params.append((t[0], int(t[1]), int(t[2]), float(t[3])))
+            if t and t[0] == "Particles":
+                break
+    deck = Deck(shape_file="shapes.shp", dt=dt, tmax=1e3, interVerlet=1e3, interConf=1e3, DVerlet=DV,
+                dVerlet=float(hdr["dVerlet"][0]), density={0: 2700.0, 1: 2700.0},
+                gravity=(g * np.sin(th), 0.0, -g * np.cos(th)), update_nl="linkCells", nDriven=len(walls),
+                force_law=hdr["forceLaw"][0], params=params)
+    with open(settled_conf) as fh:
+        dens = {}
+        for ln in fh:
+            t = ln.split()
+            if t and t[0] == "density":
+                dens[int(t[1])] = float(t[2])
+            if t and t[0] == "Particles":
+                break
+    if dens:
+        deck.density = dens
+    cs = [float(x) for x in hdr["cellMinSizes"]]
+    deck.cell_min_sizes = tuple(cs)
+    deck.box_for_linkcells_opt = 1
+    deck.dynamic_update_nl = 1
+    deck.disp_update_nl = disp_frac * DV
+    deck.angle_update_nl = angle_deg
+    deck.bodies = walls
+    deck.body_block = {"names": reps(B["names"]), "group": reps(B["group"]), "cluster": reps(B["cluster"]),
+                       "h": reps(B["h"]), "pos": pos, "vel": vel, "Q": reps(B["Q"])}
+    os.makedirs(dirpath, exist_ok=True)
+    write_shp(os.path.join(dirpath, "shapes.shp"), [wall] + list(pool))
+    write_conf(os.path.join(dirpath, name), deck)
+    return os.path.join(dirpath, name)

@@ -74,3 +74,18 @@ def test_null_engine_is_rejected_without_a_device_call():
     assert L.rk_reset_kernel_time(None) == -1
     v = ctypes.c_double()
     assert L.rk_get_option(None, b"fuse_law", ctypes.byref(v)) == -1
+
+
+def test_cxx_caller_builds_and_fails_cleanly_without_gpu(tmp_path):
+    """The C++ example caller links against the library through the header
+    alone; without a device it reports the failure and exits with status 1
+    (no abort, no CPU fallback)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    exe = os.path.join(ROOT, "rockable_amd", "rk_run")
+    assert os.path.exists(exe), "build first: python -m rockable_amd.build"
+    r = subprocess.run([exe, "nothing.conf", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "no usable device" in r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

@@ -106,6 +106,27 @@ def test_sphere_lattice_trajectory_bit_exact(tmp_path):
     assert o.nactive > 50
 
 
+def test_cpu_runnable_configuration_verbatim(tmp_path):
+    """The CPU-runnable configuration exactly as specified: 512 spheres
+    (R = 0.5 mm, density 2700) on an 8 x 8 x 8 lattice of spacing 2.2 R,
+    velocities uniform in [-0.01, 0.01] m/s from seed 1, dropping under
+    1 g onto a fixed floor inside four fixed walls; bruteForce neighbour list
+    (rebuilt every 100 steps), ForceLaw Default with kn = 1e6 N/m,
+    kt = 0.8 kn, en = 0.5 (en2 0.25), mu = 0.4; dt = 1e-6 s, 10 000 steps.
+    Bit-identical to the oracle at every 1000-step checkpoint (bodies and
+    active interactions), the bottom layers in contact with the floor and
+    each other by the end."""
+    from rockable_amd import decks as D
+    shapes, deck = D.sphere_lattice_deck(n_side=8, R=0.5e-3, spacing_factor=2.2, seed=1, vmax=0.01, kn=1e6,
+                                         en=0.5, mu=0.4, dt=1e-6, steps=10000, nl="bruteForce")
+    assert deck.gravity == (0.0, 0.0, -9.81) and len(deck.bodies) == 517 and deck.nDriven == 5
+    path = D.write_deck(str(tmp_path), shapes, deck)
+    e, o = _run_pair(path, 10000, chunk=1000)
+    I = o.interactions(active=True)
+    assert len(I) > 40 and np.any(I["j"] < 5) and np.any(I["j"] >= 5)
+    assert o.nl_updates == e.counts()["nl_updates"] >= 100
+
+
 @pytest.mark.parametrize("law", ["Default", "Avalanche"])
 def test_hull_packing_linkcells_bit_exact(tmp_path, law):
     """600 convex sphero-polyhedra (hulls of 8-14 points) thrown at each
