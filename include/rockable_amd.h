@@ -24,7 +24,10 @@
  *                              UpdateNL_linkCells (Rockable.cpp:1614-1809)
  *   rk_accelerations           Rockable::accelerations (Rockable.cpp:3562-3719)
  *   rk_step                    body of the integrate() loop (Rockable.cpp:2584-2745)
- *   rk_integrate               Rockable::integrate (Rockable.cpp:2538-2746)
+ *   rk_integrate               Rockable::integrate (Rockable.cpp:2538-2746), with
+ *                              its output files perf.txt, kineticEnergy.txt and
+ *                              staticBalance.txt in the output directory
+ *   rk_load_data_extractors    Rockable::readDataExtractors (Rockable.cpp:1152-1180)
  *   rk_save_conf               Rockable::saveConf (Rockable.cpp:440-610)
  *   rk_get_bodies              Particles[i].pos/Q/vel/vrot/acc/arot/force/moment
  *   rk_get_interactions        the Interactions sets (Rockable.hpp:107) and
@@ -116,6 +119,13 @@ int rk_load_batch(rk_engine* eng, const char* const* conf_paths, const char* con
  * out[2] first free body, out[3] free count (one packing after rk_load_conf). */
 int rk_batch_packing(rk_engine* eng, int p, int64_t* out);
 int rk_load_driving_system(rk_engine* eng, const char* path);
+/* readDataExtractors (Rockable.cpp:1152-1180): a dataExtractors.txt file of
+ * "dnStat <file> <nrec>" and "TrackDamage <file> <nrec>" lines (other names
+ * are skipped, as the reference skips an unknown name); the files are opened
+ * now, relative to the directory of path, and rk_integrate records them every
+ * nrec steps of its step counter (Rockable.cpp:2719-2726).  A missing file
+ * adds no extractor. */
+int rk_load_data_extractors(rk_engine* eng, const char* path);
 /* Returns 1 (not an error) when the name is unknown and Default is used. */
 int rk_set_force_law(rk_engine* eng, const char* name);
 /* name: AttractingPoint (x y z acceleration), PreferredDirection (axisBody.xyz

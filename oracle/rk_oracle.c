@@ -495,6 +495,17 @@ struct rko_sim {
   double ramp_vb, ramp_ve, ramp_tb, ramp_te, ramp_slope;
   int ramp_slope_set;                  /* the slope is a function-static of the first call */
   char driving_path[1024];             /* re-read at each conf save */
+  /* integrate()'s output files (Rockable.cpp:215-222) and the DataExtractors
+     of a dataExtractors.txt file (Rockable.cpp:1152-1180; dnStat, TrackDamage) */
+  FILE *perf_file, *ke_file, *sb_file;
+  int nextr;
+  struct {
+    int kind; /* 1 dnStat, 2 TrackDamage */
+    int nrec;
+    FILE* f;
+  } extr[16];
+  double inAreaInterfaces;
+  int64_t ext_step; /* integrate()'s step counter (Rockable.cpp:2578, 2742) */
   /* body force */
   int bf;
   v3 bf_point;
@@ -1157,7 +1168,7 @@ static void law_sticked(rko_sim* s, inter_t* I) {
     if (I->fn < 0.0) I->fn = 0.0;
     v3 vt = vsub(I->vel, vsc(vn, I->n));
     I->ft = vadd(I->ft, vsc(kt, vsc(s->dt, vt)));
-    double threshold = fabs(mu * I->fn);
+    double threshold = fabs(mu * ((g_var[VAR_MUTANT] == 16) ? fne : I->fn));
     double ft_square = vdot(I->ft, I->ft);
     if (ft_square > 0.0 && ft_square >= threshold * threshold) I->ft = vsc(threshold / sqrt(ft_square), I->ft);
     I->mom = vadd(I->mom, vsc(s->dt, vsc(kr, vsub(s->b[I->j].vrot, s->b[I->i].vrot))));
@@ -1318,7 +1329,7 @@ static void breakage(rko_sim* s) { /* Rockable.cpp:3350-3413; interfaces visited
       inter_t* I = find_inter(s, BI->i, BI->j, BI->bkey[3 * q], BI->bkey[3 * q + 1], BI->bkey[3 * q + 2]);
       if (!I) continue;
       I->stick = -1;
-      if (I->dn >= 0.0) {
+      if (I->dn >= 0.0 || g_var[VAR_MUTANT] == 17) {
         I->fn = 0.0;
         I->ft = V(0, 0, 0);
       } else {
@@ -2539,6 +2550,180 @@ static void integrate_prepare(rko_sim* s, int reset) {
   s->dt2 = s->dt * s->dt;
 }
 
+/* getKineticEnergy over every body (Rockable.cpp:3958-3970, first = 0) */
+static void kinetic_energy(const rko_sim* s, double* Et, double* Er) {
+  double et = 0.0, er = 0.0;
+  for (int i = 0; i < s->nb; i++) {
+    const body_t* p = &s->b[i];
+    et += p->mass * vdot(p->vel, p->vel);
+    v3 w = qrot(qconj(p->Q), p->vrot);
+    er += (p->inertia.x * w.x * w.x + p->inertia.y * w.y * w.y + p->inertia.z * w.z * w.z);
+  }
+  *Et = et * 0.5;
+  *Er = er * 0.5;
+}
+
+/* getResultantQuickStats from nDriven (Rockable.cpp:4145-4206) and
+   getInteractionQuickStats (4217-4252), on the active interactions in their
+   serial order */
+static void quick_stats(rko_sim* s, double* Fmax, double* F_fnmax, double* fnMax_out, double* fnMean_out) {
+  double* fnMax = (double*)calloc((size_t)s->nb + 1, sizeof(double));
+  size_t* nbCtc = (size_t*)calloc((size_t)s->nb + 1, sizeof(size_t));
+  for (size_t k = 0; k < s->nactive; k++) {
+    const inter_t* I = s->active[k];
+    nbCtc[I->i] += 1;
+    nbCtc[I->j] += 1;
+    if (fnMax[I->i] < I->fn) fnMax[I->i] = I->fn;
+    if (fnMax[I->j] < I->fn) fnMax[I->j] = I->fn;
+  }
+  const int first = s->nDriven, last = s->nb - 1;
+  double F = first <= last ? sqrt(vnorm2(s->b[first].force)) : 0.0;
+  *Fmax = F;
+  *F_fnmax = 0.0;
+  for (int i = first; i <= last; i++) {
+    if (nbCtc[i] <= 1) continue;
+    F = sqrt(vnorm2(s->b[i].force));
+    if (F > *Fmax) *Fmax = F;
+    if (fnMax[i] > 0.0) {
+      F /= fnMax[i];
+      if (F > *F_fnmax) *F_fnmax = F;
+    }
+  }
+  free(fnMax);
+  free(nbCtc);
+  if (s->nactive == 0) {
+    *fnMax_out = *fnMean_out = 0.0;
+    return;
+  }
+  double fn = s->active[0]->fn, mx = fn, mean = 0.0;
+  size_t n = 1;
+  if (g_var[VAR_MUTANT] == 18) { /* a planted mistake: the mean over every fn */
+    mean = fn;
+    n = 2;
+  }
+  for (size_t k = 1; k < s->nactive; k++) {
+    n += 1;
+    fn = s->active[k]->fn;
+    mean += fn;
+    if (fn > mx) mx = fn;
+  }
+  if (n > 1) mean /= (double)(n - 1);
+  *fnMax_out = mx;
+  *fnMean_out = mean;
+}
+
+/* The records integrate() writes at each conf save (Rockable.cpp:2600-2668):
+   perf.txt (times: this restatement's own wall clock), staticBalance.txt,
+   kineticEnergy.txt; numbers as an std::ostream prints them (%g). */
+static void conf_outputs(rko_sim* s, double elapsed) {
+  if (s->perf_file) {
+    fprintf(s->perf_file, "%g %g %g %g\n", s->t, elapsed > 0.0 ? s->interConf / elapsed : 0.0, 0.0, 0.0);
+    fflush(s->perf_file);
+  }
+  double Fmax, F_fnmax, fnMax, fnMean;
+  quick_stats(s, &Fmax, &F_fnmax, &fnMax, &fnMean);
+  if (s->sb_file && fnMax != 0.0 && fnMean != 0.0) {
+    fprintf(s->sb_file, "%g %g %g\n", s->t, F_fnmax, fabs(Fmax / fnMean));
+    fflush(s->sb_file);
+  }
+  if (s->ke_file) {
+    double Et, Er;
+    kinetic_energy(s, &Et, &Er);
+    fprintf(s->ke_file, "%g %g %g\n", s->t, Et, Er);
+    fflush(s->ke_file);
+  }
+}
+
+/* DataExtractor::record of dnStat (DataExtractor_dnStat.cpp:69-105) and
+   TrackDamage (DataExtractor_TrackDamage.cpp:47-50) */
+static void extractors_record(rko_sim* s, int64_t step) {
+  for (int q = 0; q < s->nextr; q++) {
+    if (s->extr[q].nrec <= 0 || step % s->extr[q].nrec != 0) continue;
+    FILE* f = s->extr[q].f;
+    if (s->extr[q].kind == 1) {
+      double dnMin = 1e20, dnMax = -1e20, dnMeanNeg = 0.0, dnMeanPos = 0.0, nbNeg = 0.0, nbPos = 0.0, dnMean = 0.0;
+      v3 posMin = V(0.0, 0.0, 0.0);
+      for (size_t k = 0; k < s->nactive; k++) {
+        const inter_t* I = s->active[k];
+        const double dn = I->dn;
+        dnMean += dn;
+        if (dn < 0.0) {
+          dnMeanNeg += dn;
+          nbNeg += 1.0;
+        } else {
+          dnMeanPos += dn;
+          nbPos += 1.0;
+        }
+        if (dn < dnMin) {
+          dnMin = dn;
+          posMin = I->pos;
+        }
+        if (dn > dnMax) dnMax = dn;
+      }
+      if (nbNeg > 0.0) dnMeanNeg /= nbNeg;
+      if (nbPos > 0.0) dnMeanPos /= nbPos;
+      if (nbNeg > 0.0 || nbPos > 0.0) dnMean /= (nbNeg + nbPos);
+      fprintf(f, "%g %g %g %g %g %g %d %d %g %g %g\n", s->t, dnMin, dnMax, dnMean, dnMeanNeg, dnMeanPos, (int)nbNeg,
+              (int)nbPos, posMin.x, posMin.y, posMin.z);
+    } else {
+      fprintf(f, "%g %g\n", s->t, s->totalBrokenArea / s->inAreaInterfaces);
+    }
+    fflush(f);
+  }
+}
+
+static void close_outputs(rko_sim* s) {
+  FILE** fs[] = {&s->perf_file, &s->ke_file, &s->sb_file};
+  for (int k = 0; k < 3; k++)
+    if (*fs[k]) {
+      fclose(*fs[k]);
+      *fs[k] = NULL;
+    }
+}
+
+/* readDataExtractors (Rockable.cpp:1152-1180): names and their arguments,
+   output files relative to the directory of the file; an unknown name is
+   skipped token by token, as the reference's stream reading skips it.
+   TrackDamage::init sums the live interfaces' areas at this point. */
+int rko_load_data_extractors(rko_sim* s, const char* path) {
+  reader_t r;
+  if (rd_open(&r, path) != 0) return 0;
+  char dir[1024] = {0};
+  dirname_of(path, dir, sizeof dir);
+  char tok[1024];
+  while (rd_tok(&r, tok, sizeof tok)) {
+    if (tok[0] == '/' || tok[0] == '#' || tok[0] == '!') {
+      rd_skipline(&r);
+      continue;
+    }
+    int kind = !strcmp(tok, "dnStat") ? 1 : !strcmp(tok, "TrackDamage") ? 2 : 0;
+    if (!kind || s->nextr >= 16) continue;
+    char name[1024], full[2300];
+    rd_tok(&r, name, sizeof name);
+    int nrec = (int)rd_l(&r);
+    if (name[0] == '/')
+      snprintf(full, sizeof full, "%s", name);
+    else
+      snprintf(full, sizeof full, "%s%s", dir, name);
+    FILE* f = fopen(full, "w");
+    if (!f) {
+      rd_close(&r);
+      return seterr(s, "cannot write %s", full);
+    }
+    s->extr[s->nextr].kind = kind;
+    s->extr[s->nextr].nrec = nrec;
+    s->extr[s->nextr].f = f;
+    s->nextr++;
+    if (kind == 2) {
+      s->inAreaInterfaces = 0.0;
+      for (int k = 0; k < s->nifc; k++)
+        if (s->ifc[k].alive) s->inAreaInterfaces += s->ifc[k].area;
+    }
+  }
+  rd_close(&r);
+  return 0;
+}
+
 static int loop_body(rko_sim* s, const char* outdir) {
   s->t += s->dt;
   s->interConfC += s->dt;
@@ -2549,6 +2734,7 @@ static int loop_body(rko_sim* s, const char* outdir) {
     if (outdir) {
       char fn[1024];
       snprintf(fn, sizeof fn, "%s/conf%d", outdir, s->iconf);
+      conf_outputs(s, 0.0);
       if (rko_save_conf(s, fn) != 0) return -1;
     }
     /* re-read the driving file, then the servo on the new controls
@@ -2564,7 +2750,9 @@ static int loop_body(rko_sim* s, const char* outdir) {
     else
       s->needUpdate = 0;
   }
+  if (outdir) extractors_record(s, s->ext_step);
   if (s->dynamicUpdateNL != 0) dynamic_check(s);
+  s->ext_step++;
   return 0;
 }
 
@@ -2574,7 +2762,15 @@ int64_t rko_integrate(rko_sim* s, const char* outdir, int64_t max_steps) {
     char fn[1024];
     snprintf(fn, sizeof fn, "%s/conf%d", outdir, s->iconf);
     if (rko_save_conf(s, fn) != 0) return -1;
+    const char* names[3] = {"perf.txt", "kineticEnergy.txt", "staticBalance.txt"};
+    FILE** fs[3] = {&s->perf_file, &s->ke_file, &s->sb_file};
+    close_outputs(s);
+    for (int k = 0; k < 3; k++) {
+      snprintf(fn, sizeof fn, "%s/%s", outdir, names[k]);
+      *fs[k] = fopen(fn, "w");
+    }
   }
+  s->ext_step = (int64_t)(s->t / s->dt); /* Rockable.cpp:2578 */
   int64_t n = 0;
   while (s->t <= s->tmax && (max_steps < 0 || n < max_steps)) {
     if (loop_body(s, outdir) != 0) return -1;
@@ -2619,6 +2815,11 @@ rko_sim* rko_create(void) {
 }
 
 void rko_destroy(rko_sim* s) {
+  if (s) {
+    close_outputs(s);
+    for (int q = 0; q < s->nextr; q++)
+      if (s->extr[q].f) fclose(s->extr[q].f);
+  }
   if (!s) return;
   for (int k = 0; k < s->nshapes; k++) shape_free(&s->shapes[k]);
   free(s->shapes);

@@ -42,8 +42,13 @@ void rko_set_verbose(rko_sim* s, int v);
  * "fake step" when run_fake_step != 0. */
 int rko_load_conf(rko_sim* s, const char* path, int run_fake_step);
 /* DrivingSystem::read (DrivingSystem.cpp:44-497): Control lines and the
- * tritriIsostaticCompression servo. */
+ * servos (tritri*, shakers, ramp); the file is read again at each conf save
+ * of rko_integrate / rko_run_steps. */
 int rko_read_driving(rko_sim* s, const char* path);
+
+/* readDataExtractors (Rockable.cpp:1152-1180): dnStat and TrackDamage, their
+ * files relative to the directory of path; recorded by rko_integrate. */
+int rko_load_data_extractors(rko_sim* s, const char* path);
 
 void rko_update_nl(rko_sim* s);            /* UpdateNL() */
 void rko_accelerations(rko_sim* s);        /* accelerations() */

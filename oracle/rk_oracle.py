@@ -58,6 +58,7 @@ def lib():
         L.rko_set_verbose.argtypes = [P, ctypes.c_int]
         L.rko_load_conf.argtypes = [P, ctypes.c_char_p, ctypes.c_int]
         L.rko_read_driving.argtypes = [P, ctypes.c_char_p]
+        L.rko_load_data_extractors.argtypes = [P, ctypes.c_char_p]
         L.rko_update_nl.argtypes = [P]
         L.rko_accelerations.argtypes = [P]
         L.rko_update_kinematics.argtypes = [P]
@@ -107,6 +108,11 @@ class Oracle:
         if driving:
             if self.L.rko_read_driving(self.s, driving.encode()) != 0:
                 raise RuntimeError("oracle driving: " + self.L.rko_last_error(self.s).decode())
+
+    def load_data_extractors(self, path):
+        """dataExtractors.txt (dnStat, TrackDamage), recorded by integrate()."""
+        if self.L.rko_load_data_extractors(self.s, path.encode()) != 0:
+            raise RuntimeError(self.L.rko_last_error(self.s).decode())
 
     def close(self):
         if self.s:

@@ -166,6 +166,18 @@ struct Dev {
   double* edamp;
   int32_t* estick;
   uint8_t* eact;
+  // Far-entry skip.  cbound[b] accumulates a bound on how far any vertex of
+  // body b has moved since the list was built (|dx| + 2 |dQ| rmax[b] per step,
+  // rmax the largest vertex distance from the body's centre; +inf for a
+  // driven body with a control).  An inactive entry with zero history keeps
+  // ekey = dn + cbound[i] + cbound[j] of the step it was last evaluated: its
+  // gap cannot have fallen by more than the growth of cbound[i] + cbound[j]
+  // since, so while ekey - cbound[i] - cbound[j] > kSkipMargin it is still
+  // positive and the entry still inactive, and the step leaves it alone.
+  double* ekey;
+  double* cbound;
+  const double* rmax;
+  int skip_far;           // 1: skip such entries (only when the step writes no geometry of inactive entries)
   const int32_t* istart;
   const int32_t* jstart;
   const int32_t* jlist;
@@ -232,6 +244,10 @@ struct Dev {
   int32_t* flags;  // [0] needUpdate
 };
 
+// Margin of the far-entry skip (m): covers the rounding of the gaps and of
+// the motion bounds, far below any gap of interest.
+constexpr double kSkipMargin = 1e-9;
+
 struct StepParams {
   double dt, dt_2, dt2_2;
   double gx, gy, gz;
@@ -289,6 +305,12 @@ void launch_predict(const Dev& d, const StepParams& sp, int glob, hipStream_t s)
 // all: every body's vertices; else those of the bodies with listed slots
 void launch_glob(const Dev& d, int all, hipStream_t s);
 void launch_interact(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s);
+// The gap of every inactive entry with zero history from the current poses,
+// as the last step computed it (an entry the step skipped keeps an older gap
+// until then); before the list is read out.
+void launch_refresh_gaps(const Dev& d, hipStream_t s);
+// Every entry evaluated at the next step (after the poses were set from outside).
+void launch_reset_keys(const Dev& d, hipStream_t s);
 // kick: the closing half kick; guard: also the displacement check's
 // accumulation (k_guard_check then follows)
 void launch_gather(const Dev& d, const StepParams& sp, int kick, int guard, hipStream_t s);

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <string>
 #include <vector>
@@ -66,6 +67,8 @@ struct rk_engine {
   BodyKin* kin = nullptr;
   double *acc = nullptr, *arot = nullptr, *force = nullptr, *moment = nullptr;
   double *mass = nullptr, *inertia = nullptr, *bf_surface = nullptr;
+  double *cbound = nullptr, *rmax = nullptr;  // far-entry skip: per-body motion bounds
+  int skip_far = 0;
   int32_t* cluster = nullptr;
   int32_t *sh_nv = nullptr, *sh_ne = nullptr, *sh_nf = nullptr, *sh_voff = nullptr, *sh_eoff = nullptr,
           *sh_foff = nullptr, *sh_edge = nullptr, *sh_fstart = nullptr, *sh_fidx = nullptr;
@@ -139,6 +142,19 @@ struct rk_engine {
   int64_t steps = 0, nl_updates = 0;
   int live_interfaces = 0;
   std::vector<double> ctrl_sig;  // the driving file's content last uploaded
+  // integrate()'s output files in its output directory (perf.txt,
+  // kineticEnergy.txt, staticBalance.txt) and the data extractors
+  FILE *perf_file = nullptr, *ke_file = nullptr, *sb_file = nullptr;
+  struct Extractor {
+    int kind;  // 1 dnStat, 2 TrackDamage
+    int nrec;
+    FILE* f;
+  };
+  std::vector<Extractor> extractors;
+  double inAreaInterfaces = 0.0;
+  int64_t ext_step = 0;  // integrate()'s step counter (Rockable.cpp:2578, 2742)
+  std::chrono::steady_clock::time_point conf_clock;
+  double nl_since_conf_s = 0.0, force_since_conf_s = 0.0;
 };
 
 namespace {
@@ -185,12 +201,13 @@ void free_device(rk_engine* eng) {
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
                 eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
                 eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,      eng->gpart,
-                eng->beoff,    eng->bfoff,
+                eng->beoff,    eng->bfoff,     eng->cbound,    eng->rmax,
                 eng->feat,     eng->pred,      eng->feat_part, eng->stat_part, eng->stat_out};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   eng->kin = nullptr;
   eng->acc = eng->arot = eng->force = eng->moment = eng->mass = eng->inertia = eng->bf_surface = nullptr;
+  eng->cbound = eng->rmax = nullptr;
   eng->cluster = nullptr;
   eng->sh_nv = eng->sh_ne = eng->sh_nf = eng->sh_voff = eng->sh_eoff = eng->sh_foff = eng->sh_edge = nullptr;
   eng->sh_fstart = eng->sh_fidx = nullptr;
@@ -250,6 +267,15 @@ std::vector<uint8_t> passive_flags(const Model& m) {
     if (fm && c.i >= 0 && c.i < nb) p[c.i] = 0;
   }
   return p;
+}
+
+// Initial far-entry bounds: 0, or +inf for a driven body that a control
+// moves or turns (its entries are then never skipped).
+std::vector<double> initial_bounds(const Model& m) {
+  std::vector<double> c(std::max<size_t>(1, m.bodies.size()), 0.0);
+  for (const Control& k : m.ctrl)
+    if (k.i >= 0 && k.i < (int)m.bodies.size()) c[k.i] = INFINITY;
+  return c;
 }
 
 // Room in gpart for the gather's workgroups under any launch shape (their
@@ -319,6 +345,10 @@ void assemble_dev(rk_engine* eng) {
   d.edamp = E.edamp;
   d.estick = E.estick;
   d.eact = E.eact;
+  d.ekey = E.ekey;
+  d.cbound = eng->cbound;
+  d.rmax = eng->rmax;
+  d.skip_far = eng->skip_far;
   d.spos = E.spos;
   d.soff = E.soff;
   d.sact = E.sact;
@@ -536,6 +566,16 @@ int upload_model(rk_engine* eng) {
       upload(eng, eng->force, zero3) || upload(eng, eng->moment, zero3) || upload(eng, eng->mass, mass) ||
       upload(eng, eng->inertia, inertia) || upload(eng, eng->cluster, cl) || upload(eng, eng->bf_surface, surf))
     return -1;
+  {
+    // far-entry skip: each body's largest vertex distance from its centre
+    std::vector<double> rmax(std::max(1, nb), 0.0);
+    for (int b = 0; b < nb; b++) {
+      double r2 = 0.0;
+      for (const v3& v : m.shapes[m.bodies[b].shape].vertex) r2 = std::max(r2, v.x * v.x + v.y * v.y + v.z * v.z);
+      rmax[b] = m.bodies[b].homothety * sqrt(r2);
+    }
+    if (upload(eng, eng->rmax, rmax) || upload(eng, eng->cbound, initial_bounds(m))) return -1;
+  }
   // global-vertex table: body b's vertices from bvoff[b]
   {
     std::vector<int32_t> bvoff(nb + 1, 0), bvbody;
@@ -687,7 +727,9 @@ int upload_model(rk_engine* eng) {
       st[k] = I.stick;
       EState s;
       dn[k] = I.dn;
-      act[k] = ((I.dn < 0.0 || I.stick >= 0) ? 1 : 0) | 2;
+      const bool hist = !(I.fn == 0.0 && I.ft.x == 0.0 && I.ft.y == 0.0 && I.ft.z == 0.0 && I.mom.x == 0.0 &&
+                          I.mom.y == 0.0 && I.mom.z == 0.0);
+      act[k] = ((I.dn < 0.0 || I.stick >= 0) ? 1 : 0) | (hist ? 2 : 0);
       s.fn = I.fn;
       s.ftx = I.ft.x;
       s.fty = I.ft.y;
@@ -731,6 +773,7 @@ int upload_model(rk_engine* eng) {
   if (ensure_gpart(eng) != 0) return -1;
   assemble_dev(eng);
   launch_entry_desc(eng->d, eng->stream);
+  launch_reset_keys(eng->d, eng->stream);
   HIPCHK(hipStreamSynchronize(eng->stream));
   return 0;
 }
@@ -829,6 +872,11 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom, bool guard = fal
 int do_update_nl(rk_engine* eng) {
   if (eng->tuning) HIPCHK(hipStreamSynchronize(eng->stream));
   const auto t0 = std::chrono::steady_clock::now();
+  struct NlClock {  // host wall time of the rebuild (it ends with a synchronisation), for perf.txt
+    rk_engine* e;
+    std::chrono::steady_clock::time_point t0;
+    ~NlClock() { e->nl_since_conf_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  } nl_clock{eng, t0};
   ensure_gv(eng, true);
   tic(eng, eng->profile);
   const int nxt = 1 - eng->cur;
@@ -1028,6 +1076,13 @@ int download_interactions(rk_engine* eng, int active_only, std::vector<Inter>& o
   const int n = E.n;
   out.clear();
   if (n == 0) return 0;
+  if (!active_only) {
+    // gaps of the entries the far-entry skip left untouched
+    if (flush_steps(eng) != 0) return -1;
+    ensure_gv(eng, false);
+    launch_refresh_gaps(eng->d, eng->stream);
+    HIPCHK(hipGetLastError());
+  }
   std::vector<int32_t> ei(n), ej(n), st(n);
   std::vector<uint32_t> sub(n);
   std::vector<EState> es(n);
@@ -1068,13 +1123,170 @@ int download_interactions(rk_engine* eng, int active_only, std::vector<Inter>& o
   return 0;
 }
 
-int save_conf_now(rk_engine* eng, const std::string& path) {
+// getKineticEnergy over every body (Rockable.cpp:3958-3970, first = 0).
+void kinetic_energy(const Model& m, double* Et, double* Er) {
+  double et = 0.0, er = 0.0;
+  for (const Body& p : m.bodies) {
+    et += p.mass * vdot(p.vel, p.vel);
+    const v3 w = qrot(qconj(p.Q), p.vrot);
+    er += (p.inertia.x * w.x * w.x + p.inertia.y * w.y * w.y + p.inertia.z * w.z * w.z);
+  }
+  *Et = et * 0.5;
+  *Er = er * 0.5;
+}
+
+// getResultantQuickStats from nDriven (Rockable.cpp:4145-4206) and
+// getInteractionQuickStats (4217-4252) over the active interactions, in
+// their serial order.
+void quick_stats(const Model& m, const std::vector<Inter>& act, double* Fmax, double* F_fnmax, double* fnMaxOut,
+                 double* fnMeanOut) {
+  const int nb = (int)m.bodies.size();
+  std::vector<double> fnMax(nb + 1, 0.0);
+  std::vector<size_t> nbCtc(nb + 1, 0);
+  for (const Inter& I : act) {
+    nbCtc[I.i] += 1;
+    nbCtc[I.j] += 1;
+    if (fnMax[I.i] < I.fn) fnMax[I.i] = I.fn;
+    if (fnMax[I.j] < I.fn) fnMax[I.j] = I.fn;
+  }
+  const int first = m.nDriven, last = nb - 1;
+  double F = first <= last ? sqrt(vnorm2(m.bodies[first].force)) : 0.0;
+  *Fmax = F;
+  *F_fnmax = 0.0;
+  for (int i = first; i <= last; i++) {
+    if (nbCtc[i] <= 1) continue;
+    F = sqrt(vnorm2(m.bodies[i].force));
+    if (F > *Fmax) *Fmax = F;
+    if (fnMax[i] > 0.0) {
+      F /= fnMax[i];
+      if (F > *F_fnmax) *F_fnmax = F;
+    }
+  }
+  if (act.empty()) {
+    *fnMaxOut = *fnMeanOut = 0.0;
+    return;
+  }
+  double fn = act[0].fn, mx = fn, mean = 0.0;
+  size_t n = 1;
+  for (size_t k = 1; k < act.size(); k++) {
+    n += 1;
+    fn = act[k].fn;
+    mean += fn;
+    if (fn > mx) mx = fn;
+  }
+  if (n > 1) mean /= (double)(n - 1);
+  *fnMaxOut = mx;
+  *fnMeanOut = mean;
+}
+
+// The records integrate() writes at each conf save (Rockable.cpp:2600-2668):
+// perf.txt (t, simulated interConf per wall second, % of that wall time in
+// neighbour-list updates and in force phases, as the host saw them),
+// staticBalance.txt, kineticEnergy.txt; numbers as an std::ostream prints them.
+void conf_outputs(rk_engine* eng, const std::vector<Inter>& act) {
+  const Model& m = eng->m;
+  if (eng->perf_file) {
+    const auto now = std::chrono::steady_clock::now();
+    const double el = std::chrono::duration<double>(now - eng->conf_clock).count();
+    const double nlp = el > 0.0 ? std::round(1000.0 * eng->nl_since_conf_s / el) / 10.0 : 0.0;
+    const double fp = el > 0.0 ? std::round(1000.0 * eng->force_since_conf_s / el) / 10.0 : 0.0;
+    fprintf(eng->perf_file, "%g %g %g %g\n", m.t, el > 0.0 ? m.interConf / el : 0.0, nlp, fp);
+    fflush(eng->perf_file);
+    eng->conf_clock = now;
+    eng->nl_since_conf_s = eng->force_since_conf_s = 0.0;
+  }
+  double Fmax, F_fnmax, fnMax, fnMean;
+  quick_stats(m, act, &Fmax, &F_fnmax, &fnMax, &fnMean);
+  if (eng->sb_file && fnMax != 0.0 && fnMean != 0.0) {
+    fprintf(eng->sb_file, "%g %g %g\n", m.t, F_fnmax, fabs(Fmax / fnMean));
+    fflush(eng->sb_file);
+  }
+  if (eng->ke_file) {
+    double Et, Er;
+    kinetic_energy(m, &Et, &Er);
+    fprintf(eng->ke_file, "%g %g %g\n", m.t, Et, Er);
+    fflush(eng->ke_file);
+  }
+}
+
+int save_conf_now(rk_engine* eng, const std::string& path, bool outputs = false) {
   if (download_bodies(eng) != 0) return -1;
   std::vector<Inter> act;
   if (download_interactions(eng, 1, act) != 0) return -1;
+  if (outputs) conf_outputs(eng, act);
   std::string err;
   if (save_conf(eng->m, act, path, err) != 0) return fail(eng, err);
   return 0;
+}
+
+// A record of the data extractors is due at integrate()'s current step.
+bool record_due(const rk_engine* eng, bool* needs_geometry) {
+  bool due = false, geo = false;
+  for (const auto& x : eng->extractors)
+    if (x.nrec > 0 && eng->ext_step % x.nrec == 0) {
+      due = true;
+      geo = geo || x.kind == 1;
+    }
+  if (needs_geometry) *needs_geometry = geo;
+  return due;
+}
+
+// DataExtractor::record of dnStat (DataExtractor_dnStat.cpp:69-105) and
+// TrackDamage (DataExtractor_TrackDamage.cpp:47-50) at integrate()'s step.
+int extractors_record(rk_engine* eng) {
+  if (!record_due(eng, nullptr)) return 0;
+  std::vector<Inter> act;
+  bool need_act = false;
+  for (const auto& x : eng->extractors)
+    if (x.kind == 1 && x.nrec > 0 && eng->ext_step % x.nrec == 0) need_act = true;
+  if (need_act && download_interactions(eng, 1, act) != 0) return -1;
+  const double t = eng->m.t;
+  for (const auto& x : eng->extractors) {
+    if (x.nrec <= 0 || eng->ext_step % x.nrec != 0) continue;
+    if (x.kind == 1) {
+      double dnMin = 1e20, dnMax = -1e20, dnMeanNeg = 0.0, dnMeanPos = 0.0, nbNeg = 0.0, nbPos = 0.0, dnMean = 0.0;
+      v3 posMin = V(0.0, 0.0, 0.0);
+      for (const Inter& I : act) {
+        const double dn = I.dn;
+        dnMean += dn;
+        if (dn < 0.0) {
+          dnMeanNeg += dn;
+          nbNeg += 1.0;
+        } else {
+          dnMeanPos += dn;
+          nbPos += 1.0;
+        }
+        if (dn < dnMin) {
+          dnMin = dn;
+          posMin = I.pos;
+        }
+        if (dn > dnMax) dnMax = dn;
+      }
+      if (nbNeg > 0.0) dnMeanNeg /= nbNeg;
+      if (nbPos > 0.0) dnMeanPos /= nbPos;
+      if (nbNeg > 0.0 || nbPos > 0.0) dnMean /= (nbNeg + nbPos);
+      fprintf(x.f, "%g %g %g %g %g %g %d %d %g %g %g\n", t, dnMin, dnMax, dnMean, dnMeanNeg, dnMeanPos, (int)nbNeg,
+              (int)nbPos, posMin.x, posMin.y, posMin.z);
+    } else {
+      fprintf(x.f, "%g %g\n", t, eng->m.totalBrokenArea / eng->inAreaInterfaces);
+    }
+    fflush(x.f);
+  }
+  return 0;
+}
+
+void close_outputs(rk_engine* eng) {
+  for (FILE** f : {&eng->perf_file, &eng->ke_file, &eng->sb_file})
+    if (*f) {
+      fclose(*f);
+      *f = nullptr;
+    }
+}
+
+void close_extractors(rk_engine* eng) {
+  for (auto& x : eng->extractors)
+    if (x.f) fclose(x.f);
+  eng->extractors.clear();
 }
 
 int check_loaded(rk_engine* eng) {
@@ -1141,14 +1353,16 @@ int loop_body(rk_engine* eng, const char* outdir) {
   m.interConfC += m.dt;
   m.interVerletC += m.dt;
   const bool conf_due = m.interConfC >= m.interConf - m.dt_2;
-  if (enqueue_step(eng, conf_due && outdir, false) != 0) return -1;
+  bool rec_geo = false;
+  const bool rec = outdir && record_due(eng, &rec_geo);
+  if (enqueue_step(eng, (conf_due && outdir) || rec_geo, false) != 0) return -1;
   eng->steps++;
   if (conf_due) {
     m.iconf++;
     if (outdir) {
       char fn[4096];
       snprintf(fn, sizeof fn, "%s/conf%d", outdir, m.iconf);
-      if (save_conf_now(eng, fn) != 0) return -1;
+      if (save_conf_now(eng, fn, true) != 0) return -1;
     }
     if (reread_driving(eng) != 0) return -1;
     m.interConfC = 0.0;
@@ -1161,6 +1375,11 @@ int loop_body(rk_engine* eng, const char* outdir) {
     else
       eng->needUpdate = false;
   }
+  if (rec) {
+    if (flush_steps(eng) != 0) return -1;
+    if (extractors_record(eng) != 0) return -1;
+  }
+  eng->ext_step++;
   return 0;
 }
 
@@ -1169,10 +1388,10 @@ int loop_body(rk_engine* eng, const char* outdir) {
 struct LoopScalars {
   double t, interConfC, interVerletC;
   int iconf;
-  int64_t steps;
+  int64_t steps, ext_step;
 };
 LoopScalars loop_scalars(const rk_engine* eng) {
-  return {eng->m.t, eng->m.interConfC, eng->m.interVerletC, eng->m.iconf, eng->steps};
+  return {eng->m.t, eng->m.interConfC, eng->m.interVerletC, eng->m.iconf, eng->steps, eng->ext_step};
 }
 void set_loop_scalars(rk_engine* eng, const LoopScalars& s) {
   eng->m.t = s.t;
@@ -1180,6 +1399,7 @@ void set_loop_scalars(rk_engine* eng, const LoopScalars& s) {
   eng->m.interVerletC = s.interVerletC;
   eng->m.iconf = s.iconf;
   eng->steps = s.steps;
+  eng->ext_step = s.ext_step;
 }
 
 // The 16 network inputs of the current state into eng->feat (device).
@@ -1256,16 +1476,19 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       m.interConfC += m.dt;
       m.interVerletC += m.dt;
       const bool conf_due = m.interConfC >= m.interConf - m.dt_2;
-      if (enqueue_step(eng, conf_due && outdir, true) != 0) return -1;
+      bool rec_geo = false;
+      const bool rec = outdir && record_due(eng, &rec_geo);
+      if (enqueue_step(eng, (conf_due && outdir) || rec_geo, true) != 0) return -1;
       eng->steps++;
       if (conf_due) {
         m.iconf++;
         m.interConfC = 0.0;
       }
+      eng->ext_step++;
       hist.push_back(loop_scalars(eng));
       conf_at.push_back(conf_due ? 1 : 0);
       k++;
-      if ((conf_due && (outdir || !m.driving_path.empty())) || eng->needUpdate ||
+      if ((conf_due && (outdir || !m.driving_path.empty())) || rec || eng->needUpdate ||
           m.interVerletC >= m.interVerlet - m.dt_2 || (until_tmax && m.t > m.tmax))
         break;
       if (pipe && --chunk_left == 0) {
@@ -1309,13 +1532,20 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     if (conf_at[executed] && outdir) {
       char fn[4096];
       snprintf(fn, sizeof fn, "%s/conf%d", outdir, m.iconf);
-      if (save_conf_now(eng, fn) != 0) return -1;
+      if (save_conf_now(eng, fn, true) != 0) return -1;
     }
     if (conf_at[executed] && reread_driving(eng) != 0) return -1;
     if (need || m.interVerletC >= m.interVerlet - m.dt_2) {
       if (do_update_nl(eng) != 0) return -1;
       if (!need) m.interVerletC = 0.0;
       eng->needUpdate = false;
+    }
+    if (outdir) {
+      // the data extractors of the last executed step (its counter was
+      // advanced with it)
+      eng->ext_step--;
+      if (extractors_record(eng) != 0) return -1;
+      eng->ext_step++;
     }
     done += executed;
     if (eng->model && !g.frozen) eng->model_trust = std::min(1.0, eng->model_trust * 1.1);
@@ -1382,6 +1612,8 @@ int rk_engine_create(rk_engine** out, int device) {
 
 void rk_engine_destroy(rk_engine* eng) {
   if (!eng) return;
+  close_outputs(eng);
+  close_extractors(eng);
   (void)hipSetDevice(eng->device);
   if (eng->stream) (void)hipStreamSynchronize(eng->stream);
   free_device(eng);
@@ -1400,6 +1632,8 @@ const char* rk_last_error(const rk_engine* eng) { return eng ? eng->err.c_str() 
 namespace {
 void reset_engine(rk_engine* eng) {
   (void)hipSetDevice(eng->device);
+  close_outputs(eng);
+  close_extractors(eng);
   free_device(eng);
   eng->m = Model();
   eng->loaded = false;
@@ -1494,6 +1728,13 @@ int upload_controls(rk_engine* eng) {
   }
   if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
   if (upload(eng, eng->passive, passive_flags(eng->m))) return -1;
+  if (nd > 0) {
+    // the driven bodies' far-entry bounds follow their controls; every entry
+    // is evaluated at the next step
+    const std::vector<double> cb = initial_bounds(eng->m);
+    HIPCHK(hipMemcpyAsync(eng->cbound, cb.data(), sizeof(double) * nd, hipMemcpyHostToDevice, eng->stream));
+  }
+  launch_reset_keys(eng->d, eng->stream);
   // the heavy bodies' summation paths depend on the passive flags
   if (nl_build_heavy(eng->w, (int)eng->m.bodies.size(), eng->E[eng->cur], eng->passive, eng->stream) != 0)
     return fail(eng, eng->w.err);
@@ -1504,6 +1745,39 @@ int upload_controls(rk_engine* eng) {
   return 0;
 }
 }  // namespace
+
+int rk_load_data_extractors(rk_engine* eng, const char* path) {
+  if (check_loaded(eng)) return -1;
+  if (!path) return fail(eng, "no path given");
+  std::ifstream is(path);
+  if (!is) return 0;  // no file: no extractor (Rockable.cpp:1157-1159)
+  close_extractors(eng);
+  const std::string p(path);
+  const size_t slash = p.rfind('/');
+  const std::string dir = slash == std::string::npos ? std::string() : p.substr(0, slash + 1);
+  std::string tok;
+  while (is >> tok) {
+    if (tok[0] == '/' || tok[0] == '#' || tok[0] == '!') {
+      std::getline(is, tok);
+      continue;
+    }
+    const int kind = tok == "dnStat" ? 1 : tok == "TrackDamage" ? 2 : 0;
+    if (!kind) continue;  // unknown: the reference warns, and reads on token by token
+    std::string name;
+    int nrec = 0;
+    is >> name >> nrec;
+    const std::string full = (!name.empty() && name[0] == '/') ? name : dir + name;
+    FILE* f = fopen(full.c_str(), "w");
+    if (!f) return fail(eng, "cannot write " + full);
+    eng->extractors.push_back({kind, nrec, f});
+    if (kind == 2) {  // TrackDamage::init: the live interfaces' total area
+      eng->inAreaInterfaces = 0.0;
+      for (const Interface& BI : eng->m.ifc)
+        if (BI.alive) eng->inAreaInterfaces += BI.area;
+    }
+  }
+  return 0;
+}
 
 int rk_set_force_law(rk_engine* eng, const char* name) {
   if (check_loaded(eng)) return -1;
@@ -1568,6 +1842,17 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
   else if (n == "exact_sums") {
     eng->exact_sums = value != 0.0;
     if (eng->loaded) assemble_dev(eng);
+  } else if (n == "skip_far") {
+    if (!(value == 0.0 || value == 1.0)) return fail(eng, "skip_far must be 0 or 1");
+    if (eng->loaded) {
+      (void)hipSetDevice(eng->device);
+      if (flush_steps(eng) != 0) return -1;
+    }
+    eng->skip_far = (int)value;
+    if (eng->loaded) {
+      assemble_dev(eng);
+      launch_reset_keys(eng->d, eng->stream);  // keys are kept only while skipping is on
+    }
   } else if (n == "gravity_x" || n == "gravity_y" || n == "gravity_z") {
     // the conf keyword "gravity" (Rockable.cpp:978-980), changed between steps
     if (check_loaded(eng)) return -1;
@@ -1769,6 +2054,8 @@ int rk_get_option(rk_engine* eng, const char* name, double* value) {
     *value = eng->profile;
   else if (n == "exact_sums")
     *value = eng->exact_sums;
+  else if (n == "skip_far")
+    *value = eng->skip_far;
   else if (n == "heavy_min")
     *value = eng->w.heavy_min;
   else if (n == "guard_batch")
@@ -1806,7 +2093,19 @@ int rk_integrate(rk_engine* eng, const char* outdir, int64_t max_steps, int64_t*
     char fn[4096];
     snprintf(fn, sizeof fn, "%s/conf%d", outdir, eng->m.iconf);
     if (save_conf_now(eng, fn) != 0) return -1;
+    // the output files of initOutputFiles (Rockable.cpp:215-222), in the output directory
+    close_outputs(eng);
+    const char* names[3] = {"perf.txt", "kineticEnergy.txt", "staticBalance.txt"};
+    FILE** fs[3] = {&eng->perf_file, &eng->ke_file, &eng->sb_file};
+    for (int k = 0; k < 3; k++) {
+      snprintf(fn, sizeof fn, "%s/%s", outdir, names[k]);
+      *fs[k] = fopen(fn, "w");
+      if (!*fs[k]) return fail(eng, std::string("cannot write ") + fn);
+    }
+    eng->conf_clock = std::chrono::steady_clock::now();
+    eng->nl_since_conf_s = eng->force_since_conf_s = 0.0;
   }
+  eng->ext_step = (int64_t)(eng->m.t / eng->m.dt);  // Rockable.cpp:2578
   int64_t n = 0;
   if (eng->guard) {
     if (run_guarded(eng, outdir, max_steps < 0 ? INT64_MAX : max_steps, true, &n) != 0) return -1;
@@ -1960,6 +2259,8 @@ int rk_set_bodies(rk_engine* eng, const double* pos, const double* Q, const doub
   }
   HIPCHK(hipMemcpy(eng->kin, kin.data(), sizeof(BodyKin) * nb, hipMemcpyHostToDevice));
   eng->gv_state = rk_engine::GV_STALE;
+  launch_reset_keys(eng->d, eng->stream);  // poses set from outside: no motion bound holds
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -1061,6 +1061,7 @@ struct MergeOut {
   double* edamp;
   int32_t* estick;
   uint8_t* eact;
+  double* ekey;
 };
 
 // Damp of the body pair (Rockable.cpp:1356-1376), pair (i < j)
@@ -1179,13 +1180,18 @@ __global__ void k_merge_fill(Dev d, int n, MergeIn in, MergeOut out) {
   } else {
     const double dn = in.oedn[o];
     const int stick = in.ostick[o];
-    out.est[e] = in.oest[o];
+    const EState st = in.oest[o];
+    out.est[e] = st;
     out.edn[e] = dn;
     out.egeo[e] = in.ogeo[o];
     out.edamp[e] = in.odamp[o];
     out.estick[e] = stick;
-    out.eact[e] = ((dn < 0.0 || stick >= 0) ? 1 : 0) | 2;
+    // bit 1: the history is not all zero (a NaN counts as non-zero)
+    const bool hist = !(st.fn == 0.0 && st.ftx == 0.0 && st.fty == 0.0 && st.ftz == 0.0 && st.momx == 0.0 &&
+                        st.momy == 0.0 && st.momz == 0.0);
+    out.eact[e] = ((dn < 0.0 || stick >= 0) ? 1 : 0) | (hist ? 2 : 0);
   }
+  out.ekey[e] = -INFINITY;  // every entry of a new list is evaluated at its first step
 }
 
 // --- CSR -----------------------------------------------------------------
@@ -1691,6 +1697,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     out.edamp = nxt.edamp;
     out.estick = nxt.estick;
     out.eact = nxt.eact;
+    out.ekey = nxt.ekey;
     if (ensure(w, w.msrc, w.msrc_cap, (size_t)nnew)) return -1;
     out.src = w.msrc;
     hipLaunchKernelGGL(k_merge<true>, dim3(nblk(ng)), dim3(kBlock), 0, s, d, in, out);
@@ -1712,6 +1719,7 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
             hipMalloc(&E.edn, c * sizeof(double)) == hipSuccess &&
             hipMalloc(&E.egeo, c * sizeof(EGeom)) == hipSuccess && hipMalloc(&E.edamp, c * sizeof(double)) == hipSuccess &&
             hipMalloc(&E.estick, c * sizeof(int32_t)) == hipSuccess && hipMalloc(&E.eact, c * sizeof(uint8_t)) == hipSuccess &&
+            hipMalloc(&E.ekey, c * sizeof(double)) == hipSuccess &&
             hipMalloc(&E.spos, c * 2 * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&E.sact, c * 2 * sizeof(uint8_t)) == hipSuccess &&
             hipMalloc(&E.sval, c * 12 * sizeof(double)) == hipSuccess &&
@@ -1727,9 +1735,11 @@ int entries_reserve(NLWork& w, EntryArrays& E, int n) {
 }
 
 void entries_free(EntryArrays& E) {
-  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.edn, E.egeo, E.edamp, E.estick, E.eact, E.spos, E.sact, E.sval, E.akin, E.acount};
+  void* ps[] = {E.ei, E.ej, E.esub, E.est, E.edn, E.egeo, E.edamp, E.estick, E.eact, E.ekey, E.spos, E.sact, E.sval,
+                E.akin, E.acount};
   for (void* q : ps)
     if (q) (void)hipFree(q);
+  E.ekey = nullptr;
   E.ei = E.ej = nullptr;
   E.esub = nullptr;
   E.est = nullptr;

@@ -23,6 +23,9 @@ struct EntryArrays {
   double* edamp = nullptr;
   int32_t* estick = nullptr;
   uint8_t* eact = nullptr;
+  // far-entry skip: for an inactive entry with zero history, its last gap
+  // plus both bodies' motion bounds at that step (-inf: evaluate)
+  double* ekey = nullptr;
   int32_t* spos = nullptr;   // 2 per entry
   uint8_t* sact = nullptr;   // 2 per entry
   double* sval = nullptr;    // 12 per entry

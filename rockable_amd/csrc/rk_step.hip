@@ -400,9 +400,16 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
   // runs one branch of update_kinematics instead of all four; without it
   // each thread takes its own entry, one dependent load and two barriers
   // sooner, its wavefront running the branches of the types it holds.
+  // The far-entry skip (only when no geometry of inactive entries is
+  // written): an inactive entry with zero history whose gap is proven
+  // positive is left as it is; it sorts past the end.
+  const bool skipping = !GEOM_ALL && d.skip_far;
   if (SORT) {
     const int e0 = blk * BS + (int)threadIdx.x;
-    const int ty = e0 < d.ne ? (int)(d.esub[e0] >> 30) : 4;
+    int ty = e0 < d.ne ? (int)(d.esub[e0] >> 30) : 4;
+    if (skipping && e0 < d.ne && d.eact[e0] == 0 &&
+        d.ekey[e0] - d.cbound[d.ei[e0]] - d.cbound[d.ej[e0]] > kSkipMargin)
+      ty = 4;
     unsigned long long mine = 0ull;
 #pragma unroll
     for (int t = 0; t < 5; t++) {
@@ -420,7 +427,10 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
   }
   const int e = blk * BS + (SORT ? (int)perm[threadIdx.x] : (int)threadIdx.x);
   bool act = false;
-  if (e < d.ne) {
+  bool skip = false;
+  if (skipping && e < d.ne && d.eact[e] == 0)
+    skip = d.ekey[e] - d.cbound[d.ei[e]] - d.cbound[d.ej[e]] > kSkipMargin;
+  if (e < d.ne && !skip) {
     const int i = d.ei[e], j = d.ej[e];
     const int type = (int)(d.esub[e] >> 30);
     const BodyKin& Pi = d.kin[i];
@@ -465,8 +475,15 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
           now = 0;
         }
         d.edn[e] = 0.0;
+        if (skipping) d.ekey[e] = -INFINITY;
       } else {
         d.edn[e] = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
+        if (skipping && now == 0) {
+          // the key of a zero-history entry (a body without a finite bound
+          // never lets its entries be skipped)
+          const double ci = d.cbound[i], cj = d.cbound[j];
+          d.ekey[e] = (ci < INFINITY && cj < INFINITY) ? k.dn + ci + cj : -INFINITY;
+        }
         if (GEOM_ALL) {
           const v3 vel = rel_vel(k.pos, Pi, Pj);
           EGeom g;
@@ -676,6 +693,24 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
     if ((int)threadIdx.x < total) law_item<LAW>(d, sp, write_geom_all, pii, sa[threadIdx.x]);
     __syncthreads();
   }
+}
+
+// The gaps of the inactive entries with zero history from the current
+// poses: what the last step computed for them (Interaction::UpdateDispatcher,
+// deactivate() giving 0), including the entries that step skipped.
+__global__ void __launch_bounds__(kBlock) k_refresh_gaps(Dev d) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.ne; e += gridDim.x * blockDim.x) {
+    if (d.eact[e] != 0) continue;
+    Kin k;
+    k.deact = false;
+    k.prev_dn = 0.0;
+    update_kinematics<false>(d, (int)(d.esub[e] >> 30), d.edesc[e], d.kin[d.ei[e]], d.kin[d.ej[e]], k);
+    d.edn[e] = k.deact ? 0.0 : k.dn;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_reset_keys(Dev d) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.ne; e += gridDim.x * blockDim.x) d.ekey[e] = -INFINITY;
 }
 
 __device__ __forceinline__ v3 euler_domega(v3 I, v3 w, v3 M) {
@@ -1474,6 +1509,18 @@ __device__ __forceinline__ void predict_free(const Dev& d, const StepParams& sp,
   P.wy = w2.y;
   P.wz = w2.z;
 }
+// The far-entry skip's motion bound of one predicted body: no vertex of it
+// moved by more than |dx| + 2 |dQ| rmax in this step (for unit quaternions
+// |Q' v Q'* - Q v Q*| <= 2 |Q' - Q| |v|), plus a rounding allowance.
+__device__ __forceinline__ void add_motion_bound(const Dev& d, int b, const BodyKin& old, const BodyKin& P) {
+  if (!d.skip_far) return;
+  const double dx = P.px - old.px, dy = P.py - old.py, dz = P.pz - old.pz;
+  const double qw = P.qw - old.qw, qx = P.qx - old.qx, qy = P.qy - old.qy, qz = P.qz - old.qz;
+  const double step = sqrt(dx * dx + dy * dy + dz * dz) + 2.0 * sqrt(qw * qw + qx * qx + qy * qy + qz * qz) * d.rmax[b];
+  const double mag = fabs(P.px) + fabs(P.py) + fabs(P.pz);
+  d.cbound[b] += step + 1e-15 * (1.0 + mag);
+}
+
 __device__ __forceinline__ void store_predicted(const Dev& d, int b, const BodyKin& P) {
   BodyKin* out = d.kin + b;
   out->px = P.px;
@@ -1501,8 +1548,10 @@ __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp, int gl
   for (int b = gid; b < d.nb; b += gridDim.x * blockDim.x) {
     if (b < d.nDriven) continue;
     BodyKin P = d.kin[b];
+    const BodyKin old = P;
     predict_free(d, sp, b, P);
     store_predicted(d, b, P);
+    add_motion_bound(d, b, old, P);
     if (glob && d.soff[b + 1] != d.soff[b]) {
       const int v0 = d.bvoff[b], nv = d.bvoff[b + 1] - v0;
       for (int v = 0; v < nv; v++) st3(d.gv + 3 * (size_t)(v0 + v), glob_vertex(d.sh, P, v));
@@ -1542,8 +1591,10 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
       shp[t] = -1;
       if (b < bend) {
         BodyKin P = d.kin[b];
+        const BodyKin old = P;
         predict_free(d, sp, b, P);
         store_predicted(d, b, P);
+        add_motion_bound(d, b, old, P);
         if (d.soff[b + 1] != d.soff[b]) {
           pose[t][0] = P.px;
           pose[t][1] = P.py;
@@ -1598,8 +1649,12 @@ __global__ void __launch_bounds__(kBlock) k_predict_vtx(Dev d, StepParams sp) {
     const bool listed = d.soff[b + 1] != d.soff[b];
     if (v != 0 && !listed) continue;
     BodyKin P = d.kin[b];
+    const BodyKin old = P;
     predict_free(d, sp, b, P);
-    if (v == 0) store_predicted(d, b, P);
+    if (v == 0) {
+      store_predicted(d, b, P);
+      add_motion_bound(d, b, old, P);
+    }
     if (listed) st3(d.gv + 3 * (size_t)k, glob_vertex(d.sh, P, v));
   }
 }
@@ -1666,6 +1721,16 @@ void launch_glob(const Dev& d, int all, int nv, hipStream_t s) {
 }
 
 void launch_glob(const Dev& d, int all, hipStream_t s) { launch_glob(d, all, d.nvtot, s); }
+
+void launch_refresh_gaps(const Dev& d, hipStream_t s) {
+  if (d.ne <= 0 || !d.skip_far) return;
+  hipLaunchKernelGGL(k_refresh_gaps, dim3(std::min((d.ne + kBlock - 1) / kBlock, 65536)), dim3(kBlock), 0, s, d);
+}
+
+void launch_reset_keys(const Dev& d, hipStream_t s) {
+  if (d.ne <= 0 || !d.ekey) return;
+  hipLaunchKernelGGL(k_reset_keys, dim3(std::min((d.ne + kBlock - 1) / kBlock, 65536)), dim3(kBlock), 0, s, d);
+}
 
 void launch_entry_desc(const Dev& d, hipStream_t s) {
   if (d.ne <= 0 || !d.edesc) return;

@@ -46,6 +46,7 @@ class RkConfInfo(ctypes.Structure):
 
 
 EXPORTED = ["rk_inspect_conf", "rk_engine_create", "rk_engine_destroy", "rk_last_error", "rk_load_conf", "rk_load_driving_system",
+            "rk_load_data_extractors",
             "rk_set_force_law", "rk_set_body_force", "rk_set_option", "rk_update_nl", "rk_accelerations",
             "rk_step", "rk_integrate", "rk_save_conf", "rk_synchronize", "rk_get_counts", "rk_get_bodies",
             "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time",
@@ -73,6 +74,7 @@ def load_library(path=LIB_PATH):
     L.rk_last_error.restype = ctypes.c_char_p
     L.rk_load_conf.argtypes = [P, ctypes.c_char_p]
     L.rk_load_driving_system.argtypes = [P, ctypes.c_char_p]
+    L.rk_load_data_extractors.argtypes = [P, ctypes.c_char_p]
     L.rk_set_force_law.argtypes = [P, ctypes.c_char_p]
     L.rk_set_body_force.argtypes = [P, ctypes.c_char_p, P, ctypes.c_int]
     L.rk_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_double]
@@ -183,6 +185,10 @@ class Engine:
 
     def load_driving_system(self, path):
         self._check(self.L.rk_load_driving_system(self.h, os.fspath(path).encode()))
+
+    def load_data_extractors(self, path):
+        """dataExtractors.txt (dnStat, TrackDamage), recorded by integrate()."""
+        self._check(self.L.rk_load_data_extractors(self.h, os.fspath(path).encode()))
 
     def set_force_law(self, name):
         return self._check(self.L.rk_set_force_law(self.h, name.encode()))

@@ -100,21 +100,22 @@ def hull_pair_decks(dirpath, n_bodies=300, law="Default"):
 
 
 def spun_bond(dirpath, R=0.5, w=2.0, kn=1e5, kt=8e4, kr=0.0, fn0=10.0, ft0=1.0, mom0=1.0, power=2.0, damp=0.0,
-              dt=1e-6, density=1e15):
-    """Two touching spheres of huge mass (dn = dn0 = 0) joined by one
-    StickedLinks bond with its parameters in the interface
-    (ParamsInInterfaces 1); the bonded vertex-vertex entry carries the
-    damping coefficient damp.  Body 1 spins at w about z, body 0 is at rest."""
+              dt=1e-6, density=1e15, overlap=0.0):
+    """Two touching spheres of huge mass (dn = dn0 = 0; or overlapping by
+    `overlap`, dn0 still 0: a bond in compression) joined by one StickedLinks
+    bond with its parameters in the interface (ParamsInInterfaces 1); the
+    bonded vertex-vertex entry carries the damping coefficient damp.  Body 1
+    spins at w about z, body 0 is at rest."""
     shapes = [D.sphere_shape("S", R)]
     D.write_shp(os.path.join(dirpath, "shapes.shp"), shapes)
     deck = D.Deck(dt=dt, tmax=1e3, interVerlet=1e-4, interConf=1e3, DVerlet=0.1 * R, dVerlet=0.05 * R,
                   density={0: density}, gravity=(0.0, 0.0, 0.0), force_law="StickedLinks",
                   params=D.contact_params(kn, 0.8 * kn, 0.25, 0.3))
-    x0, x1 = np.zeros(3), np.array([2 * R, 0.0, 0.0])
+    x0, x1 = np.zeros(3), np.array([2 * R - overlap, 0.0, 0.0])
     deck.bodies = [D.Body("S", pos=tuple(x0)), D.Body("S", pos=tuple(x1), vrot=(0.0, 0.0, w))]
     n = (x0 - x1) / np.linalg.norm(x0 - x1)
     pos = x0 - n * R
-    rec = (0, 1, 0, 0, 0) + tuple(n) + (0.0,) + tuple(pos) + (0.0, 0.0, 0.0) + (0.0,) + (0.0,) * 6 + (damp,)
+    rec = (0, 1, 0, 0, 0) + tuple(n) + (-overlap,) + tuple(pos) + (0.0, 0.0, 0.0) + (0.0,) + (0.0,) * 6 + (damp,)
     deck.interactions = [rec]
     deck.params_in_interfaces = 1
     deck.interfaces = [(0, 1, 0.0, (kn, kt, kr, fn0, ft0, mom0, power, 0.0), [(0, 0, 0)])]

@@ -1,15 +1,16 @@
 """The decks of the oracle's known-answer pins (tests/test_oracle_pins.py) on
 the GPU: the Default friction cap with damping on, critical damping for
 en2 <= 0, the Avalanche rolling-moment cap, bonds breaking on their
-tangential, viscous and moment terms (power 2 and 1.5), the biaxial servo and
-moment controls.  The engine is bit-identical to the oracle on every one, at
+tangential, viscous and moment terms (power 2 and 1.5), the StickedLinks
+contact cap, a compressed bond breaking into a capped contact, the biaxial
+servo and moment controls.  The engine is bit-identical to the oracle on every one, at
 every checkpoint, with the same live interfaces.
 
 PreferredDirection is the exception: its restoring angle is acos of a
 cosine, and the device's acos and glibc's may differ in the last bit.  The
 bar there is the tolerance of the path's contract: body moments within 1e-10
-relative of the oracle after every checkpoint and positions / orientations
-within 1e-8 relative over 1000 steps (here 128 000 steps)."""
+of the oscillation's moment amplitude at every checkpoint, orientations and
+spins within 1e-8 of theirs, over 128 000 steps."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,13 @@ def _case(name, d):
         return K.spun_bond(d, w=0.4, kt=1.0, kr=3.1e2, ft0=1e6, mom0=0.25, power=2.0), None, 2200
     if name == "bond_power":
         return K.spun_bond(d, w=0.4, kt=1.5e4, damp=0.02, kr=1.3e2, ft0=1.0, mom0=0.5, power=1.5), None, 300
+    if name == "sticked_contact_cap":
+        import os
+        from rockable_amd import decks as Dk
+        path, drv, steps = P.default_cap_deck(d)
+        return Dk.conf_with(path, os.path.join(d, "sticked.conf"), forceLaw="StickedLinks"), drv, 2 * steps
+    if name == "bond_compressed":
+        return K.spun_bond(d, w=0.4, kt=2e4, ft0=1.0, mom0=1.0, power=2.0, overlap=1e-5), None, 600
     if name == "biaxial_servo":
         return P.servo_box_deck(d, "tritriBiaxialCompression 50.0 0.2")
     if name in ("moment_x", "moment_xyz"):
@@ -45,8 +53,8 @@ def _case(name, d):
 
 
 BIT_EXACT = ["default_cap", "critical_damping_0", "critical_damping_m1", "rolling_normal", "rolling_oblique",
-             "rolling_parallel", "bond_viscous", "bond_moment", "bond_power", "biaxial_servo", "moment_x",
-             "moment_xyz"]
+             "rolling_parallel", "bond_viscous", "bond_moment", "bond_power", "sticked_contact_cap", "bond_compressed",
+             "biaxial_servo", "moment_x", "moment_xyz"]
 
 
 @pytest.mark.parametrize("name", BIT_EXACT)
@@ -81,14 +89,18 @@ def test_preferred_direction_within_tolerance(tmp_path, antiparallel):
     o = Oracle(path)
     e.update_nl()
     o.update_nl()
+    # amplitudes of the oscillation: moment Kr theta0, spin theta0 w
+    import math
+    R, momentMax, theta0 = 0.5, 2.0e3, 0.4
+    Kr = 2.0 * momentMax / math.pi
+    w = math.sqrt(Kr / (0.4 * R * R * K.sphere_mass(R)))
     differing = 0
     for _ in range(8):
         e.step(steps // 8)
         o.run_steps(steps // 8)
         g, r = e.bodies(), o.bodies()
-        scale = np.abs(r["moment"]).max()
-        assert np.abs(g["moment"] - r["moment"]).max() <= 1e-10 * scale
+        assert np.abs(g["moment"] - r["moment"]).max() <= 1e-10 * Kr * theta0
         assert np.abs(g["Q"] - r["Q"]).max() <= 1e-8
-        assert np.abs(g["vrot"] - r["vrot"]).max() <= 1e-8 * max(np.abs(r["vrot"]).max(), 1e-300)
+        assert np.abs(g["vrot"] - r["vrot"]).max() <= 1e-8 * theta0 * w
         differing += int(not np.array_equal(g["Q"], r["Q"]))
     print("checkpoints with orientations differing in some bit: %d of 8" % differing)

@@ -457,3 +457,69 @@ def test_moment_control_spins_up_at_M_over_I(oracle_lib, tmp_path, kind):
 
 def test_moment_control_catches_unrotated_moment(oracle_lib, tmp_path):
     assert moment_control_error(oracle_lib, tmp_path, "xyz", mutant=15) > 1e-3
+
+
+# --- StickedLinks: the contact branch and the breakage of a compressed bond --------
+def sticked_contact_cap_ok(rk, tmp_path, mutant=0):
+    """The Default-cap deck under StickedLinks (no interface: every entry is a
+    contact): that branch caps the friction at mu |fn| with fn the total
+    normal force, damping included (ForceLaw_StickedLinks.cpp:185-197), so
+    |ft| saturates at mu (fne + fnv) = 2 mu fne here -- twice Default's."""
+    path, _, steps = default_cap_deck(str(tmp_path))
+    from rockable_amd import decks as Dk
+    path = Dk.conf_with(path, os.path.join(str(tmp_path), "sticked.conf"), forceLaw="StickedLinks")
+    kn, mu = 1.0e5, 0.4
+
+    def run():
+        o = rk.Oracle(path)
+        o.update_nl()
+        o.run_steps(2 * steps)
+        return o.interactions(active=True)
+    I = _with_mutant(rk, mutant, run)
+    if len(I) != 1:
+        return False
+    fne = -kn * I["dn"][0]
+    fn = I["fn"][0]
+    return abs(fn / fne - 2.0) < 1e-6 and abs(np.linalg.norm(I["ft"][0]) / (mu * fn) - 1.0) < 1e-12
+
+
+def test_sticked_contact_cap_on_total_force(oracle_lib, tmp_path):
+    assert sticked_contact_cap_ok(oracle_lib, tmp_path)
+
+
+def test_sticked_contact_cap_catches_elastic_threshold(oracle_lib, tmp_path):
+    assert not sticked_contact_cap_ok(oracle_lib, tmp_path, mutant=16)
+
+
+def compressed_bond_break_ok(rk, tmp_path, mutant=0):
+    """A bond pre-compressed by 1e-5 (dn = -1e-5, dn0 = 0, so fn = kn 1e-5)
+    and sheared by the spin of body j breaks on its tangential term at the
+    step the closed form gives; the broken interface's contact keeps fn and
+    has its friction cut to mu |fn| (Rockable.cpp:3366-3380: dn < 0), not
+    zeroed as a bond in tension is."""
+    R, w, dt, kn, kt, mu = 0.5, 0.4, 1e-6, 1e5, 2e4, 0.3
+    path = K.spun_bond(str(tmp_path), R=R, w=w, kn=kn, kt=kt, fn0=10.0, ft0=1.0, mom0=1.0, power=2.0, damp=0.0,
+                       dt=dt, overlap=1e-5)
+
+    def run():
+        o = rk.Oracle(path)
+        o.update_nl()
+        for step in range(1, 2000):
+            o.run_steps(1)
+            if o.ninterfaces == 0:
+                return step, o.interactions()
+        return None, None
+    step, I = _with_mutant(rk, mutant, run)
+    if step is None or len(I) != 1:
+        return False
+    fn = I["fn"][0]
+    return (I["stick"][0] == -1 and abs(fn / (kn * 1e-5) - 1.0) < 1e-6
+            and abs(np.linalg.norm(I["ft"][0]) / (mu * fn) - 1.0) < 1e-12)
+
+
+def test_compressed_bond_breaks_to_a_capped_contact(oracle_lib, tmp_path):
+    assert compressed_bond_break_ok(oracle_lib, tmp_path)
+
+
+def test_compressed_bond_pin_catches_zeroed_contact(oracle_lib, tmp_path):
+    assert not compressed_bond_break_ok(oracle_lib, tmp_path, mutant=17)
