@@ -190,6 +190,16 @@ int rk_synchronize(rk_engine* eng);
 
 int rk_get_counts(rk_engine* eng, rk_counts* out);
 int rk_get_stats(rk_engine* eng, rk_stats* out);
+/* rk_get_stats without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * reduction's partial sums live in a fixed scratch array of 2^20 doubles
+ * (8 MiB) per device, held until the process ends; a packing whose
+ * statistics need more (about 4 per 256 bodies plus 5 per 256 listed
+ * interactions, over 40 million listed interactions) is rejected with an
+ * error.  The array is shared by every engine on the device: calls that
+ * overlap on one device race.  Results equal rk_get_stats' bit for bit. */
+int rk_get_stats_noalloc(rk_engine* eng, rk_stats* out);
 /* Any pointer may be NULL; arrays hold 3 doubles per body (Q: w x y z). */
 int rk_get_bodies(rk_engine* eng, double* pos, double* Q, double* vel, double* vrot, double* acc, double* arot,
                   double* force, double* moment, size_t nbodies);

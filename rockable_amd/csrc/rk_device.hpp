@@ -322,6 +322,11 @@ void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s);
 // min dn over contacts)
 int stat_parts(const Dev& d);
 void launch_stats(const Dev& d, double* part, double* out, hipStream_t s);
+// The statistics' fixed scratch (no allocation): its capacity in doubles
+// (partials and the 9 outputs), and its device address on the current
+// device, looked up once per device (0, or -1 with err set).
+constexpr int kStatScratchDoubles = 1 << 20;
+int stat_scratch(double** ptr, char* err, size_t errlen);
 void launch_guard(const Dev& d, hipStream_t s);
 // the step index the next guard check ends (the checks count on from it)
 void launch_guard_set_step(const Dev& d, long long step, hipStream_t s);

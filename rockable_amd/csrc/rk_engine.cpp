@@ -2165,6 +2165,40 @@ int rk_get_stats(rk_engine* eng, rk_stats* out) {
   return 0;
 }
 
+int rk_get_stats_noalloc(rk_engine* eng, rk_stats* out) {
+  if (check_loaded(eng) || !out) return -1;
+  HIPCHK(hipSetDevice(eng->device));
+  const long need = (long)stat_parts(eng->d) + 16;
+  if (need > kStatScratchDoubles)
+    return fail(eng, "rk_get_stats_noalloc: the packing needs " + std::to_string(need) +
+                         " doubles of scratch, more than the fixed " + std::to_string(kStatScratchDoubles) +
+                         " (use rk_get_stats)");
+  double* base = nullptr;
+  char msg[256];
+  if (stat_scratch(&base, msg, sizeof msg) != 0) return fail(eng, msg);
+  double* part = base;
+  double* res = base + (kStatScratchDoubles - 16);
+  launch_stats(eng->d, part, res, eng->stream);
+  HIPCHK(hipGetLastError());
+  double v[9];
+  HIPCHK(hipMemcpyAsync(v, res, sizeof v, hipMemcpyDeviceToHost, eng->stream));
+  HIPCHK(hipStreamSynchronize(eng->stream));
+  out->kinetic_translational = v[0];
+  out->kinetic_rotational = v[1];
+  out->vmax = v[2];
+  out->nfree = (int64_t)v[3];
+  out->nactive = (int64_t)v[4];
+  out->ncontacts = (int64_t)v[5];
+  out->active_pairs = (int64_t)v[6];
+  out->coordination = v[3] > 0.0 ? 2.0 * v[6] / v[3] : 0.0;
+  out->dn_mean = v[5] > 0.0 ? v[7] / v[5] : 0.0;
+  out->dn_min = v[5] > 0.0 ? v[8] : 0.0;
+  out->ninterfaces = eng->live_interfaces;
+  out->broken_interfaces = eng->loaded_interfaces - eng->live_interfaces;
+  out->total_broken_area = eng->m.totalBrokenArea;
+  return 0;
+}
+
 int rk_get_counts(rk_engine* eng, rk_counts* out) {
   if (check_loaded(eng) || !out) return -1;
   const EntryArrays& E = eng->E[eng->cur];

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <mutex>
 
 #include "rk_device.hpp"
 #include "rk_model.hpp"
@@ -1951,6 +1953,37 @@ void launch_stats(const Dev& d, double* part, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_stat_bodies, dim3(nbp), dim3(kBlock), 0, s, d, part);
   hipLaunchKernelGGL(k_stat_entries, dim3(nep), dim3(kBlock), 0, s, d, pe);
   hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(64), 0, s, part, nbp, pe, nep, out);
+}
+
+// Scratch of rk_get_stats_noalloc: partial sums and outputs of the
+// statistics kernels in a fixed array of each device (8 MiB, held until the
+// process ends), so that the call allocates nothing.  Host code never uses
+// the array's name as an address: stat_scratch() looks the device address up
+// once per device (hipGetSymbolAddress, with that device current) and the
+// kernels receive that address.  Every call on one device shares the array;
+// calls that overlap on one device race.
+static __device__ __align__(256) double scratch[kStatScratchDoubles];
+
+int stat_scratch(double** ptr, char* err, size_t errlen) {
+  static std::mutex mu;
+  static double* addr[64] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    snprintf(err, errlen, "stat_scratch: no current device");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      snprintf(err, errlen, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+      return -1;
+    }
+    addr[dev] = static_cast<double*>(p);
+  }
+  *ptr = addr[dev];
+  return 0;
 }
 
 __global__ void k_guard_set_step(Dev d, long long step) {

@@ -52,7 +52,7 @@ EXPORTED = ["rk_inspect_conf", "rk_engine_create", "rk_engine_destroy", "rk_last
             "rk_set_bodies", "rk_get_interactions", "rk_get_kernel_time", "rk_reset_kernel_time",
             "rk_mlp_create", "rk_mlp_destroy", "rk_mlp_last_error", "rk_mlp_nparams", "rk_mlp_get_params",
             "rk_mlp_set_params", "rk_mlp_forward", "rk_mlp_loss_grad", "rk_mlp_train", "rk_set_rebuild_model",
-            "rk_guard_features", "rk_get_option", "rk_autotune", "rk_get_stats", "rk_load_batch",
+            "rk_guard_features", "rk_get_option", "rk_autotune", "rk_get_stats", "rk_get_stats_noalloc", "rk_load_batch",
             "rk_batch_packing"]
 
 _lib = None
@@ -108,6 +108,7 @@ def load_library(path=LIB_PATH):
     L.rk_get_option.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
     L.rk_autotune.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     L.rk_get_stats.argtypes = [P, ctypes.POINTER(RkStats)]
+    L.rk_get_stats_noalloc.argtypes = [P, ctypes.POINTER(RkStats)]
     L.rk_load_batch.argtypes = [P, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]
     L.rk_batch_packing.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     _lib = L
