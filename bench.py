@@ -104,6 +104,18 @@ def make_settled_deck(n_bodies, seed, dirpath, settle_steps, relax_steps, log):
     return path, int((~inside).sum())
 
 
+# options of every timed engine: no geometry records (the step writes the
+# contact geometry only when an output is due), and the far-entry skip when
+# asked for (--skip-far; otherwise the engine's default)
+SKIP_FAR = None
+
+
+def timed_options(e):
+    e.set_option("record_geometry", 0)
+    if SKIP_FAR is not None:
+        e.set_option("skip_far", SKIP_FAR)
+
+
 def interverlet_steps(e):
     """Steps between two scheduled rebuilds of the engine's deck
     (interVerlet / dt, the loop's own test at Rockable.cpp:2693)."""
@@ -273,7 +285,7 @@ def batched_form(a, log):
     log("batched deck: %d packings, %d bodies, written in %.1f s" % (len(info), sum(p["bodies"] for p in info),
                                                                      time.perf_counter() - t0))
     e = Engine(path, device=0)
-    e.set_option("record_geometry", 0)
+    timed_options(e)
     e.update_nl()
     ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
     log("autotune: %d steps, %s" % (ntune, choice))
@@ -295,7 +307,7 @@ def batched_form(a, log):
         s2, d2 = D.single_packing_of(shapes, deck, info, p)
         sp = D.write_deck(os.path.join(d, "p%d" % p), s2, d2)
         q = Engine(sp, device=0)
-        q.set_option("record_geometry", 0)
+        timed_options(q)
         q.update_nl()
         q.step(100)
         q.synchronize()
@@ -336,7 +348,7 @@ def bonds_form(a, log):
     path = D.write_deck(d, shapes, deck)
     log("bonded-cluster deck: %d clusters, written in %.1f s" % (a.bond_clusters, time.perf_counter() - t0))
     e = Engine(path, device=0)
-    e.set_option("record_geometry", 0)
+    timed_options(e)
     e.update_nl()
     ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
     log("autotune: %d steps, %s" % (ntune, choice))
@@ -395,7 +407,7 @@ def avalanche_forms(a, log, settled_path):
     mlp = Mlp(device=0)
     mlp.set_params(np.load(os.path.join(ROOT, "rockable_amd", "data", "rebuild_mlp.npy")))
     e = Engine(path, device=0)
-    e.set_option("record_geometry", 0)
+    timed_options(e)
     e.update_nl()
     ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
     e.step(a.avalanche_warmup)
@@ -407,7 +419,7 @@ def avalanche_forms(a, log, settled_path):
     forms, finals = {}, {}
     for name in ("plain", "fixed_batches", "batched_learned"):
         e = Engine(start_conf, device=0)
-        e.set_option("record_geometry", 0)
+        timed_options(e)
         for k, v in choice.items():
             e.set_option(k, v)
         if name == "plain":
@@ -497,6 +509,8 @@ def main():
                     help="only print the device copy rate of MIB MiB buffers (run as a child process)")
     ap.add_argument("--no-copy-rate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--skip-far", type=int, default=None, choices=(0, 1),
+                    help="the engine's skip_far option on every timed engine (default: the engine's default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (calibrates the HBM byte counters)")
     a = ap.parse_args()
@@ -519,6 +533,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
         return
+    global SKIP_FAR
+    SKIP_FAR = a.skip_far
     if a.gpus != 1:
         print("bench.py: the engine runs on one GPU; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
@@ -544,7 +560,7 @@ def main():
         return
 
     e = Engine(path, device=0)
-    e.set_option("record_geometry", 0)
+    timed_options(e)
     e.update_nl()
     t0 = time.perf_counter()
     tuned, choice = (0, None) if a.no_autotune else e.autotune(steps_per_trial=a.tune_trial)
@@ -590,6 +606,7 @@ def main():
     log("timed: %d steps (%d rebuilds), %.4f ms/step with force-phase events, %.4f ms/step without" % (
         K, rebuilds, 1e3 * wall / K, 1e3 * wall_noev / K))
 
+    skip_far = int(e.get_option("skip_far"))
     # roofline of the force phase (interact + gather), per launch
     n_force, ms_force = kt["force"]
     nl_listed = 0.5 * (c0["nlisted"] + c1["nlisted"])
@@ -614,6 +631,7 @@ def main():
                    "kernel_ms_per_launch": {k: (v[1] / max(v[0], 1)) for k, v in kt2.items() if v[0]},
                    "ms_per_step_without_events": 1e3 * wall_noev / K,
                    "autotuned": choice,
+                   "skip_far": skip_far,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,

@@ -149,6 +149,16 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * the default: every kernel launched on its own),
  * "model_scale" (the rebuild-interval network's proposals are multiplied by
  * it; default 0.75),
+ * "skip_far" (1, the default: in steps that write no geometry of inactive
+ * entries, record_geometry 0, an inactive entry with zero history whose gap
+ * is proven positive by the bodies' accumulated motion bounds is not
+ * evaluated; its gap is recomputed when the list is read out or rebuilt;
+ * 0: every listed entry every step), "skip_segment" (entries a workgroup
+ * tests and compacts before evaluating them under skip_far: 512, 1024, 2048;
+ * 256: no compaction; default 1024), "count_skips" (1: the geometry pass
+ * counts entries skipped, inactive with zero history, and looked at, read
+ * back with rk_get_option as "skipped_entries", "zero_history_entries" and
+ * "entries_looked_at"; setting it again clears them; 0 stops counting),
  * launch shapes: "fuse_law" (1, the default: contact kinematics and force law
  * in one kernel; 0: two), "type_sort" (1, the default: the fused kernel takes
  * a block's entries by contact type; 0: in list order), "law_group" (geometry blocks per force-law
@@ -174,7 +184,7 @@ int rk_set_option(rk_engine* eng, const char* name, double value);
 int rk_get_option(rk_engine* eng, const char* name, double* value);
 /* Autotuner (no counterpart in the reference) of fuse_law, law_group,
  * geom_block, gather_lanes, gather_groups, fuse_glob, gather_waves,
- * body_block, persistent, graph_steps and type_sort, one after the other: the
+ * body_block, persistent, graph_steps, type_sort and skip_segment, one after the other: the
  * simulation runs on, trial after trial of steps_per_trial steps, each
  * candidate value twice, and the fastest (wall time per step, rebuilds
  * excluded) is kept.  Every candidate computes the same state, so the steps

@@ -369,6 +369,7 @@ typedef struct {
   v3 ft, mom;
   double damp;
   int stick; /* index of the BreakableInterface, -1 when none */
+  int act;   /* in activeInteractions of the last accelerations() */
 } inter_t;
 
 typedef struct {
@@ -1260,7 +1261,8 @@ static void update_interactions(rko_sim* s) { /* Rockable.cpp:2968-2974 */
   }
 }
 
-static void build_active(rko_sim* s) { /* Rockable.cpp:3005-3029 */
+/* activeInteractions from the entries' flags, in list order */
+static void active_from_flags(rko_sim* s) {
   size_t tot = 0;
   for (int k = 0; k < s->nb; k++) tot += (size_t)s->inter[k].n;
   if (tot > s->cap_active) {
@@ -1271,9 +1273,24 @@ static void build_active(rko_sim* s) { /* Rockable.cpp:3005-3029 */
   for (int k = 0; k < s->nb; k++) {
     ilist_t* L = &s->inter[k];
     for (int q = 0; q < L->n; q++)
-      if (L->a[q].dn < 0.0 || L->a[q].stick >= 0) s->active[na++] = &L->a[q];
+      if (L->a[q].act) s->active[na++] = &L->a[q];
   }
   s->nactive = na;
+}
+
+/* build_activeInteractions (Rockable.cpp:3005-3029): the listed entries with
+   dn < 0 or a bond.  The reference keeps the vector's pointers until the next
+   call, through UpdateNL, which keeps the interactions it does not erase in
+   place (std::set nodes): the active set, a bond broken in this step
+   included, holds until the next accelerations().  Each entry carries the
+   membership as a flag, which the list updates move with it; the outputs
+   (saveConf, the statistics, dnStat) read the set back from the flags. */
+static void build_active(rko_sim* s) {
+  for (int k = 0; k < s->nb; k++) {
+    ilist_t* L = &s->inter[k];
+    for (int q = 0; q < L->n; q++) L->a[q].act = (L->a[q].dn < 0.0 || L->a[q].stick >= 0);
+  }
+  active_from_flags(s);
 }
 
 static void compute_forces(rko_sim* s) { /* Rockable.cpp:3035-3046 */
@@ -2409,7 +2426,7 @@ static void fput_v3e(FILE* f, int prec, v3 v) {
 
 /* saveConf (Rockable.cpp:440-610) */
 int rko_save_conf(rko_sim* s, const char* path) {
-  build_active(s); /* the listed entries that were active at the last accelerations() */
+  active_from_flags(s); /* the listed entries that were active at the last accelerations() */
   FILE* f = fopen(path, "w");
   if (!f) return seterr(s, "cannot write %s", path);
   fprintf(f, "Rockable 21-08-2022\n");
@@ -2567,6 +2584,7 @@ static void kinetic_energy(const rko_sim* s, double* Et, double* Er) {
    getInteractionQuickStats (4217-4252), on the active interactions in their
    serial order */
 static void quick_stats(rko_sim* s, double* Fmax, double* F_fnmax, double* fnMax_out, double* fnMean_out) {
+  active_from_flags(s);
   double* fnMax = (double*)calloc((size_t)s->nb + 1, sizeof(double));
   size_t* nbCtc = (size_t*)calloc((size_t)s->nb + 1, sizeof(size_t));
   for (size_t k = 0; k < s->nactive; k++) {
@@ -2641,6 +2659,7 @@ static void extractors_record(rko_sim* s, int64_t step) {
     if (s->extr[q].nrec <= 0 || step % s->extr[q].nrec != 0) continue;
     FILE* f = s->extr[q].f;
     if (s->extr[q].kind == 1) {
+      active_from_flags(s);
       double dnMin = 1e20, dnMax = -1e20, dnMeanNeg = 0.0, dnMeanPos = 0.0, nbNeg = 0.0, nbPos = 0.0, dnMean = 0.0;
       v3 posMin = V(0.0, 0.0, 0.0);
       for (size_t k = 0; k < s->nactive; k++) {
@@ -2850,7 +2869,7 @@ size_t rko_nlisted(rko_sim* s) {
   return n;
 }
 size_t rko_nactive(rko_sim* s) {
-  build_active(s);
+  active_from_flags(s);
   return s->nactive;
 }
 size_t rko_ninterfaces(rko_sim* s) {
@@ -2948,7 +2967,7 @@ size_t rko_get_interactions(rko_sim* s, int active, rko_interaction* out, size_t
     ilist_t* L = &s->inter[k];
     for (int q = 0; q < L->n; q++) {
       const inter_t* I = &L->a[q];
-      if (active && !(I->dn < 0.0 || I->stick >= 0)) continue;
+      if (active && !I->act) continue;
       if (out && n < cap) to_record(I, &out[n]);
       n++;
     }

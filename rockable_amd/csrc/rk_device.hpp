@@ -178,6 +178,10 @@ struct Dev {
   double* cbound;
   const double* rmax;
   int skip_far;           // 1: skip such entries (only when the step writes no geometry of inactive entries)
+  int skip_seg;           // entries a workgroup tests and compacts before evaluating (512, 1024, 2048; else none)
+  // with the count_skips option: [0] entries skipped, [1] inactive entries
+  // with zero history, [2] entries looked at, summed over the steps
+  unsigned long long* skip_count;
   const int32_t* istart;
   const int32_t* jstart;
   const int32_t* jlist;
@@ -218,6 +222,8 @@ struct Dev {
   double* gdq;            // 4 per body
   unsigned long long* gpart;  // per gather workgroup: max |dx|^2 key, min w key
   double disp, wstar;
+  int disp_check;         // the guard runs the displacement check (dynamicUpdateNL); without it, it
+                          // only turns a bond breakage (flags[0]) into a rebuild due after the step
   // active entries of the current force phase: geometry block b's in
   // akin[b * geom_block, b * geom_block + acount[b])
   ActiveKin* akin;

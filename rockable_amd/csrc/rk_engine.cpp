@@ -68,7 +68,9 @@ struct rk_engine {
   double *acc = nullptr, *arot = nullptr, *force = nullptr, *moment = nullptr;
   double *mass = nullptr, *inertia = nullptr, *bf_surface = nullptr;
   double *cbound = nullptr, *rmax = nullptr;  // far-entry skip: per-body motion bounds
-  int skip_far = 0;
+  int skip_far = 1;
+  int skip_seg = 1024;  // entries tested and compacted per workgroup under the skip (256: no compaction)
+  unsigned long long* skip_count = nullptr;  // count_skips: the geometry pass's counts
   int32_t* cluster = nullptr;
   int32_t *sh_nv = nullptr, *sh_ne = nullptr, *sh_nf = nullptr, *sh_voff = nullptr, *sh_eoff = nullptr,
           *sh_foff = nullptr, *sh_edge = nullptr, *sh_fstart = nullptr, *sh_fidx = nullptr;
@@ -92,7 +94,8 @@ struct rk_engine {
   // global vertices: of no body, of the listed bodies (those the force phase
   // reads), or of every body (the neighbour list reads them) current
   enum { GV_STALE, GV_LISTED, GV_ALL } gv_state = GV_STALE;
-  // device displacement guard (dynamicUpdateNL)
+  // device guard: the displacement check (dynamicUpdateNL) and the freeze
+  // of the steps after a bond breakage (StickedLinks with interfaces)
   DevGuard* guard = nullptr;
   double *gdpos = nullptr, *gdq = nullptr;
   unsigned long long* gpart = nullptr;  // the gather workgroups' guard maxima
@@ -281,7 +284,7 @@ std::vector<double> initial_bounds(const Model& m) {
 // Room in gpart for the gather's workgroups under any launch shape (their
 // count grows with the heavy bodies' chunks, so after each list change).
 int ensure_gpart(rk_engine* eng) {
-  if (!eng->guard) return 0;
+  if (!eng->guard || eng->m.dynamicUpdateNL == 0) return 0;
   const int need = gather_blocks_max((int)eng->m.bodies.size(), eng->E[eng->cur].nhch);
   if (need <= eng->gpart_cap) return 0;
   if (eng->gpart) (void)hipFree(eng->gpart);
@@ -349,6 +352,8 @@ void assemble_dev(rk_engine* eng) {
   d.cbound = eng->cbound;
   d.rmax = eng->rmax;
   d.skip_far = eng->skip_far;
+  d.skip_count = eng->skip_count;
+  d.skip_seg = eng->skip_seg;
   d.spos = E.spos;
   d.soff = E.soff;
   d.sact = E.sact;
@@ -370,6 +375,7 @@ void assemble_dev(rk_engine* eng) {
   d.gpart = eng->gpart;
   d.disp = eng->m.dispUpdateNL;
   d.wstar = eng->wstar;
+  d.disp_check = eng->m.dynamicUpdateNL != 0 ? 1 : 0;
   d.acount = E.acount;
   d.exact_sums = eng->exact_sums;
   d.law_group = eng->law_group;
@@ -617,8 +623,10 @@ int upload_model(rk_engine* eng) {
     eng->nftot = bfoff[nb];
     if (upload(eng, eng->beoff, beoff) || upload(eng, eng->bfoff, bfoff)) return -1;
   }
-  // device displacement guard
-  if (m.dynamicUpdateNL != 0) {
+  // device guard: with dynamicUpdateNL, or with bonds that may break (the
+  // steps after a breakage wait for the rebuild on the device, so bonded
+  // runs need no host read per step)
+  if (m.dynamicUpdateNL != 0 || (m.law == LAW_STICKED && !m.ifc.empty())) {
     std::vector<double> q(4 * (size_t)std::max(1, nb), 0.0);
     for (int b = 0; b < nb; b++) q[4 * (size_t)b] = 1.0;
     if (upload(eng, eng->gdpos, std::vector<double>(3 * (size_t)std::max(1, nb), 0.0)) || upload(eng, eng->gdq, q))
@@ -828,8 +836,31 @@ void ensure_gv(rk_engine* eng, bool all) {
   eng->gv_state = all ? rk_engine::GV_ALL : rk_engine::GV_LISTED;
 }
 
+// Ordered accounting of the interfaces broken by the last executed step
+// (breakage_of_interfaces, Rockable.cpp:3391-3410): the broken areas add to
+// totalBrokenArea in interface order, the host's alive flags and count follow
+// the device's, the breakage marks and flag are cleared, a rebuild is due.
+int account_breakage(rk_engine* eng) {
+  std::vector<int32_t> alive(eng->m.ifc.size());
+  HIPCHK(hipMemcpy(alive.data(), eng->ifc_alive, 4 * alive.size(), hipMemcpyDeviceToHost));
+  eng->live_interfaces = 0;
+  for (size_t k = 0; k < alive.size(); k++) {
+    if (eng->m.ifc[k].alive && !alive[k]) eng->m.totalBrokenArea += eng->m.ifc[k].area;
+    eng->m.ifc[k].alive = alive[k];
+    eng->live_interfaces += alive[k];
+  }
+  std::vector<uint8_t> zero(eng->m.ifc.size(), 0);
+  HIPCHK(hipMemcpy(eng->ifc_brk, zero.data(), zero.size(), hipMemcpyHostToDevice));
+  int32_t z = 0;
+  HIPCHK(hipMemcpy(eng->flags, &z, sizeof(z), hipMemcpyHostToDevice));
+  eng->needUpdate = true;
+  return 0;
+}
+
 // Rockable::accelerations (Rockable.cpp:3562-3719); kick fuses the closing
-// half kick of velocityVerletStep.
+// half kick of velocityVerletStep.  guard: a step of the guarded loop (the
+// guard kernel follows; it turns a breakage into a freeze that the host
+// reads with the guard); other calls account a breakage at once.
 int run_accelerations(rk_engine* eng, int kick, int write_geom, bool guard = false) {
   const StepParams sp = step_params(eng);
   ensure_gv(eng, false);
@@ -838,31 +869,18 @@ int run_accelerations(rk_engine* eng, int kick, int write_geom, bool guard = fal
   launch_interact(eng->d, sp, write_geom || eng->record_geometry, eng->stream);
   toc(eng, "interact", 2);
   tic(eng, 2);
-  launch_gather(eng->d, sp, kick, guard ? 1 : 0, eng->stream);
+  launch_gather(eng->d, sp, kick, guard && eng->d.disp_check ? 1 : 0, eng->stream);
   toc(eng, "gather", 2);
   toc(eng, "force", 1);
   if (eng->m.law == LAW_STICKED && eng->live_interfaces > 0) {
     tic(eng, 2);
     launch_breakage(eng->d, sp, eng->stream);
     toc(eng, "breakage", 2);
-    int32_t fl = 0;
-    HIPCHK(hipMemcpyAsync(&fl, eng->flags, sizeof(int32_t), hipMemcpyDeviceToHost, eng->stream));
-    HIPCHK(hipStreamSynchronize(eng->stream));
-    if (fl) {
-      // ordered accounting of the broken interfaces (Rockable.cpp:3391-3410)
-      std::vector<int32_t> alive(eng->m.ifc.size());
-      HIPCHK(hipMemcpy(alive.data(), eng->ifc_alive, 4 * alive.size(), hipMemcpyDeviceToHost));
-      eng->live_interfaces = 0;
-      for (size_t k = 0; k < alive.size(); k++) {
-        if (eng->m.ifc[k].alive && !alive[k]) eng->m.totalBrokenArea += eng->m.ifc[k].area;
-        eng->m.ifc[k].alive = alive[k];
-        eng->live_interfaces += alive[k];
-      }
-      std::vector<uint8_t> zero(eng->m.ifc.size(), 0);
-      HIPCHK(hipMemcpy(eng->ifc_brk, zero.data(), zero.size(), hipMemcpyHostToDevice));
-      int32_t z = 0;
-      HIPCHK(hipMemcpy(eng->flags, &z, sizeof(z), hipMemcpyHostToDevice));
-      eng->needUpdate = true;
+    if (!guard) {
+      int32_t fl = 0;
+      HIPCHK(hipMemcpyAsync(&fl, eng->flags, sizeof(int32_t), hipMemcpyDeviceToHost, eng->stream));
+      HIPCHK(hipStreamSynchronize(eng->stream));
+      if (fl && account_breakage(eng) != 0) return -1;
     }
   }
   HIPCHK(hipGetLastError());
@@ -878,6 +896,9 @@ int do_update_nl(rk_engine* eng) {
     ~NlClock() { e->nl_since_conf_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
   } nl_clock{eng, t0};
   ensure_gv(eng, true);
+  // the entries the last step skipped get their gaps before the merge
+  // copies them into the new list
+  launch_refresh_gaps(eng->d, eng->stream);
   tic(eng, eng->profile);
   const int nxt = 1 - eng->cur;
   if (nl_rebuild(eng->w, eng->d, eng->nlp, eng->E[eng->cur], eng->E[nxt], eng->stream) != 0)
@@ -897,11 +918,13 @@ int do_update_nl(rk_engine* eng) {
   return 0;
 }
 
-// guard: the step ends with the displacement check (dynamicUpdateNL), its
-// accumulation fused into the resultant kernels
+// guard: the step ends with the guard kernel (the displacement check of
+// dynamicUpdateNL, its accumulation fused into the resultant kernels, and
+// the freeze after a bond breakage)
 int vv_step(rk_engine* eng, int write_geom, bool guard) {
   const StepParams sp = step_params(eng);
-  if (guard && (!eng->guard || eng->gpart_cap < gather_blocks_max(eng->d.nb, eng->d.nhch)))
+  if (guard && !eng->guard) return fail(eng, "device guard: not allocated");
+  if (guard && eng->d.disp_check && eng->gpart_cap < gather_blocks_max(eng->d.nb, eng->d.nhch))
     return fail(eng, "device guard: no room for the gather's maxima");
   tic(eng, 2);
   launch_predict(eng->d, sp, 1, eng->stream);
@@ -919,7 +942,7 @@ int vv_step(rk_engine* eng, int write_geom, bool guard) {
 }
 
 // One step of the loop as the device runs it (write_geom 0): the
-// velocity-Verlet step, and the displacement check with dynamicUpdateNL.
+// velocity-Verlet step, and the guard kernel in the guarded loop.
 int launch_one_step(rk_engine* eng, bool with_guard) {
   return vv_step(eng, 0, with_guard);
 }
@@ -1027,10 +1050,11 @@ int flush_steps(rk_engine* eng) {
 }
 
 // One step of the loop from the host's side: deferred when nothing on the
-// host depends on it (no conf file written at it, no bonds whose breakage
-// the host accounts), launched now otherwise.
+// host depends on it (no conf file written at it, no bonds outside the
+// guarded loop, whose breakage the host accounts after the step), launched
+// now otherwise.
 int enqueue_step(rk_engine* eng, bool write_geom, bool with_guard) {
-  const bool bonds = eng->m.law == LAW_STICKED && eng->live_interfaces > 0;
+  const bool bonds = !with_guard && eng->m.law == LAW_STICKED && eng->live_interfaces > 0;
   if (eng->pending > 0 && eng->pending_guard != with_guard && flush_steps(eng) != 0) return -1;
   if (write_geom || bonds || eng->graph_steps <= 1 || eng->m.servo_timed()) {
     if (flush_steps(eng) != 0) return -1;
@@ -1435,15 +1459,17 @@ int propose_batch(rk_engine* eng) {
   return 0;
 }
 
-// The integrate() loop with dynamicUpdateNL, its displacement check on the
-// device.  Iterations are enqueued in batches without host synchronisation;
-// a batch ends where the host has work (a conf file, a scheduled rebuild,
-// bond breakage accounting), after guard_batch iterations (no network), or
-// once an asynchronous poll has seen the check fire (with a network).  When
-// the device check finds a rebuild due after iteration s, the step kernels of
-// the later iterations do nothing: the host reads the guard at the end of the
-// batch, keeps the loop scalars of iteration s, rebuilds and continues.  The
-// schedule of rebuilds is exactly the reference's.
+// The integrate() loop with the device guard (dynamicUpdateNL, its
+// displacement check on the device, or bonds that may break).  Iterations
+// are enqueued in batches without host synchronisation; a batch ends where
+// the host has work (a conf file, a scheduled rebuild), after guard_batch
+// iterations (no network), or once an asynchronous poll has seen the check
+// fire (with a network).  When the device finds a rebuild due after
+// iteration s (the check fired the iteration before, or a bond broke in s),
+// the step kernels of the later iterations do nothing: the host reads the
+// guard at the end of the batch, keeps the loop scalars of iteration s,
+// accounts the broken interfaces, rebuilds and continues.  The schedule of
+// rebuilds is exactly the reference's.
 int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tmax, int64_t* ndone) {
   Model& m = eng->m;
   int64_t done = 0;
@@ -1457,9 +1483,9 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     // kPollAhead polls in flight, so the host never waits on the GPU's
     // current chunk).  Chunks follow the network's estimate of the steps to
     // the next rebuild: long far from it, short near it.
-    const bool pipe = eng->model != nullptr && !bonds && eng->poll_host != nullptr;
+    const bool pipe = eng->model != nullptr && eng->poll_host != nullptr;
     const int64_t want = pipe ? nmax - done : eng->model ? eng->pred_batch : eng->guard_batch;
-    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(bonds ? 1 : want, nmax - done));
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(want, nmax - done));
     hist.assign(1, loop_scalars(eng));
     conf_at.assign(1, 0);
     launch_guard_set_step(eng->d, (long long)eng->steps, eng->stream);
@@ -1514,7 +1540,9 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     }
     if (flush_steps(eng) != 0) return -1;
     DevGuard g;
+    int32_t broke = 0;
     HIPCHK(hipMemcpyAsync(&g, eng->guard, sizeof g, hipMemcpyDeviceToHost, eng->stream));
+    if (bonds) HIPCHK(hipMemcpyAsync(&broke, eng->flags, sizeof broke, hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
     eng->guard_reads++;
     int64_t executed = k;
@@ -1528,6 +1556,11 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       need = true;
       const int zero = 0;
       HIPCHK(hipMemcpy(&eng->guard->frozen, &zero, sizeof zero, hipMemcpyHostToDevice));
+    }
+    if (broke) {
+      // a breakage freezes the steps after its own
+      if (!g.frozen) return fail(eng, "device guard: a bond broke without a freeze");
+      if (account_breakage(eng) != 0) return -1;
     }
     if (conf_at[executed] && outdir) {
       char fn[4096];
@@ -1617,6 +1650,7 @@ void rk_engine_destroy(rk_engine* eng) {
   (void)hipSetDevice(eng->device);
   if (eng->stream) (void)hipStreamSynchronize(eng->stream);
   free_device(eng);
+  if (eng->skip_count) (void)hipFree(eng->skip_count);
   for (hipEvent_t e : eng->ev_pool) (void)hipEventDestroy(e);
   if (eng->ev0) (void)hipEventDestroy(eng->ev0);
   if (eng->ev1) (void)hipEventDestroy(eng->ev1);
@@ -1847,12 +1881,42 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
     if (eng->loaded) {
       (void)hipSetDevice(eng->device);
       if (flush_steps(eng) != 0) return -1;
+      ensure_gv(eng, false);
+      launch_refresh_gaps(eng->d, eng->stream);  // the gaps the last step skipped
     }
     eng->skip_far = (int)value;
     if (eng->loaded) {
       assemble_dev(eng);
       launch_reset_keys(eng->d, eng->stream);  // keys are kept only while skipping is on
     }
+  } else if (n == "skip_segment") {
+    if (!(value == 256.0 || value == 512.0 || value == 1024.0 || value == 2048.0))
+      return fail(eng, "skip_segment must be 256, 512, 1024 or 2048");
+    if (eng->loaded) {
+      (void)hipSetDevice(eng->device);
+      if (flush_steps(eng) != 0) return -1;
+    }
+    eng->skip_seg = (int)value;
+    if (eng->loaded) assemble_dev(eng);
+  } else if (n == "count_skips") {
+    // 1: the geometry pass counts the entries it skips, the inactive ones with
+    // zero history and all it looks at (three integer atomics per wavefront;
+    // read back as skipped_entries, zero_history_entries, entries_looked_at);
+    // setting it again clears the counts; 0 stops counting
+    if (!(value == 0.0 || value == 1.0)) return fail(eng, "count_skips must be 0 or 1");
+    if (eng->loaded) {
+      (void)hipSetDevice(eng->device);
+      if (flush_steps(eng) != 0) return -1;
+      HIPCHK(hipStreamSynchronize(eng->stream));
+    }
+    if (value == 0.0 && eng->skip_count) {
+      (void)hipFree(eng->skip_count);
+      eng->skip_count = nullptr;
+    } else if (value == 1.0) {
+      if (!eng->skip_count) HIPCHK(hipMalloc(&eng->skip_count, 3 * sizeof(unsigned long long)));
+      HIPCHK(hipMemset(eng->skip_count, 0, 3 * sizeof(unsigned long long)));
+    }
+    if (eng->loaded) assemble_dev(eng);
   } else if (n == "gravity_x" || n == "gravity_y" || n == "gravity_z") {
     // the conf keyword "gravity" (Rockable.cpp:978-980), changed between steps
     if (check_loaded(eng)) return -1;
@@ -1987,6 +2051,7 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
   };
   const Knob knobs[] = {{"fuse_law", {0, 1}},
                         {"type_sort", {1, 0}},
+                        {"skip_segment", {1024, 512, 2048, 256}},
                         {"law_group", {1, 2, 4, 8}},
                         {"geom_block", {64, 128, 256, 512}},
                         {"gather_lanes", {8, 16}},
@@ -2056,6 +2121,19 @@ int rk_get_option(rk_engine* eng, const char* name, double* value) {
     *value = eng->exact_sums;
   else if (n == "skip_far")
     *value = eng->skip_far;
+  else if (n == "count_skips")
+    *value = eng->skip_count != nullptr;
+  else if (n == "skip_segment")
+    *value = eng->skip_seg;
+  else if (n == "skipped_entries" || n == "zero_history_entries" || n == "entries_looked_at") {
+    unsigned long long c[3] = {0ull, 0ull, 0ull};
+    if (eng->skip_count) {
+      if (flush_steps(eng) != 0) return -1;
+      HIPCHK(hipMemcpyAsync(c, eng->skip_count, sizeof c, hipMemcpyDeviceToHost, eng->stream));
+      HIPCHK(hipStreamSynchronize(eng->stream));
+    }
+    *value = (double)c[n == "skipped_entries" ? 0 : n == "zero_history_entries" ? 1 : 2];
+  }
   else if (n == "heavy_min")
     *value = eng->w.heavy_min;
   else if (n == "guard_batch")

@@ -1047,6 +1047,7 @@ struct MergeIn {
   const EGeom* ogeo;
   const double* odamp;
   const int32_t* ostick;
+  const uint8_t* oeact;
 };
 struct MergeOut {
   int32_t* count;   // per group (count pass)
@@ -1186,10 +1187,13 @@ __global__ void k_merge_fill(Dev d, int n, MergeIn in, MergeOut out) {
     out.egeo[e] = in.ogeo[o];
     out.edamp[e] = in.odamp[o];
     out.estick[e] = stick;
-    // bit 1: the history is not all zero (a NaN counts as non-zero)
+    // bit 1: the history is not all zero (a NaN counts as non-zero); bit 0
+    // is kept: the entries active at the last force phase stay the active
+    // set until the next one (activeInteractions holds its pointers through
+    // UpdateNL, Rockable.cpp:3005-3029), a bond broken in that phase included
     const bool hist = !(st.fn == 0.0 && st.ftx == 0.0 && st.fty == 0.0 && st.ftz == 0.0 && st.momx == 0.0 &&
                         st.momy == 0.0 && st.momz == 0.0);
-    out.eact[e] = ((dn < 0.0 || stick >= 0) ? 1 : 0) | (hist ? 2 : 0);
+    out.eact[e] = (in.oeact[o] & 1) | (hist ? 2 : 0);
   }
   out.ekey[e] = -INFINITY;  // every entry of a new list is evaluated at its first step
 }
@@ -1674,6 +1678,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
   in.ogeo = cur.egeo;
   in.odamp = cur.edamp;
   in.ostick = cur.estick;
+  in.oeact = cur.eact;
   MergeOut out;
   out.count = w.gcount;
   out.offset = w.goff;

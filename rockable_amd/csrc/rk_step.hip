@@ -378,6 +378,93 @@ __device__ void law_sticked(const Dev& d, const StepParams& sp, const Kin& k, do
   }
 }
 
+// The geometry pass of one listed entry e (Interaction::UpdateDispatcher,
+// Rockable.cpp:2934-2977): its gap, and for an active entry its kinematics in
+// a (returns true); an inactive entry gets its gap and activity byte here,
+// and with skipping its far-entry key.
+template <bool GEOM_ALL>
+__device__ __forceinline__ bool geom_entry(const Dev& d, int e, bool skipping, ActiveKin& a) {
+  bool act = false;
+  const int i = d.ei[e], j = d.ej[e];
+  const int type = (int)(d.esub[e] >> 30);
+  const BodyKin& Pi = d.kin[i];
+  const BodyKin& Pj = d.kin[j];
+  const int stick = d.ifc.n > 0 ? d.estick[e] : -1;  // bonds exist only with interfaces
+  Kin k;
+  k.deact = false;
+  k.prev_dn = 0.0;  // the previous dn is read by the law, for active entries only
+  if (stick >= 0) {  // a bond may run on stale geometry after a deactivation
+    const EGeom& g = d.egeo[e];
+    k.n = V(g.nx, g.ny, g.nz);
+    k.pos = V(g.px, g.py, g.pz);
+  }
+  update_kinematics<false>(d, type, d.edesc[e], Pi, Pj, k);
+  if (k.deact) k.dn = 0.0;  // Interaction::deactivate (Interaction.cpp:123-128)
+  act = (k.dn < 0.0) || (stick >= 0);
+  if (act) {
+    a.nx = k.n.x;
+    a.ny = k.n.y;
+    a.nz = k.n.z;
+    a.px = k.pos.x;
+    a.py = k.pos.y;
+    a.pz = k.pos.z;
+    a.dn = k.dn;
+    a.prev_dn = 0.0;  // (unused: the law reads the previous dn itself)
+    a.e = e;
+    a.deact = k.deact ? 1 : 0;
+    a.i = i;
+    a.j = j;
+  } else {
+    // activity byte: bit 0 active, bit 1 history maybe non-zero; the
+    // history and the slot flags are written only when they change
+    const uint8_t old = d.eact[e];
+    uint8_t now = old & 2;
+    if (k.deact) {
+      if (old & 2) {
+        EState o;
+        o.fn = 0.0;
+        o.ftx = o.fty = o.ftz = 0.0;
+        o.momx = o.momy = o.momz = 0.0;
+        d.est[e] = o;
+        now = 0;
+      }
+      d.edn[e] = 0.0;
+      if (skipping) d.ekey[e] = -INFINITY;
+    } else {
+      d.edn[e] = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
+      if (skipping && now == 0) {
+        // the key of a zero-history entry (a body without a finite bound
+        // never lets its entries be skipped)
+        const double ci = d.cbound[i], cj = d.cbound[j];
+        d.ekey[e] = (ci < INFINITY && cj < INFINITY) ? k.dn + ci + cj : -INFINITY;
+      }
+      if (GEOM_ALL) {
+        const v3 vel = rel_vel(k.pos, Pi, Pj);
+        EGeom g;
+        g.nx = k.n.x;
+        g.ny = k.n.y;
+        g.nz = k.n.z;
+        g.px = k.pos.x;
+        g.py = k.pos.y;
+        g.pz = k.pos.z;
+        g.vx = vel.x;
+        g.vy = vel.y;
+        g.vz = vel.z;
+        d.egeo[e] = g;
+      }
+    }
+    if (now != old) {
+      d.eact[e] = now;
+      if (old & 1) {
+        const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
+        d.sact[slots.x] = 0;
+        d.sact[slots.y] = 0;
+      }
+    }
+  }
+  return act;
+}
+
 // The force phase: geometry, then law, as two kernels or fused in one.
 //  geometry  every listed entry: contact geometry from the body poses
 //            (Interaction::UpdateDispatcher without the relative velocity).
@@ -432,85 +519,16 @@ __device__ __forceinline__ bool geom_pass(const Dev& d, int blk, int (*wtype)[5]
   bool skip = false;
   if (skipping && e < d.ne && d.eact[e] == 0)
     skip = d.ekey[e] - d.cbound[d.ei[e]] - d.cbound[d.ej[e]] > kSkipMargin;
-  if (e < d.ne && !skip) {
-    const int i = d.ei[e], j = d.ej[e];
-    const int type = (int)(d.esub[e] >> 30);
-    const BodyKin& Pi = d.kin[i];
-    const BodyKin& Pj = d.kin[j];
-    const int stick = d.ifc.n > 0 ? d.estick[e] : -1;  // bonds exist only with interfaces
-    Kin k;
-    k.deact = false;
-    k.prev_dn = 0.0;  // the previous dn is read by the law, for active entries only
-    if (stick >= 0) {  // a bond may run on stale geometry after a deactivation
-      const EGeom& g = d.egeo[e];
-      k.n = V(g.nx, g.ny, g.nz);
-      k.pos = V(g.px, g.py, g.pz);
-    }
-    update_kinematics<false>(d, type, d.edesc[e], Pi, Pj, k);
-    if (k.deact) k.dn = 0.0;  // Interaction::deactivate (Interaction.cpp:123-128)
-    act = (k.dn < 0.0) || (stick >= 0);
-    if (act) {
-      a.nx = k.n.x;
-      a.ny = k.n.y;
-      a.nz = k.n.z;
-      a.px = k.pos.x;
-      a.py = k.pos.y;
-      a.pz = k.pos.z;
-      a.dn = k.dn;
-      a.prev_dn = 0.0;  // (unused: the law reads the previous dn itself)
-      a.e = e;
-      a.deact = k.deact ? 1 : 0;
-      a.i = i;
-      a.j = j;
-    } else {
-      // activity byte: bit 0 active, bit 1 history maybe non-zero; the
-      // history and the slot flags are written only when they change
-      const uint8_t old = d.eact[e];
-      uint8_t now = old & 2;
-      if (k.deact) {
-        if (old & 2) {
-          EState o;
-          o.fn = 0.0;
-          o.ftx = o.fty = o.ftz = 0.0;
-          o.momx = o.momy = o.momz = 0.0;
-          d.est[e] = o;
-          now = 0;
-        }
-        d.edn[e] = 0.0;
-        if (skipping) d.ekey[e] = -INFINITY;
-      } else {
-        d.edn[e] = k.dn;  // stale fn/ft/mom are kept (Rockable.cpp:3022)
-        if (skipping && now == 0) {
-          // the key of a zero-history entry (a body without a finite bound
-          // never lets its entries be skipped)
-          const double ci = d.cbound[i], cj = d.cbound[j];
-          d.ekey[e] = (ci < INFINITY && cj < INFINITY) ? k.dn + ci + cj : -INFINITY;
-        }
-        if (GEOM_ALL) {
-          const v3 vel = rel_vel(k.pos, Pi, Pj);
-          EGeom g;
-          g.nx = k.n.x;
-          g.ny = k.n.y;
-          g.nz = k.n.z;
-          g.px = k.pos.x;
-          g.py = k.pos.y;
-          g.pz = k.pos.z;
-          g.vx = vel.x;
-          g.vy = vel.y;
-          g.vz = vel.z;
-          d.egeo[e] = g;
-        }
-      }
-      if (now != old) {
-        d.eact[e] = now;
-        if (old & 1) {
-          const int2 slots = reinterpret_cast<const int2*>(d.spos)[e];
-          d.sact[slots.x] = 0;
-          d.sact[slots.y] = 0;
-        }
-      }
+  if (d.skip_count) {  // counts for the measurement of the skip (off in production)
+    const unsigned long long ms = __ballot(skip), mz = __ballot(e < d.ne && d.eact[e] == 0),
+                             ml = __ballot(e < d.ne);
+    if (lane == 0) {
+      atomicAdd(d.skip_count, (unsigned long long)__popcll(ms));
+      atomicAdd(d.skip_count + 1, (unsigned long long)__popcll(mz));
+      atomicAdd(d.skip_count + 2, (unsigned long long)__popcll(ml));
     }
   }
+  if (e < d.ne && !skip) act = geom_entry<GEOM_ALL>(d, e, skipping, a);
   return act;
 }
 
@@ -697,12 +715,79 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
   }
 }
 
-// The gaps of the inactive entries with zero history from the current
-// poses: what the last step computed for them (Interaction::UpdateDispatcher,
-// deactivate() giving 0), including the entries that step skipped.
+// The far-entry skip with compaction: a workgroup takes SEG consecutive
+// entries, tests each against the motion bounds (coalesced reads of the
+// activity byte, the key and the pair; the bounds are per body and stay in
+// L2), gathers the entries still to evaluate into an LDS list, and runs the
+// geometry and the force law on that list 256 at a time.  In a settled
+// packing about one entry in ten is evaluated: the list fills whole
+// wavefronts, where the per-entry test of k_interact leaves most lanes of
+// every wavefront idle for the length of the busiest lane's chain.  The
+// entries are independent (each writes its own state and its own two
+// resultant slots), so the order of the list changes no result.
+template <int LAW, int SEG>
+__global__ void __launch_bounds__(kBlock) k_interact_seg(Dev d, StepParams sp, int pii) {
+  constexpr int M = SEG / kBlock;
+  __shared__ int wcount[kBlock / 64];
+  __shared__ int list[SEG];
+  __shared__ ActiveKin sa[kBlock];
+  if (frozen(d)) return;
+  const int lane = threadIdx.x & 63;
+  for (TileLoop L = xcd_tiles((d.ne + SEG - 1) / SEG); L.t < L.end; L.t += L.step) {
+    const int e0 = L.t * SEG + (int)threadIdx.x;
+    bool need[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      const int e = e0 + r * kBlock;
+      bool nd = e < d.ne;
+      if (nd && d.eact[e] == 0) nd = !(d.ekey[e] - d.cbound[d.ei[e]] - d.cbound[d.ej[e]] > kSkipMargin);
+      need[r] = nd;
+    }
+    if (d.skip_count) {  // counts for the measurement of the skip (off in production)
+#pragma unroll
+      for (int r = 0; r < M; r++) {
+        const int e = e0 + r * kBlock;
+        const unsigned long long ms = __ballot(e < d.ne && !need[r]), mz = __ballot(e < d.ne && d.eact[e] == 0),
+                                 ml = __ballot(e < d.ne);
+        if (lane == 0) {
+          atomicAdd(d.skip_count, (unsigned long long)__popcll(ms));
+          atomicAdd(d.skip_count + 1, (unsigned long long)__popcll(mz));
+          atomicAdd(d.skip_count + 2, (unsigned long long)__popcll(ml));
+        }
+      }
+    }
+    int n = 0;
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      int tot = 0;
+      const int rank = active_rank<kBlock>(need[r], wcount, &tot);
+      if (need[r]) list[n + rank] = e0 + r * kBlock;
+      n += tot;
+      __syncthreads();  // wcount is read by every thread before the next ranking writes it
+    }
+    for (int c = 0; c < n; c += kBlock) {
+      ActiveKin a;
+      bool act = false;
+      if (c + (int)threadIdx.x < n) act = geom_entry<false>(d, list[c + threadIdx.x], true, a);
+      int total = 0;
+      const int r = active_rank<kBlock>(act, wcount, &total);
+      if (act) sa[r] = a;
+      __syncthreads();
+      if ((int)threadIdx.x < total) law_item<LAW>(d, sp, 0, pii, sa[threadIdx.x]);
+      __syncthreads();
+    }
+  }
+}
+
+// The gaps of the entries the last step skipped, from the current poses:
+// what that step would have computed for them (Interaction::UpdateDispatcher,
+// deactivate() giving 0).  The test is the skip's own with the bounds of the
+// last step, so it also takes entries that step evaluated with a positive
+// gap (recomputed to the same bits) and never an entry not evaluated since
+// its key was cleared (a new list's entries keep the gap the merge gave).
 __global__ void __launch_bounds__(kBlock) k_refresh_gaps(Dev d) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.ne; e += gridDim.x * blockDim.x) {
-    if (d.eact[e] != 0) continue;
+    if (d.eact[e] != 0 || !(d.ekey[e] - d.cbound[d.ei[e]] - d.cbound[d.ej[e]] > kSkipMargin)) continue;
     Kin k;
     k.deact = false;
     k.prev_dn = 0.0;
@@ -1223,22 +1308,27 @@ __global__ void __launch_bounds__(kCheckBlock) k_guard_check(Dev d, int nparts) 
     kw = red[1][k] < kw ? red[1][k] : kw;
   }
   DevGuard* G = d.guard;
-  if (kp > G->maxpos2) G->maxpos2 = kp;
-  if (kw < G->minw) G->minw = kw;
   const long long step = G->step;
   G->step = step + 1;
   const int rebuild_now = G->nu;
   G->nu = 0;
-  G->reset = 0;
-  const double mp = sqrt(__longlong_as_double((long long)G->maxpos2));
-  const bool rot = G->minw != ~0ull && G->minw <= wkey(d.wstar);
-  if (mp >= d.disp || rot) {
-    G->nu = 1;
-    G->reset = 1;
-    G->maxpos2 = 0ull;
-    G->minw = ~0ull;
+  if (d.disp_check) {
+    if (kp > G->maxpos2) G->maxpos2 = kp;
+    if (kw < G->minw) G->minw = kw;
+    G->reset = 0;
+    const double mp = sqrt(__longlong_as_double((long long)G->maxpos2));
+    const bool rot = G->minw != ~0ull && G->minw <= wkey(d.wstar);
+    if (mp >= d.disp || rot) {
+      G->nu = 1;
+      G->reset = 1;
+      G->maxpos2 = 0ull;
+      G->minw = ~0ull;
+    }
   }
-  if (rebuild_now) {
+  // a rebuild is due after this step: the check of the step before asked
+  // for it, or an interface broke in this step (breakage_of_interfaces sets
+  // needUpdate, Rockable.cpp:3410, and the loop rebuilds at once, 2693)
+  if (rebuild_now || *(volatile const int*)d.flags != 0) {
     G->frozen = 1;
     G->frozen_step = step;
   }
@@ -1677,6 +1767,7 @@ __device__ int find_entry(const Dev& d, int owner, int other, uint32_t sub) {
 }
 
 __global__ void k_breakage(Dev d) {
+  if (frozen(d)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.ifc.n || !d.ifc.brk[k] || !d.ifc.alive[k]) return;
   const uint64_t key = d.ifc.key[k];
@@ -1784,8 +1875,26 @@ void launch_fused_sorted(const Dev& d, const StepParams& sp, int write_geom_all,
     hipLaunchKernelGGL((k_interact<LAW_DEFAULT, GEOM_ALL, SORT>), g, bl, 0, s, d, sp, write_geom_all, 0);
 }
 
+template <int SEG>
+void launch_seg(const Dev& d, const StepParams& sp, hipStream_t s) {
+  const dim3 g = grid_for(d, (d.ne + SEG - 1) / SEG), bl(kBlock);
+  if (sp.law == LAW_AVALANCHE)
+    hipLaunchKernelGGL((k_interact_seg<LAW_AVALANCHE, SEG>), g, bl, 0, s, d, sp, 0);
+  else if (sp.law == LAW_STICKED)
+    hipLaunchKernelGGL((k_interact_seg<LAW_STICKED, SEG>), g, bl, 0, s, d, sp, d.paramsInInterfaces);
+  else
+    hipLaunchKernelGGL((k_interact_seg<LAW_DEFAULT, SEG>), g, bl, 0, s, d, sp, 0);
+}
+
 template <bool GEOM_ALL>
 void launch_fused(const Dev& d, const StepParams& sp, int write_geom_all, hipStream_t s) {
+  if (!GEOM_ALL && d.skip_far && d.skip_seg >= 512) {
+    switch (d.skip_seg) {
+      case 512: launch_seg<512>(d, sp, s); return;
+      case 2048: launch_seg<2048>(d, sp, s); return;
+      default: launch_seg<1024>(d, sp, s); return;
+    }
+  }
   if (d.type_sort)
     launch_fused_sorted<GEOM_ALL, true>(d, sp, write_geom_all, s);
   else
@@ -1991,7 +2100,8 @@ __global__ void k_guard_set_step(Dev d, long long step) {
 }
 
 void launch_guard(const Dev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(kCheckBlock), 0, s, d, d.nb > 0 ? gather_blocks(d) : 0);
+  const int nparts = (d.disp_check && d.nb > 0) ? gather_blocks(d) : 0;
+  hipLaunchKernelGGL(k_guard_check, dim3(1), dim3(kCheckBlock), 0, s, d, nparts);
 }
 
 void launch_guard_set_step(const Dev& d, long long step, hipStream_t s) {

@@ -217,7 +217,7 @@ class Engine:
         dVerlet or tmax."""
         return self.get_option(name)
 
-    TUNED = ("fuse_law", "type_sort", "law_group", "geom_block", "gather_lanes", "gather_groups", "fuse_glob", "gather_waves",
+    TUNED = ("fuse_law", "type_sort", "skip_segment", "law_group", "geom_block", "gather_lanes", "gather_groups", "fuse_glob", "gather_waves",
              "body_block", "persistent",
              "graph_steps")
 

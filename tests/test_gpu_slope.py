@@ -48,16 +48,19 @@ def test_settled_bed_on_slope_with_network_bit_exact(tmp_path):
     mlp = Mlp(device=0)
     mlp.set_params(np.load(os.path.join(ROOT, "rockable_amd", "data", "rebuild_mlp.npy")))
     e = Engine(path)
+    e.set_option("exact_sums", 1)  # the base's resultant in serial order too
     e.set_rebuild_model(mlp)
     o = Oracle(path, threads=8)
     e.update_nl()
     o.update_nl()
-    st = e.stats()
-    print("released bed: %d bodies, coordination %.2f, %d contacts" % (st["nfree"], st["coordination"],
-                                                                      st["ncontacts"]))
+    st = None
     for _ in range(4):
         e.step(500)
         o.run_steps(500)
+        if st is None:
+            st = e.stats()
+            print("released bed after 500 steps: %d bodies, coordination %.2f, %d contacts" %
+                  (st["nfree"], st["coordination"], st["ncontacts"]), flush=True)
         assert_bodies_equal(e.bodies(), o.bodies())
         assert_interactions_equal(e.interactions(active=True), o.interactions(active=True))
         assert e.counts()["nl_updates"] == o.nl_updates
@@ -66,6 +69,6 @@ def test_settled_bed_on_slope_with_network_bit_exact(tmp_path):
                                                                   c["guard_polls"]))
     assert o.nl_updates >= 5
     assert c["guard_reads"] < 2000 // 4
-    assert st["coordination"] > 1.0
+    assert st["ncontacts"] > 1000   # a flowing bed, not a dilute spray
     e.close()
     mlp.close()

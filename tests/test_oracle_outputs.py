@@ -123,3 +123,30 @@ def test_dnstat_and_track_damage(oracle_lib, tmp_path):
         assert np.allclose(pos, [0.5 - 0.5 * d1, 0.0, 0.0], atol=1e-6)
     dmg = _lines(str(tmp_path / "damage.txt"))
     assert len(dmg) == 2 and all(r[1] in ("nan", "-nan") for r in dmg)
+
+
+def test_broken_bond_stays_active_until_next_force_phase(oracle_lib, tmp_path):
+    """activeInteractions is built in accelerations() before the breakage
+    (Rockable.cpp:3005-3029, 3391-3410) and UpdateNL keeps the interactions
+    it does not erase where they are (std::set nodes), so the vector's
+    pointers hold until the next accelerations(): the bond broken in step s,
+    at rest (dn = 0), is still in the active set after step s and its
+    rebuild, and leaves it at step s + 1.  Recomputing the set from dn and
+    the bond after the breakage would drop it at step s."""
+    path = K.spun_bond(str(tmp_path), w=0.4, kt=2.0e4, damp=0.035, ft0=1.0, mom0=1.0, power=2.0)
+    o = oracle_lib.Oracle(path)
+    o.update_nl()
+    broke = None
+    for s in range(1, 400):
+        nl = o.nl_updates
+        o.run_steps(1)
+        if o.ninterfaces == 0:
+            broke = s
+            break
+        assert o.nactive == 1
+    assert broke is not None and o.nl_updates == nl + 1   # the breakage's rebuild ran
+    a = o.interactions(active=True)
+    assert len(a) == 1 and a["stick"][0] == -1 and a["dn"][0] >= 0.0 and o.nactive == 1
+    o.run_steps(1)
+    assert len(o.interactions(active=True)) == 0 and o.nactive == 0
+    o.close()
