@@ -142,10 +142,36 @@ def form_roofline(e, steps, law):
     b = force_phase_bytes(c1["nbodies"], nl_listed, law)
     t = ms / max(n, 1) * 1e-3
     ach = b / t / 1e9 if t > 0 else 0.0
-    return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
-            "traffic": None, "kernel": "force phase (%s law)" % law, "algorithmic_bytes_per_launch": b,
-            "force_phase_ms": 1e3 * t, "launches": n, "listed_per_body": nl_listed / c1["nbodies"],
-            "note": "HIP events over %d steps right after the timed region, same state" % steps}
+    out = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+           "traffic": None, "kernel": "force phase (%s law)" % law, "algorithmic_bytes_per_launch": b,
+           "force_phase_ms": 1e3 * t, "launches": n, "listed_per_body": nl_listed / c1["nbodies"],
+           "note": "HIP events over %d steps right after the timed region, same state" % steps}
+    sk = skip_accounting(e, min(steps, 200), c1["nbodies"], nl_listed, law, t)
+    if sk is not None:
+        out["far_entry_skip"] = sk
+    return out
+
+
+def skip_accounting(e, steps, nb, nl, law, t_force):
+    """With the far-entry skip on, the share of listed entries the force
+    phase leaves alone (counted over `steps` further steps, outside every
+    timed region) and the bytes a launch then touches: 17 B per entry tested
+    (activity byte, key, pair), the full per-entry figure for the evaluated
+    ones, the bodies' 156 B.  The roofline's `achieved` keeps the per-listed-
+    interaction figure of the reference's algorithm, which evaluates every
+    listed entry; this is the skip's own count beside it."""
+    if not int(e.get_option("skip_far")):
+        return None
+    e.set_option("count_skips", 1)
+    e.step(steps)
+    skipped, looked = e.get_option("skipped_entries"), e.get_option("entries_looked_at")
+    e.set_option("count_skips", 0)
+    f = skipped / looked if looked > 0 else 0.0
+    per_inter = {"Default": 116, "Avalanche": 148, "StickedLinks": 184}.get(law, 116)
+    touched = 156 * nb + 17 * nl + per_inter * (1.0 - f) * nl
+    ach = touched / t_force / 1e9 if t_force > 0 else 0.0
+    return {"skipped_fraction": f, "steps_counted": steps, "bytes_touched_per_launch": touched,
+            "achieved_touched": ach, "frac_touched": ach / PEAK_HBM_GBS}
 
 
 def force_phase_bytes(nb, nl, law):
@@ -509,6 +535,8 @@ def main():
                     help="only print the device copy rate of MIB MiB buffers (run as a child process)")
     ap.add_argument("--no-copy-rate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--set", default="", metavar="K=V,...",
+                    help="launch options of the headline engine, in place of autotuning")
     ap.add_argument("--skip-far", type=int, default=None, choices=(0, 1),
                     help="the engine's skip_far option on every timed engine (default: the engine's default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -563,7 +591,17 @@ def main():
     timed_options(e)
     e.update_nl()
     t0 = time.perf_counter()
-    tuned, choice = (0, None) if a.no_autotune else e.autotune(steps_per_trial=a.tune_trial)
+    if a.set:
+        # launch options given explicitly (a profiled run repeating what the
+        # autotuner chose): no autotuning
+        choice = {}
+        for kv in a.set.split(","):
+            k, v = kv.split("=")
+            e.set_option(k.strip(), float(v))
+            choice[k.strip()] = int(float(v))
+        tuned = 0
+    else:
+        tuned, choice = (0, None) if a.no_autotune else e.autotune(steps_per_trial=a.tune_trial)
     e.step(max(0, a.warmup - tuned))
     e.synchronize()
     log("warm-up: %d steps (%d of them autotuning: %s) in %.2f s" % (max(a.warmup, tuned), tuned, choice,
@@ -638,6 +676,9 @@ def main():
                      "kernel": "force phase (k_interact, or k_geom + k_law, then k_gather + k_heavy_finish)",
                      "algorithmic_bytes_per_launch": bytes_per},
     }
+    sk = skip_accounting(e, period, nb, nl_listed, "Avalanche", t_force)
+    if sk is not None:
+        out["roofline"]["far_entry_skip"] = sk
     e.close()
     if not a.no_copy_rate:
         try:

@@ -154,8 +154,8 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * is proven positive by the bodies' accumulated motion bounds is not
  * evaluated; its gap is recomputed when the list is read out or rebuilt;
  * 0: every listed entry every step), "skip_segment" (entries a workgroup
- * tests and compacts before evaluating them under skip_far: 512, 1024, 2048;
- * 256: no compaction; default 1024), "count_skips" (1: the geometry pass
+ * tests and compacts before evaluating them under skip_far: 512, 1024, 2048,
+ * 4096; 256: no compaction; default 1024), "count_skips" (1: the geometry pass
  * counts entries skipped, inactive with zero history, and looked at, read
  * back with rk_get_option as "skipped_entries", "zero_history_entries" and
  * "entries_looked_at"; setting it again clears them; 0 stops counting),

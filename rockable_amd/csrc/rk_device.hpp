@@ -178,7 +178,7 @@ struct Dev {
   double* cbound;
   const double* rmax;
   int skip_far;           // 1: skip such entries (only when the step writes no geometry of inactive entries)
-  int skip_seg;           // entries a workgroup tests and compacts before evaluating (512, 1024, 2048; else none)
+  int skip_seg;           // entries a workgroup tests and compacts before evaluating (512 ... 4096; else none)
   // with the count_skips option: [0] entries skipped, [1] inactive entries
   // with zero history, [2] entries looked at, summed over the steps
   unsigned long long* skip_count;

@@ -1890,8 +1890,8 @@ int rk_set_option(rk_engine* eng, const char* name, double value) {
       launch_reset_keys(eng->d, eng->stream);  // keys are kept only while skipping is on
     }
   } else if (n == "skip_segment") {
-    if (!(value == 256.0 || value == 512.0 || value == 1024.0 || value == 2048.0))
-      return fail(eng, "skip_segment must be 256, 512, 1024 or 2048");
+    if (!(value == 256.0 || value == 512.0 || value == 1024.0 || value == 2048.0 || value == 4096.0))
+      return fail(eng, "skip_segment must be 256, 512, 1024, 2048 or 4096");
     if (eng->loaded) {
       (void)hipSetDevice(eng->device);
       if (flush_steps(eng) != 0) return -1;
@@ -2051,7 +2051,7 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
   };
   const Knob knobs[] = {{"fuse_law", {0, 1}},
                         {"type_sort", {1, 0}},
-                        {"skip_segment", {1024, 512, 2048, 256}},
+                        {"skip_segment", {1024, 2048, 512, 4096, 256}},
                         {"law_group", {1, 2, 4, 8}},
                         {"geom_block", {64, 128, 256, 512}},
                         {"gather_lanes", {8, 16}},

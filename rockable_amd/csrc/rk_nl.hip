@@ -1048,6 +1048,7 @@ struct MergeIn {
   const double* odamp;
   const int32_t* ostick;
   const uint8_t* oeact;
+  const double* oekey;
 };
 struct MergeOut {
   int32_t* count;   // per group (count pass)
@@ -1195,7 +1196,10 @@ __global__ void k_merge_fill(Dev d, int n, MergeIn in, MergeOut out) {
                         st.momy == 0.0 && st.momz == 0.0);
     out.eact[e] = (in.oeact[o] & 1) | (hist ? 2 : 0);
   }
-  out.ekey[e] = -INFINITY;  // every entry of a new list is evaluated at its first step
+  // the far-entry key: a kept entry keeps its own (the key bounds the
+  // entry's gap through the bodies' accumulated motion, which a rebuild does
+  // not reset); a new entry is evaluated at its first step
+  out.ekey[e] = o >= 0 ? in.oekey[o] : -INFINITY;
 }
 
 // --- CSR -----------------------------------------------------------------
@@ -1679,6 +1683,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
   in.odamp = cur.edamp;
   in.ostick = cur.estick;
   in.oeact = cur.eact;
+  in.oekey = cur.ekey;
   MergeOut out;
   out.count = w.gcount;
   out.offset = w.goff;

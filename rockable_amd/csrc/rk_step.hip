@@ -661,7 +661,7 @@ __device__ __forceinline__ void law_item(const Dev& d, const StepParams& sp, int
 constexpr int kLawGroup = 8;
 
 template <int LAW>
-__global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_geom_all, int pii, int ngeom, int W) {
+__global__ void __launch_bounds__(kBlock, LAW == LAW_STICKED ? 2 : 4) k_law(Dev d, StepParams sp, int write_geom_all, int pii, int ngeom, int W) {
   if (frozen(d)) return;
   const int waves = (ngeom + W - 1) / W;
   for (TileLoop L = xcd_tiles((waves + kBlock / 64 - 1) / (kBlock / 64)); L.t < L.end; L.t += L.step) {
@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(kBlock) k_law(Dev d, StepParams sp, int write_
 // them after the geometry pass (no second launch, no round trip of the
 // active kinematics through memory).
 template <int LAW, bool GEOM_ALL, bool SORT>
-__global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int write_geom_all, int pii) {
+__global__ void __launch_bounds__(kBlock, LAW == LAW_STICKED ? 2 : 4) k_interact(Dev d, StepParams sp, int write_geom_all, int pii) {
   __shared__ int wcount[kBlock / 64];
   __shared__ int wtype[kBlock / 64][5];
   __shared__ short perm[kBlock];
@@ -719,64 +719,80 @@ __global__ void __launch_bounds__(kBlock) k_interact(Dev d, StepParams sp, int w
 // entries, tests each against the motion bounds (coalesced reads of the
 // activity byte, the key and the pair; the bounds are per body and stay in
 // L2), gathers the entries still to evaluate into an LDS list, and runs the
-// geometry and the force law on that list 256 at a time.  In a settled
-// packing about one entry in ten is evaluated: the list fills whole
-// wavefronts, where the per-entry test of k_interact leaves most lanes of
-// every wavefront idle for the length of the busiest lane's chain.  The
+// geometry and the force law on that list, each lane taking the law of its
+// own entry.  In a settled packing about one entry in ten is evaluated and
+// most of those are active: the list fills whole wavefronts, where the
+// per-entry test of k_interact leaves most lanes of every wavefront idle for
+// the length of the busiest lane's chain.  The
 // entries are independent (each writes its own state and its own two
 // resultant slots), so the order of the list changes no result.
 template <int LAW, int SEG>
-__global__ void __launch_bounds__(kBlock) k_interact_seg(Dev d, StepParams sp, int pii) {
+__global__ void __launch_bounds__(kBlock, LAW == LAW_STICKED ? 2 : 4) k_interact_seg(Dev d, StepParams sp, int pii) {
   constexpr int M = SEG / kBlock;
-  __shared__ int wcount[kBlock / 64];
+  __shared__ int wcnt[M][kBlock / 64];
   __shared__ int list[SEG];
-  __shared__ ActiveKin sa[kBlock];
   if (frozen(d)) return;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long cnt[3] = {0ull, 0ull, 0ull};  // the wavefront's counts (count_skips)
   for (TileLoop L = xcd_tiles((d.ne + SEG - 1) / SEG); L.t < L.end; L.t += L.step) {
     const int e0 = L.t * SEG + (int)threadIdx.x;
     bool need[M];
 #pragma unroll
     for (int r = 0; r < M; r++) {
+      // the activity byte, key and pair together (one round trip), then the bounds
       const int e = e0 + r * kBlock;
       bool nd = e < d.ne;
-      if (nd && d.eact[e] == 0) nd = !(d.ekey[e] - d.cbound[d.ei[e]] - d.cbound[d.ej[e]] > kSkipMargin);
+      if (nd) {
+        const uint8_t ea = d.eact[e];
+        const double key = d.ekey[e];
+        const int bi = d.ei[e], bj = d.ej[e];
+        nd = !(ea == 0 && key - d.cbound[bi] - d.cbound[bj] > kSkipMargin);
+      }
       need[r] = nd;
     }
     if (d.skip_count) {  // counts for the measurement of the skip (off in production)
 #pragma unroll
       for (int r = 0; r < M; r++) {
         const int e = e0 + r * kBlock;
-        const unsigned long long ms = __ballot(e < d.ne && !need[r]), mz = __ballot(e < d.ne && d.eact[e] == 0),
-                                 ml = __ballot(e < d.ne);
-        if (lane == 0) {
-          atomicAdd(d.skip_count, (unsigned long long)__popcll(ms));
-          atomicAdd(d.skip_count + 1, (unsigned long long)__popcll(mz));
-          atomicAdd(d.skip_count + 2, (unsigned long long)__popcll(ml));
-        }
+        cnt[0] += __popcll(__ballot(e < d.ne && !need[r]));
+        cnt[1] += __popcll(__ballot(e < d.ne && d.eact[e] == 0));
+        cnt[2] += __popcll(__ballot(e < d.ne));
       }
     }
+    // rank the entries to evaluate, in entry order: every wavefront's counts
+    // for the M rows first, one barrier, then each thread its own places
+    unsigned long long mask[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      mask[r] = __ballot(need[r]);
+      if (lane == 0) wcnt[r][wave] = __popcll(mask[r]);
+    }
+    __syncthreads();
     int n = 0;
 #pragma unroll
     for (int r = 0; r < M; r++) {
-      int tot = 0;
-      const int rank = active_rank<kBlock>(need[r], wcount, &tot);
-      if (need[r]) list[n + rank] = e0 + r * kBlock;
+      int before = 0, tot = 0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; w++) {
+        const int c = wcnt[r][w];
+        before += w < wave ? c : 0;
+        tot += c;
+      }
+      if (need[r]) list[n + before + __popcll(mask[r] & ((1ull << lane) - 1ull))] = e0 + r * kBlock;
       n += tot;
-      __syncthreads();  // wcount is read by every thread before the next ranking writes it
     }
-    for (int c = 0; c < n; c += kBlock) {
+    __syncthreads();
+    // the evaluated entries are dense in the list and most of them are
+    // active: each lane applies the law to its own entry right after its
+    // geometry (the body records still in registers), no second ranking
+    for (int c = (int)threadIdx.x; c < n; c += kBlock) {
       ActiveKin a;
-      bool act = false;
-      if (c + (int)threadIdx.x < n) act = geom_entry<false>(d, list[c + threadIdx.x], true, a);
-      int total = 0;
-      const int r = active_rank<kBlock>(act, wcount, &total);
-      if (act) sa[r] = a;
-      __syncthreads();
-      if ((int)threadIdx.x < total) law_item<LAW>(d, sp, 0, pii, sa[threadIdx.x]);
-      __syncthreads();
+      if (geom_entry<false>(d, list[c], true, a)) law_item<LAW>(d, sp, 0, pii, a);
     }
+    __syncthreads();  // the next tile's ranking rewrites wcnt and list
   }
+  if (d.skip_count && lane == 0)  // one atomic per wavefront and count, not per tile
+    for (int k = 0; k < 3; k++) atomicAdd(d.skip_count + k, cnt[k]);
 }
 
 // The gaps of the entries the last step skipped, from the current poses:
@@ -1892,6 +1908,7 @@ void launch_fused(const Dev& d, const StepParams& sp, int write_geom_all, hipStr
     switch (d.skip_seg) {
       case 512: launch_seg<512>(d, sp, s); return;
       case 2048: launch_seg<2048>(d, sp, s); return;
+      case 4096: launch_seg<4096>(d, sp, s); return;
       default: launch_seg<1024>(d, sp, s); return;
     }
   }
