@@ -400,19 +400,24 @@ __global__ void k_old_compact(int n, const int32_t* ei, const int32_t* ej, const
 }
 // combined key list: candidates in both directions, then old groups
 constexpr uint32_t kOldTag = 0x80000000u;
+// The group keys (owner, other) packed as owner << bb | other, bb the bits
+// of a body index, so that the radix sort runs over 2 bb bits instead of 64
+// (the order is the same: owner first, then other).
+__device__ __forceinline__ uint64_t pack_pair(uint32_t i, uint32_t j, int bb) { return ((uint64_t)i << bb) | j; }
 __global__ void k_fill_groups(int nc, const uint64_t* cand, int ng_old, const uint64_t* okey, uint64_t* gk,
-                              uint32_t* gv) {
+                              uint32_t* gv, int bb) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nc) {
     const uint64_t k = cand[t];
     const uint32_t i = (uint32_t)(k >> 32), j = (uint32_t)(k & 0xffffffffu);
-    gk[2 * t] = k;
+    gk[2 * t] = pack_pair(i, j, bb);
     gv[2 * t] = (uint32_t)t << 1;
-    gk[2 * t + 1] = ((uint64_t)j << 32) | i;
+    gk[2 * t + 1] = pack_pair(j, i, bb);
     gv[2 * t + 1] = ((uint32_t)t << 1) | 1u;
   } else if (t < nc + ng_old) {
     const int g = t - nc;
-    gk[2 * nc + g] = okey[g];
+    const uint64_t k = okey[g];
+    gk[2 * nc + g] = pack_pair((uint32_t)(k >> 32), (uint32_t)(k & 0xffffffffu), bb);
     gv[2 * nc + g] = kOldTag | (uint32_t)g;
   }
 }
@@ -422,11 +427,12 @@ __global__ void k_group_heads(int n, const uint64_t* gk, int32_t* flag) {
   flag[t] = (t == 0 || gk[t] != gk[t - 1]) ? 1 : 0;
 }
 __global__ void k_group_compact(int n, const uint64_t* gk, const uint32_t* gv, const int32_t* flag,
-                                const int32_t* pos, uint64_t* ukey, int32_t* ucand, int32_t* uold) {
+                                const int32_t* pos, uint64_t* ukey, int32_t* ucand, int32_t* uold, int bb) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || !flag[t]) return;
   const int u = pos[t];
-  ukey[u] = gk[t];
+  // back to (owner << 32) | other
+  ukey[u] = ((gk[t] >> bb) << 32) | (gk[t] & ((1ull << bb) - 1ull));
   int32_t c = -1, o = -1;
   for (int q = t; q < n && q < t + 2 && gk[q] == gk[t]; q++) {
     if (gv[q] & kOldTag)
@@ -1647,13 +1653,15 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       ensure(w, w.goff, w.goff_cap, (size_t)std::max(ngk, 1) + 1))
     return -1;
   if (ngk > 0) {
+    int bb = 1;
+    while ((1ll << bb) < (long long)nb) bb++;
     hipLaunchKernelGGL(k_fill_groups, dim3(nblk(nc + ng_old)), dim3(kBlock), 0, s, nc, w.cand, ng_old, w.okey, w.gk,
-                       w.gv);
-    if (sort_pairs(w, w.gk, w.gk2, w.gv, w.gv2, ngk, 64, s)) return -1;
+                       w.gv, bb);
+    if (sort_pairs(w, w.gk, w.gk2, w.gv, w.gv2, ngk, 2 * bb, s)) return -1;
     hipLaunchKernelGGL(k_group_heads, dim3(nblk(ngk)), dim3(kBlock), 0, s, ngk, w.gk2, w.gflag);
     if (scan_exclusive(w, w.gflag, w.gpos, ngk, s)) return -1;
     hipLaunchKernelGGL(k_group_compact, dim3(nblk(ngk)), dim3(kBlock), 0, s, ngk, w.gk2, w.gv2, w.gflag, w.gpos,
-                       w.ukey, w.ucand, w.uold);
+                       w.ukey, w.ucand, w.uold, bb);
     int last[2];
     RK_CHECK(hipMemcpyAsync(&last[0], w.gpos + ngk - 1, sizeof(int), hipMemcpyDeviceToHost, s));
     RK_CHECK(hipMemcpyAsync(&last[1], w.gflag + ngk - 1, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -791,8 +791,17 @@ __global__ void __launch_bounds__(kBlock, LAW == LAW_STICKED ? 2 : 4) k_interact
     }
     __syncthreads();  // the next tile's ranking rewrites wcnt and list
   }
-  if (d.skip_count && lane == 0)  // one atomic per wavefront and count, not per tile
-    for (int k = 0; k < 3; k++) atomicAdd(d.skip_count + k, cnt[k]);
+  if (d.skip_count) {  // one atomic per workgroup and count (the counts summed in LDS first)
+    __shared__ unsigned long long wsum[3][kBlock / 64];
+    if (lane == 0)
+      for (int k = 0; k < 3; k++) wsum[k][wave] = cnt[k];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      unsigned long long t = 0ull;
+      for (int w = 0; w < kBlock / 64; w++) t += wsum[threadIdx.x][w];
+      atomicAdd(d.skip_count + threadIdx.x, t);
+    }
+  }
 }
 
 // The gaps of the entries the last step skipped, from the current poses:
