@@ -50,11 +50,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per force-phase launch (k_interact + k_gather + k_heavy_finish on
-# the 50 005-body settled pour): rocprofv3 FETCH_SIZE and WRITE_SIZE, each
-# scaled by what it reads for a plain 256 MiB device copy (factors 2.0 and
-# 1.0), 171.3 MB read + 36.3 MB written; profiles/c2_force_phase_traffic.md
-TRAFFIC_BYTES_PER_LAUNCH = 207.6e6
+# HBM bytes per force-phase launch on the 50 005-body settled pour:
+# rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
+# plain 256 MiB device copy (factors 2.0 and 1.0).  With the far-entry skip
+# (k_interact_seg + k_gather + k_heavy_finish): 137.0 MB read + 27.9 MB
+# written, profiles/c2_force_phase_traffic.md; without it (k_interact +
+# k_gather + k_heavy_finish, measured before the skip): 171.3 + 36.3 MB.
+TRAFFIC_BYTES_PER_LAUNCH = {1: 164.9e6, 0: 207.6e6}
 
 
 def make_settled_deck(n_bodies, seed, dirpath, settle_steps, relax_steps, log):
@@ -114,6 +116,14 @@ def timed_options(e):
     e.set_option("record_geometry", 0)
     if SKIP_FAR is not None:
         e.set_option("skip_far", SKIP_FAR)
+
+
+def form_autotune(e, a):
+    """The forms' launch shapes: autotuned, or with --no-autotune the
+    engine's defaults (no captured graphs), as a profiled run takes them."""
+    if a.no_autotune:
+        return 0, {k: int(e.get_option(k)) for k in e.TUNED}
+    return e.autotune(steps_per_trial=a.tune_trial)
 
 
 def interverlet_steps(e):
@@ -313,7 +323,7 @@ def batched_form(a, log):
     e = Engine(path, device=0)
     timed_options(e)
     e.update_nl()
-    ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
+    ntune, choice = form_autotune(e, a)
     log("autotune: %d steps, %s" % (ntune, choice))
     c0 = e.counts()
     e.synchronize()
@@ -376,7 +386,7 @@ def bonds_form(a, log):
     e = Engine(path, device=0)
     timed_options(e)
     e.update_nl()
-    ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
+    ntune, choice = form_autotune(e, a)
     log("autotune: %d steps, %s" % (ntune, choice))
     c0 = e.counts()
     e.synchronize()
@@ -435,7 +445,7 @@ def avalanche_forms(a, log, settled_path):
     e = Engine(path, device=0)
     timed_options(e)
     e.update_nl()
-    ntune, choice = e.autotune(steps_per_trial=a.tune_trial)
+    ntune, choice = form_autotune(e, a)
     e.step(a.avalanche_warmup)
     start = ntune + a.avalanche_warmup
     log("autotune: %d steps, %s" % (ntune, choice))
@@ -529,6 +539,8 @@ def main():
     ap.add_argument("--bond-steps", type=int, default=2000, help="timed steps of the bonds form")
     ap.add_argument("--forms", default="batched,avalanche,bonds",
                     help="extra forms reported beside the plain one (comma list; empty for none)")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="run the --forms alone, without the headline (profiled runs of one form)")
     ap.add_argument("--prepare-only", action="store_true",
                     help="settle the packing into --deck-dir and stop (later runs reuse it)")
     ap.add_argument("--copy-rate", type=int, default=0, metavar="MIB",
@@ -585,6 +597,18 @@ def main():
         json.dump(meta, open(meta_path, "w"))
     if a.prepare_only:
         log("settled packing in %s" % d)
+        return
+    if a.forms_only:
+        # the forms alone (a profiled run of one form); no headline line
+        forms = [f for f in a.forms.split(",") if f]
+        res = {}
+        if "batched" in forms:
+            res["batched"] = batched_form(a, log)
+        if "avalanche" in forms:
+            res["avalanche"] = avalanche_forms(a, log, path)
+        if "bonds" in forms:
+            res["bonds"] = bonds_form(a, log)
+        print(json.dumps({"forms": res}), flush=True)
         return
 
     e = Engine(path, device=0)
@@ -672,8 +696,9 @@ def main():
                    "skip_far": skip_far,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "kernel": "force phase (k_interact, or k_geom + k_law, then k_gather + k_heavy_finish)",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH[skip_far],
+                     "kernel": "force phase (k_interact_seg with the far-entry skip, else k_interact or k_geom + "
+                               "k_law; then k_gather + k_heavy_finish)",
                      "algorithmic_bytes_per_launch": bytes_per},
     }
     sk = skip_accounting(e, period, nb, nl_listed, "Avalanche", t_force)

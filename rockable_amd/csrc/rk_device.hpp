@@ -232,6 +232,7 @@ struct Dev {
   int nctrl;
   DevControl* ctrl;
   const int32_t* ctrl_off;   // per driven body, [ctrl_off[b], ctrl_off[b+1]) into ctrl_idx
+  int ctrl_parallel;         // every control acts on a driven body: one thread per driven body
   const int32_t* ctrl_idx;
   // ServoFunction: 0 none, 1 tritriIsostaticCompression, 2 tritriBiaxialCompression,
   // 3 tritriCustom, 4 tritriLodeAngle (the walls' gaps, on the device);

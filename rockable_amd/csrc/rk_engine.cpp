@@ -83,6 +83,7 @@ struct rk_engine {
   uint8_t* ifc_brk = nullptr;
   DevControl* ctrl = nullptr;
   int32_t *ctrl_off = nullptr, *ctrl_idx = nullptr;
+  int ctrl_parallel = 0;  // every control acts on a driven body
   int32_t* flags = nullptr;
   uint8_t* passive = nullptr;
   double* gv = nullptr;
@@ -272,6 +273,24 @@ std::vector<uint8_t> passive_flags(const Model& m) {
   return p;
 }
 
+// The controls of each driven body, in list order (off: nDriven + 1 starts
+// into idx).  Returns 1 when every control acts on a driven body (each body's
+// controls can then run on a thread of their own), else 0.
+int control_csr(const Model& m, std::vector<int32_t>& off, std::vector<int32_t>& idx) {
+  const int nd = std::max(0, std::min(m.nDriven, (int)m.bodies.size()));
+  off.assign(nd + 1, 0);
+  for (const Control& k : m.ctrl)
+    if (k.i >= 0 && k.i < nd) off[k.i + 1]++;
+  for (int b = 0; b < nd; b++) off[b + 1] += off[b];
+  idx.assign(off[nd], 0);
+  std::vector<int32_t> fill(off.begin(), off.end() - 1);
+  for (size_t q = 0; q < m.ctrl.size(); q++) {
+    const int b = m.ctrl[q].i;
+    if (b >= 0 && b < nd) idx[fill[b]++] = (int)q;
+  }
+  return off[nd] == (int)m.ctrl.size() ? 1 : 0;
+}
+
 // Initial far-entry bounds: 0, or +inf for a driven body that a control
 // moves or turns (its entries are then never skipped).
 std::vector<double> initial_bounds(const Model& m) {
@@ -399,6 +418,7 @@ void assemble_dev(rk_engine* eng) {
   d.ctrl = eng->ctrl;
   d.ctrl_off = eng->ctrl_off;
   d.ctrl_idx = eng->ctrl_idx;
+  d.ctrl_parallel = eng->ctrl_parallel;
   d.servo = m.servo;
   d.servo_icontr = m.servo_icontr;
   for (int k = 0; k < 6; k++) d.servo_ids[k] = m.servo_ids[k];
@@ -702,13 +722,8 @@ int upload_model(rk_engine* eng) {
       dc.vz = x.vec.z;
       c.push_back(dc);
     }
-    const int nd = std::max(0, std::min(m.nDriven, nb));
-    std::vector<int32_t> off(nd + 1, 0), idx;
-    for (int b = 0; b < nd; b++) {
-      for (size_t q = 0; q < m.ctrl.size(); q++)
-        if (m.ctrl[q].i == b) idx.push_back((int)q);
-      off[b + 1] = (int)idx.size();
-    }
+    std::vector<int32_t> off, idx;
+    eng->ctrl_parallel = control_csr(m, off, idx);
     if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
     if (upload(eng, eng->passive, passive_flags(m))) return -1;
   }
@@ -1754,12 +1769,8 @@ int upload_controls(rk_engine* eng) {
     c.push_back(dc);
   }
   const int nd = std::max(0, std::min(eng->m.nDriven, (int)eng->m.bodies.size()));
-  std::vector<int32_t> off(nd + 1, 0), idx;
-  for (int b = 0; b < nd; b++) {
-    for (size_t q = 0; q < eng->m.ctrl.size(); q++)
-      if (eng->m.ctrl[q].i == b) idx.push_back((int)q);
-    off[b + 1] = (int)idx.size();
-  }
+  std::vector<int32_t> off, idx;
+  eng->ctrl_parallel = control_csr(eng->m, off, idx);
   if (upload(eng, eng->ctrl, c) || upload(eng, eng->ctrl_off, off) || upload(eng, eng->ctrl_idx, idx)) return -1;
   if (upload(eng, eng->passive, passive_flags(eng->m))) return -1;
   if (nd > 0) {

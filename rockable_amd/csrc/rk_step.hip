@@ -1461,7 +1461,9 @@ __global__ void __launch_bounds__(kBlock) k_glob(Dev d, int all, int nv) {
 // list order (Rockable.cpp:1856-1900, 1988-2033), all by one thread.  The
 // tritri servos read the walls' inner faces here; the shakers' and the
 // ramp's values (functions of t only) come from the host in sp.servo_v.
-__device__ void drive_controls(const Dev& d, const StepParams& sp) {
+// The servo of the driving system (DrivingSystem.cpp:242-496): the control
+// values it imposes, from the walls' positions before any control moves them.
+__device__ void drive_servo(const Dev& d, const StepParams& sp) {
   const int ic = d.servo_icontr;
   if (d.servo >= 1 && d.servo <= 4 && ic + 5 < d.nctrl) {
     const int* id = d.servo_ids;
@@ -1503,99 +1505,129 @@ __device__ void drive_controls(const Dev& d, const StepParams& sp) {
   } else if (d.servo == 8 && ic < d.nctrl) {  // ramp
     d.ctrl[ic].value = sp.servo_v[0];
   }
-  for (int c = 0; c < d.nctrl; c++) {
-    const DevControl C = d.ctrl[c];
-    BodyKin* p = d.kin + C.i;
-    switch (C.type) {
-      case C_X_VEL:
-        p->vx = C.value;
-        p->px += sp.dt * C.value;
-        break;
-      case C_Y_VEL:
-        p->vy = C.value;
-        p->py += sp.dt * C.value;
-        break;
-      case C_Z_VEL:
-        p->vz = C.value;
-        p->pz += sp.dt * C.value;
-        break;
-      case C_XROT_VEL:
-      case C_YROT_VEL:
-      case C_ZROT_VEL:
-      case C_XYZROT_VEL: {
-        v3 w;
-        if (C.type == C_XROT_VEL)
-          w = V(C.value, 0.0, 0.0);
-        else if (C.type == C_YROT_VEL)
-          w = V(0.0, C.value, 0.0);
-        else if (C.type == C_ZROT_VEL)
-          w = V(0.0, 0.0, C.value);
-        else
-          w = V(C.vx, C.vy, C.vz);
-        qt Q = kq(*p);
-        Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
-        qnormalize(&Q);
-        p->qw = Q.w;
-        p->qx = Q.x;
-        p->qy = Q.y;
-        p->qz = Q.z;
-      } break;
-      default: break;
-    }
+}
+
+// The controls of the predictor (Rockable.cpp:1979-2035): first the
+// velocity controls of every control in list order, then the force and
+// moment ones.  A control reads and writes only its own body, so the two
+// passes restricted to one body give that body's state whatever the other
+// bodies' controls do.
+__device__ __forceinline__ void control_vel(const Dev& d, const StepParams& sp, const DevControl& C) {
+  BodyKin* p = d.kin + C.i;
+  switch (C.type) {
+    case C_X_VEL:
+      p->vx = C.value;
+      p->px += sp.dt * C.value;
+      break;
+    case C_Y_VEL:
+      p->vy = C.value;
+      p->py += sp.dt * C.value;
+      break;
+    case C_Z_VEL:
+      p->vz = C.value;
+      p->pz += sp.dt * C.value;
+      break;
+    case C_XROT_VEL:
+    case C_YROT_VEL:
+    case C_ZROT_VEL:
+    case C_XYZROT_VEL: {
+      v3 w;
+      if (C.type == C_XROT_VEL)
+        w = V(C.value, 0.0, 0.0);
+      else if (C.type == C_YROT_VEL)
+        w = V(0.0, C.value, 0.0);
+      else if (C.type == C_ZROT_VEL)
+        w = V(0.0, 0.0, C.value);
+      else
+        w = V(C.vx, C.vy, C.vz);
+      qt Q = kq(*p);
+      Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+      qnormalize(&Q);
+      p->qw = Q.w;
+      p->qx = Q.x;
+      p->qy = Q.y;
+      p->qz = Q.z;
+    } break;
+    default: break;
   }
-  for (int c = 0; c < d.nctrl; c++) {
-    const DevControl C = d.ctrl[c];
-    BodyKin* p = d.kin + C.i;
-    const double* a = d.acc + 3 * (size_t)C.i;
-    const double* ar = d.arot + 3 * (size_t)C.i;
-    switch (C.type) {
-      case C_X_FOR:
-        p->px += sp.dt * p->vx + sp.dt2_2 * a[0];
-        p->vx += sp.dt_2 * a[0];
-        break;
-      case C_Y_FOR:
-        p->py += sp.dt * p->vy + sp.dt2_2 * a[1];
-        p->vy += sp.dt_2 * a[1];
-        break;
-      case C_Z_FOR:
-        p->pz += sp.dt * p->vz + sp.dt2_2 * a[2];
-        p->vz += sp.dt_2 * a[2];
-        break;
-      case C_XROT_MOM:
-      case C_YROT_MOM:
-      case C_ZROT_MOM:
-      case C_XYZROT_MOM: {
-        v3 w;
-        if (C.type == C_XROT_MOM)
-          w = V(p->wx * 1.0, p->wx * 0.0, p->wx * 0.0);
-        else if (C.type == C_YROT_MOM)
-          w = V(p->wy * 0.0, p->wy * 1.0, p->wy * 0.0);
-        else if (C.type == C_ZROT_MOM)
-          w = V(p->wz * 0.0, p->wz * 0.0, p->wz * 1.0);
-        else
-          w = V(p->wx, p->wy, p->wz);
-        qt Q = kq(*p);
-        Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
-        qnormalize(&Q);
-        p->qw = Q.w;
-        p->qx = Q.x;
-        p->qy = Q.y;
-        p->qz = Q.z;
-        if (C.type == C_XROT_MOM)
-          p->wx += sp.dt_2 * ar[0];
-        else if (C.type == C_YROT_MOM)
-          p->wy += sp.dt_2 * ar[1];
-        else if (C.type == C_ZROT_MOM)
-          p->wz += sp.dt_2 * ar[2];
-        else {
-          v3 nw = vadd(V(p->wx, p->wy, p->wz), vsc(sp.dt_2, V(ar[0], ar[1], ar[2])));
-          p->wx = nw.x;
-          p->wy = nw.y;
-          p->wz = nw.z;
-        }
-      } break;
-      default: break;
-    }
+}
+__device__ __forceinline__ void control_force(const Dev& d, const StepParams& sp, const DevControl& C) {
+  BodyKin* p = d.kin + C.i;
+  const double* a = d.acc + 3 * (size_t)C.i;
+  const double* ar = d.arot + 3 * (size_t)C.i;
+  switch (C.type) {
+    case C_X_FOR:
+      p->px += sp.dt * p->vx + sp.dt2_2 * a[0];
+      p->vx += sp.dt_2 * a[0];
+      break;
+    case C_Y_FOR:
+      p->py += sp.dt * p->vy + sp.dt2_2 * a[1];
+      p->vy += sp.dt_2 * a[1];
+      break;
+    case C_Z_FOR:
+      p->pz += sp.dt * p->vz + sp.dt2_2 * a[2];
+      p->vz += sp.dt_2 * a[2];
+      break;
+    case C_XROT_MOM:
+    case C_YROT_MOM:
+    case C_ZROT_MOM:
+    case C_XYZROT_MOM: {
+      v3 w;
+      if (C.type == C_XROT_MOM)
+        w = V(p->wx * 1.0, p->wx * 0.0, p->wx * 0.0);
+      else if (C.type == C_YROT_MOM)
+        w = V(p->wy * 0.0, p->wy * 1.0, p->wy * 0.0);
+      else if (C.type == C_ZROT_MOM)
+        w = V(p->wz * 0.0, p->wz * 0.0, p->wz * 1.0);
+      else
+        w = V(p->wx, p->wy, p->wz);
+      qt Q = kq(*p);
+      Q = qadd_scaled(Q, qdot(Q, w), sp.dt);
+      qnormalize(&Q);
+      p->qw = Q.w;
+      p->qx = Q.x;
+      p->qy = Q.y;
+      p->qz = Q.z;
+      if (C.type == C_XROT_MOM)
+        p->wx += sp.dt_2 * ar[0];
+      else if (C.type == C_YROT_MOM)
+        p->wy += sp.dt_2 * ar[1];
+      else if (C.type == C_ZROT_MOM)
+        p->wz += sp.dt_2 * ar[2];
+      else {
+        v3 nw = vadd(V(p->wx, p->wy, p->wz), vsc(sp.dt_2, V(ar[0], ar[1], ar[2])));
+        p->wx = nw.x;
+        p->wy = nw.y;
+        p->wz = nw.z;
+      }
+    } break;
+    default: break;
+  }
+}
+
+// All controls by one thread in list order (the servo first).
+__device__ void drive_controls(const Dev& d, const StepParams& sp) {
+  drive_servo(d, sp);
+  for (int c = 0; c < d.nctrl; c++) control_vel(d, sp, d.ctrl[c]);
+  for (int c = 0; c < d.nctrl; c++) control_force(d, sp, d.ctrl[c]);
+}
+
+// The controls by the threads of one workgroup (every thread calls it): the
+// servo by thread 0, then each driven body's controls in list order by one
+// thread (ctrl_parallel: every control acts on a driven body), else all of
+// them by thread 0.
+__device__ void drive_controls_block(const Dev& d, const StepParams& sp) {
+  if (d.nctrl <= 0) return;
+  if (!d.ctrl_parallel) {
+    if (threadIdx.x == 0) drive_controls(d, sp);
+    return;
+  }
+  if (threadIdx.x == 0) drive_servo(d, sp);
+  __syncthreads();  // the servo's values are read by the other threads
+  for (int b = threadIdx.x; b < d.nDriven; b += blockDim.x) {
+    const int q0 = d.ctrl_off[b], q1 = d.ctrl_off[b + 1];
+    for (int q = q0; q < q1; q++) control_vel(d, sp, d.ctrl[d.ctrl_idx[q]]);
+    for (int q = q0; q < q1; q++) control_force(d, sp, d.ctrl[d.ctrl_idx[q]]);
   }
 }
 
@@ -1660,7 +1692,7 @@ __device__ __forceinline__ void store_predicted(const Dev& d, int b, const BodyK
 __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp, int glob) {
   if (frozen(d)) return;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid == 0 && d.nctrl > 0) drive_controls(d, sp);
+  if (blockIdx.x == 0) drive_controls_block(d, sp);
   // free bodies (Rockable.cpp:2036-2055), a grid-stride loop
   for (int b = gid; b < d.nb; b += gridDim.x * blockDim.x) {
     if (b < d.nDriven) continue;
@@ -1688,7 +1720,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
   if (frozen(d)) return;
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
-    if (t == 0 && d.nctrl > 0) drive_controls(d, sp);
+    drive_controls_block(d, sp);
     if (driven_glob) {
       __syncthreads();  // the driven poses, written by thread 0, are visible to the workgroup
       for (int k = t; k < d.nvdriven; k += blockDim.x) {
@@ -1755,7 +1787,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
 // k_glob after it.
 __global__ void __launch_bounds__(kBlock) k_predict_vtx(Dev d, StepParams sp) {
   if (frozen(d)) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && d.nctrl > 0) drive_controls(d, sp);
+  if (blockIdx.x == 0) drive_controls_block(d, sp);
   const int lane = threadIdx.x & 63;
   for (int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wv < d.nwv;
        wv += (int)((gridDim.x * blockDim.x) >> 6)) {
