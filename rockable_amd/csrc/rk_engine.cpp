@@ -119,6 +119,13 @@ struct rk_engine {
   int64_t guard_reads = 0, guard_wasted = 0, guard_polls = 0;
   // asynchronous polls of the device check: pinned flags and their events
   int* poll_host = nullptr;
+  // pinned staging of the guard read at the end of each batch (the guard
+  // record and the breakage flag)
+  struct GuardRead {
+    DevGuard g;
+    int32_t broke;
+  };
+  GuardRead* guard_host = nullptr;
   hipEvent_t poll_ev[8] = {};
   int64_t last_rebuild_step = 0, last_interval = 0;
   int exact_sums = 0;
@@ -1556,9 +1563,15 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     if (flush_steps(eng) != 0) return -1;
     DevGuard g;
     int32_t broke = 0;
-    HIPCHK(hipMemcpyAsync(&g, eng->guard, sizeof g, hipMemcpyDeviceToHost, eng->stream));
-    if (bonds) HIPCHK(hipMemcpyAsync(&broke, eng->flags, sizeof broke, hipMemcpyDeviceToHost, eng->stream));
+    rk_engine::GuardRead* gh = eng->guard_host;
+    HIPCHK(hipMemcpyAsync(gh ? &gh->g : &g, eng->guard, sizeof g, hipMemcpyDeviceToHost, eng->stream));
+    if (bonds)
+      HIPCHK(hipMemcpyAsync(gh ? &gh->broke : &broke, eng->flags, sizeof broke, hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
+    if (gh) {
+      g = gh->g;
+      if (bonds) broke = gh->broke;
+    }
     eng->guard_reads++;
     int64_t executed = k;
     bool need = eng->needUpdate;
@@ -1654,6 +1667,8 @@ int rk_engine_create(rk_engine** out, int device) {
   } else {
     eng->poll_host = nullptr;
   }
+  if (hipHostMalloc((void**)&eng->guard_host, sizeof(rk_engine::GuardRead), hipHostMallocDefault) != hipSuccess)
+    eng->guard_host = nullptr;
   *out = eng;
   return 0;
 }
@@ -1672,6 +1687,7 @@ void rk_engine_destroy(rk_engine* eng) {
   for (hipEvent_t e : eng->poll_ev)
     if (e) (void)hipEventDestroy(e);
   if (eng->poll_host) (void)hipHostFree(eng->poll_host);
+  if (eng->guard_host) (void)hipHostFree(eng->guard_host);
   if (eng->stream) (void)hipStreamDestroy(eng->stream);
   delete eng;
 }

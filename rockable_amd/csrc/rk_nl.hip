@@ -33,6 +33,9 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
+#include <initializer_list>
+#include <utility>
 #include <vector>
 
 #include "rk_device.hpp"
@@ -1115,7 +1118,6 @@ __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
   const BodyKin Po = d.kin[owner];
   obb_t oth;
   if (candidate) oth = load_obb(in.bobb + 15 * (size_t)other);
-  const int pi = owner < other ? owner : other, pj = owner < other ? other : owner;
   int cnt = WRITE ? out.offset[g] : 0;
   int a = 0, b = 0;
   while (a < on || b < nn) {
@@ -1399,6 +1401,33 @@ int nl_build_csr(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipS
   return nl_build_heavy(w, nb, E, passive, s);
 }
 
+// Reads of small device values through the pinned staging buffer: copy_in
+// queues the copies (byte offsets into the buffer), wait synchronises the
+// stream and copy_out takes the values.
+constexpr size_t kPinBytes = 8192;
+static char* pin_buf(NLWork& w) {
+  if (!w.pin && hipHostMalloc((void**)&w.pin, kPinBytes, hipHostMallocDefault) != hipSuccess) w.pin = nullptr;
+  return w.pin;
+}
+static int read_small(NLWork& w, hipStream_t s, std::initializer_list<std::pair<void*, std::pair<const void*, size_t>>> xs) {
+  char* P = pin_buf(w);
+  size_t off = 0;
+  for (const auto& x : xs) {
+    if (P && off + x.second.second <= kPinBytes)
+      RK_CHECK(hipMemcpyAsync(P + off, x.second.first, x.second.second, hipMemcpyDeviceToHost, s));
+    else
+      RK_CHECK(hipMemcpyAsync(x.first, x.second.first, x.second.second, hipMemcpyDeviceToHost, s));
+    off += (x.second.second + 7) & ~(size_t)7;
+  }
+  RK_CHECK(hipStreamSynchronize(s));
+  off = 0;
+  for (const auto& x : xs) {
+    if (P && off + x.second.second <= kPinBytes) memcpy(x.first, P + off, x.second.second);
+    off += (x.second.second + 7) & ~(size_t)7;
+  }
+  return 0;
+}
+
 int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hipStream_t s) {
   E.nheavy = 0;
   E.nhch = 0;
@@ -1411,9 +1440,8 @@ int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hi
   if (scan_exclusive(w, w.i32a, w.i32c, nb, s)) return -1;
   hipLaunchKernelGGL(k_heavy_list, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, w.i32a, w.i32c, E.heavy);
   int32_t tail[2] = {0, 0};
-  RK_CHECK(hipMemcpyAsync(&tail[0], w.i32c + nb - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  RK_CHECK(hipMemcpyAsync(&tail[1], w.i32a + nb - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  RK_CHECK(hipStreamSynchronize(s));
+  if (read_small(w, s, {{&tail[0], {w.i32c + nb - 1, sizeof(int32_t)}}, {&tail[1], {w.i32a + nb - 1, sizeof(int32_t)}}}))
+    return -1;
   E.nheavy = tail[0] + tail[1];
   if (E.nheavy == 0) return 0;
   // chunk table: at most (slots of all bodies) / 64 + one per heavy body
@@ -1432,8 +1460,7 @@ int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hi
   hipLaunchKernelGGL(k_heavy_chunks, dim3(1), dim3(kBlock), 0, s, E.heavy, E.nheavy, E.soff, passive, E.hch_off,
                      E.hch, (int)cap, w.i32c);
   int32_t cnt[2] = {0, 0};
-  RK_CHECK(hipMemcpyAsync(cnt, w.i32c, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  RK_CHECK(hipStreamSynchronize(s));
+  if (read_small(w, s, {{cnt, {w.i32c, 2 * sizeof(int32_t)}}})) return -1;
   if ((size_t)cnt[0] > cap) {
     snprintf(w.err, sizeof(w.err), "heavy-body chunk table overflow");
     return -1;
@@ -1477,8 +1504,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       if (ensure(w, w.f64a, w.f64a_cap, (size_t)6 * pb)) return -1;
       hipLaunchKernelGGL(k_box_partial, dim3(pb), dim3(kBlock), 0, s, first, nb, w.baabb, w.f64a);
       std::vector<double> part(6 * pb);
-      RK_CHECK(hipMemcpyAsync(part.data(), w.f64a, sizeof(double) * 6 * pb, hipMemcpyDeviceToHost, s));
-      RK_CHECK(hipStreamSynchronize(s));
+      if (read_small(w, s, {{part.data(), {w.f64a, sizeof(double) * 6 * pb}}})) return -1;
       double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
       for (int q = 0; q < pb; q++)
         for (int k = 0; k < 3; k++) {
@@ -1513,8 +1539,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       if (scan_exclusive(w, w.ccount, w.cstart, (int)ncell + 1, s)) return -1;
       hipLaunchKernelGGL(k_cell_fill, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, w.cell_of, w.cstart, w.cfill, w.cbody);
       int hv[2];
-      RK_CHECK(hipMemcpyAsync(hv, w.counter + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-      RK_CHECK(hipStreamSynchronize(s));
+      if (read_small(w, s, {{hv, {w.counter + 1, sizeof(int)}}})) return -1;
       const int nover = hv[0];
       const int ncb = nb - nover;
       hipLaunchKernelGGL(k_broad_cells, dim3(nblk(32L * nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
@@ -1527,8 +1552,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     }
     hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
     int tm[2] = {0, 0};
-    RK_CHECK(hipMemcpyAsync(tm, w.counter + 2, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-    RK_CHECK(hipStreamSynchronize(s));
+    if (read_small(w, s, {{tm, {w.counter + 2, sizeof(int) * 2}}})) return -1;
     nc = tm[0];
     if (tm[1] <= rcap) {
       if (ensure(w, w.cand, w.cand_cap, (size_t)std::max(nc, 1))) return -1;
@@ -1589,9 +1613,8 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
         if (scan_exclusive(w, w.pflag, w.ppos, nc, s)) return -1;
         hipLaunchKernelGGL(k_heavy_list, dim3(nblk(nc)), dim3(kBlock), 0, s, nc, w.pflag, w.ppos, w.plist);
         int last[2] = {0, 0};
-        RK_CHECK(hipMemcpyAsync(&last[0], w.ppos + nc - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-        RK_CHECK(hipMemcpyAsync(&last[1], w.pflag + nc - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-        RK_CHECK(hipStreamSynchronize(s));
+        if (read_small(w, s, {{&last[0], {w.ppos + nc - 1, sizeof(int)}}, {&last[1], {w.pflag + nc - 1, sizeof(int)}}}))
+          return -1;
         nrest = last[0] + last[1];
         clist = w.plist;
       }
@@ -1605,8 +1628,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       RK_CHECK(hipGetLastError());
       hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
       int tm[2] = {0, 0};
-      RK_CHECK(hipMemcpyAsync(tm, w.counter + 2, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-      RK_CHECK(hipStreamSynchronize(s));
+      if (read_small(w, s, {{tm, {w.counter + 2, sizeof(int) * 2}}})) return -1;
       nnear = tm[0];
       if (tm[1] <= rcap) break;
       w.near_hint = kStripes * (tm[1] + tm[1] / 4 + 64);
@@ -1632,9 +1654,8 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     hipLaunchKernelGGL(k_old_compact, dim3(nblk(nold)), dim3(kBlock), 0, s, nold, cur.ei, cur.ej, w.i32b, w.i32c,
                        w.okey, w.ostart);
     int last[2];
-    RK_CHECK(hipMemcpyAsync(&last[0], w.i32c + nold - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    RK_CHECK(hipMemcpyAsync(&last[1], w.i32b + nold - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    RK_CHECK(hipStreamSynchronize(s));
+    if (read_small(w, s, {{&last[0], {w.i32c + nold - 1, sizeof(int)}}, {&last[1], {w.i32b + nold - 1, sizeof(int)}}}))
+      return -1;
     ng_old = last[0] + last[1];
   }
   RK_CHECK(hipMemcpyAsync(w.ostart + ng_old, &nold, sizeof(int), hipMemcpyHostToDevice, s));
@@ -1663,9 +1684,8 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     hipLaunchKernelGGL(k_group_compact, dim3(nblk(ngk)), dim3(kBlock), 0, s, ngk, w.gk2, w.gv2, w.gflag, w.gpos,
                        w.ukey, w.ucand, w.uold, bb);
     int last[2];
-    RK_CHECK(hipMemcpyAsync(&last[0], w.gpos + ngk - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    RK_CHECK(hipMemcpyAsync(&last[1], w.gflag + ngk - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    RK_CHECK(hipStreamSynchronize(s));
+    if (read_small(w, s, {{&last[0], {w.gpos + ngk - 1, sizeof(int)}}, {&last[1], {w.gflag + ngk - 1, sizeof(int)}}}))
+      return -1;
     ng = last[0] + last[1];
   }
 
@@ -1700,8 +1720,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     RK_CHECK(hipMemsetAsync(w.gcount + ng, 0, sizeof(int32_t), s));
     hipLaunchKernelGGL(k_merge<false>, dim3(nblk(ng)), dim3(kBlock), 0, s, d, in, out);
     if (scan_exclusive(w, w.gcount, w.goff, ng + 1, s)) return -1;
-    RK_CHECK(hipMemcpyAsync(&nnew, w.goff + ng, sizeof(int), hipMemcpyDeviceToHost, s));
-    RK_CHECK(hipStreamSynchronize(s));
+    if (read_small(w, s, {{&nnew, {w.goff + ng, sizeof(int)}}})) return -1;
   }
   if (entries_reserve(w, nxt, nnew) != 0) return -1;
   nxt.n = nnew;
@@ -1782,6 +1801,7 @@ void nl_work_free(NLWork& w) {
                 w.cbody, w.over, w.gedge, w.gface, w.kout, w.msrc};
   for (void* q : ps)
     if (q) (void)hipFree(q);
+  if (w.pin) (void)hipHostFree(w.pin);
   NLWork z;
   w = z;
 }

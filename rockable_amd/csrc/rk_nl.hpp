@@ -77,6 +77,9 @@ inline size_t narrow_lds_bytes(const NLParams& p) {
 
 struct NLWork {
   char err[256] = {0};
+  // pinned host staging of the small device-to-host reads (counts, the box
+  // partials): a pageable destination makes the runtime stage each copy
+  char* pin = nullptr;
   int cand_hint = 0, near_hint = 0, last_candidates = 0;
   int heavy_min = 256;  // slot count above which a body's sum runs block-wide
   int narrow_pairs = 0;  // candidate pairs per narrow-phase wavefront (1, 2; 0: by shape size)
