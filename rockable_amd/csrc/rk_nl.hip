@@ -226,10 +226,12 @@ __global__ void k_broad_brute(Dev d, const double* bobb, double half, uint64_t* 
   const int jb = j0 + (int)blockIdx.y * kBruteChunk;
   const int je = min(jb + kBruteChunk, d.nb);
   for (int j = jb; j < je; j++) {
-    const bool emit = !has_live_interface(d.ifc, i, j) && pair_overlap(bobb, i, j, half);
+    const bool emit = pair_overlap(bobb, i, j, half) && !has_live_interface(d.ifc, i, j);
     emit_pair(emit, cand, cnt, rcap, i, j);
   }
 }
+// (the OBB test before the interface search in every broad-phase kernel: both
+// are pure, and the search costs a chain of dependent loads per pair)
 
 // --- linkCells broad phase ---------------------------------------------
 __global__ void k_box_partial(int first, int nb, const double* baabb, double* part) {
@@ -337,8 +339,8 @@ __global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, co
   const int nc = (nx * g.N[1] + ny) * g.N[2] + nz;
   for (int r = cstart[nc]; r < cstart[nc + 1]; r++) {
     const int j = cbody[r];
-    const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
-                      pair_overlap(bobb, i, j, half);
+    const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && pair_overlap(bobb, i, j, half) &&
+                      !has_live_interface(d.ifc, i, j);
     emit_pair(emit, cand, cnt, rcap, i, j);
   }
 }
@@ -379,8 +381,8 @@ __global__ void k_broad_over(Dev d, const double* bobb, double half, Grid g, con
       int j = 0;
       if (r < r1) {
         j = cbody[r];
-        emit = !(j < d.nDriven && i < d.nDriven) && j > i && !has_live_interface(d.ifc, i, j) &&
-               pair_overlap(bobb, i, j, half);
+        emit = !(j < d.nDriven && i < d.nDriven) && j > i && pair_overlap(bobb, i, j, half) &&
+               !has_live_interface(d.ifc, i, j);
       }
       emit_pair(emit, cand, cnt, rcap, i, j);
     }
@@ -1545,7 +1547,10 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       hipLaunchKernelGGL(k_broad_cells, dim3(nblk(32L * nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
                          w.cbody, w.cstripe, w.scnt, rcap);
       if (nover > 0 && ncb > 0) {
-        const int split = std::max(1, std::min(64, (4096 + nover - 1) / nover));
+        // enough workgroups to fill the chip when one body (a wall or a plane
+        // over the whole domain) meets every cell: each takes every split-th
+        // column of cells its box covers
+        const int split = std::max(1, std::min(4096, (8192 + nover - 1) / nover));
         hipLaunchKernelGGL(k_broad_over, dim3(nover, split), dim3(kBlock), 0, s, d, w.bobb, half, g, w.over, nover,
                            w.cstart, w.cbody, reach, w.cstripe, w.scnt, rcap);
       }
