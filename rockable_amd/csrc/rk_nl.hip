@@ -894,24 +894,38 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
-  n = nvi_g * nej_g;
-  inv = 1.0f / (float)max(nej_g, 1);
-  for (int k = sl; k < n; k += SW) {
-    const int q = quot_small(k, nej_g, inv);
-    const int a = vi_l[q], b = ej_l[k - q * nej_g] - nei;
-    if (near_ve(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
-      const int bit = s1 + a * nej + b;
-      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+  // vertex-edge both ways in one pass (i's vertices against j's edges, then
+  // j's against i's, the same predicate with the roles swapped per lane),
+  // and vertex-face likewise: a wavefront runs each predicate once where
+  // the two directions' tests fit its lanes together
+  {
+    const int n1 = nvi_g * nej_g, n2 = nvj_g * nei_g;
+    const float inv1 = 1.0f / (float)max(nej_g, 1), inv2 = 1.0f / (float)max(nei_g, 1);
+    for (int k = sl; k < n1 + n2; k += SW) {
+      const bool fw = k < n1;
+      const int kk = fw ? k : k - n1, no = fw ? nej_g : nei_g;
+      const int q = quot_small(kk, no, fw ? inv1 : inv2);
+      const int a = fw ? vi_l[q] : vj_l[q] - nvi;
+      const int b = fw ? ej_l[kk - q * no] - nei : ei_l[kk - q * no];
+      if (near_ve(d.sh, fw ? shj : shi, fw ? gvi : gvj, fw ? gvj : gvi, fw ? Ri : Rj, fw ? Rj : Ri, a, b, dmax)) {
+        const int bit = fw ? s1 + a * nej + b : s4 + a * nei + b;
+        atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+      }
     }
   }
-  n = nvi_g * nfj_g;
-  inv = 1.0f / (float)max(nfj_g, 1);
-  for (int k = sl; k < n; k += SW) {
-    const int q = quot_small(k, nfj_g, inv);
-    const int a = vi_l[q], b = fj_l[k - q * nfj_g] - ne2 - nfi;
-    if (near_vf(d.sh, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
-      const int bit = s2 + a * nfj + b;
-      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+  {
+    const int n1 = nvi_g * nfj_g, n2 = nvj_g * nfi_g;
+    const float inv1 = 1.0f / (float)max(nfj_g, 1), inv2 = 1.0f / (float)max(nfi_g, 1);
+    for (int k = sl; k < n1 + n2; k += SW) {
+      const bool fw = k < n1;
+      const int kk = fw ? k : k - n1, no = fw ? nfj_g : nfi_g;
+      const int q = quot_small(kk, no, fw ? inv1 : inv2);
+      const int a = fw ? vi_l[q] : vj_l[q] - nvi;
+      const int b = fw ? fj_l[kk - q * no] - ne2 - nfi : fi_l[kk - q * no] - ne2;
+      if (near_vf(d.sh, fw ? shj : shi, fw ? gvi : gvj, fw ? gvj : gvi, fw ? Ri : Rj, fw ? Rj : Ri, a, b, dmax)) {
+        const int bit = fw ? s2 + a * nfj + b : s5 + a * nfi + b;
+        atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
+      }
     }
   }
   n = nei_g * nej_g;
@@ -921,26 +935,6 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
     const int a = ei_l[q], b = ej_l[k - q * nej_g] - nei;
     if (near_ee(d.sh, shi, shj, gvi, gvj, Ri, Rj, a, b, dmax)) {
       const int bit = s3 + a * nej + b;
-      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
-    }
-  }
-  n = nvj_g * nei_g;
-  inv = 1.0f / (float)max(nei_g, 1);
-  for (int k = sl; k < n; k += SW) {
-    const int q = quot_small(k, nei_g, inv);
-    const int a = vj_l[q] - nvi, b = ei_l[k - q * nei_g];
-    if (near_ve(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
-      const int bit = s4 + a * nei + b;
-      atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
-    }
-  }
-  n = nvj_g * nfi_g;
-  inv = 1.0f / (float)max(nfi_g, 1);
-  for (int k = sl; k < n; k += SW) {
-    const int q = quot_small(k, nfi_g, inv);
-    const int a = vj_l[q] - nvi, b = fi_l[k - q * nfi_g] - ne2;
-    if (near_vf(d.sh, shi, gvj, gvi, Rj, Ri, a, b, dmax)) {
-      const int bit = s5 + a * nfi + b;
       atomicOr(bits + (bit >> 6), 1ull << (bit & 63));
     }
   }
