@@ -2103,7 +2103,26 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
     return 0;
   };
   int rc = 0;
+  // the starting shapes (the defaults or the caller's), timed first: if the
+  // search ends slower than them (its trials are short and one knob at a
+  // time), they are kept
+  std::vector<int> start;
   for (const Knob& k : knobs) {
+    double v = 0.0;
+    if (rk_get_option(eng, k.name, &v) != 0) rc = -1;
+    start.push_back((int)v);
+  }
+  double t_start = 1e30;
+  for (int rep = 0; rep < 2 && rc == 0; rep++) {
+    double t = 0.0;
+    const int warm = std::max(1, eng->graph_steps);
+    if (rk_step(eng, warm) != 0 || trial(&t) != 0) rc = -1;
+    done += warm;
+    t_start = std::min(t_start, t);
+  }
+  double t_last = 1e30;
+  for (const Knob& k : knobs) {
+    if (rc != 0) break;
     std::vector<double> best(k.values.size(), 1e30);
     for (int rep = 0; rep < 2 && rc == 0; rep++)
       for (size_t c = 0; c < k.values.size() && rc == 0; c++) {
@@ -2125,10 +2144,26 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
     size_t b = 0;
     for (size_t c = 1; c < best.size(); c++)
       if (best[c] < best[b]) b = c;
+    t_last = best[b];
     if (rk_set_option(eng, k.name, k.values[b]) != 0) {
       rc = -1;
       break;
     }
+  }
+  if (rc == 0) {
+    // the chosen shapes timed again against the starting ones
+    double t_end = 1e30;
+    for (int rep = 0; rep < 2 && rc == 0; rep++) {
+      double t = 0.0;
+      const int warm = std::max(1, eng->graph_steps);
+      if (rk_step(eng, warm) != 0 || trial(&t) != 0) rc = -1;
+      done += warm;
+      t_end = std::min(t_end, t);
+    }
+    (void)t_last;
+    if (rc == 0 && t_start < t_end)
+      for (size_t q = 0; q < start.size() && rc == 0; q++)
+        if (rk_set_option(eng, knobs[q].name, start[q]) != 0) rc = -1;
   }
   eng->tuning = false;
   if (steps_done) *steps_done = done;
