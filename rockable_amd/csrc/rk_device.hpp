@@ -110,6 +110,11 @@ struct DevShapes {
   const int32_t* fidx;    // local vertex indices
   const double* radius;
   const double* obb;      // 15 per shape: c, e0, e1, e2, ext
+  // 2 per shape, at homothety 1: the longest edge and the largest distance
+  // of a face's vertex from the plane of its first two and last vertices
+  // (+inf for a degenerate face); they bound how far the narrow phase's
+  // near tests reach beyond the vertices' convex hull
+  const double* reach;
 };
 
 // Group-pair parameter table: tab[(id * G + g1) * G + g2].

@@ -74,7 +74,7 @@ struct rk_engine {
   int32_t* cluster = nullptr;
   int32_t *sh_nv = nullptr, *sh_ne = nullptr, *sh_nf = nullptr, *sh_voff = nullptr, *sh_eoff = nullptr,
           *sh_foff = nullptr, *sh_edge = nullptr, *sh_fstart = nullptr, *sh_fidx = nullptr;
-  double *sh_vtx = nullptr, *sh_radius = nullptr, *sh_obb = nullptr;
+  double *sh_vtx = nullptr, *sh_radius = nullptr, *sh_obb = nullptr, *sh_reach = nullptr;
   double *ptab = nullptr, *damp_rate = nullptr;
   int32_t* damp_kind = nullptr;
   uint64_t* ifc_key = nullptr;
@@ -208,7 +208,7 @@ void free_device(rk_engine* eng) {
   void* ps[] = {eng->kin,      eng->acc,       eng->arot,      eng->force,     eng->moment,   eng->mass,
                 eng->inertia,  eng->bf_surface, eng->cluster,  eng->sh_nv,     eng->sh_ne,    eng->sh_nf,
                 eng->sh_voff,  eng->sh_eoff,   eng->sh_foff,   eng->sh_edge,   eng->sh_fstart, eng->sh_fidx,
-                eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->ptab,      eng->damp_rate, eng->damp_kind,
+                eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->sh_reach,  eng->ptab,      eng->damp_rate, eng->damp_kind,
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
                 eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
                 eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,      eng->gpart,
@@ -222,7 +222,7 @@ void free_device(rk_engine* eng) {
   eng->cluster = nullptr;
   eng->sh_nv = eng->sh_ne = eng->sh_nf = eng->sh_voff = eng->sh_eoff = eng->sh_foff = eng->sh_edge = nullptr;
   eng->sh_fstart = eng->sh_fidx = nullptr;
-  eng->sh_vtx = eng->sh_radius = eng->sh_obb = nullptr;
+  eng->sh_vtx = eng->sh_radius = eng->sh_obb = eng->sh_reach = nullptr;
   eng->ptab = eng->damp_rate = nullptr;
   eng->damp_kind = nullptr;
   eng->ifc_key = nullptr;
@@ -359,6 +359,7 @@ void assemble_dev(rk_engine* eng) {
   d.sh.fidx = eng->sh_fidx;
   d.sh.radius = eng->sh_radius;
   d.sh.obb = eng->sh_obb;
+  d.sh.reach = eng->sh_reach;
   d.par.tab = eng->ptab;
   d.par.damp_rate = eng->damp_rate;
   d.par.damp_kind = eng->damp_kind;
@@ -506,13 +507,48 @@ double wstar_for(double angle) {
   return unkey(lo);
 }
 
+// How far a shape's near tests (Particle.cpp:88-209) reach beyond the convex
+// hull of its vertices, at homothety 1: a vertex-edge test accepts the
+// vertex's foot on the edge's line up to dVerlet edge lengths past either
+// end (so the longest edge), and a vertex-face test accepts a foot inside the
+// face's outline projected on the plane of its first two and last vertices
+// (so the largest distance of a face vertex from that plane).  A degenerate
+// face (no plane) gives +inf, which turns the narrow phase's separation test
+// off for the shape.
+using v2d = std::pair<double, double>;
+v2d shape_reach(const Shape& s) {
+  double lmax = 0.0, off = 0.0;
+  for (int e = 0; e < s.ne(); e++) {
+    const v3 E = vsub(s.vertex[s.edge[2 * e + 1]], s.vertex[s.edge[2 * e]]);
+    lmax = std::max(lmax, std::sqrt(vdot(E, E)));
+  }
+  for (int f = 0; f < s.nf(); f++) {
+    const int f0 = s.face_start[f], f1 = s.face_start[f + 1];
+    if (f1 - f0 < 3) {
+      off = INFINITY;
+      continue;
+    }
+    const v3 A = s.vertex[s.face_idx[f0]];
+    v3 v1 = vsub(s.vertex[s.face_idx[f0 + 1]], A);
+    v1 = vsc(1.0 / std::sqrt(vdot(v1, v1)), v1);
+    v3 n = vcross(v1, vsub(s.vertex[s.face_idx[f1 - 1]], A));
+    n = vsc(1.0 / std::sqrt(vdot(n, n)), n);
+    for (int k = f0; k < f1; k++) {
+      const double h = std::fabs(vdot(vsub(s.vertex[s.face_idx[k]], A), n));
+      off = (h == h) ? std::max(off, h) : INFINITY;
+    }
+  }
+  if (!(lmax == lmax)) lmax = INFINITY;
+  return {lmax, off};
+}
+
 // upload everything the model holds (after a conf load)
 int upload_model(rk_engine* eng) {
   Model& m = eng->m;
   const int nb = (int)m.bodies.size();
   // shapes
   std::vector<int32_t> nv, ne, nf, voff, eoff, foff, edge, fstart, fidx;
-  std::vector<double> vtx, radius, obb;
+  std::vector<double> vtx, radius, obb, reach;
   int maxv = 1;
   int maxne = 0, maxnf = 0;
   for (const Shape& s : m.shapes) {
@@ -539,6 +575,9 @@ int upload_model(rk_engine* eng) {
     const double ob[15] = {o.c.x,    o.c.y,    o.c.z,    o.e[0].x, o.e[0].y, o.e[0].z, o.e[1].x, o.e[1].y,
                            o.e[1].z, o.e[2].x, o.e[2].y, o.e[2].z, o.ext.x,  o.ext.y,  o.ext.z};
     obb.insert(obb.end(), ob, ob + 15);
+    const v2d r = shape_reach(s);
+    reach.push_back(r.first);
+    reach.push_back(r.second);
   }
   fstart.push_back((int)fidx.size());
   if (maxv > 2048 || maxne > 8192 || maxnf > 8192) return fail(eng, "shape too large for the narrow phase");
@@ -555,7 +594,8 @@ int upload_model(rk_engine* eng) {
   if (upload(eng, eng->sh_nv, nv) || upload(eng, eng->sh_ne, ne) || upload(eng, eng->sh_nf, nf) ||
       upload(eng, eng->sh_voff, voff) || upload(eng, eng->sh_eoff, eoff) || upload(eng, eng->sh_foff, foff) ||
       upload(eng, eng->sh_edge, edge) || upload(eng, eng->sh_fstart, fstart) || upload(eng, eng->sh_fidx, fidx) ||
-      upload(eng, eng->sh_vtx, vtx) || upload(eng, eng->sh_radius, radius) || upload(eng, eng->sh_obb, obb))
+      upload(eng, eng->sh_vtx, vtx) || upload(eng, eng->sh_radius, radius) || upload(eng, eng->sh_obb, obb) ||
+      upload(eng, eng->sh_reach, reach))
     return -1;
   // bodies
   std::vector<BodyKin> kin(nb);

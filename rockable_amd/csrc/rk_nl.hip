@@ -702,6 +702,16 @@ __device__ __forceinline__ int group_sum(int x) {
   for (int off = SW / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, SW);
   return x;
 }
+template <int SW>
+__device__ __forceinline__ double group_max(double x) {
+  for (int off = SW / 2; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, SW));
+  return x;
+}
+template <int SW>
+__device__ __forceinline__ double group_min(double x) {
+  for (int off = SW / 2; off > 0; off >>= 1) x = fmin(x, __shfl_xor(x, off, SW));
+  return x;
+}
 
 // Candidate pairs of two point shapes (one vertex, no edge or face: the
 // spheres), one pair per lane.  Their only test is vertex-vertex, so their
@@ -834,9 +844,55 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
   const double sum = Ri + Rj + dmax;
   const double margin = 1e-9 * (1.0 + xmax + sum);
 
+  // Every near test of the pair is false when a plane separates the two
+  // bodies' vertices by more than the tests reach: a vertex-vertex or
+  // edge-edge test compares the distance of two points of the vertices'
+  // convex hulls with sum; a vertex-edge test takes a foot on the edge's line
+  // up to dVerlet edge lengths past its ends; a vertex-face test takes a
+  // foot in the face's outline projected on a plane the face's vertices lie
+  // within reach[1] of.  The planes tried are those normal to the line of
+  // centres, whose projection is 1-Lipschitz, so a gap along it above sum +
+  // those allowances + margin (rounding) leaves no test true.  Such pairs are
+  // most candidates in a settled packing; they get zero counts without a
+  // gate or test.  A NaN coordinate (or a zero axis) turns the test off.
+  bool apart = false;
+  if (valid) {
+    const BodyKin& Ki = d.kin[i];
+    const BodyKin& Kj = d.kin[j];
+    v3 ax = V(Kj.px - Ki.px, Kj.py - Ki.py, Kj.pz - Ki.pz);
+    const double a2 = vdot(ax, ax);
+    if (a2 > 0.0) {
+      ax = vsc(1.0 / sqrt(a2), ax);
+      double hi_i = -INFINITY, lo_j = INFINITY;
+      for (int v = sl; v < nvi; v += SW) {
+        const double q = vdot(ax, lds_v(gvi, v));
+        hi_i = q == q ? fmax(hi_i, q) : INFINITY;
+      }
+      for (int v = sl; v < nvj; v += SW) {
+        const double q = vdot(ax, lds_v(gvj, v));
+        lo_j = q == q ? fmin(lo_j, q) : -INFINITY;
+      }
+      hi_i = group_max<SW>(hi_i);
+      lo_j = group_min<SW>(lo_j);
+      const double* ri = d.sh.reach + 2 * shi;
+      const double* rj = d.sh.reach + 2 * shj;
+      const double reach = dmax * fmax(Ki.h * ri[0], Kj.h * rj[0]) + fmax(Ki.h * ri[1], Kj.h * rj[1]);
+      apart = lo_j - hi_i > sum + reach + margin;
+    }
+  }
+  if (P == 1 && apart) {
+    if (sl == 0) {
+      cbase[c] = 0;
+      cnf[c] = 0;
+      cnb[c] = 0;
+    }
+    return;
+  }
+
   // reference vertex gates of both bodies in one pass: i's vertices against
   // j's OBB, j's against i's
   const int nvg = group_compact<SW>(nvi + nvj, sl, g, vg_l, [&](int v) {
+    if (apart) return false;
     return v < nvi ? vertex_gate(lds_v(gvi, v), Ri, load_obb(obj), dmax)
                    : vertex_gate(lds_v(gvj, v - nvi), Rj, load_obb(obi), dmax);
   });
@@ -848,6 +904,7 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
   const float* gfi = cfg.gface + kGate * (size_t)d.bfoff[i];
   const float* gfj = cfg.gface + kGate * (size_t)d.bfoff[j];
   const int nefg = group_compact<SW>(nef, sl, g, ef_l, [&](int q) {
+    if (apart) return false;
     if (q < nei) return edge_gate_pre(gei + kGate * q, sum, dmax, margin, bxj, bxj + 3);
     if (q < ne2) return edge_gate_pre(gej + kGate * (q - nei), sum, dmax, margin, bxi, bxi + 3);
     if (q < ne2 + nfi) return face_gate_pre(gfi + kGate * (q - ne2), sum, margin, bxj, bxj + 3);
