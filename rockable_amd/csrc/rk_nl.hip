@@ -672,6 +672,19 @@ __device__ __forceinline__ bool face_gate_pre(const float* g, double sum, double
 
 
 
+// The largest (hi) or smallest projection on axis a of a point of an edge's
+// or face's box (the gate data: lo[3], hi[3] as floats).
+__device__ __forceinline__ double gate_box_reach(const float* g, v3 a, bool hi) {
+  const float4 p = reinterpret_cast<const float4*>(g)[0], q = reinterpret_cast<const float4*>(g)[1];
+  const double lo[3] = {p.x, p.y, p.z}, up[3] = {p.w, q.x, q.y}, ak[3] = {a.x, a.y, a.z};
+  double s = 0.0;
+  for (int k = 0; k < 3; k++) {
+    const double x = ak[k] * lo[k], y = ak[k] * up[k];
+    s += hi ? fmax(x, y) : fmin(x, y);
+  }
+  return s;
+}
+
 // floor(k / n) for 0 <= k < 2^20 and n >= 1, with inv = 1 / n in FP32:
 // (k + 0.5) / n lies at least 0.5 / n from an integer, and the FP32 product
 // is within 3e-7 of it relative, less than that for k < 2^20.  Larger k
@@ -855,29 +868,54 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
   // those allowances + margin (rounding) leaves no test true.  Such pairs are
   // most candidates in a settled packing; they get zero counts without a
   // gate or test.  A NaN coordinate (or a zero axis) turns the test off.
+  // The same bound culls single elements of the other pairs: a test of an
+  // element of i needs its hull's projection to reach lo_j - slack (lim_i),
+  // one of j's to reach down to hi_i + slack (lim_j); the culls apply it to
+  // the vertices and to the edges' and faces' boxes, which hold them.
   bool apart = false;
+  v3 ax = V(0.0, 0.0, 0.0);
+  double lim_i = -INFINITY, lim_j = INFINITY;
+  // bit v: vertex v within the limit (all set where not computed, shapes of
+  // more than SW vertices)
+  unsigned long long keep_i = ~0ull, keep_j = ~0ull;
   if (valid) {
     const BodyKin& Ki = d.kin[i];
     const BodyKin& Kj = d.kin[j];
-    v3 ax = V(Kj.px - Ki.px, Kj.py - Ki.py, Kj.pz - Ki.pz);
+    ax = V(Kj.px - Ki.px, Kj.py - Ki.py, Kj.pz - Ki.pz);
     const double a2 = vdot(ax, ax);
     if (a2 > 0.0) {
       ax = vsc(1.0 / sqrt(a2), ax);
-      double hi_i = -INFINITY, lo_j = INFINITY;
+      double hi_i = -INFINITY, lo_j = INFINITY, qi = 0.0, qj = 0.0;
       for (int v = sl; v < nvi; v += SW) {
-        const double q = vdot(ax, lds_v(gvi, v));
-        hi_i = q == q ? fmax(hi_i, q) : INFINITY;
+        qi = vdot(ax, lds_v(gvi, v));
+        hi_i = qi == qi ? fmax(hi_i, qi) : INFINITY;
       }
       for (int v = sl; v < nvj; v += SW) {
-        const double q = vdot(ax, lds_v(gvj, v));
-        lo_j = q == q ? fmin(lo_j, q) : -INFINITY;
+        qj = vdot(ax, lds_v(gvj, v));
+        lo_j = qj == qj ? fmin(lo_j, qj) : -INFINITY;
       }
       hi_i = group_max<SW>(hi_i);
       lo_j = group_min<SW>(lo_j);
       const double* ri = d.sh.reach + 2 * shi;
       const double* rj = d.sh.reach + 2 * shj;
       const double reach = dmax * fmax(Ki.h * ri[0], Kj.h * rj[0]) + fmax(Ki.h * ri[1], Kj.h * rj[1]);
-      apart = lo_j - hi_i > sum + reach + margin;
+      const double slack = sum + reach + margin;
+      apart = lo_j - hi_i > slack;
+      const double li = lo_j - slack, lj = hi_i + slack;
+      if (li == li) lim_i = li;
+      if (lj == lj) lim_j = lj;
+      if (nvi <= SW && nvj <= SW) {
+        // (qi, qj: the lane's own vertex; a NaN one is dropped, as its tests are false)
+        unsigned long long bi = __ballot(sl < nvi && qi >= lim_i), bj = __ballot(sl < nvj && qj <= lim_j);
+        if constexpr (SW < 64) {
+          bi = (bi >> (g * SW)) & ((1ull << SW) - 1ull);
+          bj = (bj >> (g * SW)) & ((1ull << SW) - 1ull);
+        }
+        keep_i = bi;
+        keep_j = bj;
+      }
+    } else {
+      ax = V(0.0, 0.0, 0.0);
     }
   }
   if (P == 1 && apart) {
@@ -889,15 +927,9 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
     return;
   }
 
-  // reference vertex gates of both bodies in one pass: i's vertices against
-  // j's OBB, j's against i's
-  const int nvg = group_compact<SW>(nvi + nvj, sl, g, vg_l, [&](int v) {
-    if (apart) return false;
-    return v < nvi ? vertex_gate(lds_v(gvi, v), Ri, load_obb(obj), dmax)
-                   : vertex_gate(lds_v(gvj, v - nvi), Rj, load_obb(obi), dmax);
-  });
-  // edges and faces of both bodies in one pass; tag = kind * 2^20 + index,
-  // kinds 0 edge of i, 1 edge of j, 2 face of i, 3 face of j
+  // edges and faces of both bodies in one pass, index q over i's edges, j's
+  // edges, i's faces, j's faces (before the vertices, so that ax and the
+  // limits are dead in the vertex gates' registers)
   const int ne2 = nei + nej, nef = ne2 + nfi + nfj;
   const float* gei = cfg.gedge + kGate * (size_t)d.beoff[i];
   const float* gej = cfg.gedge + kGate * (size_t)d.beoff[j];
@@ -905,17 +937,35 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
   const float* gfj = cfg.gface + kGate * (size_t)d.bfoff[j];
   const int nefg = group_compact<SW>(nef, sl, g, ef_l, [&](int q) {
     if (apart) return false;
-    if (q < nei) return edge_gate_pre(gei + kGate * q, sum, dmax, margin, bxj, bxj + 3);
-    if (q < ne2) return edge_gate_pre(gej + kGate * (q - nei), sum, dmax, margin, bxi, bxi + 3);
-    if (q < ne2 + nfi) return face_gate_pre(gfi + kGate * (q - ne2), sum, margin, bxj, bxj + 3);
-    return face_gate_pre(gfj + kGate * (q - ne2 - nfi), sum, margin, bxi, bxi + 3);
+    if (q < nei) {
+      const float* gb = gei + kGate * q;
+      return gate_box_reach(gb, ax, true) >= lim_i && edge_gate_pre(gb, sum, dmax, margin, bxj, bxj + 3);
+    }
+    if (q < ne2) {
+      const float* gb = gej + kGate * (q - nei);
+      return gate_box_reach(gb, ax, false) <= lim_j && edge_gate_pre(gb, sum, dmax, margin, bxi, bxi + 3);
+    }
+    if (q < ne2 + nfi) {
+      const float* gb = gfi + kGate * (q - ne2);
+      return gate_box_reach(gb, ax, true) >= lim_i && face_gate_pre(gb, sum, margin, bxj, bxj + 3);
+    }
+    const float* gb = gfj + kGate * (q - ne2 - nfi);
+    return gate_box_reach(gb, ax, false) <= lim_j && face_gate_pre(gb, sum, margin, bxi, bxi + 3);
+  });
+  // reference vertex gates of both bodies in one pass: i's vertices against
+  // j's OBB, j's against i's
+  const int nvg = group_compact<SW>(nvi + nvj, sl, g, vg_l, [&](int v) {
+    if (apart) return false;
+    if (v < nvi) return (v >= 64 || ((keep_i >> v) & 1ull)) && vertex_gate(lds_v(gvi, v), Ri, load_obb(obj), dmax);
+    const int u = v - nvi;
+    return (u >= 64 || ((keep_j >> u) & 1ull)) && vertex_gate(lds_v(gvj, u), Rj, load_obb(obi), dmax);
   });
   // vertex-vertex partners: vertices of j within sum of i's box
   const double gvv = sum + margin;
   const int nvj_n = group_compact<SW>(nvj, sl, g, vjn_l, [&](int v) {
     const v3 p = lds_v(gvj, v);
     return p.x >= bxi[0] - gvv && p.x <= bxi[3] + gvv && p.y >= bxi[1] - gvv && p.y <= bxi[4] + gvv &&
-           p.z >= bxi[2] - gvv && p.z <= bxi[5] + gvv;
+           p.z >= bxi[2] - gvv && p.z <= bxi[5] + gvv && (v >= 64 || ((keep_j >> v) & 1ull));
   });
   // split the compacted lists by kind (they are in ascending q, so each kind
   // is a contiguous run)
