@@ -37,6 +37,15 @@
 
 namespace rk {
 
+// A word pair in host-mapped memory that the displacement check writes for
+// the host to read without a copy or an event: done, the steps the device
+// has completed; frozen, set when it freezes the later steps.
+struct HostNote {
+  int32_t frozen;
+  int32_t pad;
+  long long done;
+};
+
 struct alignas(16) BodyKin {
   double px, py, pz;
   double qw, qx, qy, qz;
@@ -223,6 +232,7 @@ struct Dev {
   // device displacement guard (dynamicUpdateNL)
   int guard_on;
   DevGuard* guard;
+  HostNote* note;  // host-mapped (may be null): the check's progress and freeze, for the host
   double* gdpos;          // 3 per body
   double* gdq;            // 4 per body
   unsigned long long* gpart;  // per gather workgroup: max |dx|^2 key, min w key

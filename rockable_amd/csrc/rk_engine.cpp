@@ -17,6 +17,7 @@
 #include <fstream>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rockable_amd.h"
@@ -119,6 +120,11 @@ struct rk_engine {
   int64_t guard_reads = 0, guard_wasted = 0, guard_polls = 0;
   // asynchronous polls of the device check: pinned flags and their events
   int* poll_host = nullptr;
+  // host-mapped progress and freeze words of the check (the device writes
+  // them, the host reads them without a copy or an event); note_dev is the
+  // device address
+  HostNote* note = nullptr;
+  HostNote* note_dev = nullptr;
   // pinned staging of the guard read at the end of each batch (the guard
   // record and the breakage flag)
   struct GuardRead {
@@ -397,6 +403,7 @@ void assemble_dev(rk_engine* eng) {
   d.akin = E.akin;
   d.guard_on = eng->guard != nullptr;
   d.guard = eng->guard;
+  d.note = eng->note_dev;
   d.gdpos = eng->gdpos;
   d.gdq = eng->gdq;
   d.gpart = eng->gpart;
@@ -702,6 +709,7 @@ int upload_model(rk_engine* eng) {
     memset(&g0, 0, sizeof g0);
     g0.minw = ~0ull;
     if (upload(eng, eng->guard, std::vector<DevGuard>(1, g0))) return -1;
+    if (eng->note) *(volatile int32_t*)&eng->note->frozen = 0;
     eng->wstar = wstar_for(m.angleUpdateNL);
   } else {
     void* ps[] = {eng->guard, eng->gdpos, eng->gdq, eng->gpart};
@@ -1521,6 +1529,25 @@ int propose_batch(rk_engine* eng) {
   return 0;
 }
 
+// Waits until the device has completed target steps (the check's progress
+// word) or frozen the steps; 1 if frozen, 0 if not, -1 on an error.  The
+// deferred steps are launched first; a drained stream ends the wait (the
+// words are then final).
+int wait_progress(rk_engine* eng, int64_t target) {
+  if (flush_steps(eng) != 0) return -1;
+  const volatile HostNote* n = eng->note;
+  for (unsigned spin = 0;; spin++) {
+    if (n->frozen) return 1;
+    if (n->done >= target) return 0;
+    if ((spin & 255u) == 255u) {
+      const hipError_t q = hipStreamQuery(eng->stream);
+      if (q == hipSuccess) return n->frozen ? 1 : 0;
+      if (q != hipErrorNotReady) HIPCHK(q);
+    }
+    std::this_thread::yield();
+  }
+}
+
 // The integrate() loop with the device guard (dynamicUpdateNL, its
 // displacement check on the device, or bonds that may break).  Iterations
 // are enqueued in batches without host synchronisation; a batch ends where
@@ -1540,11 +1567,14 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
   while (done < nmax && (!until_tmax || m.t <= m.tmax)) {
     const bool bonds = m.law == LAW_STICKED && eng->live_interfaces > 0;
     // With the network the batch runs on until a host action or a rebuild is
-    // seen, the check polled asynchronously after each chunk of steps (a
-    // copy of the frozen flag to pinned memory and an event; at most
-    // kPollAhead polls in flight, so the host never waits on the GPU's
-    // current chunk).  Chunks follow the network's estimate of the steps to
-    // the next rebuild: long far from it, short near it.
+    // seen.  The check writes its progress and its freeze to host-mapped
+    // words, which the host reads after every step it enqueues; it lets the
+    // device fall at most a lag of steps behind (waiting on the progress
+    // word), the lag following the network's estimate of the steps to the
+    // next rebuild: long far from it, short near it, so that few steps are
+    // enqueued past a freeze.  Without the mapped words: a copy of the frozen
+    // flag to pinned memory and an event after each chunk of steps of that
+    // length, at most kPollAhead chunks in flight.
     const bool pipe = eng->model != nullptr && eng->poll_host != nullptr;
     const int64_t want = pipe ? nmax - done : eng->model ? eng->pred_batch : eng->guard_batch;
     const int64_t B = std::max<int64_t>(1, std::min<int64_t>(want, nmax - done));
@@ -1579,6 +1609,18 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       if ((conf_due && (outdir || !m.driving_path.empty())) || rec || eng->needUpdate ||
           m.interVerletC >= m.interVerlet - m.dt_2 || (until_tmax && m.t > m.tmax))
         break;
+      if (pipe && eng->note) {
+        const volatile HostNote* n = eng->note;
+        if (n->frozen) break;
+        const int64_t target = (int64_t)eng->steps - chunk_for(k);
+        if (n->done < target) {
+          eng->guard_polls++;
+          const int r = wait_progress(eng, target);
+          if (r < 0) return -1;
+          if (r == 1) break;
+        }
+        continue;
+      }
       if (pipe && --chunk_left == 0) {
         if (flush_steps(eng) != 0) return -1;
         const int slot = (head + inflight) % kPollSlots;
@@ -1624,6 +1666,7 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
       need = true;
       const int zero = 0;
       HIPCHK(hipMemcpy(&eng->guard->frozen, &zero, sizeof zero, hipMemcpyHostToDevice));
+      if (eng->note) *(volatile int32_t*)&eng->note->frozen = 0;
     }
     if (broke) {
       // a breakage freezes the steps after its own
@@ -1709,6 +1752,15 @@ int rk_engine_create(rk_engine** out, int device) {
   }
   if (hipHostMalloc((void**)&eng->guard_host, sizeof(rk_engine::GuardRead), hipHostMallocDefault) != hipSuccess)
     eng->guard_host = nullptr;
+  if (hipHostMalloc((void**)&eng->note, sizeof(HostNote), hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
+    memset(eng->note, 0, sizeof(HostNote));
+    if (hipHostGetDevicePointer((void**)&eng->note_dev, eng->note, 0) != hipSuccess) {
+      (void)hipHostFree(eng->note);
+      eng->note = eng->note_dev = nullptr;
+    }
+  } else {
+    eng->note = nullptr;
+  }
   *out = eng;
   return 0;
 }
@@ -1728,6 +1780,7 @@ void rk_engine_destroy(rk_engine* eng) {
     if (e) (void)hipEventDestroy(e);
   if (eng->poll_host) (void)hipHostFree(eng->poll_host);
   if (eng->guard_host) (void)hipHostFree(eng->guard_host);
+  if (eng->note) (void)hipHostFree(eng->note);
   if (eng->stream) (void)hipStreamDestroy(eng->stream);
   delete eng;
 }

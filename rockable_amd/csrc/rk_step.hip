@@ -1335,6 +1335,7 @@ __global__ void __launch_bounds__(kCheckBlock) k_guard_check(Dev d, int nparts) 
   DevGuard* G = d.guard;
   const long long step = G->step;
   G->step = step + 1;
+  if (d.note) *(volatile long long*)&d.note->done = step + 1;
   const int rebuild_now = G->nu;
   G->nu = 0;
   if (d.disp_check) {
@@ -1356,13 +1357,14 @@ __global__ void __launch_bounds__(kCheckBlock) k_guard_check(Dev d, int nparts) 
   if (rebuild_now || *(volatile const int*)d.flags != 0) {
     G->frozen = 1;
     G->frozen_step = step;
+    if (d.note) *(volatile int32_t*)&d.note->frozen = 1;
   }
 }
 
 // The 16 statistics the rebuild-interval network reads (per-body speed, spin,
 // acceleration and accumulated displacement, and the guard's running
 // maxima), scaled by the guard's thresholds.  Pass 1: per-block partials;
-// pass 2: one workgroup sums them in block order (no atomics).
+// pass 2: one workgroup reduces them in a fixed order (no atomics).
 constexpr int kFeatPart = 9;
 __global__ void __launch_bounds__(kBlock) k_guard_feat_part(Dev d, StepParams sp, double disp, double* part) {
   __shared__ double red[kFeatPart][kBlock];
@@ -1396,17 +1398,35 @@ __global__ void __launch_bounds__(kBlock) k_guard_feat_part(Dev d, StepParams sp
   if (t < kFeatPart) part[(size_t)blockIdx.x * kFeatPart + t] = red[t][0];
 }
 
-__global__ void __launch_bounds__(64) k_guard_feat_final(Dev d, StepParams sp, const double* part, int nparts,
-                                                         double disp, double angle, long long since,
-                                                         long long last, float* feat) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(kBlock) k_guard_feat_final(Dev d, StepParams sp, const double* part, int nparts,
+                                                             double disp, double angle, long long since,
+                                                             long long last, float* feat) {
+  // thread t takes the partials t, t + 256, ... in turn, then a fixed tree
+  // over the threads (the same order whatever the scheduling)
+  __shared__ double red[kFeatPart][kBlock];
+  const int t = threadIdx.x;
   double a[kFeatPart] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int q = 0; q < nparts; q++)
+  for (int q = t; q < nparts; q += kBlock)
+#pragma unroll
     for (int k = 0; k < kFeatPart; k++) {
       const double x = part[(size_t)q * kFeatPart + k];
       const bool mx = (k == 0 || k == 3 || k == 5);
       a[k] = mx ? fmax(a[k], x) : a[k] + x;
     }
+#pragma unroll
+  for (int k = 0; k < kFeatPart; k++) red[k][t] = a[k];
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    __syncthreads();
+    if (t < h)
+#pragma unroll
+      for (int k = 0; k < kFeatPart; k++) {
+        const bool mx = (k == 0 || k == 3 || k == 5);
+        red[k][t] = mx ? fmax(red[k][t], red[k][t + h]) : red[k][t] + red[k][t + h];
+      }
+  }
+  if (t != 0) return;
+#pragma unroll
+  for (int k = 0; k < kFeatPart; k++) a[k] = red[k][0];
   const double nb = d.nb > 0 ? (double)d.nb : 1.0;
   const double ds = disp > 0.0 ? disp : 1.0, an = angle > 0.0 ? angle : 1.0, dt = sp.dt;
   double dxmax = 0.0, rot = 0.0;
@@ -2170,7 +2190,7 @@ void launch_guard_features(const Dev& d, const StepParams& sp, double disp, doub
                            long long last, double* part, float* feat, hipStream_t s) {
   const int np = std::max(1, (d.nb + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_guard_feat_part, dim3(np), dim3(kBlock), 0, s, d, sp, disp, part);
-  hipLaunchKernelGGL(k_guard_feat_final, dim3(1), dim3(64), 0, s, d, sp, part, np, disp, angle, since, last, feat);
+  hipLaunchKernelGGL(k_guard_feat_final, dim3(1), dim3(kBlock), 0, s, d, sp, part, np, disp, angle, since, last, feat);
 }
 
 void launch_breakage(const Dev& d, const StepParams& sp, hipStream_t s) {
