@@ -308,9 +308,11 @@ def batched_form(a, log):
     walls, velocities in [-0.01, 0.01] m/s), each with its own Default-law mu
     and en, all in one engine: one launch per kernel and step covers every
     packing.  The launch shapes and the graph length are picked by the
-    autotuner on the running batch, then a.batched_steps steps are timed.
-    Beside it, a sample of the same packings run one engine after another
-    (the unbatched way)."""
+    autotuner on a first load of the batch, then a fresh load runs the
+    configuration's a.batched_steps steps from the start, timed (so that the
+    autotuned and the default shapes time the same work).  Beside it, a
+    sample of the same packings run one engine after another (the unbatched
+    way)."""
     import numpy as np
     from rockable_amd import decks as D
     from rockable_amd.engine import Engine
@@ -325,6 +327,12 @@ def batched_form(a, log):
     e.update_nl()
     ntune, choice = form_autotune(e, a)
     log("autotune: %d steps, %s" % (ntune, choice))
+    e.close()
+    e = Engine(path, device=0)
+    timed_options(e)
+    for k, v in choice.items():
+        e.set_option(k, v)
+    e.update_nl()
     c0 = e.counts()
     e.synchronize()
     t0 = time.perf_counter()
