@@ -1728,15 +1728,22 @@ __global__ void __launch_bounds__(kBlock) k_predict(Dev d, StepParams sp, int gl
   }
 }
 
-// The predictor by workgroups of 64 free bodies: one thread per body
-// computes and stores its new state and keeps the pose in LDS, then the 256
-// threads share those bodies' vertices (global vertices of the listed ones).
+// The predictor by workgroups of 64 free bodies.  The tile's 64 body records
+// (128 bytes each) are read and written whole by the 256 threads in
+// contiguous doubles through LDS (17 doubles per record there, so that the
+// per-body reads spread over the banks); one thread per body computes its
+// new state in between and keeps the pose in LDS; then the 256 threads share
+// those bodies' vertices (global vertices of the listed ones).  Rewriting a
+// record's unchanged fields (R, h, shape, group) stores the bits it read.
 // The driven bodies are moved by thread 0 of workgroup 0 (controls); with
 // driven_glob, workgroup 0 then writes their listed global vertices (else
 // k_glob does after this kernel).
 __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, int driven_glob) {
-  __shared__ double pose[64][8];  // px py pz qw qx qy qz h
-  __shared__ int shp[64];         // shape, or -1 if the body is not listed
+  constexpr int kRec = 17;          // LDS doubles per record
+  __shared__ double rec[64 * kRec];
+  __shared__ double pose[64][8];    // px py pz qw qx qy qz h
+  __shared__ int shp[64];           // shape, or -1 if the body is not listed
+  static_assert(sizeof(BodyKin) == 16 * sizeof(double), "a body record is 16 doubles");
   if (frozen(d)) return;
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
@@ -1755,14 +1762,48 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
   for (int tile = blockIdx.x; tile * 64 < nfree; tile += gridDim.x) {
     const int b0 = d.nDriven + tile * 64;
     const int bend = min(b0 + 64, d.nb);
+    const int nd = 16 * (bend - b0);
+    double* kd = reinterpret_cast<double*>(d.kin + b0);
+    for (int k = t; k < nd; k += blockDim.x) rec[(k >> 4) * kRec + (k & 15)] = kd[k];
+    __syncthreads();
     if (t < 64) {
       const int b = b0 + t;
       shp[t] = -1;
       if (b < bend) {
-        BodyKin P = d.kin[b];
+        double* r = rec + t * kRec;
+        BodyKin P;
+        P.px = r[0];
+        P.py = r[1];
+        P.pz = r[2];
+        P.qw = r[3];
+        P.qx = r[4];
+        P.qy = r[5];
+        P.qz = r[6];
+        P.vx = r[7];
+        P.vy = r[8];
+        P.vz = r[9];
+        P.wx = r[10];
+        P.wy = r[11];
+        P.wz = r[12];
+        P.R = r[13];
+        P.h = r[14];
+        P.shape = __double2loint(r[15]);
+        P.group = __double2hiint(r[15]);
         const BodyKin old = P;
         predict_free(d, sp, b, P);
-        store_predicted(d, b, P);
+        r[0] = P.px;
+        r[1] = P.py;
+        r[2] = P.pz;
+        r[3] = P.qw;
+        r[4] = P.qx;
+        r[5] = P.qy;
+        r[6] = P.qz;
+        r[7] = P.vx;
+        r[8] = P.vy;
+        r[9] = P.vz;
+        r[10] = P.wx;
+        r[11] = P.wy;
+        r[12] = P.wz;
         add_motion_bound(d, b, old, P);
         if (d.soff[b + 1] != d.soff[b]) {
           pose[t][0] = P.px;
@@ -1778,6 +1819,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
       }
     }
     __syncthreads();
+    for (int k = t; k < nd; k += blockDim.x) kd[k] = rec[(k >> 4) * kRec + (k & 15)];
     for (int k = d.bvoff[b0] + t; k < d.bvoff[bend]; k += blockDim.x) {
       const int b = d.bvbody[k], lb = b - b0;
       if (shp[lb] < 0) continue;
