@@ -186,9 +186,11 @@ int rk_get_option(rk_engine* eng, const char* name, double* value);
  * geom_block, gather_lanes, gather_groups, fuse_glob, gather_waves,
  * body_block, persistent, graph_steps, type_sort and skip_segment, one after the other: the
  * simulation runs on, trial after trial of steps_per_trial steps, each
- * candidate value twice, and the fastest (wall time per step, rebuilds
- * excluded) is kept; the shapes it started from are timed before and after
- * the search and kept if the search ends slower.  Every candidate computes
+ * candidate value twice, and the fastest is kept: by the step kernels' own
+ * durations per step (events around each kernel) for the kernel shapes, by
+ * wall time per step (rebuilds excluded) for persistent and graph_steps;
+ * the shapes it started from are timed (wall time) before and after the
+ * search and kept if the search ends slower.  Every candidate computes
  * the same state, so the steps taken are those rk_step would take;
  * *steps_done receives their number. */
 int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done);

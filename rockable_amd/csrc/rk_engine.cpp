@@ -2195,6 +2195,30 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
     *per_step = (w - (eng->nl_wall_s - nl0)) / steps_per_trial;
     return 0;
   };
+  // Per-kernel events (profile level 2) time the kernel-shape knobs: the sum of
+  // the step kernels' durations per step, which a trial's host work, guard
+  // reads and pipeline drains around a rebuild do not disturb.  The knobs of
+  // the launches themselves (persistent grids, graph length) are timed by
+  // wall time.  The caller's timers are kept aside meanwhile.
+  auto trial_kernels = [&](double* per_step) -> int {
+    resolve_events(eng);
+    auto saved = eng->timers;
+    eng->timers.clear();
+    eng->profile = 2;
+    const int r = rk_step(eng, steps_per_trial);
+    resolve_events(eng);
+    eng->profile = 0;
+    double ms = 0.0;
+    for (const char* n : {"predict", "interact", "gather", "guard"}) {
+      auto it = eng->timers.find(n);
+      if (it != eng->timers.end()) ms += it->second.ms;
+    }
+    eng->timers = saved;
+    if (r != 0) return -1;
+    done += steps_per_trial;
+    *per_step = 1e-3 * ms / steps_per_trial;
+    return 0;
+  };
   int rc = 0;
   // the starting shapes (the defaults or the caller's), timed first: if the
   // search ends slower than them (its trials are short and one knob at a
@@ -2226,7 +2250,8 @@ int rk_autotune(rk_engine* eng, int steps_per_trial, int64_t* steps_done) {
         double t = 0.0;
         // a first short run captures the graphs of the new shape, then the trial
         const int warm = std::max(1, eng->graph_steps);
-        if (rk_step(eng, warm) != 0 || trial(&t) != 0) {
+        const bool launch_knob = !strcmp(k.name, "persistent") || !strcmp(k.name, "graph_steps");
+        if (rk_step(eng, warm) != 0 || (launch_knob ? trial(&t) : trial_kernels(&t)) != 0) {
           rc = -1;
           break;
         }
