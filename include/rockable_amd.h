@@ -170,7 +170,7 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * 262144 bodies, gather_lanes from there), "fuse_glob" (1: the predictor
  * writes the free bodies' global vertices, one thread per body; 2: one
  * thread per vertex; 3: workgroups of 64 bodies, their vertices shared by the
- * workgroup; 0: a kernel of its own; -1, the default: 1 from 262144
+ * workgroup; 0: a kernel of its own; -1, the default: 0 from 262144
  * bodies, else 3), "gather_waves" (wavefronts per resultant
  * workgroup: 1, 2, 4; default 4), "body_block" (per-body kernels: 64, 128,
  * 256; default 256), "persistent" (0, the default: one tile per workgroup;

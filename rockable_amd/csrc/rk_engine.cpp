@@ -423,7 +423,7 @@ void assemble_dev(rk_engine* eng) {
     const bool large = d.nb >= kLargePacking;
     const int g = eng->gather_groups > 0 ? eng->gather_groups : large ? eng->gather_lanes : 1;
     d.gather_groups = std::min(g, eng->gather_lanes);
-    d.fuse_glob = eng->fuse_glob >= 0 ? eng->fuse_glob : large ? 1 : 3;
+    d.fuse_glob = eng->fuse_glob >= 0 ? eng->fuse_glob : large ? 0 : 3;
   }
   d.persist = eng->persist;
   d.istart = E.istart;

@@ -2086,8 +2086,9 @@ void launch_gather(const Dev& d, const StepParams& sp, int kick, int guard, hipS
 }
 
 // --- statistics of the packing (DataExtractor-style quantities) -----------
-// Partial sums per workgroup (a fixed tree in LDS), then one thread adds the
-// partials in block order: the results do not depend on scheduling.
+// Partial sums per workgroup (a fixed tree in LDS), then one workgroup
+// reduces the partials in a fixed order: the results do not depend on
+// scheduling.
 constexpr int kStatBody = 4;   // KE translation, KE rotation, max |v|, free bodies
 constexpr int kStatEntry = 5;  // active, contacts (dn < 0), active pairs, sum dn, min dn
 
@@ -2156,20 +2157,37 @@ __global__ void __launch_bounds__(kBlock) k_stat_entries(Dev d, double* part) {
 }
 
 // out[0..3] body statistics, out[4..8] entry statistics
-__global__ void k_stat_final(const double* pb, int nbp, const double* pe, int nep, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s[kStatBody + kStatEntry] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
-  for (int q = 0; q < nbp; q++) {
+__global__ void __launch_bounds__(kBlock) k_stat_final(const double* pb, int nbp, const double* pe, int nep,
+                                                       double* out) {
+  // thread t takes the partials t, t + 256, ... in turn, then a fixed tree
+  // over the threads (the same order whatever the scheduling)
+  constexpr int NS = kStatBody + kStatEntry;
+  __shared__ double red[NS][kBlock];
+  const int t = threadIdx.x;
+  double s[NS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
+  for (int q = t; q < nbp; q += kBlock) {
     s[0] += pb[kStatBody * q];
     s[1] += pb[kStatBody * q + 1];
     s[2] = fmax(s[2], pb[kStatBody * q + 2]);
     s[3] += pb[kStatBody * q + 3];
   }
-  for (int q = 0; q < nep; q++) {
+  for (int q = t; q < nep; q += kBlock) {
     for (int k = 0; k < 4; k++) s[kStatBody + k] += pe[kStatEntry * q + k];
     s[kStatBody + 4] = fmin(s[kStatBody + 4], pe[kStatEntry * q + 4]);
   }
-  for (int k = 0; k < kStatBody + kStatEntry; k++) out[k] = s[k];
+#pragma unroll
+  for (int k = 0; k < NS; k++) red[k][t] = s[k];
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    __syncthreads();
+    if (t < h)
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        const double a = red[k][t], b = red[k][t + h];
+        red[k][t] = k == 2 ? fmax(a, b) : k == NS - 1 ? fmin(a, b) : a + b;
+      }
+  }
+  __syncthreads();
+  if (t < NS) out[t] = red[t][0];
 }
 
 int stat_parts(const Dev& d) {
@@ -2181,7 +2199,7 @@ void launch_stats(const Dev& d, double* part, double* out, hipStream_t s) {
   double* pe = part + (size_t)nbp * kStatBody;
   hipLaunchKernelGGL(k_stat_bodies, dim3(nbp), dim3(kBlock), 0, s, d, part);
   hipLaunchKernelGGL(k_stat_entries, dim3(nep), dim3(kBlock), 0, s, d, pe);
-  hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(64), 0, s, part, nbp, pe, nep, out);
+  hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(kBlock), 0, s, part, nbp, pe, nep, out);
 }
 
 // Scratch of rk_get_stats_noalloc: partial sums and outputs of the
