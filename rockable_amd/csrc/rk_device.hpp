@@ -140,7 +140,9 @@ struct DevInterfaces {
   const double* par;    // 8 per interface: kn kt kr fn0 ft0 mom0 power dn0
   const int32_t* isInner;
   int32_t* alive;
-  uint8_t* brk;         // set by the force law when the yield criterion is met
+  uint8_t* brk;         // set by the force law when the yield criterion is met (cleared by k_breakage)
+  int32_t* broken;      // the interfaces k_breakage broke since the host last accounted them
+  int32_t* nbroken;     // their count
   const int32_t* bond_off;  // bonds of interface k: bond_key[bond_off[k] .. bond_off[k+1])
   const int32_t* bond_key;  // 3 per bond: type isub jsub
 };

@@ -82,6 +82,7 @@ struct rk_engine {
   double* ifc_par = nullptr;
   int32_t *ifc_inner = nullptr, *ifc_alive = nullptr, *ifc_bond_off = nullptr, *ifc_bond_key = nullptr;
   uint8_t* ifc_brk = nullptr;
+  int32_t *ifc_broken = nullptr, *ifc_nbroken = nullptr;  // k_breakage's list of broken interfaces
   DevControl* ctrl = nullptr;
   int32_t *ctrl_off = nullptr, *ctrl_idx = nullptr;
   int ctrl_parallel = 0;  // every control acts on a driven body
@@ -216,7 +217,7 @@ void free_device(rk_engine* eng) {
                 eng->sh_voff,  eng->sh_eoff,   eng->sh_foff,   eng->sh_edge,   eng->sh_fstart, eng->sh_fidx,
                 eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->sh_reach,  eng->ptab,      eng->damp_rate, eng->damp_kind,
                 eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
-                eng->ifc_brk,  eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
+                eng->ifc_brk,  eng->ifc_broken, eng->ifc_nbroken, eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
                 eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,      eng->gpart,
                 eng->beoff,    eng->bfoff,     eng->cbound,    eng->rmax,
                 eng->feat,     eng->pred,      eng->feat_part, eng->stat_part, eng->stat_out};
@@ -235,6 +236,7 @@ void free_device(rk_engine* eng) {
   eng->ifc_par = nullptr;
   eng->ifc_inner = eng->ifc_alive = eng->ifc_bond_off = eng->ifc_bond_key = nullptr;
   eng->ifc_brk = nullptr;
+  eng->ifc_broken = eng->ifc_nbroken = nullptr;
   eng->ctrl = nullptr;
   eng->ctrl_off = eng->ctrl_idx = nullptr;
   eng->flags = nullptr;
@@ -462,6 +464,8 @@ void assemble_dev(rk_engine* eng) {
   d.ifc.isInner = eng->ifc_inner;
   d.ifc.alive = eng->ifc_alive;
   d.ifc.brk = eng->ifc_brk;
+  d.ifc.broken = eng->ifc_broken;
+  d.ifc.nbroken = eng->ifc_nbroken;
   d.ifc.bond_off = eng->ifc_bond_off;
   d.ifc.bond_key = eng->ifc_bond_key;
   d.paramsInInterfaces = m.paramsInInterfaces;
@@ -758,7 +762,9 @@ int upload_model(rk_engine* eng) {
     std::vector<uint8_t> brk(m.ifc.size(), 0);
     if (upload(eng, eng->ifc_key, key) || upload(eng, eng->ifc_par, par) || upload(eng, eng->ifc_inner, inner) ||
         upload(eng, eng->ifc_alive, alive) || upload(eng, eng->ifc_bond_off, boff) ||
-        upload(eng, eng->ifc_bond_key, bkey) || upload(eng, eng->ifc_brk, brk))
+        upload(eng, eng->ifc_bond_key, bkey) || upload(eng, eng->ifc_brk, brk) ||
+        upload(eng, eng->ifc_broken, std::vector<int32_t>(std::max<size_t>(1, m.ifc.size()), 0)) ||
+        upload(eng, eng->ifc_nbroken, std::vector<int32_t>(1, 0)))
       return -1;
     eng->live_interfaces = 0;
     for (const Interface& BI : m.ifc) eng->live_interfaces += BI.alive;
@@ -910,19 +916,25 @@ void ensure_gv(rk_engine* eng, bool all) {
 // (breakage_of_interfaces, Rockable.cpp:3391-3410): the broken areas add to
 // totalBrokenArea in interface order, the host's alive flags and count follow
 // the device's, the breakage marks and flag are cleared, a rebuild is due.
+// The device lists the interfaces it broke (k_breakage, in no particular
+// order): the host reads that short list, not every alive flag, and sorts it.
 int account_breakage(rk_engine* eng) {
-  std::vector<int32_t> alive(eng->m.ifc.size());
-  HIPCHK(hipMemcpy(alive.data(), eng->ifc_alive, 4 * alive.size(), hipMemcpyDeviceToHost));
-  eng->live_interfaces = 0;
-  for (size_t k = 0; k < alive.size(); k++) {
-    if (eng->m.ifc[k].alive && !alive[k]) eng->m.totalBrokenArea += eng->m.ifc[k].area;
-    eng->m.ifc[k].alive = alive[k];
-    eng->live_interfaces += alive[k];
+  int32_t n = 0;
+  HIPCHK(hipMemcpy(&n, eng->ifc_nbroken, sizeof n, hipMemcpyDeviceToHost));
+  if (n < 0 || (size_t)n > eng->m.ifc.size()) return fail(eng, "breakage: inconsistent count of broken interfaces");
+  std::vector<int32_t> broken(n);
+  if (n > 0) HIPCHK(hipMemcpy(broken.data(), eng->ifc_broken, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  std::sort(broken.begin(), broken.end());
+  for (int32_t k : broken) {
+    if (k < 0 || (size_t)k >= eng->m.ifc.size() || !eng->m.ifc[k].alive)
+      return fail(eng, "breakage: a listed interface is not a live one");
+    eng->m.totalBrokenArea += eng->m.ifc[k].area;
+    eng->m.ifc[k].alive = 0;
+    eng->live_interfaces--;
   }
-  std::vector<uint8_t> zero(eng->m.ifc.size(), 0);
-  HIPCHK(hipMemcpy(eng->ifc_brk, zero.data(), zero.size(), hipMemcpyHostToDevice));
-  int32_t z = 0;
-  HIPCHK(hipMemcpy(eng->flags, &z, sizeof(z), hipMemcpyHostToDevice));
+  const int32_t z[2] = {0, 0};
+  HIPCHK(hipMemcpy(eng->ifc_nbroken, z, sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(eng->flags, z, sizeof(int32_t), hipMemcpyHostToDevice));
   eng->needUpdate = true;
   return 0;
 }

@@ -918,8 +918,10 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
       ax = V(0.0, 0.0, 0.0);
     }
   }
-  if (P == 1 && apart) {
-    if (sl == 0) {
+  // (with several pairs per wavefront, when every pair of it is apart; the
+  // workgroup is this one wavefront)
+  if (P == 1 ? apart : __all(apart || !valid)) {
+    if (valid && sl == 0) {
       cbase[c] = 0;
       cnf[c] = 0;
       cnb[c] = 0;

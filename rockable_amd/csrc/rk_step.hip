@@ -1888,7 +1888,9 @@ __device__ int find_entry(const Dev& d, int owner, int other, uint32_t sub) {
 __global__ void k_breakage(Dev d) {
   if (frozen(d)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.ifc.n || !d.ifc.brk[k] || !d.ifc.alive[k]) return;
+  if (k >= d.ifc.n || !d.ifc.brk[k]) return;
+  d.ifc.brk[k] = 0;
+  if (!d.ifc.alive[k]) return;
   const uint64_t key = d.ifc.key[k];
   const int bi = (int)(key >> 32), bj = (int)(key & 0xffffffffu);
   const int g1 = d.kin[bi].group, g2 = d.kin[bj].group;
@@ -1916,6 +1918,9 @@ __global__ void k_breakage(Dev d) {
     d.est[e] = s;
   }
   d.ifc.alive[k] = 0;
+  // for the host's accounting (it sorts them: interface order); an interface
+  // breaks once, so the list holds at most n
+  d.ifc.broken[atomicAdd(d.ifc.nbroken, 1)] = k;
   d.flags[0] = 1;  // needUpdate
 }
 
