@@ -142,8 +142,10 @@ int rk_set_body_force(rk_engine* eng, const char* name, const double* args, int 
  * tree, which changes only their reported force and moment by rounding),
  * "gravity_x", "gravity_y", "gravity_z" (the conf keyword gravity, changed
  * between steps),
- * "guard_batch" (with dynamicUpdateNL and no network: iterations enqueued
- * between two host reads of the device displacement check; default 16),
+ * "guard_batch" (with dynamicUpdateNL or live bonds and no network:
+ * iterations enqueued between two host reads of the device check; default 16;
+ * while bonds live the batches adapt between 2 and it: twice the iterations
+ * a batch ended by a freeze executed, doubled after one that was not),
  * "graph_steps" (n > 1: runs of n steps with no host action between them go
  * out as one HIP graph, captured once per device view and replayed; 0 or 1,
  * the default: every kernel launched on its own),

@@ -105,6 +105,10 @@ struct rk_engine {
   int gpart_cap = 0;                    // pairs
   double wstar = -2.0;
   int guard_batch = 16;  // steps enqueued between two reads of the guard
+  // while bonds live: the batch length, from 2 up to guard_batch, twice the
+  // steps a batch ended by a freeze executed, doubled after one that was not
+  // (breakages come in bursts, and each frozen step still costs launches)
+  int64_t bond_batch = 16;
   // rebuild-interval network (not owned) and its inputs / output on the device
   rk_mlp* model = nullptr;
   float* feat = nullptr;   // 16
@@ -1588,7 +1592,9 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     // flag to pinned memory and an event after each chunk of steps of that
     // length, at most kPollAhead chunks in flight.
     const bool pipe = eng->model != nullptr && eng->poll_host != nullptr;
-    const int64_t want = pipe ? nmax - done : eng->model ? eng->pred_batch : eng->guard_batch;
+    const int64_t want = pipe ? nmax - done
+                         : eng->model ? eng->pred_batch
+                         : bonds ? std::min<int64_t>(eng->bond_batch, eng->guard_batch) : eng->guard_batch;
     const int64_t B = std::max<int64_t>(1, std::min<int64_t>(want, nmax - done));
     hist.assign(1, loop_scalars(eng));
     conf_at.assign(1, 0);
@@ -1705,6 +1711,9 @@ int run_guarded(rk_engine* eng, const char* outdir, int64_t nmax, bool until_tma
     }
     done += executed;
     if (eng->model && !g.frozen) eng->model_trust = std::min(1.0, eng->model_trust * 1.1);
+    if (bonds)
+      eng->bond_batch = g.frozen ? std::max<int64_t>(2, 2 * executed)
+                                 : std::min<int64_t>(eng->guard_batch, 2 * std::max<int64_t>(1, eng->bond_batch));
     if (eng->model && propose_batch(eng) != 0) return -1;
   }
   if (ndone) *ndone = done;
