@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <utility>
 #include <vector>
@@ -1141,7 +1142,9 @@ __global__ void __launch_bounds__(64, P == 1 ? 6 : 3) k_narrow(Dev d, const doub
 
 // --- merge ---------------------------------------------------------------
 struct MergeIn {
-  int ng;
+  int ng;                    // groups (count pass with ng_pos: the groups' upper bound)
+  const int32_t* ng_pos;     // count pass: the group count is *ng_pos + *ng_flag (the
+  const int32_t* ng_flag;    // heads' scan and flag at the last key), else null
   const uint64_t* ukey;
   const int32_t* ucand;
   const int32_t* uold;
@@ -1199,10 +1202,24 @@ __device__ double pair_damp(const Dev& d, int i, int j) {
   return 2.0 * sqrt(kn * meff);
 }
 
+// The group count and the merged list's length (the count scan at it), for
+// one read with the narrow phase's totals.
+__global__ void k_merge_total(const int32_t* pos_last, const int32_t* flag_last, const int32_t* goff, int32_t* out) {
+  const int ng = *pos_last + *flag_last;
+  out[0] = ng;
+  out[1] = goff[ng];
+}
+
 template <bool WRITE>
 __global__ void k_merge(Dev d, MergeIn in, MergeOut out) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= in.ng) return;
+  // the count pass runs over the groups' upper bound (+1, the scan's
+  // trailing zero) and zeroes the counts past the last group
+  const int ng = (!WRITE && in.ng_pos) ? *in.ng_pos + *in.ng_flag : in.ng;
+  if (g >= ng) {
+    if (!WRITE && g <= in.ng) out.count[g] = 0;
+    return;
+  }
   const uint64_t key = in.ukey[g];
   const int owner = (int)(key >> 32), other = (int)(key & 0xffffffffu);
   const int cref = in.ucand[g], oref = in.uold[g];
@@ -1400,8 +1417,13 @@ __global__ void k_heavy_list(int nb, const int32_t* flag, const int32_t* pos, in
 // Chunks of 64 slots of the heavy bodies, and the count of heavy bodies that
 // are not passive.  One workgroup: thread 0 lays out the chunk offsets
 // (a few thousand heavy bodies at most), then the threads fill the chunks.
-__global__ void k_heavy_chunks(const int32_t* heavy, int nheavy, const int32_t* soff, const uint8_t* passive,
-                               int32_t* hch_off, int2* hch, int cap, int32_t* out) {
+// The heavy count is the flag scan's last value plus the last flag; out
+// receives the chunk count, the count of heavy bodies that are not passive
+// and the heavy count.
+__global__ void k_heavy_chunks(const int32_t* heavy, const int32_t* pos_last, const int32_t* flag_last,
+                               const int32_t* soff, const uint8_t* passive, int32_t* hch_off, int2* hch, int cap,
+                               int32_t* out) {
+  const int nheavy = *pos_last + *flag_last;
   if (threadIdx.x == 0) {
     int off = 0, np = 0;
     for (int h = 0; h < nheavy; h++) {
@@ -1413,6 +1435,7 @@ __global__ void k_heavy_chunks(const int32_t* heavy, int nheavy, const int32_t* 
     hch_off[nheavy] = off;
     out[0] = off;
     out[1] = np;
+    out[2] = nheavy;
   }
   __syncthreads();
   for (int h = threadIdx.x; h < nheavy; h += blockDim.x) {
@@ -1539,19 +1562,16 @@ int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hi
   E.nheavy_np = 0;
   if (nb <= 0 || !E.soff) return 0;
   if (ensure(w, E.heavy, E.heavy_cap, (size_t)nb) || ensure(w, w.i32a, w.i32a_cap, (size_t)nb + 1) ||
-      ensure(w, w.i32c, w.i32c_cap, (size_t)nb + 1))
+      ensure(w, w.i32c, w.i32c_cap, (size_t)nb + 1) || ensure(w, w.counter, w.counter_cap, 4))
     return -1;
   hipLaunchKernelGGL(k_heavy_flag, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, E.soff, w.heavy_min, w.i32a);
   if (scan_exclusive(w, w.i32a, w.i32c, nb, s)) return -1;
   hipLaunchKernelGGL(k_heavy_list, dim3(nblk(nb)), dim3(kBlock), 0, s, nb, w.i32a, w.i32c, E.heavy);
-  int32_t tail[2] = {0, 0};
-  if (read_small(w, s, {{&tail[0], {w.i32c + nb - 1, sizeof(int32_t)}}, {&tail[1], {w.i32a + nb - 1, sizeof(int32_t)}}}))
-    return -1;
-  E.nheavy = tail[0] + tail[1];
-  if (E.nheavy == 0) return 0;
-  // chunk table: at most (slots of all bodies) / 64 + one per heavy body
-  const size_t cap = (size_t)E.cap * 2 / 64 + (size_t)E.nheavy + 1;
-  if (ensure(w, E.hch, E.hch_cap, cap) || ensure(w, E.hch_off, E.hch_off_cap, (size_t)E.nheavy + 1)) return -1;
+  // chunk table sized as if every body were heavy: at most (slots of all
+  // bodies) / 64 + one per body, so that the heavy count is read once, with
+  // the chunks'
+  const size_t cap = (size_t)E.cap * 2 / 64 + (size_t)nb + 1;
+  if (ensure(w, E.hch, E.hch_cap, cap) || ensure(w, E.hch_off, E.hch_off_cap, (size_t)nb + 1)) return -1;
   if (E.hpart_cap < cap) {
     if (E.hpart) (void)hipFree(E.hpart);
     E.hpart = nullptr;
@@ -1562,16 +1582,17 @@ int nl_build_heavy(NLWork& w, int nb, EntryArrays& E, const uint8_t* passive, hi
     }
     E.hpart_cap = cap;
   }
-  hipLaunchKernelGGL(k_heavy_chunks, dim3(1), dim3(kBlock), 0, s, E.heavy, E.nheavy, E.soff, passive, E.hch_off,
-                     E.hch, (int)cap, w.i32c);
-  int32_t cnt[2] = {0, 0};
-  if (read_small(w, s, {{cnt, {w.i32c, 2 * sizeof(int32_t)}}})) return -1;
+  hipLaunchKernelGGL(k_heavy_chunks, dim3(1), dim3(kBlock), 0, s, E.heavy, w.i32c + nb - 1, w.i32a + nb - 1, E.soff,
+                     passive, E.hch_off, E.hch, (int)cap, w.counter);
+  int32_t cnt[3] = {0, 0, 0};
+  if (read_small(w, s, {{cnt, {w.counter, 3 * sizeof(int32_t)}}})) return -1;
   if ((size_t)cnt[0] > cap) {
     snprintf(w.err, sizeof(w.err), "heavy-body chunk table overflow");
     return -1;
   }
   E.nhch = cnt[0];
   E.nheavy_np = cnt[1];
+  E.nheavy = cnt[2];
   return 0;
 }
 
@@ -1680,7 +1701,8 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
   if (ensure(w, w.cbase, w.cbase_cap, (size_t)std::max(nc, 1)) || ensure(w, w.cnf, w.cnf_cap, (size_t)std::max(nc, 1)) ||
       ensure(w, w.cnb, w.cnb_cap, (size_t)std::max(nc, 1)))
     return -1;
-  int nnear = 0;
+  std::function<int(int)> launch_narrow;
+  int narrow_rcap = 0;
   if (nc > 0) {
     NarrowCfg cfg;
     cfg.maxv = p.maxv;
@@ -1704,8 +1726,9 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     if (P <= 0) P = 2 * p.maxv <= 16 ? 2 : 1;
     while (P > 1 && (size_t)P * stride > 65536) P >>= 1;
     const size_t lds = (size_t)P * stride;
-    for (int attempt = 0; attempt < 3; attempt++) {
-      const int rcap = std::max(64, (w.near_hint + kStripes - 1) / kStripes);
+    // one run of the narrow phase into stripes of rcap keys; the stripe
+    // totals (all, largest) land in counter[2..3]
+    launch_narrow = [&, cfg, P, stride, lds](int rcap) -> int {
       if (ensure(w, w.nearbuf, w.nearbuf_cap, (size_t)kStripes * rcap)) return -1;
       RK_CHECK(hipMemsetAsync(w.scnt, 0, sizeof(int) * kStripes * kStripeStride, s));
       // pairs of point shapes one per lane; the others (all of them without
@@ -1735,17 +1758,12 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
                            w.scnt, rcap, w.cbase, w.cnf, w.cnb, (int)stride, clist);
       RK_CHECK(hipGetLastError());
       hipLaunchKernelGGL(k_stripe_scan, dim3(1), dim3(kStripes), 0, s, w.scnt, w.soff, w.counter + 2);
-      int tm[2] = {0, 0};
-      if (read_small(w, s, {{tm, {w.counter + 2, sizeof(int) * 2}}})) return -1;
-      nnear = tm[0];
-      if (tm[1] <= rcap) break;
-      w.near_hint = kStripes * (tm[1] + tm[1] / 4 + 64);
-      if (attempt == 2) {
-        snprintf(w.err, sizeof(w.err), "narrow phase overflow");
-        return -1;
-      }
-    }
-    (void)nnear;
+      return 0;
+    };
+    // the stripes' overflow is checked with the merge's totals (one read);
+    // on overflow the phase runs again with room, before the merge counts
+    narrow_rcap = std::max(64, (w.near_hint + kStripes - 1) / kStripes);
+    if (launch_narrow(narrow_rcap)) return -1;
   }
 
   // ---- old groups
@@ -1791,15 +1809,13 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     if (scan_exclusive(w, w.gflag, w.gpos, ngk, s)) return -1;
     hipLaunchKernelGGL(k_group_compact, dim3(nblk(ngk)), dim3(kBlock), 0, s, ngk, w.gk2, w.gv2, w.gflag, w.gpos,
                        w.ukey, w.ucand, w.uold, bb);
-    int last[2];
-    if (read_small(w, s, {{&last[0], {w.gpos + ngk - 1, sizeof(int)}}, {&last[1], {w.gflag + ngk - 1, sizeof(int)}}}))
-      return -1;
-    ng = last[0] + last[1];
   }
 
   // ---- merge: count, scan, allocate, write
   MergeIn in;
-  in.ng = ng;
+  in.ng = ngk;
+  in.ng_pos = ngk > 0 ? w.gpos + ngk - 1 : nullptr;
+  in.ng_flag = ngk > 0 ? w.gflag + ngk - 1 : nullptr;
   in.ukey = w.ukey;
   in.ucand = w.ucand;
   in.uold = w.uold;
@@ -1824,12 +1840,39 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
   out.count = w.gcount;
   out.offset = w.goff;
   int nnew = 0;
-  if (ng > 0) {
-    RK_CHECK(hipMemsetAsync(w.gcount + ng, 0, sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_merge<false>, dim3(nblk(ng)), dim3(kBlock), 0, s, d, in, out);
-    if (scan_exclusive(w, w.gcount, w.goff, ng + 1, s)) return -1;
-    if (read_small(w, s, {{&nnew, {w.goff + ng, sizeof(int)}}})) return -1;
+  if (ngk > 0) {
+    // count pass over the groups' upper bound, its scan, then one read: the
+    // group count, the merged length and the narrow phase's stripe totals
+    auto count_pass = [&]() -> int {
+      hipLaunchKernelGGL(k_merge<false>, dim3(nblk((long)ngk + 1)), dim3(kBlock), 0, s, d, in, out);
+      if (scan_exclusive(w, w.gcount, w.goff, ngk + 1, s)) return -1;
+      hipLaunchKernelGGL(k_merge_total, dim3(1), dim3(1), 0, s, in.ng_pos, in.ng_flag, w.goff, w.counter);
+      return 0;
+    };
+    if (count_pass()) return -1;
+    int rd[4] = {0, 0, 0, 0};
+    if (read_small(w, s, {{rd, {w.counter, 4 * sizeof(int)}}})) return -1;
+    if (launch_narrow && rd[3] > narrow_rcap) {
+      for (int attempt = 0;; attempt++) {
+        if (attempt == 2) {
+          snprintf(w.err, sizeof(w.err), "narrow phase overflow");
+          return -1;
+        }
+        w.near_hint = kStripes * (rd[3] + rd[3] / 4 + 64);
+        narrow_rcap = std::max(64, (w.near_hint + kStripes - 1) / kStripes);
+        if (launch_narrow(narrow_rcap)) return -1;
+        if (read_small(w, s, {{rd + 2, {w.counter + 2, 2 * sizeof(int)}}})) return -1;
+        if (rd[3] <= narrow_rcap) break;
+      }
+      in.nearbuf = w.nearbuf;
+      if (count_pass()) return -1;
+      if (read_small(w, s, {{rd, {w.counter, 2 * sizeof(int)}}})) return -1;
+    }
+    ng = rd[0];
+    nnew = rd[1];
   }
+  in.ng = ng;
+  in.ng_pos = in.ng_flag = nullptr;
   if (entries_reserve(w, nxt, nnew) != 0) return -1;
   nxt.n = nnew;
   if (ng > 0 && nnew > 0) {
