@@ -1653,9 +1653,7 @@ __device__ void drive_controls_block(const Dev& d, const StepParams& sp) {
 
 // predict_free: the new pose and velocities of free body b (Rockable.cpp:
 // 2036-2055) in P; store_predicted writes them back
-__device__ __forceinline__ void predict_free(const Dev& d, const StepParams& sp, int b, BodyKin& P) {
-  const v3 a = ld3(d.acc + 3 * (size_t)b);
-  const v3 ar = ld3(d.arot + 3 * (size_t)b);
+__device__ __forceinline__ void predict_free_with(const StepParams& sp, BodyKin& P, v3 a, v3 ar) {
   v3 pos = vadd(kpos(P), vadd(vsc(sp.dt, kvel(P)), vsc(sp.dt2_2, a)));
   v3 vel = vadd(kvel(P), vsc(sp.dt_2, a));
   qt Q = kq(P);
@@ -1678,16 +1676,24 @@ __device__ __forceinline__ void predict_free(const Dev& d, const StepParams& sp,
   P.wy = w2.y;
   P.wz = w2.z;
 }
+__device__ __forceinline__ void predict_free(const Dev& d, const StepParams& sp, int b, BodyKin& P) {
+  predict_free_with(sp, P, ld3(d.acc + 3 * (size_t)b), ld3(d.arot + 3 * (size_t)b));
+}
 // The far-entry skip's motion bound of one predicted body: no vertex of it
 // moved by more than |dx| + 2 |dQ| rmax in this step (for unit quaternions
 // |Q' v Q'* - Q v Q*| <= 2 |Q' - Q| |v|), plus a rounding allowance.
-__device__ __forceinline__ void add_motion_bound(const Dev& d, int b, const BodyKin& old, const BodyKin& P) {
+__device__ __forceinline__ void add_motion_bound_with(const Dev& d, int b, const BodyKin& old, const BodyKin& P,
+                                                      double cb, double rmax) {
   if (!d.skip_far) return;
   const double dx = P.px - old.px, dy = P.py - old.py, dz = P.pz - old.pz;
   const double qw = P.qw - old.qw, qx = P.qx - old.qx, qy = P.qy - old.qy, qz = P.qz - old.qz;
-  const double step = sqrt(dx * dx + dy * dy + dz * dz) + 2.0 * sqrt(qw * qw + qx * qx + qy * qy + qz * qz) * d.rmax[b];
+  const double step = sqrt(dx * dx + dy * dy + dz * dz) + 2.0 * sqrt(qw * qw + qx * qx + qy * qy + qz * qz) * rmax;
   const double mag = fabs(P.px) + fabs(P.py) + fabs(P.pz);
-  d.cbound[b] += step + 1e-15 * (1.0 + mag);
+  d.cbound[b] = cb + (step + 1e-15 * (1.0 + mag));
+}
+__device__ __forceinline__ void add_motion_bound(const Dev& d, int b, const BodyKin& old, const BodyKin& P) {
+  if (!d.skip_far) return;
+  add_motion_bound_with(d, b, old, P, d.cbound[b], d.rmax[b]);
 }
 
 __device__ __forceinline__ void store_predicted(const Dev& d, int b, const BodyKin& P) {
@@ -1765,6 +1771,21 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
     const int nd = 16 * (bend - b0);
     double* kd = reinterpret_cast<double*>(d.kin + b0);
     for (int k = t; k < nd; k += blockDim.x) rec[(k >> 4) * kRec + (k & 15)] = kd[k];
+    // the body's other inputs loaded with the records, so that their
+    // latencies overlap
+    v3 acc = V(0.0, 0.0, 0.0), arot = acc;
+    double cb = 0.0, rmx = 0.0;
+    bool listed = false;
+    if (t < 64 && b0 + t < bend) {
+      const int b = b0 + t;
+      acc = ld3(d.acc + 3 * (size_t)b);
+      arot = ld3(d.arot + 3 * (size_t)b);
+      if (d.skip_far) {
+        cb = d.cbound[b];
+        rmx = d.rmax[b];
+      }
+      listed = d.soff[b + 1] != d.soff[b];
+    }
     __syncthreads();
     if (t < 64) {
       const int b = b0 + t;
@@ -1790,7 +1811,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
         P.shape = __double2loint(r[15]);
         P.group = __double2hiint(r[15]);
         const BodyKin old = P;
-        predict_free(d, sp, b, P);
+        predict_free_with(sp, P, acc, arot);
         r[0] = P.px;
         r[1] = P.py;
         r[2] = P.pz;
@@ -1804,8 +1825,8 @@ __global__ void __launch_bounds__(kBlock) k_predict_blk(Dev d, StepParams sp, in
         r[10] = P.wx;
         r[11] = P.wy;
         r[12] = P.wz;
-        add_motion_bound(d, b, old, P);
-        if (d.soff[b + 1] != d.soff[b]) {
+        add_motion_bound_with(d, b, old, P, cb, rmx);
+        if (listed) {
           pose[t][0] = P.px;
           pose[t][1] = P.py;
           pose[t][2] = P.pz;
