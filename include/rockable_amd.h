@@ -61,7 +61,7 @@ typedef struct {
   int64_t guard_wasted;    /* iterations enqueued after a rebuild fell due (done nothing) */
   int64_t graph_launches;  /* runs of graph_steps steps launched as one captured graph */
   int64_t graph_captures;  /* graphs captured (one per device view, cached) */
-  int64_t guard_polls;     /* asynchronous polls of the guard (with a rebuild network) */
+  int64_t guard_polls;     /* host waits on the guard's progress word (with a rebuild network) */
 } rk_counts;
 
 /* Statistics of the packing, reduced on the device in a fixed order

@@ -1568,8 +1568,8 @@ int wait_progress(rk_engine* eng, int64_t target) {
 // displacement check on the device, or bonds that may break).  Iterations
 // are enqueued in batches without host synchronisation; a batch ends where
 // the host has work (a conf file, a scheduled rebuild), after guard_batch
-// iterations (no network), or once an asynchronous poll has seen the check
-// fire (with a network).  When the device finds a rebuild due after
+// iterations (no network), or once the host has seen the check freeze the
+// steps (with a network).  When the device finds a rebuild due after
 // iteration s (the check fired the iteration before, or a bond broke in s),
 // the step kernels of the later iterations do nothing: the host reads the
 // guard at the end of the batch, keeps the loop scalars of iteration s,
