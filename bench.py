@@ -328,7 +328,9 @@ def batched_form(a, log):
     ntune, choice = form_autotune(e, a)
     log("autotune: %d steps, %s" % (ntune, choice))
     e.close()
+    t_load = time.perf_counter()
     e = Engine(path, device=0)
+    load_s = time.perf_counter() - t_load
     timed_options(e)
     for k, v in choice.items():
         e.set_option(k, v)
@@ -366,7 +368,7 @@ def batched_form(a, log):
             "workload": "%d packings of 500-20000 spheres (R 0.5 mm, 2.2 R lattice, floor + 4 walls each), "
                         "per-packing Default-law mu in [0.2, 0.8] and en in [0.1, 0.9], linkCells, "
                         "dt 1e-6 s, NL every 100 steps" % len(info),
-            "autotuned": choice, "autotune_steps": ntune, "roofline": roof,
+            "autotuned": choice, "autotune_steps": ntune, "roofline": roof, "deck_load_s": load_s,
             "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
             "graph_launches": c1["graph_launches"] - c0["graph_launches"],
             "unbatched_sample": {"packings": sample, "value": work / swall, "unit": "body-steps/s",
@@ -399,7 +401,9 @@ def bonds_form(a, log):
     ntune, choice = form_autotune(e, a)
     log("autotune: %d steps, %s" % (ntune, choice))
     e.close()
+    t_load = time.perf_counter()
     e = Engine(path, device=0)
+    load_s = time.perf_counter() - t_load
     timed_options(e)
     for k, v in choice.items():
         e.set_option(k, v)
@@ -417,6 +421,7 @@ def bonds_form(a, log):
     e.close()
     return {"value": nb * a.bond_steps / wall, "unit": "body-steps/s", "ms_per_step": 1e3 * wall / a.bond_steps,
             "steps": a.bond_steps, "bodies": nb, "autotuned": choice, "autotune_steps": ntune, "roofline": roof,
+            "deck_load_s": load_s,
             "interfaces_before": c0["ninterfaces"], "interfaces_after": c1["ninterfaces"],
             "broken_area": c1["total_broken_area"], "nl_rebuilds": c1["nl_updates"] - c0["nl_updates"],
             "active_per_body": c1["nactive"] / nb, "kinetic_energy": st["kinetic_translational"] +

@@ -15,8 +15,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
-#include <sstream>
 #include <string>
 #include <unordered_set>
 
@@ -48,24 +46,42 @@ void Model::set_param(int id, int g1, int g2, double v) {
 
 namespace {
 
+// The whole file in memory, NUL-terminated; numbers are parsed in place
+// (strtod / strtol stop at the whitespace that ends the token, so the value
+// is the one they give for the token alone).
 struct Reader {
   std::string buf;
   size_t pos = 0;
   bool open(const std::string& file) {
-    std::ifstream f(file, std::ios::binary);
+    FILE* f = fopen(file.c_str(), "rb");
     if (!f) return false;
-    std::ostringstream ss;
-    ss << f.rdbuf();
-    buf = ss.str();
+    buf.clear();
+    char chunk[1 << 16];
+    if (fseek(f, 0, SEEK_END) == 0) {
+      const long n = ftell(f);
+      if (n > 0) buf.reserve((size_t)n + 1);
+      rewind(f);
+    }
+    size_t k;
+    while ((k = fread(chunk, 1, sizeof chunk, f)) > 0) buf.append(chunk, k);
+    const bool ok = !ferror(f);
+    fclose(f);
     pos = 0;
-    return true;
+    return ok;
   }
   static bool ws(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; }
-  bool tok(std::string& out) {
+  // start of the next token, or false at the end of the file
+  bool skip() {
     while (pos < buf.size() && ws(buf[pos])) pos++;
-    if (pos >= buf.size()) return false;
-    size_t b = pos;
+    return pos < buf.size();
+  }
+  void past() {
     while (pos < buf.size() && !ws(buf[pos])) pos++;
+  }
+  bool tok(std::string& out) {
+    if (!skip()) return false;
+    size_t b = pos;
+    past();
     out.assign(buf, b, pos - b);
     return true;
   }
@@ -73,14 +89,16 @@ struct Reader {
     while (pos < buf.size() && buf[pos] != '\n') pos++;
   }
   double d() {
-    std::string t;
-    if (!tok(t)) return 0.0;
-    return strtod(t.c_str(), nullptr);
+    if (!skip()) return 0.0;
+    const double v = strtod(buf.c_str() + pos, nullptr);
+    past();
+    return v;
   }
   long l() {
-    std::string t;
-    if (!tok(t)) return 0;
-    return strtol(t.c_str(), nullptr, 10);
+    if (!skip()) return 0;
+    const long v = strtol(buf.c_str() + pos, nullptr, 10);
+    past();
+    return v;
   }
   v3 vec() {
     v3 v;
