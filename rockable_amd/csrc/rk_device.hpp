@@ -137,6 +137,7 @@ struct DevParams {
 struct DevInterfaces {
   int n;
   const uint64_t* key;  // (i << 32) | j, sorted
+  const int32_t* istart;  // the interfaces whose first body is i: [istart[i], istart[i + 1])
   const double* par;    // 8 per interface: kn kt kr fn0 ft0 mom0 power dn0
   const int32_t* isInner;
   int32_t* alive;

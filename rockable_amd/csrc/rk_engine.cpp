@@ -81,6 +81,7 @@ struct rk_engine {
   uint64_t* ifc_key = nullptr;
   double* ifc_par = nullptr;
   int32_t *ifc_inner = nullptr, *ifc_alive = nullptr, *ifc_bond_off = nullptr, *ifc_bond_key = nullptr;
+  int32_t* ifc_istart = nullptr;  // per body: its first interface as the lower index
   uint8_t* ifc_brk = nullptr;
   int32_t *ifc_broken = nullptr, *ifc_nbroken = nullptr;  // k_breakage's list of broken interfaces
   DevControl* ctrl = nullptr;
@@ -220,7 +221,7 @@ void free_device(rk_engine* eng) {
                 eng->inertia,  eng->bf_surface, eng->cluster,  eng->sh_nv,     eng->sh_ne,    eng->sh_nf,
                 eng->sh_voff,  eng->sh_eoff,   eng->sh_foff,   eng->sh_edge,   eng->sh_fstart, eng->sh_fidx,
                 eng->sh_vtx,   eng->sh_radius, eng->sh_obb,    eng->sh_reach,  eng->ptab,      eng->damp_rate, eng->damp_kind,
-                eng->ifc_key,  eng->ifc_par,   eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
+                eng->ifc_key,  eng->ifc_istart, eng->ifc_par,  eng->ifc_inner, eng->ifc_alive, eng->ifc_bond_off, eng->ifc_bond_key,
                 eng->ifc_brk,  eng->ifc_broken, eng->ifc_nbroken, eng->ctrl,      eng->ctrl_off,  eng->ctrl_idx,  eng->flags,    eng->passive,
                 eng->gv,       eng->bvoff,     eng->bvbody,    eng->guard,     eng->gdpos,    eng->gdq,      eng->gpart,
                 eng->beoff,    eng->bfoff,     eng->cbound,    eng->rmax,
@@ -237,6 +238,7 @@ void free_device(rk_engine* eng) {
   eng->ptab = eng->damp_rate = nullptr;
   eng->damp_kind = nullptr;
   eng->ifc_key = nullptr;
+  eng->ifc_istart = nullptr;
   eng->ifc_par = nullptr;
   eng->ifc_inner = eng->ifc_alive = eng->ifc_bond_off = eng->ifc_bond_key = nullptr;
   eng->ifc_brk = nullptr;
@@ -464,6 +466,7 @@ void assemble_dev(rk_engine* eng) {
   d.bf_surface = eng->bf_surface;
   d.ifc.n = (int)m.ifc.size();
   d.ifc.key = eng->ifc_key;
+  d.ifc.istart = eng->ifc_istart;
   d.ifc.par = eng->ifc_par;
   d.ifc.isInner = eng->ifc_inner;
   d.ifc.alive = eng->ifc_alive;
@@ -763,8 +766,12 @@ int upload_model(rk_engine* eng) {
       for (int x : BI.bkey) bkey.push_back(x);
       boff.push_back((int)bkey.size() / 3);
     }
+    // keys sorted by (i, j): a counting pass gives each body's range
+    std::vector<int32_t> istart((size_t)nb + 1, 0);
+    for (const Interface& BI : m.ifc) istart[(size_t)BI.i + 1]++;
+    for (int b = 0; b < nb; b++) istart[(size_t)b + 1] += istart[b];
     std::vector<uint8_t> brk(m.ifc.size(), 0);
-    if (upload(eng, eng->ifc_key, key) || upload(eng, eng->ifc_par, par) || upload(eng, eng->ifc_inner, inner) ||
+    if (upload(eng, eng->ifc_key, key) || upload(eng, eng->ifc_istart, istart) || upload(eng, eng->ifc_par, par) || upload(eng, eng->ifc_inner, inner) ||
         upload(eng, eng->ifc_alive, alive) || upload(eng, eng->ifc_bond_off, boff) ||
         upload(eng, eng->ifc_bond_key, bkey) || upload(eng, eng->ifc_brk, brk) ||
         upload(eng, eng->ifc_broken, std::vector<int32_t>(std::max<size_t>(1, m.ifc.size()), 0)) ||

@@ -135,18 +135,18 @@ __global__ void k_obb(Dev d, double* bobb, double* baabb) {
 }
 
 // Interfaces[i].find((i, j)) on the sorted live-interface keys.
+// Whether (i, j) has a live interface: the first key not below (i, j) among
+// i's interfaces (their keys are sorted by j), as a binary search over all
+// keys would find it, without its chain of dependent loads.
 __device__ __forceinline__ bool has_live_interface(const DevInterfaces& f, int i, int j) {
   if (f.n == 0) return false;
-  const uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
-  int lo = 0, hi = f.n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (f.key[mid] < key)
-      lo = mid + 1;
-    else
-      hi = mid;
+  const uint32_t uj = (uint32_t)j;
+  const int hi = f.istart[i + 1];
+  for (int k = f.istart[i]; k < hi; k++) {
+    const uint32_t kj = (uint32_t)(f.key[k] & 0xffffffffull);
+    if (kj >= uj) return kj == uj && f.alive[k];
   }
-  return lo < f.n && f.key[lo] == key && f.alive[lo];
+  return false;
 }
 
 // Output counters are striped: stripe s owns counter cnt[s * kStripeStride]
