@@ -138,11 +138,19 @@ __global__ void k_obb(Dev d, double* bobb, double* baabb) {
 // Whether (i, j) has a live interface: the first key not below (i, j) among
 // i's interfaces (their keys are sorted by j), as a binary search over all
 // keys would find it, without its chain of dependent loads.
-__device__ __forceinline__ bool has_live_interface(const DevInterfaces& f, int i, int j) {
-  if (f.n == 0) return false;
+// IfcRange holds i's range, loaded once for all of i's pairs.
+struct IfcRange {
+  int lo, hi;
+};
+__device__ __forceinline__ IfcRange ifc_range(const DevInterfaces& f, int i) {
+  IfcRange r;
+  r.lo = f.n ? f.istart[i] : 0;
+  r.hi = f.n ? f.istart[i + 1] : 0;
+  return r;
+}
+__device__ __forceinline__ bool has_live_interface(const DevInterfaces& f, IfcRange r, int j) {
   const uint32_t uj = (uint32_t)j;
-  const int hi = f.istart[i + 1];
-  for (int k = f.istart[i]; k < hi; k++) {
+  for (int k = r.lo; k < r.hi; k++) {
     const uint32_t kj = (uint32_t)(f.key[k] & 0xffffffffull);
     if (kj >= uj) return kj == uj && f.alive[k];
   }
@@ -226,8 +234,9 @@ __global__ void k_broad_brute(Dev d, const double* bobb, double half, uint64_t* 
   if (j0 < d.nDriven) j0 = d.nDriven;
   const int jb = j0 + (int)blockIdx.y * kBruteChunk;
   const int je = min(jb + kBruteChunk, d.nb);
+  const IfcRange ir = ifc_range(d.ifc, i);
   for (int j = jb; j < je; j++) {
-    const bool emit = pair_overlap(bobb, i, j, half) && !has_live_interface(d.ifc, i, j);
+    const bool emit = pair_overlap(bobb, i, j, half) && !has_live_interface(d.ifc, ir, j);
     emit_pair(emit, cand, cnt, rcap, i, j);
   }
 }
@@ -338,10 +347,11 @@ __global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, co
   const int nx = cx + q / 9 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q % 3 - 1;
   if (nx < 0 || ny < 0 || nz < 0 || nx >= g.N[0] || ny >= g.N[1] || nz >= g.N[2]) return;
   const int nc = (nx * g.N[1] + ny) * g.N[2] + nz;
+  const IfcRange ir = ifc_range(d.ifc, i);
   for (int r = cstart[nc]; r < cstart[nc + 1]; r++) {
     const int j = cbody[r];
     const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && pair_overlap(bobb, i, j, half) &&
-                      !has_live_interface(d.ifc, i, j);
+                      !has_live_interface(d.ifc, ir, j);
     emit_pair(emit, cand, cnt, rcap, i, j);
   }
 }
@@ -372,6 +382,7 @@ __global__ void k_broad_over(Dev d, const double* bobb, double half, Grid g, con
   }
   const int ny = hi[1] - lo[1] + 1;
   const int ncol = (hi[0] - lo[0] + 1) * ny;
+  const IfcRange ir = ifc_range(d.ifc, i);
   for (int col = blockIdx.y; col < ncol; col += gridDim.y) {
     const int cx = lo[0] + col / ny, cy = lo[1] + col % ny;
     const int cbase = (cx * g.N[1] + cy) * g.N[2];
@@ -383,7 +394,7 @@ __global__ void k_broad_over(Dev d, const double* bobb, double half, Grid g, con
       if (r < r1) {
         j = cbody[r];
         emit = !(j < d.nDriven && i < d.nDriven) && j > i && pair_overlap(bobb, i, j, half) &&
-               !has_live_interface(d.ifc, i, j);
+               !has_live_interface(d.ifc, ir, j);
       }
       emit_pair(emit, cand, cnt, rcap, i, j);
     }
@@ -601,14 +612,17 @@ __device__ __forceinline__ void store_gate(float* o, const double lo[3], const d
   reinterpret_cast<float4*>(o)[0] = a;
   reinterpret_cast<float4*>(o)[1] = b;
 }
-__global__ void k_gate_boxes(Dev d, float* gedge, float* gface) {
-  const int b = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
-  const int lane = threadIdx.x & 63;
+// 2^lg lanes per body (lg from 3 to 6: the shapes' largest edge plus face
+// count rounded up), each lane taking every 2^lg-th of its body's elements.
+__global__ void k_gate_boxes(Dev d, float* gedge, float* gface, int lg) {
+  const unsigned t = blockIdx.x * (unsigned)blockDim.x + threadIdx.x;
+  const int b = (int)(t >> lg);
+  const int lane = (int)(t & ((1u << lg) - 1u));
   if (b >= d.nb) return;
   const int shp = d.kin[b].shape;
   const int ne = d.sh.ne[shp], nf = d.sh.nf[shp];
   const double* gv = d.gv + 3 * (size_t)d.bvoff[b];
-  for (int k = lane; k < ne + nf; k += 64) {
+  for (int k = lane; k < ne + nf; k += 1 << lg) {
     double lo[3], hi[3], x;
     if (k < ne) {
       const int* ed = d.sh.edge + 2 * (d.sh.eoff[shp] + k);
@@ -1714,7 +1728,10 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
     if (ensure(w, w.gedge, w.gedge_cap, (size_t)kGate * std::max(d.netot, 1)) ||
         ensure(w, w.gface, w.gface_cap, (size_t)kGate * std::max(d.nftot, 1)))
       return -1;
-    hipLaunchKernelGGL(k_gate_boxes, dim3((d.nb + 3) / 4), dim3(kBlock), 0, s, d, w.gedge, w.gface);
+    int glg = 3;
+    while (glg < 6 && (1 << glg) < p.maxe + p.maxf) glg++;
+    hipLaunchKernelGGL(k_gate_boxes, dim3((int)((((long)d.nb << glg) + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d,
+                       w.gedge, w.gface, glg);
     cfg.gedge = w.gedge;
     cfg.gface = w.gface;
     // candidate pairs per wavefront: by default 2 for shapes of up to 8
