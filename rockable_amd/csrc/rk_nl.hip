@@ -331,27 +331,63 @@ __global__ void k_cell_fill(int nb, const int* cell_of, const int* cstart, int* 
   cbody[cstart[c] + atomicAdd(cfill + c, 1)] = b;
 }
 
-// main loop of UpdateNL_linkCells (Rockable.cpp:1697-1751): 32 threads per
-// body, thread q < 27 taking neighbour cell q (a thread per body alone
-// leaves the GPU nearly idle: a few hundred wavefronts, each looping over
-// 27 cells).  The candidates are sorted afterwards, so the order in which
+// main loop of UpdateNL_linkCells (Rockable.cpp:1697-1751): 32 lanes per
+// body.  Body i's 27 neighbour cells are 9 runs of consecutive cells (the
+// cells of a column along z are consecutive in the cell order), so its
+// candidates are the bodies of 9 ranges of cbody; lane q < 9 finds range q,
+// and the 32 lanes then take the ranges' bodies in turn, so that a lane is
+// idle only past the last one (a lane per cell, looping over that cell's
+// bodies, leaves the lanes of emptier cells idle while the fullest is
+// walked).  The candidates are sorted afterwards, so the order in which
 // they are found does not matter.
+constexpr int kCellLanes = 32;
 __global__ void k_broad_cells(Dev d, const double* bobb, double half, Grid g, const int* cell_of, const int* cstart,
                               const int* cbody, uint64_t* cand, int* cnt, int rcap) {
+  constexpr int L = kCellLanes;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = (int)(t >> 5), q = (int)(t & 31);
-  if (i >= d.nb || q >= 27) return;
-  const int c = cell_of[i];
-  if (c < 0) return;
-  const int cz = c % g.N[2], cy = (c / g.N[2]) % g.N[1], cx = c / (g.N[2] * g.N[1]);
-  const int nx = cx + q / 9 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q % 3 - 1;
-  if (nx < 0 || ny < 0 || nz < 0 || nx >= g.N[0] || ny >= g.N[1] || nz >= g.N[2]) return;
-  const int nc = (nx * g.N[1] + ny) * g.N[2] + nz;
-  const IfcRange ir = ifc_range(d.ifc, i);
-  for (int r = cstart[nc]; r < cstart[nc + 1]; r++) {
-    const int j = cbody[r];
-    const bool emit = !(j < d.nDriven && i < d.nDriven) && j > i && pair_overlap(bobb, i, j, half) &&
-                      !has_live_interface(d.ifc, ir, j);
+  const int i = (int)(t / L), q = (int)(t % L);
+  const int c = i < d.nb ? cell_of[i] : -1;
+  int r0 = 0, len = 0;
+  if (c >= 0 && q < 9) {
+    const int cz = c % g.N[2], cy = (c / g.N[2]) % g.N[1], cx = c / (g.N[2] * g.N[1]);
+    const int nx = cx + q / 3 - 1, ny = cy + q % 3 - 1;
+    if (nx >= 0 && ny >= 0 && nx < g.N[0] && ny < g.N[1]) {
+      const int base = (nx * g.N[1] + ny) * g.N[2];
+      const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz + 1 < g.N[2] ? cz + 1 : g.N[2] - 1;
+      r0 = cstart[base + z0];
+      len = cstart[base + z1 + 1] - r0;
+    }
+  }
+  // inclusive prefix of the range lengths over lanes 0-8 of the body
+  int pre = len;
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) {
+    const int x = __shfl_up(pre, o, L);
+    if (q >= o) pre += x;
+  }
+  // every lane: range k's end in the concatenation and its shift into cbody
+  int pend[9], poff[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    pend[k] = __shfl(pre, k, L);
+    poff[k] = __shfl(r0 - (pre - len), k, L);
+  }
+  const int total = pend[8];
+  const bool own = c >= 0;
+  const IfcRange ir = own ? ifc_range(d.ifc, i) : IfcRange{0, 0};
+  for (int base = 0; __any(own && base < total); base += L) {
+    const int x = base + q;
+    bool emit = false;
+    int j = 0;
+    if (own && x < total) {
+      int off = poff[8];
+#pragma unroll
+      for (int k = 7; k >= 0; k--)
+        if (x < pend[k]) off = poff[k];
+      j = cbody[x + off];
+      emit = !(j < d.nDriven && i < d.nDriven) && j > i && pair_overlap(bobb, i, j, half) &&
+             !has_live_interface(d.ifc, ir, j);
+    }
     emit_pair(emit, cand, cnt, rcap, i, j);
   }
 }
@@ -1682,7 +1718,7 @@ int nl_rebuild(NLWork& w, Dev& d, const NLParams& p, EntryArrays& cur, EntryArra
       if (read_small(w, s, {{hv, {w.counter + 1, sizeof(int)}}})) return -1;
       const int nover = hv[0];
       const int ncb = nb - nover;
-      hipLaunchKernelGGL(k_broad_cells, dim3(nblk(32L * nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
+      hipLaunchKernelGGL(k_broad_cells, dim3(nblk((long)kCellLanes * nb)), dim3(kBlock), 0, s, d, w.bobb, half, g, w.cell_of, w.cstart,
                          w.cbody, w.cstripe, w.scnt, rcap);
       if (nover > 0 && ncb > 0) {
         // enough workgroups to fill the chip when one body (a wall or a plane
