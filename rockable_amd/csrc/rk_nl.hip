@@ -1385,21 +1385,49 @@ __global__ void k_merge_fill(Dev d, int n, MergeIn in, MergeOut out) {
 // --- CSR -----------------------------------------------------------------
 // Counts of a sorted key array without atomics (a wall owning thousands of
 // entries would serialise them): the last entry of each run writes the run's
-// length, its first index found by binary search.
+// length.  Its first index is found by 8-way searches, each round's 8 loads
+// independent: probes 1..8 spans back from the end, the span growing 8-fold
+// until a probe leaves the run, then the interval holding the run's start
+// cut into 8 per round (a few rounds of load latency for a run of tens of
+// entries, against a binary search's chain of log2(n) dependent loads).
 __global__ void k_count_sorted(int n, const int32_t* key, int32_t* cnt) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int k = key[e];
   if (e + 1 < n && key[e + 1] == k) return;
-  int lo = 0, hi = e;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (key[mid] < k)
-      lo = mid + 1;
-    else
-      hi = mid;
+  // invariant: key[hi] == k, and lo < hi is -1 or an index whose key is not k
+  int hi = e, lo = -1;
+  for (long span = 1;; span *= 8) {
+    int out = 9;
+#pragma unroll
+    for (int q = 8; q >= 1; q--) {
+      const long x = (long)e - span * q;
+      if (x < 0 || key[x] != k) out = q;
+    }
+    if (out <= 8) {
+      hi = (int)((long)e - span * (out - 1));
+      const long l = (long)e - span * out;
+      lo = l < -1 ? -1 : (int)l;
+      break;
+    }
   }
-  cnt[k] = e - lo + 1;
+  while (hi - lo > 1) {
+    const int st = (hi - lo + 7) / 8;
+    int nlo = lo, nhi = hi;
+#pragma unroll
+    for (int q = 1; q < 8; q++) {
+      const int x = lo + st * q;
+      if (x < hi) {
+        if (key[x] == k)
+          nhi = x < nhi ? x : nhi;
+        else
+          nlo = x > nlo ? x : nlo;
+      }
+    }
+    lo = nlo;
+    hi = nhi;
+  }
+  cnt[k] = e - hi + 1;
 }
 __global__ void k_iota(int n, int32_t* v) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
