@@ -66,3 +66,17 @@ def test_half_a_million_interfaces_load_in_linear_time(tmp_path):
     elapsed = time.perf_counter() - t0
     assert info["ninterfaces"] == 500000
     assert elapsed < 20.0, elapsed
+
+
+def test_reader_takes_tabs_crlf_and_a_last_token_without_newline(tmp_path):
+    """The reader parses numbers in place in the file's buffer: a deck with
+    CRLF line ends, tabs between fields and no newline after its last token
+    reads as the plain one does."""
+    path = _spheres_deck(str(tmp_path), 5)
+    plain = inspect_conf(path)
+    text = open(path).read()
+    odd = text.replace(" ", "\t").replace("\n", "\r\n").rstrip()
+    odd_path = os.path.join(str(tmp_path), "odd.conf")
+    with open(odd_path, "w", newline="") as f:
+        f.write(odd)
+    assert inspect_conf(odd_path) == plain
